@@ -1,0 +1,145 @@
+/*
+ * diablo_gpu.h -- C-ABI of the MI355X implementation of DIABLO 3's channel
+ * RK3/Crank-Nicolson substep (rk_chan_1) and of the CPU oracle that mirrors it.
+ *
+ * Reference interface being replaced (reference tree, Jamie-Hilditch/diablo3_test):
+ *   - for/channel.f90:10-917    subroutine rk_chan_1 (no arguments; module globals)
+ *   - for/diablo.f90:98-111     the driver loop that calls it 3x per step and then
+ *                               advances `time` by delta_t
+ *   - for/channel.f90:946-1023  subroutine rem_div_chan
+ *   - for/boundary.f90:824-838  subroutine apply_BC_vel_mpi_post
+ *   - for/advance.f90:17-44     subroutine pre_first_step (post BCs + rem_div)
+ * The reference passes everything through module globals; here the globals that
+ * the substep reads become `dgpu_params` + the y-grid, and the state arrays
+ * (flow.f90:20-25, 50-51) are copied in and out through dgpu_set_field /
+ * dgpu_get_field in the reference's own memory layout:
+ *
+ *   spectral field  complex128, Fortran order (0:Nx/2, 0:Nz+1, 0:Ny+1)
+ *                   == (Nx+2)*(Nz+2)*(Ny+2) doubles (fft.f90:72-97, alloc_array3D);
+ *                   only i<=Nx/3, k<=2*(Nz/3) carry data (fft.f90:300-313)
+ *   physical field  float64, Fortran order (0:Nx+1, 0:Nz+1, 0:Ny+1)   (nu_t, kappa_t)
+ *   scalars         the field array repeated n_th times (alloc_array4D, fft.f90:101-129)
+ *
+ * All entry points return 0 on success and a negative DGPU_E* code on error;
+ * dgpu_last_error() returns the message of the last failure on that handle.
+ * No C++ exception crosses this boundary.  Calls on one handle are not
+ * re-entrant; separate handles are independent.  One handle owns all of its
+ * device memory and one HIP stream; host buffers are borrowed for the duration
+ * of a call only.
+ *
+ * The CPU oracle (oracle/liboracle.so) exports the same functions with the
+ * prefix `orc_` instead of `dgpu_`; it is test infrastructure only.
+ */
+#ifndef DIABLO_GPU_H
+#define DIABLO_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DGPU_MAX_TH 4
+
+/* Error codes. */
+#define DGPU_OK 0
+#define DGPU_EINVAL (-1)   /* bad argument (size, field id, null pointer, ...) */
+#define DGPU_EHIP (-2)     /* a HIP runtime call failed */
+#define DGPU_ENOMEM (-3)   /* device allocation failed */
+#define DGPU_EUNSUP (-4)   /* configuration not supported by this build */
+#define DGPU_ESTATE (-5)   /* non-finite value detected in the state */
+
+/* Precision modes for dgpu_create. */
+#define DGPU_FP64 0  /* everything in FP64 (parity path) */
+#define DGPU_MIXED 1 /* FFTs, nonlinear products and LES in FP32; solves,
+                        mean-pressure-gradient forcing, projection and the
+                        state itself in FP64 */
+
+/* Field ids for dgpu_set_field / dgpu_get_field. */
+#define DGPU_CU1 0
+#define DGPU_CU2 1
+#define DGPU_CU3 2
+#define DGPU_CP 3
+#define DGPU_CTH 4   /* n_th scalars, one after another */
+#define DGPU_CF1 5   /* RK3 carry of the explicit terms (cf1..3, cfth) */
+#define DGPU_CF2 6
+#define DGPU_CF3 7
+#define DGPU_CFTH 8
+#define DGPU_NU_T 9      /* physical, eddy viscosity at GY points */
+#define DGPU_KAPPA_T 10  /* physical, eddy diffusivity, n_th of them */
+#define DGPU_NFIELDS 11
+
+/*
+ * Run parameters: the module globals of for/parameters.f90:23-82 that
+ * rk_chan_1 reads, with the derived values of parameters.f90:101-103 and
+ * set_parameters (parameters.f90:241-281) already applied by the caller.
+ * Boundary-condition integers: 0 = Dirichlet, 1 = Neumann (channel.f90:1545-1876).
+ * v_bc_* is the wall-normal velocity u2, w_bc_* the spanwise u3 (note the
+ * TOML V/W swap at parameters.f90:373-382: TOML "V_*" feeds w_bc_*).
+ */
+typedef struct dgpu_params {
+  int32_t nx, ny, nz, n_th;   /* Nx, Ny (wall-normal points), Nz: Nx, Nz powers of two */
+  double lx, lz;              /* periodic box lengths */
+  double re;                  /* nu = 1/re (parameters.f90:101) */
+  double nu_v_scale;          /* vertical/horizontal viscosity ratio */
+  double beta;                /* horizontal Laplacian exponent (1 = ordinary viscosity) */
+  double ro_inv, delta;       /* f-plane Coriolis 1/Ro and aspect ratio */
+  double grav_x, grav_y, grav_z;
+  double delta_t;             /* fixed time step (h_bar = delta_t*(8,2,5)/15) */
+  int32_t f_type;             /* 1 const px0, 2 oscillating px0, 4 oscillating top stress */
+  int32_t les_model_type;     /* 1 Smagorinsky, 4 AMD-Rozema, 5 AMD-Verstappen */
+  double px0, omega0, amp_omega0, force_start;
+  int32_t u_bc_ymin, v_bc_ymin, w_bc_ymin;
+  int32_t u_bc_ymax, v_bc_ymax, w_bc_ymax;
+  double u_bc_ymin_c1, v_bc_ymin_c1, w_bc_ymin_c1;
+  double u_bc_ymax_c1, v_bc_ymax_c1, w_bc_ymax_c1;
+  double pr[DGPU_MAX_TH];
+  int32_t th_bc_ymin[DGPU_MAX_TH];
+  int32_t th_bc_ymax[DGPU_MAX_TH];
+  double th_bc_ymin_c1[DGPU_MAX_TH];
+  double th_bc_ymax_c1[DGPU_MAX_TH];
+  double dthdx[DGPU_MAX_TH], dthdz[DGPU_MAX_TH]; /* background scalar gradients */
+  int32_t use_les;
+  int32_t create_new_flow;    /* LES gate: LES only if !create_new_flow || time_step>100 */
+} dgpu_params;
+
+typedef struct dgpu_handle dgpu_handle;
+
+/*
+ * Create a solver.  gy holds the Ny+1 wall-normal grid points GY(1..Ny+1)
+ * exactly as stored in grid.h5:/grids/y (domain.f90:429-483); GYF, DY, DYF and
+ * their ghost values are derived as in domain.f90:230-254, 501-527.
+ * Replaces init_mpi/create_grid_chan/init_chan_mpi/init_flow/init_fft
+ * (domain.f90:55-337, flow.f90:58-88, fft.f90:133-241) for one GPU.
+ */
+int dgpu_create(const dgpu_params* p, const double* gy, int precision, dgpu_handle** out);
+int dgpu_destroy(dgpu_handle* h);
+const char* dgpu_last_error(const dgpu_handle* h);
+
+/* Copy one field between a host buffer in the reference layout and the device. */
+int dgpu_set_field(dgpu_handle* h, int field, const double* host);
+int dgpu_get_field(dgpu_handle* h, int field, double* host);
+
+/* apply_BC_vel_mpi_post (boundary.f90:299-498, 824-838) on cu1..3. */
+int dgpu_apply_bc_vel_post(dgpu_handle* h, double time);
+/* rem_div_chan (channel.f90:946-1023): project cu1..3 onto the discrete
+   divergence-free space.  Does not touch cp. */
+int dgpu_rem_div(dgpu_handle* h);
+
+/* One RK3 substep: `call rk_chan_1` with rk_step in 1..3 (channel.f90:10-917). */
+int dgpu_rk_substep(dgpu_handle* h, int rk_step, double time, int time_step);
+/* nsteps full steps of the driver loop (diablo.f90:98-108): time_step += 1,
+   three substeps, time += delta_t.  *time and *time_step are updated. */
+int dgpu_step(dgpu_handle* h, int nsteps, double* time, int32_t* time_step);
+
+/* Wait for all work queued on the handle's stream. */
+int dgpu_synchronize(dgpu_handle* h);
+
+/* Sizes of the reference-layout buffers (in doubles) for a field id. */
+int64_t dgpu_field_doubles(const dgpu_handle* h, int field);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DIABLO_GPU_H */
