@@ -1,0 +1,160 @@
+"""Throughput of the channel RK3/CN step on one MI355X.
+
+A step is one pass of the driver loop (for/diablo.f90:98-108): three
+rk_chan_1 substeps over the whole field, which stays resident in HBM.
+The metric is Mpts.RK3-steps/s = Nx*Ny*Nz * steps / seconds / 1e6.
+
+Beside the GPU number the line carries
+* roofline: the dominant kernel class's algorithmic HBM bytes over its
+  kernel time, timed live with HIP events on the library's stream;
+* cpu_baseline: the CPU oracle (C++ FP64 restatement, OpenMP) timed on the
+  host cores on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mpts·RK3-steps/s at 512×257×512: FP64 plain vs mixed+adjoint; % HBM peak"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(p, gy, st, npts):
+    """One RK3 step (three rk_chan_1 substeps) of the same state on the CPU oracle."""
+    import ctypes
+
+    from diablo3_test_amd.binding import ChannelSolver, _declare
+
+    lib = _declare(ctypes.CDLL(os.path.join(ROOT, "oracle", "liboracle.so")), "orc_", with_precision=False)
+    o = ChannelSolver(p, gy, lib=lib, prefix="orc_")
+    o.set_state(st)
+    o.pre_first_step()
+    t0 = time.perf_counter()
+    o.step(1)
+    dt = time.perf_counter() - t0
+    o.close()
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {
+        "value": npts / dt / 1e6,
+        "unit": "Mpts·RK3-steps/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"1 RK3 step of the same state, {dt:.1f} s wall, OpenMP threads={cores}",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="dns_512x257x512")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--timing-steps", type=int, default=2)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU "
+              "sharding of this path is a design note, not a build", file=sys.stderr)
+        sys.exit(2)
+
+    from diablo3_test_amd import workloads
+    from diablo3_test_amd.binding import ChannelSolver
+
+    p, gy, ic, seed, amp = workloads.workload(args.workload)
+    npts = p["nx"] * p["ny"] * p["nz"]
+    log(f"bench: {args.workload} {p['nx']}x{p['ny']}x{p['nz']} precision={args.precision}")
+    st = workloads.initial_state(p, gy, ic, seed, amp)
+    g = ChannelSolver(p, gy, precision=args.precision)
+    g.set_state(st)
+    g.pre_first_step()
+    g.synchronize()
+    log("bench: state resident, warming up")
+    g.step(args.warmup)
+    g.synchronize()
+    t0 = time.perf_counter()
+    g.step(args.steps)
+    g.synchronize()
+    t1 = time.perf_counter()
+    sec = t1 - t0
+    ms_per_step = sec / args.steps * 1e3
+    value = npts * args.steps / sec / 1e6
+    log(f"bench: {args.steps} steps in {sec:.3f} s, {ms_per_step:.2f} ms/step")
+
+    # per-kernel-class timing on extra steps (not part of the timed region)
+    g.timing_enable(True)
+    g.step(args.timing_steps)
+    kt = g.timing_read()
+    g.timing_enable(False)
+    total_ms = sum(v[0] for v in kt.values())
+    total_bytes = sum(v[1] for v in kt.values())
+    kernels = {}
+    for name, (ms, by, ct) in kt.items():
+        if ct:
+            kernels[name] = {
+                "ms_per_step": ms / args.timing_steps,
+                "launches_per_step": ct / args.timing_steps,
+                "GBps": by / (ms * 1e-3) / 1e9 if ms > 0 else None,
+                "share": ms / total_ms if total_ms > 0 else None,
+            }
+    dom = max(kt.items(), key=lambda kv: kv[1][0])
+    dname, (dms, dbytes, dct) = dom
+    achieved = dbytes / (dms * 1e-3) / 1e9
+    traffic = None
+    roofline = {
+        "bound": "hbm",
+        "achieved": achieved,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS,
+        "traffic": traffic,
+        "kernel": dname,
+        "bytes_per_launch": dbytes / dct,
+        "avg_launch_ms": dms / dct,
+    }
+    step_bytes = total_bytes / args.timing_steps
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "Mpts·RK3-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32/f64",
+        "data": "synthetic",
+        "config": {
+            "workload": args.workload,
+            "grid": f"{p['nx']}x{p['ny']}x{p['nz']}",
+            "physics": "DNS" if not p["use_les"] else "LES",
+            "precision": args.precision,
+            "re": p["re"],
+            "n_th": p["n_th"],
+        },
+        "roofline": roofline,
+        "step_hbm_frac": step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "algorithmic_GB_per_step": step_bytes / 1e9,
+        "kernels": kernels,
+    }
+    g.close()
+    if not args.no_cpu_baseline:
+        log("bench: timing the CPU oracle on one step of the same state")
+        line["cpu_baseline"] = cpu_baseline(p, gy, st, npts)
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

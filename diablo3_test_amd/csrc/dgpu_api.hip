@@ -1,0 +1,1112 @@
+// dgpu_api.hip -- host side of the MI355X channel substep: the handle, device
+// memory, the Crank-Nicolson factor tables, the order of kernel launches for
+// one rk_chan_1 call (for/channel.f90:10-917), and the extern "C" entry points
+// declared in include/diablo_gpu.h.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/diablo_gpu.h"
+#include "dgpu_kernels.h"
+
+using namespace dgpu;
+
+namespace {
+
+struct HipError {
+  hipError_t e;
+  const char* what;
+};
+
+#define HIPCHECK(call)                             \
+  do {                                             \
+    hipError_t e_ = (call);                        \
+    if (e_ != hipSuccess) throw HipError{e_, #call}; \
+  } while (0)
+
+struct ApiError {
+  int code;
+  std::string msg;
+};
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  void alloc(size_t count) {
+    n = count;
+    if (count) HIPCHECK(hipMalloc(&p, sizeof(T) * count));
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+// Rows per segment of the CN solver: a multiple of 8, at most 56 (the rows
+// live in registers), with at most 8 segments (512 threads per block).
+int seg_len_for(int n) {
+  int S = (n + 47) / 48;
+  if (S > 8) S = (n + 55) / 56;
+  if (S > 8) return -1;
+  int L = (n + S - 1) / S;
+  return (L + 7) / 8 * 8;
+}
+
+}  // namespace
+
+struct dgpu_handle {
+  dgpu_params p;
+  int precision;
+  Geo g;
+  int nth;
+  double nu, pi;
+  std::vector<double> gy, gyf, dy, dyf, kx, kz;
+  int jstart, jend, jstart_th[DGPU_MAX_TH], jend_th[DGPU_MAX_TH];
+  double h_bar[3], beta_bar[3], zeta_bar[3];
+  double w_transient;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  // device state (layouts described in dgpu_kernels.h)
+  DevBuf<double2> cu[3], cp, cth[DGPU_MAX_TH], cf[3], cfth[DGPU_MAX_TH];
+  DevBuf<double> u[3], th[DGPU_MAX_TH], r[3], rth[DGPU_MAX_TH];
+  DevBuf<double> nu_t, kappa_t[DGPU_MAX_TH];
+  std::vector<DevBuf<double2>> tpool;
+  DevBuf<double> sb;
+  DevBuf<double2> sg;
+  DevBuf<double2> twx, twz;
+  DevBuf<double> d_kx, d_kz, d_dy, d_dyf;
+  // Crank-Nicolson factor tables: [field][rk] rows, field 0 u1, 1 u2, 2 u3, 3+n th_n
+  DevBuf<SegRow> rows;
+  int seg_L = 0, seg_S = 0;
+  int nfields = 0;
+  size_t lds_limit = 65536;
+  // optional per-kernel-class timing (dgpu_timing_enable)
+  bool timing = false;
+  struct Mark {
+    int cls;
+    double bytes;
+    hipEvent_t a, b;
+  };
+  std::vector<Mark> marks;
+  std::vector<hipEvent_t> event_pool;
+  double t_ms[DGPU_NCLASS] = {0};
+  double t_bytes[DGPU_NCLASS] = {0};
+  long long t_count[DGPU_NCLASS] = {0};
+};
+
+namespace {
+
+// ------------------------------------------------------------ geometry ----
+void setup_grid(dgpu_handle* h, const double* gy_in) {
+  const dgpu_params& p = h->p;
+  const int Ny = p.ny, n2 = Ny + 2;
+  h->gy.assign(n2, 0.0);
+  h->gyf.assign(n2, 0.0);
+  h->dy.assign(n2, 0.0);
+  h->dyf.assign(n2, 0.0);
+  // gy(1..Ny+1) from grid.h5 (domain.f90:429-483); metrics as domain.f90:230-254, 501-527
+  for (int j = 1; j <= Ny + 1; ++j) h->gy[j] = gy_in[j - 1];
+  for (int j = 1; j <= Ny; ++j) h->gyf[j] = 0.5 * (h->gy[j] + h->gy[j + 1]);
+  h->gyf[0] = 2.0 * h->gyf[1] - h->gyf[2];
+  h->gyf[Ny + 1] = 2.0 * h->gyf[Ny] - h->gyf[Ny - 1];
+  for (int j = 1; j <= Ny + 1; ++j) h->dy[j] = h->gyf[j] - h->gyf[j - 1];
+  for (int j = 1; j <= Ny; ++j) h->dyf[j] = h->gy[j + 1] - h->gy[j];
+  h->dyf[0] = h->dyf[1];
+  h->dyf[Ny + 1] = h->dyf[Ny];
+  // wavenumbers (fft.f90:152, 213-230); pi = 4*atan(1.0) in default real kind
+  h->pi = (double)(4.0f * std::atan(1.0f));
+  const int nkz = p.nz / 3;
+  h->kx.assign(h->g.nq, 0.0);
+  for (int q = 0; q < h->g.nq; ++q) h->kx[q] = (double)q * (2.0 * h->pi) / p.lx;
+  h->kz.assign(2 * nkz + 1, 0.0);
+  for (int k = 0; k <= nkz; ++k) h->kz[k] = (double)k * (2.0 * h->pi) / p.lz;
+  for (int k = 1; k <= nkz; ++k) h->kz[2 * nkz + 1 - k] = -(double)k * (2.0 * h->pi) / p.lz;
+  // j ranges at one y-rank (domain.f90:284-303): jend = Ny whatever the top BC
+  h->jend = Ny;
+  h->jstart = (p.u_bc_ymin == 1) ? 1 : 2;
+  for (int n = 0; n < p.n_th; ++n) {
+    h->jend_th[n] = Ny;
+    h->jstart_th[n] = (p.th_bc_ymin[n] == 1) ? 1 : 2;
+  }
+  // RK3 (flow.f90:78-86)
+  h->h_bar[0] = p.delta_t * (8.0 / 15.0);
+  h->h_bar[1] = p.delta_t * (2.0 / 15.0);
+  h->h_bar[2] = p.delta_t * (5.0 / 15.0);
+  h->beta_bar[0] = 1.0;
+  h->beta_bar[1] = 25.0 / 8.0;
+  h->beta_bar[2] = 9.0 / 4.0;
+  h->zeta_bar[0] = 0.0;
+  h->zeta_bar[1] = -17.0 / 8.0;
+  h->zeta_bar[2] = -5.0 / 4.0;
+  h->w_transient = p.w_bc_ymax_c1;
+}
+
+// The matrix of one real CN solve (rows j = 0..Ny+1) before factorisation.
+struct Tri {
+  std::vector<double> a, b, c, vb, wp, wm;
+  std::vector<int> kind;
+};
+
+// channel.f90:656-707 (th), :716-765 (u2), :771-822 (u1), :825-877 (u3) without
+// LES, then the BC rows of apply_BC_{1,2,3,th}_{lower,upper} (:1545-1876).
+Tri build_tri(const dgpu_handle* h, int field, int rs) {
+  const dgpu_params& p = h->p;
+  const int Ny = p.ny, n = Ny + 2;
+  Tri t;
+  t.a.assign(n, 0.0);
+  t.b.assign(n, 1.0);
+  t.c.assign(n, 0.0);
+  t.vb.assign(n, 0.0);
+  t.wp.assign(n, 0.0);
+  t.wm.assign(n, 0.0);
+  t.kind.assign(n, 0);
+  const double temp1 = 0.5 * h->nu * h->h_bar[rs];
+  const auto& dy = h->dy;
+  const auto& dyf = h->dyf;
+  int ja, jb;
+  double ce;
+  if (field == 1) {
+    ja = 2;
+    jb = Ny;
+    ce = temp1 * p.nu_v_scale;
+    for (int j = ja; j <= jb; ++j) {
+      t.a[j] = -ce / (dyf[j - 1] * dy[j]);
+      t.b[j] = 1.0 + ce / (dyf[j] * dy[j]) + ce / (dyf[j - 1] * dy[j]);
+      t.c[j] = -ce / (dyf[j] * dy[j]);
+      t.wp[j] = ce / (dyf[j] * dy[j]);
+      t.wm[j] = ce / (dyf[j - 1] * dy[j]);
+      t.kind[j] = 1;
+    }
+  } else {
+    if (field >= 3) {
+      const int nn = field - 3;
+      ja = h->jstart_th[nn];
+      jb = h->jend_th[nn];
+      ce = temp1 / p.pr[nn] * p.nu_v_scale;
+    } else {
+      ja = h->jstart;
+      jb = h->jend;
+      ce = temp1 * p.nu_v_scale;
+    }
+    for (int j = ja; j <= jb; ++j) {
+      t.a[j] = -ce / (dy[j] * dyf[j]);
+      t.b[j] = 1.0 + ce / (dy[j + 1] * dyf[j]) + ce / (dy[j] * dyf[j]);
+      t.c[j] = -ce / (dy[j + 1] * dyf[j]);
+      t.wp[j] = ce / (dy[j + 1] * dyf[j]);
+      t.wm[j] = ce / (dy[j] * dyf[j]);
+      t.kind[j] = 1;
+    }
+  }
+  auto row = [&](int j, double l, double d, double u, double v, int kind = 0) {
+    t.a[j] = l;
+    t.b[j] = d;
+    t.c[j] = u;
+    t.vb[j] = v;
+    t.kind[j] = kind;
+  };
+  if (field == 1) {
+    if (p.v_bc_ymin == 0) {
+      row(1, 0, 1, 0, p.v_bc_ymin_c1);
+      row(2, 0, 1, 0, p.v_bc_ymin_c1);
+    } else if (p.v_bc_ymin == 1) {
+      row(1, 0, -1, 1, dyf[1] * p.v_bc_ymin_c1);
+    }
+    row(0, 0, 1, 0, 0);
+    if (p.v_bc_ymax == 0) {
+      row(Ny + 1, 0, 1, 0, p.v_bc_ymax_c1);
+      row(Ny, 0, 1, 0, p.v_bc_ymax_c1);
+    } else if (p.v_bc_ymax == 1) {
+      row(Ny + 1, -1, 1, 0, dyf[Ny] * p.v_bc_ymax_c1);
+    }
+  } else {
+    int bcl, bcu;
+    double c1l, c1u;
+    bool transient = false;
+    if (field == 0) {
+      bcl = p.u_bc_ymin; bcu = p.u_bc_ymax; c1l = p.u_bc_ymin_c1; c1u = p.u_bc_ymax_c1;
+    } else if (field == 2) {
+      bcl = p.w_bc_ymin; bcu = p.w_bc_ymax; c1l = p.w_bc_ymin_c1; c1u = p.w_bc_ymax_c1;
+      transient = (bcu != 0);  // uses w_BC_Ymax_c1_transient (channel.f90:1775-1780)
+    } else {
+      const int nn = field - 3;
+      bcl = p.th_bc_ymin[nn]; bcu = p.th_bc_ymax[nn];
+      c1l = p.th_bc_ymin_c1[nn]; c1u = p.th_bc_ymax_c1[nn];
+    }
+    row(0, 0, 1, 0, 0);
+    if (bcl == 0)
+      row(1, 0, 1, 0, c1l);
+    else
+      row(1, 0, -1, 1, dy[2] * c1l);
+    if (bcu == 0) {
+      row(Ny + 1, 0, 1, 0, 0);
+      row(Ny, 0, 1, 0, c1u);
+    } else {
+      if (transient)
+        row(Ny, -1, 1, 0, dy[Ny], 2);  // vb scaled by the current transient value
+      else
+        row(Ny, -1, 1, 0, dy[Ny] * c1u);
+      row(Ny + 1, 0, 1, 0, 0);
+    }
+  }
+  return t;
+}
+
+// LU factors of the Thomas sweep and the segment carry products.
+void factor_rows(const Tri& t, int L, SegRow* out) {
+  const int n = (int)t.a.size();
+  std::vector<double> bp(n), m(n, 0.0);
+  bp[0] = t.b[0];
+  for (int j = 1; j < n; ++j) {
+    m[j] = -t.a[j] / bp[j - 1];
+    bp[j] = t.b[j] + m[j] * t.c[j - 1];
+  }
+  for (int j = 0; j < n; ++j) {
+    SegRow& r = out[j];
+    r.m = m[j];
+    r.e = 1.0 / bp[j];
+    r.f = -t.c[j] / bp[j];
+    r.wp = t.wp[j];
+    r.wm = t.wm[j];
+    r.kind = t.kind[j];
+    r.vb = t.vb[j];
+  }
+  for (int s0 = 0; s0 < n; s0 += L) {
+    const int s1 = std::min(s0 + L, n) - 1;
+    double prod = 1.0;
+    for (int j = s0; j <= s1; ++j) {
+      prod *= out[j].m;
+      out[j].pm = prod;
+    }
+    prod = 1.0;
+    for (int j = s1; j >= s0; --j) {
+      prod *= out[j].f;
+      out[j].qf = prod;
+    }
+  }
+}
+
+void twiddles(int n, std::vector<double2>& w) {
+  w.resize(n);
+  for (int t = 0; t < n; ++t) {
+    long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)t / (long double)n;
+    w[t] = make_double2((double)cosl(a), (double)sinl(a));
+  }
+}
+
+template <typename T>
+void upload(DevBuf<T>& b, const std::vector<T>& v) {
+  b.alloc(v.size());
+  HIPCHECK(hipMemcpy(b.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+}
+
+// ------------------------------------------------------------- timing ----
+hipEvent_t take_event(dgpu_handle* h) {
+  if (!h->event_pool.empty()) {
+    hipEvent_t e = h->event_pool.back();
+    h->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  HIPCHECK(hipEventCreate(&e));
+  return e;
+}
+
+// Brackets one kernel launch with events when timing is on.  `bytes` is the
+// launch's algorithmic HBM traffic (compulsory reads + writes).
+struct Timed {
+  dgpu_handle* h;
+  int cls;
+  double bytes;
+  hipEvent_t a = nullptr;
+  Timed(dgpu_handle* h_, int cls_, double bytes_) : h(h_), cls(cls_), bytes(bytes_) {
+    if (h->timing) {
+      a = take_event(h);
+      HIPCHECK(hipEventRecord(a, h->stream));
+    }
+  }
+  ~Timed() {
+    // never throws: a failed launch has already raised, and its events are
+    // simply returned to the pool
+    if (h->timing && a) {
+      hipEvent_t b = nullptr;
+      if (!h->event_pool.empty()) {
+        b = h->event_pool.back();
+        h->event_pool.pop_back();
+      } else if (hipEventCreate(&b) != hipSuccess) {
+        b = nullptr;
+      }
+      if (b && hipEventRecord(b, h->stream) == hipSuccess) {
+        h->marks.push_back({cls, bytes, a, b});
+      } else {
+        h->event_pool.push_back(a);
+        if (b) h->event_pool.push_back(b);
+      }
+    }
+  }
+};
+
+void collect_timing(dgpu_handle* h) {
+  if (h->marks.empty()) return;
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  for (auto& m : h->marks) {
+    float ms = 0.f;
+    HIPCHECK(hipEventElapsedTime(&ms, m.a, m.b));
+    h->t_ms[m.cls] += ms;
+    h->t_bytes[m.cls] += m.bytes;
+    h->t_count[m.cls] += 1;
+    h->event_pool.push_back(m.a);
+    h->event_pool.push_back(m.b);
+  }
+  h->marks.clear();
+}
+
+// ------------------------------------------------------- launch helpers ----
+// Transform groups per workgroup for the x passes: as many as fit 256
+// threads, the jobs of one tile and the LDS limit (transform buffers plus
+// `nout` staging tiles).
+int x_ngroups(const dgpu_handle* h, int jobs, int nout) {
+  const Geo& g = h->g;
+  int G = g.nx / 8;
+  int ng = 256 / G;
+  if (ng < 1) ng = 1;
+  if (ng > jobs) ng = jobs;
+  auto lds = [&](int n) { return (size_t)(n * g.nx + nout * g.nq * KT) * sizeof(double2); };
+  while (ng > 1 && lds(ng) > h->lds_limit) --ng;
+  if (lds(ng) > h->lds_limit) throw ApiError{DGPU_EUNSUP, "Nx too large for the x-transform pass"};
+  return ng;
+}
+
+int z_ngroups(const dgpu_handle* h) {
+  const Geo& g = h->g;
+  int G = g.nz / 8;
+  int ng = 256 / G;
+  if (ng < 1) ng = 1;
+  while (ng > 1 && (size_t)ng * g.nz * sizeof(double2) > h->lds_limit) --ng;
+  if ((size_t)ng * g.nz * sizeof(double2) > h->lds_limit)
+    throw ApiError{DGPU_EUNSUP, "Nz too large for the z-transform pass"};
+  return ng;
+}
+
+void launch_xfwd(dgpu_handle* h, const double* u, double2* T, int j0, int j1) {
+  if (j1 < j0) return;
+  const Geo& g = h->g;
+  const int ng = x_ngroups(h, KT / 2, 1);
+  const size_t lds = (size_t)(ng * g.nx + g.nq * KT) * sizeof(double2);
+  dim3 grid(g.nz / KT, j1 - j0 + 1), block(ng * (g.nx / 8));
+  Timed tm(h, DGPU_K_XFWD, (8.0 * g.pp + 16.0 * g.tp) * (j1 - j0 + 1));
+  hipLaunchKernelGGL(k_xfwd<double>, grid, block, lds, h->stream, u,
+                     reinterpret_cast<cpx<double>*>(T), g,
+                     reinterpret_cast<const cpx<double>*>(h->twx.p), j0);
+  HIPCHECK(hipGetLastError());
+}
+
+void launch_xinv(dgpu_handle* h, const double2* T, double* u, int j0, int j1) {
+  if (j1 < j0) return;
+  const Geo& g = h->g;
+  const int ng = x_ngroups(h, KT / 2, 1);
+  const size_t lds = (size_t)(ng * g.nx + g.nq * KT) * sizeof(double2);
+  dim3 grid(g.nz / KT, j1 - j0 + 1), block(ng * (g.nx / 8));
+  Timed tm(h, DGPU_K_XINV, (8.0 * g.pp + 16.0 * g.tp) * (j1 - j0 + 1));
+  hipLaunchKernelGGL(k_xinv<double>, grid, block, lds, h->stream,
+                     reinterpret_cast<const cpx<double>*>(T), u, g,
+                     reinterpret_cast<const cpx<double>*>(h->twx.p), j0);
+  HIPCHECK(hipGetLastError());
+}
+
+void launch_zfwd(dgpu_handle* h, const double2* T, double2* C, int j0, int j1) {
+  if (j1 < j0) return;
+  const Geo& g = h->g;
+  const int ng = z_ngroups(h);
+  const size_t lds = (size_t)ng * g.nz * sizeof(double2);
+  dim3 grid((g.nq + ng - 1) / ng, j1 - j0 + 1), block(ng * (g.nz / 8));
+  Timed tm(h, DGPU_K_ZFWD, (16.0 * g.tp + 16.0 * g.sp) * (j1 - j0 + 1));
+  hipLaunchKernelGGL(k_zfwd<double>, grid, block, lds, h->stream,
+                     reinterpret_cast<const cpx<double>*>(T), reinterpret_cast<cpx<double>*>(C), g,
+                     reinterpret_cast<const cpx<double>*>(h->twz.p), j0);
+  HIPCHECK(hipGetLastError());
+}
+
+void launch_zinv(dgpu_handle* h, const double2* C, double2* T, int j0, int j1) {
+  if (j1 < j0) return;
+  const Geo& g = h->g;
+  const int ng = z_ngroups(h);
+  const size_t lds = (size_t)ng * g.nz * sizeof(double2);
+  dim3 grid((g.nq + ng - 1) / ng, j1 - j0 + 1), block(ng * (g.nz / 8));
+  Timed tm(h, DGPU_K_ZINV, (16.0 * g.tp + 16.0 * g.sp) * (j1 - j0 + 1));
+  hipLaunchKernelGGL(k_zinv<double>, grid, block, lds, h->stream,
+                     reinterpret_cast<const cpx<double>*>(C), reinterpret_cast<cpx<double>*>(T), g,
+                     reinterpret_cast<const cpx<double>*>(h->twz.p), j0);
+  HIPCHECK(hipGetLastError());
+}
+
+// spectral -> physical for one field, all planes (fft_xz_to_physical, fft.f90:319-382)
+void to_physical(dgpu_handle* h, const double2* C, double* u, double2* T) {
+  const int n2 = h->g.ny2;
+  launch_zinv(h, C, T, 0, n2 - 1);
+  launch_xinv(h, T, u, 0, n2 - 1);
+}
+
+// physical -> spectral for one field on planes [j0, j1] (fft_xz_to_fourier, fft.f90:247-315)
+void to_fourier(dgpu_handle* h, const double* u, double2* C, double2* T, int j0, int j1) {
+  launch_xfwd(h, u, T, j0, j1);
+  launch_zfwd(h, T, C, j0, j1);
+}
+
+void launch_thomas(dgpu_handle* h, int field, int rs, const double* r, double* u, int ca, int cb,
+                   double transient) {
+  const Geo& g = h->g;
+  const int n = g.ny2;
+  const SegRow* rows = h->rows.p + ((size_t)field * 3 + rs) * n;
+  dim3 grid((unsigned)(g.pp / 64)), block(64, h->seg_S);
+  Timed tm(h, DGPU_K_THOMAS, 24.0 * g.pp * n);
+  switch (h->seg_L) {
+#define TCASE(L)                                                                              \
+  case L:                                                                                     \
+    hipLaunchKernelGGL(k_thomas_seg<L>, grid, block, 0, h->stream, r, u, rows, n, h->seg_S, \
+                       g.pp, ca, cb, transient);                                             \
+    break;
+    TCASE(8)
+    TCASE(16)
+    TCASE(24)
+    TCASE(32)
+    TCASE(40)
+    TCASE(48)
+    TCASE(56)
+#undef TCASE
+    default:
+      throw ApiError{DGPU_EUNSUP, "unsupported Thomas segment length"};
+  }
+  HIPCHECK(hipGetLastError());
+}
+
+void launch_remdiv(dgpu_handle* h, bool update_cp, double temp4) {
+  const Geo& g = h->g;
+  RemDivArgs a;
+  a.cu1 = h->cu[0].p;
+  a.cu2 = h->cu[1].p;
+  a.cu3 = h->cu[2].p;
+  a.cp = h->cp.p;
+  a.sb = h->sb.p;
+  a.sg = h->sg.p;
+  a.kx = h->d_kx.p;
+  a.kz = h->d_kz.p;
+  a.dy = h->d_dy.p;
+  a.dyf = h->d_dyf.p;
+  a.inv_h = 1.0 / temp4;
+  a.update_cp = update_cp ? 1 : 0;
+  a.js = h->jstart;
+  a.je = h->jend;
+  dim3 grid((unsigned)((g.sp + 255) / 256)), block(256);
+  Timed tm(h, DGPU_K_REMDIV, (update_cp ? 128.0 : 96.0) * g.sp * g.ny);
+  hipLaunchKernelGGL(k_remdiv, grid, block, 0, h->stream, a, g);
+  HIPCHECK(hipGetLastError());
+}
+
+void launch_bc_post(dgpu_handle* h, double time) {
+  const dgpu_params& p = h->p;
+  BcPostArgs a;
+  a.cu1 = h->cu[0].p;
+  a.cu2 = h->cu[1].p;
+  a.cu3 = h->cu[2].p;
+  a.dy = h->d_dy.p;
+  a.dyf = h->d_dyf.p;
+  a.u_lo = p.u_bc_ymin;
+  a.v_lo = p.v_bc_ymin;
+  a.w_lo = p.w_bc_ymin;
+  a.u_hi = p.u_bc_ymax;
+  a.v_hi = p.v_bc_ymax;
+  a.w_hi = p.w_bc_ymax;
+  a.u_lo_c1 = p.u_bc_ymin_c1;
+  a.v_lo_c1 = p.v_bc_ymin_c1;
+  a.w_lo_c1 = p.w_bc_ymin_c1;
+  a.u_hi_c1 = p.u_bc_ymax_c1;
+  a.v_hi_c1 = p.v_bc_ymax_c1;
+  a.w_hi_c1 = p.w_bc_ymax_c1;
+  if (p.w_bc_ymax == 1 && p.f_type == 4)
+    h->w_transient =
+        p.w_bc_ymax_c1 + p.amp_omega0 * std::sin(p.omega0 * (p.ro_inv / p.delta * time - p.force_start));
+  a.w_hi_c1_transient = h->w_transient;
+  dim3 grid((unsigned)((h->g.sp + 255) / 256)), block(256);
+  Timed tm(h, DGPU_K_BCPOST, 16.0 * 14 * h->g.sp);
+  hipLaunchKernelGGL(k_bc_post, grid, block, 0, h->stream, a, h->g);
+  HIPCHECK(hipGetLastError());
+}
+
+// ----------------------------------------------------------- the substep --
+void rk_substep(dgpu_handle* h, int rk_step, double time, int time_step) {
+  const dgpu_params& p = h->p;
+  const Geo& g = h->g;
+  const int rs = rk_step - 1;
+  const int nth = h->nth;
+  const int Ny = g.ny;
+  const double temp3 = h->zeta_bar[rs] * h->h_bar[rs];  // channel.f90:29-33
+  const double temp4 = h->h_bar[rs];
+  const double temp5 = h->beta_bar[rs] * h->h_bar[rs];
+  (void)time_step;
+  if (p.use_les) throw ApiError{DGPU_EUNSUP, "LES path not available in this build"};
+  const int nslot = 6 + 2 * nth;
+  const int ncr = 3 + nth;
+
+  // 1. state to physical space (channel.f90:279-287)
+  for (int f = 0; f < 3; ++f) to_physical(h, h->cu[f].p, h->u[f].p, h->tpool[f].p);
+  for (int n = 0; n < nth; ++n) to_physical(h, h->cth[n].p, h->th[n].p, h->tpool[3 + n].p);
+
+  // 2. products, x-transformed (channel.f90:299-459, 557-608)
+  ProdArgs pa;
+  pa.u1 = h->u[0].p;
+  pa.u2 = h->u[1].p;
+  pa.u3 = h->u[2].p;
+  pa.th = nullptr;
+  pa.dy = h->d_dy.p;
+  pa.dyf = h->d_dyf.p;
+  pa.kx = h->d_kx.p;
+  pa.js = h->jstart;
+  pa.je = h->jend;
+  pa.jst = 1 << 30;
+  pa.jet = -1;
+  {
+    const int ngx = x_ngroups(h, KT, 2);
+    const size_t lds = (size_t)(ngx * g.nx + 2 * g.nq * KT) * sizeof(double2);
+    dim3 grid(g.nz / KT, Ny), block(ngx * (g.nx / 8));
+    auto slotp = [&](int s) { return reinterpret_cast<cpx<double>*>(h->tpool[s].p); };
+    const int outs[4][2] = {{0, -1}, {1, 2}, {3, 5}, {4, -1}};
+    for (int pair = 0; pair < 4; ++pair) {
+      cpx<double>* o0 = slotp(outs[pair][0]);
+      cpx<double>* o1 = outs[pair][1] >= 0 ? slotp(outs[pair][1]) : nullptr;
+      const int nread[4] = {2, 3, 2, 2};
+      Timed tm(h, DGPU_K_XPROD, (8.0 * g.pp * nread[pair] + 16.0 * g.tp * (o1 ? 2 : 1)) * Ny);
+      hipLaunchKernelGGL(k_xprod<double>, grid, block, lds, h->stream, pa, g, pair, o0, o1,
+                         reinterpret_cast<const cpx<double>*>(h->twx.p), 1);
+      HIPCHECK(hipGetLastError());
+    }
+    for (int n = 0; n < nth; ++n) {
+      ProdArgs pt = pa;
+      pt.th = h->th[n].p;
+      pt.jst = h->jstart_th[n];
+      pt.jet = h->jend_th[n];
+      for (int pair = 4; pair < 6; ++pair) {
+        cpx<double>* o0 = slotp(6 + 2 * n + (pair - 4));
+        Timed tm(h, DGPU_K_XPROD, (8.0 * g.pp * (pair == 4 ? 3 : 2) + 16.0 * g.tp) * Ny);
+        hipLaunchKernelGGL(k_xprod<double>, grid, block, lds, h->stream, pt, g, pair, o0,
+                           (cpx<double>*)nullptr, reinterpret_cast<const cpx<double>*>(h->twx.p), 1);
+        HIPCHECK(hipGetLastError());
+      }
+    }
+  }
+
+  // 3. explicit spectral terms, cf, cr, backward z transform of cr
+  {
+    RhsArgs a;
+    std::memset(&a, 0, sizeof(a));
+    for (int s = 0; s < nslot; ++s) a.X[s] = h->tpool[s].p;
+    for (int c = 0; c < ncr; ++c) a.Tcr[c] = h->tpool[nslot + c].p;
+    a.cu1 = h->cu[0].p;
+    a.cu2 = h->cu[1].p;
+    a.cu3 = h->cu[2].p;
+    a.cp = h->cp.p;
+    a.cf1 = h->cf[0].p;
+    a.cf2 = h->cf[1].p;
+    a.cf3 = h->cf[2].p;
+    for (int n = 0; n < nth; ++n) {
+      a.cth[n] = h->cth[n].p;
+      a.cfth[n] = h->cfth[n].p;
+      a.pr[n] = p.pr[n];
+      a.dthdx[n] = p.dthdx[n];
+      a.dthdz[n] = p.dthdz[n];
+      a.jst[n] = h->jstart_th[n];
+      a.jet[n] = h->jend_th[n];
+    }
+    a.kx = h->d_kx.p;
+    a.kz = h->d_kz.p;
+    a.dy = h->d_dy.p;
+    a.dyf = h->d_dyf.p;
+    a.nu = h->nu;
+    a.beta = p.beta;
+    a.coriolis = p.ro_inv / p.delta;
+    a.grav_x = p.grav_x;
+    a.grav_y = p.grav_y;
+    a.grav_z = p.grav_z;
+    a.temp3 = temp3;
+    a.temp4 = temp4;
+    a.temp5 = temp5;
+    if (p.f_type == 1)
+      a.force = p.px0;  // channel.f90:108-112
+    else if (p.f_type == 2)
+      a.force = p.px0 + p.amp_omega0 * std::cos(p.omega0 * time);  // :113-118
+    else
+      a.force = 0.0;
+    a.rk_step = rk_step;
+    a.nth = nth;
+    a.js = h->jstart;
+    a.je = h->jend;
+    const int G = g.nz / 8;
+    int ng = 1024 / G;
+    if (ng < 1) ng = 1;
+    if (ng > nslot) ng = nslot;
+    const int rounds = (nslot + ng - 1) / ng;
+    const size_t lds = (size_t)rounds * ng * g.nz * sizeof(double2);
+    if (lds > h->lds_limit) throw ApiError{DGPU_EUNSUP, "Nz too large for the spectral RHS pass"};
+    dim3 grid(g.nq, Ny), block(ng * G);
+    // x-spectra of the products in, cr x-spectra out; cu1..3, cp, cth, cf1..3,
+    // cfth read and cf1..3, cfth written (neighbour planes come from cache)
+    Timed tm(h, DGPU_K_RHS,
+             (16.0 * g.tp * (nslot + ncr) + 16.0 * g.sp * (4 + 2 * nth + 3 + nth + 3 + nth)) * Ny);
+    hipLaunchKernelGGL(k_rhs, grid, block, lds, h->stream, a, g, h->twz.p, 1);
+    HIPCHECK(hipGetLastError());
+  }
+
+  // 4. cr to physical space (second half of fft_xz_to_physical, :485-487, :621)
+  // slot order of cr: 0 cr1, 1 cr3, 2 cr2, 3+n crth_n
+  launch_xinv(h, h->tpool[nslot + 0].p, h->r[0].p, 1, Ny);
+  launch_xinv(h, h->tpool[nslot + 1].p, h->r[2].p, 1, Ny);
+  launch_xinv(h, h->tpool[nslot + 2].p, h->r[1].p, 1, Ny);
+  for (int n = 0; n < nth; ++n) launch_xinv(h, h->tpool[nslot + 3 + n].p, h->rth[n].p, 1, Ny);
+
+  // 5. implicit vertical solves (channel.f90:656-877)
+  double transient = h->w_transient;
+  if (p.w_bc_ymax == 1 && p.f_type == 4) {
+    transient = p.w_bc_ymax_c1 + p.amp_omega0 * std::sin(p.omega0 * (p.ro_inv / p.delta * time - p.force_start));
+    h->w_transient = transient;
+  }
+  for (int n = 0; n < nth; ++n)
+    launch_thomas(h, 3 + n, rs, h->rth[n].p, h->th[n].p, h->jstart_th[n] - 1, h->jend_th[n] + 1,
+                  transient);
+  launch_thomas(h, 1, rs, h->r[1].p, h->u[1].p, 1, Ny + 1, transient);
+  launch_thomas(h, 0, rs, h->r[0].p, h->u[0].p, h->jstart - 1, h->jend + 1, transient);
+  launch_thomas(h, 2, rs, h->r[2].p, h->u[2].p, h->jstart - 1, h->jend + 1, transient);
+
+  // 6. back to Fourier space on the solved planes (channel.f90:891-896)
+  to_fourier(h, h->u[0].p, h->cu[0].p, h->tpool[0].p, h->jstart - 1, h->jend + 1);
+  to_fourier(h, h->u[1].p, h->cu[1].p, h->tpool[1].p, 1, Ny + 1);
+  to_fourier(h, h->u[2].p, h->cu[2].p, h->tpool[2].p, h->jstart - 1, h->jend + 1);
+  for (int n = 0; n < nth; ++n)
+    to_fourier(h, h->th[n].p, h->cth[n].p, h->tpool[3 + n].p, h->jstart_th[n] - 1, h->jend_th[n] + 1);
+
+  // 7. projection and pressure (channel.f90:901-911)
+  launch_remdiv(h, true, temp4);
+}
+
+// -------------------------------------------------- reference layouts ----
+size_t ref_spec_doubles(const Geo& g) { return (size_t)(g.nx + 2) * (g.nz + 2) * (g.ny + 2); }
+
+void spec_from_ref(const dgpu_handle* h, const double* ref, std::vector<double2>& out) {
+  const Geo& g = h->g;
+  out.assign((size_t)g.ny2 * g.sp, make_double2(0, 0));
+  const size_t NXC = g.nx / 2 + 1, NZ2 = g.nz + 2;
+  for (int j = 0; j < g.ny2; ++j)
+    for (int q = 0; q < g.nq; ++q)
+      for (int kk = 0; kk < g.nkk; ++kk) {
+        size_t c = q + NXC * (kk + NZ2 * j);
+        out[(size_t)j * g.sp + (size_t)q * g.nkk + kk] = make_double2(ref[2 * c], ref[2 * c + 1]);
+      }
+}
+
+void spec_to_ref(const dgpu_handle* h, const std::vector<double2>& in, double* ref) {
+  const Geo& g = h->g;
+  std::memset(ref, 0, sizeof(double) * ref_spec_doubles(g));
+  const size_t NXC = g.nx / 2 + 1, NZ2 = g.nz + 2;
+  for (int j = 0; j < g.ny2; ++j)
+    for (int q = 0; q < g.nq; ++q)
+      for (int kk = 0; kk < g.nkk; ++kk) {
+        size_t c = q + NXC * (kk + NZ2 * j);
+        double2 v = in[(size_t)j * g.sp + (size_t)q * g.nkk + kk];
+        ref[2 * c] = v.x;
+        ref[2 * c + 1] = v.y;
+      }
+}
+
+void phys_from_ref(const dgpu_handle* h, const double* ref, std::vector<double>& out) {
+  const Geo& g = h->g;
+  out.assign((size_t)g.ny2 * g.pp, 0.0);
+  const size_t NX2 = g.nx + 2, NZ2 = g.nz + 2;
+  for (int j = 0; j < g.ny2; ++j)
+    for (int k = 0; k < g.nz; ++k)
+      for (int i = 0; i < g.nx; ++i)
+        out[(size_t)j * g.pp + (size_t)k * g.nx + i] = ref[i + NX2 * (k + NZ2 * j)];
+}
+
+void phys_to_ref(const dgpu_handle* h, const std::vector<double>& in, double* ref) {
+  const Geo& g = h->g;
+  std::memset(ref, 0, sizeof(double) * ref_spec_doubles(g));
+  const size_t NX2 = g.nx + 2, NZ2 = g.nz + 2;
+  for (int j = 0; j < g.ny2; ++j)
+    for (int k = 0; k < g.nz; ++k)
+      for (int i = 0; i < g.nx; ++i)
+        ref[i + NX2 * (k + NZ2 * j)] = in[(size_t)j * g.pp + (size_t)k * g.nx + i];
+}
+
+// (buffer, is_spectral) for field id f, scalar n
+bool field_buf(dgpu_handle* h, int f, int n, void** dev, bool* spec) {
+  *spec = true;
+  switch (f) {
+    case DGPU_CU1: *dev = h->cu[0].p; return true;
+    case DGPU_CU2: *dev = h->cu[1].p; return true;
+    case DGPU_CU3: *dev = h->cu[2].p; return true;
+    case DGPU_CP: *dev = h->cp.p; return true;
+    case DGPU_CTH: *dev = h->cth[n].p; return true;
+    case DGPU_CF1: *dev = h->cf[0].p; return true;
+    case DGPU_CF2: *dev = h->cf[1].p; return true;
+    case DGPU_CF3: *dev = h->cf[2].p; return true;
+    case DGPU_CFTH: *dev = h->cfth[n].p; return true;
+    case DGPU_NU_T: *dev = h->nu_t.p; *spec = false; return true;
+    case DGPU_KAPPA_T: *dev = h->kappa_t[n].p; *spec = false; return true;
+  }
+  return false;
+}
+
+int nfield_copies(const dgpu_handle* h, int f) {
+  return (f == DGPU_CTH || f == DGPU_CFTH || f == DGPU_KAPPA_T) ? h->nth : 1;
+}
+
+int check_params(const dgpu_params* p, std::string& why) {
+  auto pow2 = [](int n) { return n >= 16 && n <= 2048 && (n & (n - 1)) == 0; };
+  if (!pow2(p->nx) || !pow2(p->nz)) { why = "nx and nz must be powers of two in [16, 2048]"; return -1; }
+  if (p->ny < 5) { why = "ny must be >= 5"; return -1; }
+  if (p->n_th < 0 || p->n_th > DGPU_MAX_TH) { why = "n_th out of range"; return -1; }
+  if (!(p->re > 0) || !(p->delta_t > 0)) { why = "re and delta_t must be positive"; return -1; }
+  if (p->delta == 0.0 && p->ro_inv != 0.0) { why = "delta must be nonzero when ro_inv is"; return -1; }
+  for (int n = 0; n < p->n_th; ++n)
+    if (!(p->pr[n] > 0)) { why = "pr must be positive"; return -1; }
+  auto bc01 = [](int b) { return b == 0 || b == 1; };
+  if (!bc01(p->u_bc_ymin) || !bc01(p->u_bc_ymax) || !bc01(p->v_bc_ymin) || !bc01(p->v_bc_ymax) ||
+      !bc01(p->w_bc_ymin) || !bc01(p->w_bc_ymax)) {
+    why = "velocity BC types must be 0 (Dirichlet) or 1 (Neumann)";
+    return -1;
+  }
+  for (int n = 0; n < p->n_th; ++n)
+    if (!bc01(p->th_bc_ymin[n]) || !bc01(p->th_bc_ymax[n])) {
+      why = "scalar BC types must be 0 or 1";
+      return -1;
+    }
+  return 0;
+}
+
+void free_all(dgpu_handle* h) {
+  for (int f = 0; f < 3; ++f) {
+    h->cu[f].release();
+    h->cf[f].release();
+    h->u[f].release();
+    h->r[f].release();
+  }
+  h->cp.release();
+  for (int n = 0; n < DGPU_MAX_TH; ++n) {
+    h->cth[n].release();
+    h->cfth[n].release();
+    h->th[n].release();
+    h->rth[n].release();
+    h->kappa_t[n].release();
+  }
+  h->nu_t.release();
+  for (auto& b : h->tpool) b.release();
+  h->tpool.clear();
+  h->sb.release();
+  h->sg.release();
+  h->twx.release();
+  h->twz.release();
+  h->d_kx.release();
+  h->d_kz.release();
+  h->d_dy.release();
+  h->d_dyf.release();
+  h->rows.release();
+}
+
+void create_impl(dgpu_handle* h, const double* gy) {
+  const dgpu_params& p = h->p;
+  Geo& g = h->g;
+  g.nx = p.nx;
+  g.ny = p.ny;
+  g.nz = p.nz;
+  g.lognx = 0;
+  while ((1 << g.lognx) < g.nx) ++g.lognx;
+  g.lognz = 0;
+  while ((1 << g.lognz) < g.nz) ++g.lognz;
+  g.nkx = p.nx / 3;
+  g.nkz = p.nz / 3;
+  g.nq = g.nkx + 1;
+  g.nkk = 2 * g.nkz + 1;
+  g.ny2 = p.ny + 2;
+  g.pp = (long long)g.nx * g.nz;
+  g.sp = (long long)g.nq * g.nkk;
+  g.tp = (long long)g.nq * g.nz;
+  h->nth = p.n_th;
+  h->nu = 1.0 / p.re;  // parameters.f90:101
+  setup_grid(h, gy);
+
+  int dev = 0;
+  HIPCHECK(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  HIPCHECK(hipGetDeviceProperties(&prop, dev));
+  h->lds_limit = prop.sharedMemPerBlock;
+  if (h->lds_limit > 65536) {
+    const void* kernels[] = {(const void*)k_rhs, (const void*)k_xfwd<double>,
+                             (const void*)k_xinv<double>, (const void*)k_zfwd<double>,
+                             (const void*)k_zinv<double>, (const void*)k_xprod<double>};
+    for (const void* k : kernels)
+      HIPCHECK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds_limit));
+  }
+  HIPCHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+
+  const size_t ns = (size_t)g.ny2 * g.sp, np = (size_t)g.ny2 * g.pp, nt = (size_t)g.ny2 * g.tp;
+  for (int f = 0; f < 3; ++f) {
+    h->cu[f].alloc(ns);
+    h->cf[f].alloc(ns);
+    h->u[f].alloc(np);
+    h->r[f].alloc(np);
+  }
+  h->cp.alloc(ns);
+  for (int n = 0; n < h->nth; ++n) {
+    h->cth[n].alloc(ns);
+    h->cfth[n].alloc(ns);
+    h->th[n].alloc(np);
+    h->rth[n].alloc(np);
+    h->kappa_t[n].alloc(np);
+  }
+  h->nu_t.alloc(np);
+  const int nslot = 6 + 2 * h->nth, ncr = 3 + h->nth;
+  h->tpool.resize(nslot + ncr);
+  for (auto& b : h->tpool) b.alloc(nt);
+  h->sb.alloc(ns);
+  h->sg.alloc(ns);
+  for (int f = 0; f < 3; ++f) {
+    HIPCHECK(hipMemset(h->cu[f].p, 0, sizeof(double2) * ns));
+    HIPCHECK(hipMemset(h->cf[f].p, 0, sizeof(double2) * ns));
+    HIPCHECK(hipMemset(h->u[f].p, 0, sizeof(double) * np));
+  }
+  HIPCHECK(hipMemset(h->cp.p, 0, sizeof(double2) * ns));
+  for (int n = 0; n < h->nth; ++n) {
+    HIPCHECK(hipMemset(h->cth[n].p, 0, sizeof(double2) * ns));
+    HIPCHECK(hipMemset(h->cfth[n].p, 0, sizeof(double2) * ns));
+    HIPCHECK(hipMemset(h->kappa_t[n].p, 0, sizeof(double) * np));
+  }
+  HIPCHECK(hipMemset(h->nu_t.p, 0, sizeof(double) * np));
+
+  std::vector<double2> w;
+  twiddles(g.nx, w);
+  upload(h->twx, w);
+  twiddles(g.nz, w);
+  upload(h->twz, w);
+  upload(h->d_kx, h->kx);
+  upload(h->d_kz, h->kz);
+  upload(h->d_dy, h->dy);
+  upload(h->d_dyf, h->dyf);
+
+  // CN factor tables
+  const int n = g.ny2;
+  h->seg_L = seg_len_for(n);
+  if (h->seg_L < 0) throw ApiError{DGPU_EUNSUP, "ny too large for the segmented solver"};
+  h->seg_S = (n + h->seg_L - 1) / h->seg_L;
+  h->nfields = 3 + h->nth;
+  std::vector<SegRow> rows((size_t)h->nfields * 3 * n);
+  for (int f = 0; f < h->nfields; ++f)
+    for (int rs = 0; rs < 3; ++rs) {
+      Tri t = build_tri(h, f, rs);
+      factor_rows(t, h->seg_L, rows.data() + ((size_t)f * 3 + rs) * n);
+    }
+  upload(h->rows, rows);
+  HIPCHECK(hipDeviceSynchronize());
+}
+
+}  // namespace
+
+// ================================================================ C-ABI ====
+
+namespace {
+int run_guarded(dgpu_handle* h, const std::function<void()>& fn) {
+  try {
+    fn();
+    return DGPU_OK;
+  } catch (const HipError& e) {
+    if (h) h->err = std::string(e.what) + ": " + hipGetErrorString(e.e);
+    return e.e == hipErrorOutOfMemory ? DGPU_ENOMEM : DGPU_EHIP;
+  } catch (const ApiError& e) {
+    if (h) h->err = e.msg;
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    if (h) h->err = "host allocation failed";
+    return DGPU_ENOMEM;
+  } catch (...) {
+    if (h) h->err = "unknown error";
+    return DGPU_EINVAL;
+  }
+}
+}  // namespace
+
+extern "C" {
+
+int dgpu_create(const dgpu_params* p, const double* gy, int precision, dgpu_handle** out) {
+  if (!p || !gy || !out) return DGPU_EINVAL;
+  *out = nullptr;
+  std::string why;
+  if (check_params(p, why)) return DGPU_EINVAL;
+  if (precision != DGPU_FP64) return DGPU_EUNSUP;
+  dgpu_handle* h = new (std::nothrow) dgpu_handle();
+  if (!h) return DGPU_ENOMEM;
+  h->p = *p;
+  h->precision = precision;
+  int rc = run_guarded(h, [&] { create_impl(h, gy); });
+  if (rc != DGPU_OK) {
+    free_all(h);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return DGPU_OK;
+}
+
+int dgpu_destroy(dgpu_handle* h) {
+  if (!h) return DGPU_EINVAL;
+  (void)hipStreamSynchronize(h->stream);
+  for (auto& m : h->marks) {
+    (void)hipEventDestroy(m.a);
+    (void)hipEventDestroy(m.b);
+  }
+  for (auto e : h->event_pool) (void)hipEventDestroy(e);
+  free_all(h);
+  (void)hipStreamDestroy(h->stream);
+  delete h;
+  return DGPU_OK;
+}
+
+const char* dgpu_last_error(const dgpu_handle* h) { return h ? h->err.c_str() : "null handle"; }
+
+int64_t dgpu_field_doubles(const dgpu_handle* h, int field) {
+  if (!h || field < 0 || field >= DGPU_NFIELDS) return -1;
+  return (int64_t)ref_spec_doubles(h->g) * nfield_copies(h, field);
+}
+
+int dgpu_set_field(dgpu_handle* h, int field, const double* host) {
+  if (!h || !host) return DGPU_EINVAL;
+  if (field < 0 || field >= DGPU_NFIELDS) {
+    h->err = "bad field id";
+    return DGPU_EINVAL;
+  }
+  return run_guarded(h, [&] {
+    const size_t per = ref_spec_doubles(h->g);
+    for (int n = 0; n < nfield_copies(h, field); ++n) {
+      void* dev;
+      bool spec;
+      field_buf(h, field, n, &dev, &spec);
+      if (spec) {
+        std::vector<double2> v;
+        spec_from_ref(h, host + n * per, v);
+        HIPCHECK(hipMemcpyAsync(dev, v.data(), sizeof(double2) * v.size(), hipMemcpyHostToDevice,
+                                h->stream));
+        HIPCHECK(hipStreamSynchronize(h->stream));
+      } else {
+        std::vector<double> v;
+        phys_from_ref(h, host + n * per, v);
+        HIPCHECK(hipMemcpyAsync(dev, v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice,
+                                h->stream));
+        HIPCHECK(hipStreamSynchronize(h->stream));
+      }
+    }
+  });
+}
+
+int dgpu_get_field(dgpu_handle* h, int field, double* host) {
+  if (!h || !host) return DGPU_EINVAL;
+  if (field < 0 || field >= DGPU_NFIELDS) {
+    h->err = "bad field id";
+    return DGPU_EINVAL;
+  }
+  return run_guarded(h, [&] {
+    const size_t per = ref_spec_doubles(h->g);
+    for (int n = 0; n < nfield_copies(h, field); ++n) {
+      void* dev;
+      bool spec;
+      field_buf(h, field, n, &dev, &spec);
+      if (spec) {
+        std::vector<double2> v((size_t)h->g.ny2 * h->g.sp);
+        HIPCHECK(hipMemcpyAsync(v.data(), dev, sizeof(double2) * v.size(), hipMemcpyDeviceToHost,
+                                h->stream));
+        HIPCHECK(hipStreamSynchronize(h->stream));
+        spec_to_ref(h, v, host + n * per);
+      } else {
+        std::vector<double> v((size_t)h->g.ny2 * h->g.pp);
+        HIPCHECK(hipMemcpyAsync(v.data(), dev, sizeof(double) * v.size(), hipMemcpyDeviceToHost,
+                                h->stream));
+        HIPCHECK(hipStreamSynchronize(h->stream));
+        phys_to_ref(h, v, host + n * per);
+      }
+    }
+  });
+}
+
+int dgpu_apply_bc_vel_post(dgpu_handle* h, double time) {
+  if (!h) return DGPU_EINVAL;
+  return run_guarded(h, [&] { launch_bc_post(h, time); });
+}
+
+int dgpu_rem_div(dgpu_handle* h) {
+  if (!h) return DGPU_EINVAL;
+  return run_guarded(h, [&] { launch_remdiv(h, false, 1.0); });
+}
+
+int dgpu_rk_substep(dgpu_handle* h, int rk_step, double time, int time_step) {
+  if (!h) return DGPU_EINVAL;
+  if (rk_step < 1 || rk_step > 3) {
+    h->err = "rk_step must be 1, 2 or 3";
+    return DGPU_EINVAL;
+  }
+  return run_guarded(h, [&] { rk_substep(h, rk_step, time, time_step); });
+}
+
+int dgpu_step(dgpu_handle* h, int nsteps, double* time, int32_t* time_step) {
+  if (!h || !time || !time_step || nsteps < 0) return DGPU_EINVAL;
+  return run_guarded(h, [&] {
+    for (int s = 0; s < nsteps; ++s) {
+      *time_step += 1;
+      for (int rk = 1; rk <= 3; ++rk) rk_substep(h, rk, *time, *time_step);
+      *time += h->p.delta_t;  // diablo.f90:108
+    }
+  });
+}
+
+int dgpu_timing_enable(dgpu_handle* h, int enable) {
+  if (!h) return DGPU_EINVAL;
+  return run_guarded(h, [&] {
+    collect_timing(h);
+    h->timing = enable != 0;
+    for (int c = 0; c < DGPU_NCLASS; ++c) {
+      h->t_ms[c] = 0.0;
+      h->t_bytes[c] = 0.0;
+      h->t_count[c] = 0;
+    }
+  });
+}
+
+int dgpu_timing_read(dgpu_handle* h, double* ms, double* bytes, int64_t* count) {
+  if (!h || !ms || !bytes || !count) return DGPU_EINVAL;
+  return run_guarded(h, [&] {
+    collect_timing(h);
+    for (int c = 0; c < DGPU_NCLASS; ++c) {
+      ms[c] = h->t_ms[c];
+      bytes[c] = h->t_bytes[c];
+      count[c] = h->t_count[c];
+    }
+  });
+}
+
+const char* dgpu_kernel_class_name(int cls) {
+  static const char* names[DGPU_NCLASS] = {"x_forward", "x_inverse", "z_forward", "z_inverse",
+                                           "x_products", "spectral_rhs", "cn_solve",
+                                           "projection", "wall_bc"};
+  return (cls >= 0 && cls < DGPU_NCLASS) ? names[cls] : "";
+}
+
+int dgpu_synchronize(dgpu_handle* h) {
+  if (!h) return DGPU_EINVAL;
+  return run_guarded(h, [&] { HIPCHECK(hipStreamSynchronize(h->stream)); });
+}
+
+}  // extern "C"

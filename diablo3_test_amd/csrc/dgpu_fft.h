@@ -1,0 +1,155 @@
+// dgpu_fft.h -- device-side FFT building blocks for gfx950 (CDNA4).
+//
+// The reference transforms every y-plane with FFTW guru plans (fft.f90:185-209):
+// a real-to-complex FFT of length Nx along x, an MPI_Alltoall, then a complex FFT
+// of length Nz along z.  Here the whole field sits in HBM, so each direction is
+// one kernel pass over the field and the transpose is the pass's own store
+// pattern.  The transforms themselves run in LDS: a Stockham autosort FFT
+// (radix 8, then one radix-4 or radix-2 pass when log2 N is not a multiple of
+// 3) executed by N/8 threads per transform, 8 complex values per thread in
+// registers between passes.
+//
+// Conventions match FFTW's: forward = exp(-2 pi i k n / N), unnormalised;
+// backward = exp(+2 pi i k n / N), unnormalised.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace dgpu {
+
+template <typename R>
+struct cpx {
+  R x, y;
+};
+
+template <typename R>
+__device__ __forceinline__ cpx<R> operator+(cpx<R> a, cpx<R> b) {
+  return {a.x + b.x, a.y + b.y};
+}
+template <typename R>
+__device__ __forceinline__ cpx<R> operator-(cpx<R> a, cpx<R> b) {
+  return {a.x - b.x, a.y - b.y};
+}
+template <typename R>
+__device__ __forceinline__ cpx<R> operator*(cpx<R> a, cpx<R> b) {
+  return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename R>
+__device__ __forceinline__ cpx<R> operator*(R s, cpx<R> a) {
+  return {s * a.x, s * a.y};
+}
+template <typename R>
+__device__ __forceinline__ cpx<R> conjg(cpx<R> a) {
+  return {a.x, -a.y};
+}
+// i*k*a : the reference's cikx(i)*a with cikx = (0, kx)  (fft.f90:217, 229)
+template <typename R>
+__device__ __forceinline__ cpx<R> ik(R k, cpx<R> a) {
+  return {-k * a.y, k * a.x};
+}
+
+// Small DFTs in registers.  S = -1 forward, +1 backward.
+template <typename R, int S>
+__device__ __forceinline__ void dft2(cpx<R>* v) {
+  cpx<R> a = v[0], b = v[1];
+  v[0] = a + b;
+  v[1] = a - b;
+}
+
+template <typename R, int S>
+__device__ __forceinline__ void dft4(cpx<R>& v0, cpx<R>& v1, cpx<R>& v2, cpx<R>& v3) {
+  cpx<R> a = v0 + v2, b = v0 - v2, c = v1 + v3, t = v1 - v3;
+  cpx<R> d = {-(R)S * t.y, (R)S * t.x};  // (S i) * t
+  v0 = a + c;
+  v1 = b + d;
+  v2 = a - c;
+  v3 = b - d;
+}
+
+template <typename R, int S>
+__device__ __forceinline__ void dft8(cpx<R>* v) {
+  const R h = (R)0.70710678118654752440084436210485;
+  cpx<R> e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+  cpx<R> o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+  dft4<R, S>(e0, e1, e2, e3);
+  dft4<R, S>(o0, o1, o2, o3);
+  // twiddles w8^m, w8 = exp(S i pi/4)
+  cpx<R> t1 = {h * (o1.x - (R)S * o1.y), h * (o1.y + (R)S * o1.x)};
+  cpx<R> t2 = {-(R)S * o2.y, (R)S * o2.x};
+  cpx<R> t3 = {h * (-o3.x - (R)S * o3.y), h * (-o3.y + (R)S * o3.x)};
+  v[0] = e0 + o0;
+  v[4] = e0 - o0;
+  v[1] = e1 + t1;
+  v[5] = e1 - t1;
+  v[2] = e2 + t2;
+  v[6] = e2 - t2;
+  v[3] = e3 + t3;
+  v[7] = e3 - t3;
+}
+
+// One Stockham pass of radix 2^LOGR over a length-N transform in LDS, after
+// the radices already applied multiply to Ns = 2^logNs.  Each of the G = N/8
+// threads owns 8 complex values (8 >> LOGR butterflies).
+template <typename R, int S, int LOGR>
+__device__ __forceinline__ void stockham_pass(cpx<R>* buf, int N, int logNs,
+                                              const cpx<R>* __restrict__ tw, int gt) {
+  constexpr int RDX = 1 << LOGR, NB = 8 >> LOGR;
+  const int G = N >> 3;
+  const int stride = N >> LOGR;
+  const int Ns = 1 << logNs;
+  const int tstep = N >> (logNs + LOGR);
+  cpx<R> v[8];
+#pragma unroll
+  for (int bb = 0; bb < NB; ++bb)
+#pragma unroll
+    for (int r = 0; r < RDX; ++r) v[bb * RDX + r] = buf[gt + bb * G + r * stride];
+  __syncthreads();
+#pragma unroll
+  for (int bb = 0; bb < NB; ++bb) {
+    const int j = gt + bb * G;
+    const int k = j & (Ns - 1);
+    if (logNs > 0) {
+#pragma unroll
+      for (int r = 1; r < RDX; ++r) {
+        cpx<R> t = tw[k * r * tstep];
+        if (S > 0) t.y = -t.y;
+        v[bb * RDX + r] = v[bb * RDX + r] * t;
+      }
+    }
+    if constexpr (LOGR == 3)
+      dft8<R, S>(v);
+    else if constexpr (LOGR == 2)
+      dft4<R, S>(v[bb * 4], v[bb * 4 + 1], v[bb * 4 + 2], v[bb * 4 + 3]);
+    else
+      dft2<R, S>(v + bb * 2);
+    const int idxD = ((j >> logNs) << (logNs + LOGR)) + k;
+#pragma unroll
+    for (int r = 0; r < RDX; ++r) buf[idxD + r * Ns] = v[bb * RDX + r];
+  }
+  __syncthreads();
+}
+
+// In-place complex FFT of length N (power of two, 8 <= N <= 4096) held in LDS
+// at `buf`, executed by the G = N/8 threads of one group (gt = 0..G-1).
+// tw[t] = exp(-2 pi i t / N), t = 0..N-1.  Every thread of the workgroup must
+// call this the same number of times with the same N: it synchronises the
+// whole workgroup (one barrier on entry, two per pass).
+template <typename R, int S>
+__device__ __forceinline__ void lds_fft(cpx<R>* buf, int N, int logN, const cpx<R>* __restrict__ tw, int gt) {
+  __syncthreads();
+  int logNs = 0;
+  while (logNs < logN) {
+    const int rem = logN - logNs;
+    if (rem >= 3) {
+      stockham_pass<R, S, 3>(buf, N, logNs, tw, gt);
+      logNs += 3;
+    } else if (rem == 2) {
+      stockham_pass<R, S, 2>(buf, N, logNs, tw, gt);
+      logNs += 2;
+    } else {
+      stockham_pass<R, S, 1>(buf, N, logNs, tw, gt);
+      logNs += 1;
+    }
+  }
+}
+
+}  // namespace dgpu

@@ -1,0 +1,809 @@
+// dgpu_kernels.h -- HIP kernels of the channel RK3/Crank-Nicolson substep
+// (reference: for/channel.f90 rk_chan_1, rem_div_chan; for/fft.f90).
+//
+// Device layouts (one GPU holds the whole field, so no pencils):
+//   physical  P[j][k][i]       j = 0..Ny+1, k = 0..Nz-1, i = 0..Nx-1   (i fastest)
+//   spectral  C[j][q][kk]      q = 0..Nkx (kx = q*2pi/Lx), kk = 0..2Nkz packed as
+//                              the reference packs kz (fft.f90:307-312)   (kk fastest)
+//   x-spectra T[j][q][k]       the half-way point of a 2-D transform: x already
+//                              transformed and dealiased, z not yet     (k fastest)
+// Only the dealiased kx block q <= Nkx is stored: the reference zeroes
+// i in (Nkx, Nx/2] on every transform (fft.f90:264-266, 374-376) and no
+// spectral operation of the substep couples kx modes, so those modes are
+// identically zero in every field.
+#pragma once
+#include "dgpu_fft.h"
+
+namespace dgpu {
+
+constexpr int KT = 8;  // rows (z lines) per workgroup in the x passes
+
+struct Geo {
+  int nx, ny, nz;     // Nx, Ny, Nz
+  int lognx, lognz;   // log2 Nx, log2 Nz
+  int nkx, nkz;       // Nx/3, Nz/3 (domain.f90:18-19)
+  int nq, nkk;        // Nkx+1, 2Nkz+1
+  int ny2;            // Ny+2 planes
+  long long pp;       // physical plane: Nx*Nz
+  long long sp;       // spectral plane: nq*nkk
+  long long tp;       // x-spectra plane: nq*Nz
+};
+
+// natural z index of packed mode kk (inverse of the pack at fft.f90:308-311)
+__device__ __forceinline__ int kz_nat(const Geo& g, int kk) {
+  return kk <= g.nkz ? kk : g.nz - 2 * g.nkz - 1 + kk;
+}
+
+// ---------------------------------------------------------------- x pass --
+// Real-pair post-processing: z = FFT(a + i b) -> A = FFT(a), B = FFT(b) at q.
+template <typename R>
+__device__ __forceinline__ void split_pair(const cpx<R>* f, int n, int q, R scale, cpx<R>& A,
+                                           cpx<R>& B) {
+  cpx<R> z = f[q], zc = conjg(f[(n - q) & (n - 1)]);
+  R h = (R)0.5 * scale;
+  A = {h * (z.x + zc.x), h * (z.y + zc.y)};
+  B = {h * (z.y - zc.y), -h * (z.x - zc.x)};
+}
+
+// Forward x transform of one field on planes j0.. : the r2c of fft.f90:258
+// with the 1/Nx scale and dealias of :259-268.  Two rows share one complex FFT.
+// grid (Nz/KT, nplanes), block 256, dynamic LDS (ngroups*Nx + nq*KT) cpx.
+template <typename R>
+__global__ void __launch_bounds__(256) k_xfwd(const R* __restrict__ u, cpx<R>* __restrict__ T,
+                                              Geo g, const cpx<R>* __restrict__ tw, int j0) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<R>* smem = reinterpret_cast<cpx<R>*>(smem_raw);
+  const int G = g.nx >> 3;
+  const int ngroups = blockDim.x / G;
+  const int grp = threadIdx.x / G, gt = threadIdx.x % G;
+  cpx<R>* fbuf = smem + grp * g.nx;
+  cpx<R>* stage = smem + ngroups * g.nx;
+  const int j = j0 + blockIdx.y, k0 = blockIdx.x * KT;
+  const R* plane = u + (long long)j * g.pp;
+  const R inv = (R)1 / (R)g.nx;
+  for (int m0 = 0; m0 < KT / 2; m0 += ngroups) {
+    const int m = m0 + grp;
+    const bool act = grp < ngroups && m < KT / 2;
+    if (act) {
+      const R* ra = plane + (long long)(k0 + 2 * m) * g.nx;
+      const R* rb = ra + g.nx;
+      for (int i = gt; i < g.nx; i += G) fbuf[i] = {ra[i], rb[i]};
+    }
+    lds_fft<R, -1>(fbuf, g.nx, g.lognx, tw, gt);
+    if (act) {
+      for (int q = gt; q < g.nq; q += G) {
+        cpx<R> A, B;
+        split_pair(fbuf, g.nx, q, inv, A, B);
+        stage[q * KT + 2 * m] = A;
+        stage[q * KT + 2 * m + 1] = B;
+      }
+    }
+    __syncthreads();
+  }
+  cpx<R>* out = T + (long long)j * g.tp + k0;
+  for (int idx = threadIdx.x; idx < g.nq * KT; idx += blockDim.x) {
+    const int q = idx / KT, kk = idx % KT;
+    out[(long long)q * g.nz + kk] = stage[idx];
+  }
+}
+
+// Inverse x transform: dealias (fft.f90:372-378) and c2r (:380), unnormalised.
+// FFTW's c2r reads only the real part of the kx = 0 input.
+template <typename R>
+__global__ void __launch_bounds__(256) k_xinv(const cpx<R>* __restrict__ T, R* __restrict__ u,
+                                              Geo g, const cpx<R>* __restrict__ tw, int j0) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<R>* smem = reinterpret_cast<cpx<R>*>(smem_raw);
+  const int G = g.nx >> 3;
+  const int ngroups = blockDim.x / G;
+  const int grp = threadIdx.x / G, gt = threadIdx.x % G;
+  cpx<R>* fbuf = smem + grp * g.nx;
+  cpx<R>* stage = smem + ngroups * g.nx;
+  const int j = j0 + blockIdx.y, k0 = blockIdx.x * KT;
+  const cpx<R>* in = T + (long long)j * g.tp + k0;
+  for (int idx = threadIdx.x; idx < g.nq * KT; idx += blockDim.x) {
+    const int q = idx / KT, kk = idx % KT;
+    stage[idx] = in[(long long)q * g.nz + kk];
+  }
+  R* plane = u + (long long)j * g.pp;
+  for (int m0 = 0; m0 < KT / 2; m0 += ngroups) {
+    const int m = m0 + grp;
+    const bool act = grp < ngroups && m < KT / 2;
+    __syncthreads();  // stage complete / previous round's reads of fbuf done
+    if (act) {
+      for (int qq = gt; qq < g.nx; qq += G) {
+        cpx<R> z = {(R)0, (R)0};
+        if (qq == 0) {
+          z = {stage[2 * m].x, stage[2 * m + 1].x};
+        } else if (qq <= g.nkx) {
+          cpx<R> A = stage[qq * KT + 2 * m], B = stage[qq * KT + 2 * m + 1];
+          z = {A.x - B.y, A.y + B.x};
+        } else if (qq >= g.nx - g.nkx) {
+          const int q = g.nx - qq;
+          cpx<R> A = stage[q * KT + 2 * m], B = stage[q * KT + 2 * m + 1];
+          z = {A.x + B.y, B.x - A.y};
+        }
+        fbuf[qq] = z;
+      }
+    }
+    lds_fft<R, 1>(fbuf, g.nx, g.lognx, tw, gt);
+    if (act) {
+      R* ra = plane + (long long)(k0 + 2 * m) * g.nx;
+      R* rb = ra + g.nx;
+      for (int i = gt; i < g.nx; i += G) {
+        cpx<R> z = fbuf[i];
+        ra[i] = z.x;
+        rb[i] = z.y;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------- z pass --
+// Forward z transform + 1/Nz + pack (fft.f90:298-313).  One column (j, q) per
+// group; grid (ceil(nq/ngroups), nplanes), block 256.
+template <typename R>
+__global__ void __launch_bounds__(256) k_zfwd(const cpx<R>* __restrict__ T, cpx<R>* __restrict__ C,
+                                              Geo g, const cpx<R>* __restrict__ tw, int j0) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<R>* smem = reinterpret_cast<cpx<R>*>(smem_raw);
+  const int G = g.nz >> 3;
+  const int ngroups = blockDim.x / G;
+  const int grp = threadIdx.x / G, gt = threadIdx.x % G;
+  cpx<R>* fbuf = smem + grp * g.nz;
+  const int j = j0 + blockIdx.y, q = blockIdx.x * ngroups + grp;
+  const bool act = grp < ngroups && q < g.nq;
+  if (act) {
+    const cpx<R>* col = T + (long long)j * g.tp + (long long)q * g.nz;
+    for (int k = gt; k < g.nz; k += G) fbuf[k] = col[k];
+  }
+  lds_fft<R, -1>(fbuf, g.nz, g.lognz, tw, gt);
+  if (act) {
+    const R inv = (R)1 / (R)g.nz;
+    cpx<R>* out = C + (long long)j * g.sp + (long long)q * g.nkk;
+    for (int kk = gt; kk < g.nkk; kk += G) out[kk] = inv * fbuf[kz_nat(g, kk)];
+  }
+}
+
+// Unpack + zero (fft.f90:330-341) + backward z transform (:345), unnormalised.
+template <typename R>
+__global__ void __launch_bounds__(256) k_zinv(const cpx<R>* __restrict__ C, cpx<R>* __restrict__ T,
+                                              Geo g, const cpx<R>* __restrict__ tw, int j0) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<R>* smem = reinterpret_cast<cpx<R>*>(smem_raw);
+  const int G = g.nz >> 3;
+  const int ngroups = blockDim.x / G;
+  const int grp = threadIdx.x / G, gt = threadIdx.x % G;
+  cpx<R>* fbuf = smem + grp * g.nz;
+  const int j = j0 + blockIdx.y, q = blockIdx.x * ngroups + grp;
+  const bool act = grp < ngroups && q < g.nq;
+  if (act) {
+    const cpx<R>* col = C + (long long)j * g.sp + (long long)q * g.nkk;
+    const int off = g.nz - 2 * g.nkz - 1;
+    for (int k = gt; k < g.nz; k += G) {
+      cpx<R> z = {(R)0, (R)0};
+      if (k <= g.nkz)
+        z = col[k];
+      else if (k >= g.nz - g.nkz)
+        z = col[k - off];
+      fbuf[k] = z;
+    }
+  }
+  lds_fft<R, 1>(fbuf, g.nz, g.lognz, tw, gt);
+  if (act) {
+    cpx<R>* out = T + (long long)j * g.tp + (long long)q * g.nz;
+    for (int k = gt; k < g.nz; k += G) out[k] = fbuf[k];
+  }
+}
+
+// ------------------------------------------------------ nonlinear x pass --
+// The products of channel.f90:299-459 (velocity) and :557-608 (scalar n),
+// computed on the fly from the physical fields and x-transformed in pairs.
+// Each pair feeds one or two x-spectra; terms that the reference multiplies
+// by ikx after its 2-D transform get that factor here (kx commutes with the z
+// transform), terms multiplied by ikz keep a spectrum of their own:
+//   pair 0: (u1u1, V1)   -> X1  = ikx F(u1u1) + F(V1)            -> cf1
+//   pair 1: (u3u1, V3)   -> X13 = F(u3u1); X3 = ikx F(u3u1) + F(V3)
+//   pair 2: (u3u3, W3)   -> X33 = F(u3u3); X32 = F(W3)            (W3 = interp(u3) u2)
+//   pair 3: (W1, V2)     -> X2  = ikx F(W1) + F(V2)              (W1 = interp(u1) u2)
+//   pair 4: (th u1, Vt)  -> Xt  = ikx F(th u1) + F(Vt)
+//   pair 5: (th u3, 0)   -> Xt3 = F(th u3)
+struct ProdArgs {
+  const double* u1;
+  const double* u2;
+  const double* u3;
+  const double* th;
+  const double* dy;
+  const double* dyf;
+  const double* kx;  // kx[q]
+  int js, je;        // jstart, jend
+  int jst, jet;      // jstart_th(n), jend_th(n)
+};
+
+template <typename R>
+__device__ __forceinline__ void prod_pair(const ProdArgs& a, const Geo& g, int pair, int j,
+                                          long long o, bool inA, bool inB, R& pa, R& pb) {
+  const long long up = o + g.pp, dn = o - g.pp;
+  double va = 0.0, vb = 0.0;
+  switch (pair) {
+    case 0:
+      if (inA) va = a.u1[o] * a.u1[o];
+      if (inB)
+        vb = (a.u1[up] * a.u2[up] + a.u1[o] * a.u2[up] - a.u1[o] * a.u2[o] - a.u1[dn] * a.u2[o]) /
+             (2.0 * a.dyf[j]);
+      break;
+    case 1:
+      if (inA) va = a.u3[o] * a.u1[o];
+      if (inB)
+        vb = (a.u3[up] * a.u2[up] + a.u3[o] * a.u2[up] - a.u3[o] * a.u2[o] - a.u3[dn] * a.u2[o]) /
+             (2.0 * a.dyf[j]);
+      break;
+    case 2:
+      if (inA) va = a.u3[o] * a.u3[o];
+      if (inB) vb = ((a.dyf[j] * a.u3[o] + a.dyf[j - 1] * a.u3[dn]) / (2.0 * a.dy[j])) * a.u2[o];
+      break;
+    case 3:
+      if (inA) va = ((a.dyf[j] * a.u1[o] + a.dyf[j - 1] * a.u1[dn]) / (2.0 * a.dy[j])) * a.u2[o];
+      if (inB) {
+        double s = a.u2[o] + a.u2[up], t = a.u2[o] + a.u2[dn];
+        vb = (0.25 * (s * s) - 0.25 * (t * t)) / a.dy[j];
+      }
+      break;
+    case 4:
+      if (inA) va = a.th[o] * a.u1[o];
+      if (inB)
+        vb = (a.th[up] * a.u2[up] + a.th[o] * a.u2[up] - a.th[o] * a.u2[o] - a.th[dn] * a.u2[o]) /
+             (2.0 * a.dyf[j]);
+      break;
+    default:
+      if (inA) va = a.th[o] * a.u3[o];
+      break;
+  }
+  pa = (R)va;
+  pb = (R)vb;
+}
+
+__device__ __forceinline__ void pair_ranges(const ProdArgs& a, const Geo& g, int pair, int j,
+                                            bool& inA, bool& inB) {
+  const bool r1 = j >= a.js && j <= a.je;
+  const bool r2 = j >= 2 && j <= g.ny;
+  const bool rt = j >= a.jst && j <= a.jet;
+  switch (pair) {
+    case 0:
+    case 1: inA = r1; inB = r1; break;
+    case 2: inA = r1; inB = r2; break;
+    case 3: inA = r2; inB = r2; break;
+    case 4: inA = rt; inB = rt; break;
+    default: inA = rt; inB = false; break;
+  }
+}
+
+// grid (Nz/KT, nplanes), block 256, LDS (ngroups*Nx + 2*nq*KT) cpx.
+template <typename R>
+__global__ void __launch_bounds__(256) k_xprod(ProdArgs a, Geo g, int pair, cpx<R>* __restrict__ O0,
+                                               cpx<R>* __restrict__ O1, const cpx<R>* __restrict__ tw,
+                                               int j0) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<R>* smem = reinterpret_cast<cpx<R>*>(smem_raw);
+  const int G = g.nx >> 3;
+  const int ngroups = blockDim.x / G;
+  const int grp = threadIdx.x / G, gt = threadIdx.x % G;
+  cpx<R>* fbuf = smem + grp * g.nx;
+  cpx<R>* st0 = smem + ngroups * g.nx;
+  cpx<R>* st1 = st0 + g.nq * KT;
+  const int j = j0 + blockIdx.y, k0 = blockIdx.x * KT;
+  bool inA, inB;
+  pair_ranges(a, g, pair, j, inA, inB);
+  const R inv = (R)1 / (R)g.nx;
+  for (int m0 = 0; m0 < KT; m0 += ngroups) {
+    const int m = m0 + grp;
+    const bool act = grp < ngroups && m < KT;
+    if (act) {
+      const long long rowo = (long long)j * g.pp + (long long)(k0 + m) * g.nx;
+      for (int i = gt; i < g.nx; i += G) {
+        R pa, pb;
+        prod_pair<R>(a, g, pair, j, rowo + i, inA, inB, pa, pb);
+        fbuf[i] = {pa, pb};
+      }
+    }
+    lds_fft<R, -1>(fbuf, g.nx, g.lognx, tw, gt);
+    if (act) {
+      for (int q = gt; q < g.nq; q += G) {
+        cpx<R> A, B;
+        split_pair(fbuf, g.nx, q, inv, A, B);
+        const R kxq = (R)a.kx[q];
+        cpx<R> ikA = ik(kxq, A);
+        switch (pair) {
+          case 0: st0[q * KT + m] = ikA + B; break;
+          case 1: st0[q * KT + m] = A; st1[q * KT + m] = ikA + B; break;
+          case 2: st0[q * KT + m] = A; st1[q * KT + m] = B; break;
+          case 3:
+          case 4: st0[q * KT + m] = ikA + B; break;
+          default: st0[q * KT + m] = A; break;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  const bool two = (pair == 1 || pair == 2);
+  cpx<R>* out0 = O0 + (long long)j * g.tp + k0;
+  cpx<R>* out1 = two ? O1 + (long long)j * g.tp + k0 : nullptr;
+  for (int idx = threadIdx.x; idx < g.nq * KT; idx += blockDim.x) {
+    const int q = idx / KT, kk = idx % KT;
+    out0[(long long)q * g.nz + kk] = st0[idx];
+    if (two) out1[(long long)q * g.nz + kk] = st1[idx];
+  }
+}
+
+// ---------------------------------------------------- spectral RHS z pass --
+// For one (j, q) column: z-transform the x-spectra of the products, assemble
+// the explicit RK terms of channel.f90:39-253 and :299-482 (velocity) and
+// :209-246, :557-617 (scalars), store the new cf, form cr, and take cr back
+// through the backward z transform (first half of fft_xz_to_physical, :485-487,
+// :621).  Slots: 0 X1, 1 X13, 2 X3, 3 X33, 4 X2, 5 X32, 6+2n Xt_n, 7+2n Xt3_n.
+// cr goes to slot 0 (cr1), 1 (cr3), 2 (cr2), 3+n (crth_n) once read.
+constexpr int MAXTH = 4;
+constexpr int MAXSLOT = 6 + 2 * MAXTH;
+
+struct RhsArgs {
+  const double2* X[MAXSLOT];      // x-spectra of the products
+  double2* Tcr[3 + MAXTH];        // out: x-spectra of cr1, cr3, cr2, crth_n (slot order)
+  const double2* cu1;
+  const double2* cu2;
+  const double2* cu3;
+  const double2* cp;
+  const double2* cth[MAXTH];
+  double2* cf1;
+  double2* cf2;
+  double2* cf3;
+  double2* cfth[MAXTH];
+  const double* kx;
+  const double* kz;
+  const double* dy;
+  const double* dyf;
+  double nu, beta, coriolis;  // coriolis = Ro_inv/delta
+  double grav_x, grav_y, grav_z;
+  double pr[MAXTH], dthdx[MAXTH], dthdz[MAXTH];
+  double temp3, temp4, temp5, force;  // force = px0 (+ oscillation), 0 if none
+  int rk_step, nth;
+  int js, je;
+  int jst[MAXTH], jet[MAXTH];
+};
+
+__global__ void __launch_bounds__(1024) k_rhs(RhsArgs a, Geo g, const double2* __restrict__ twd,
+                                              int j0) {
+  typedef cpx<double> C;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C* smem = reinterpret_cast<C*>(smem_raw);
+  const C* tw = reinterpret_cast<const C*>(twd);
+  const int G = g.nz >> 3;
+  const int ngroups = blockDim.x / G;
+  const int grp = threadIdx.x / G, gt = threadIdx.x % G;
+  const int j = j0 + blockIdx.y, q = blockIdx.x;
+  const int nslot = 6 + 2 * a.nth;
+  const bool in1 = j >= a.js && j <= a.je;
+  const bool in2 = j >= 2 && j <= g.ny;
+  bool inT[MAXTH];
+  for (int n = 0; n < MAXTH; ++n) inT[n] = n < a.nth && j >= a.jst[n] && j <= a.jet[n];
+  auto slot_used = [&](int s) -> bool {
+    if (s < 4) return in1;
+    if (s < 6) return in2;
+    return inT[(s - 6) >> 1];
+  };
+  const long long to = (long long)j * g.tp + (long long)q * g.nz;
+  // --- z-transform the product spectra
+  // The LDS holds one buffer per (round, group), so groups without a slot in
+  // a round transform an unused buffer and every thread meets every barrier.
+  for (int s0 = 0; s0 < nslot; s0 += ngroups) {
+    const int s = s0 + grp;
+    const bool act = s < nslot && slot_used(s);
+    C* buf = smem + (long long)s * g.nz;
+    if (act) {
+      const C* col = reinterpret_cast<const C*>(a.X[s]) + to;
+      for (int k = gt; k < g.nz; k += G) buf[k] = col[k];
+    }
+    lds_fft<double, -1>(buf, g.nz, g.lognz, tw, gt);
+  }
+  // --- explicit terms, mode by mode
+  const double invnz = 1.0 / g.nz;
+  const double kxq = a.kx[q];
+  const double cdy = a.dy[j], cdyf = a.dyf[j], cdyfm = a.dyf[j > 0 ? j - 1 : 0];
+  for (int kk = threadIdx.x; kk < g.nkk; kk += blockDim.x) {
+    const int nat = kz_nat(g, kk);
+    const long long o = (long long)j * g.sp + (long long)q * g.nkk + kk;
+    const double kzk = a.kz[kk];
+    const double lap = kxq * kxq + kzk * kzk;
+    const double visc = (a.beta == 1.0) ? lap : pow(lap, a.beta);
+    const bool mean = (q == 0 && kk == 0);
+    // slot s at this mode; a slot is read before the cr slot that reuses it
+    // is written (cr slot 3+n < 6+2n, and slots 0..3 belong to u1/u3)
+    auto Z = [&](int s) -> C { return invnz * smem[(long long)s * g.nz + nat]; };
+    C cu1 = {0, 0}, cu3 = {0, 0};
+    if (in1 || a.nth > 0) {
+      cu1 = reinterpret_cast<const C*>(a.cu1)[o];
+      cu3 = reinterpret_cast<const C*>(a.cu3)[o];
+    }
+    if (in1) {
+      C cp = reinterpret_cast<const C*>(a.cp)[o];
+      C f1 = (-a.nu * visc) * cu1 + a.coriolis * cu3;
+      C f3 = (-a.nu * visc) * cu3 - a.coriolis * cu1;
+      for (int n = 0; n < a.nth; ++n) {
+        C th = reinterpret_cast<const C*>(a.cth[n])[o];
+        f1 = f1 + a.grav_x * th;
+        f3 = f3 + a.grav_z * th;
+      }
+      f1 = f1 - (Z(0) + ik(kzk, Z(1)));
+      f3 = f3 - (Z(2) + ik(kzk, Z(3)));
+      C r1 = cu1, r3 = cu3;
+      if (a.rk_step > 1) {
+        r1 = r1 + a.temp3 * reinterpret_cast<const C*>(a.cf1)[o];
+        r3 = r3 + a.temp3 * reinterpret_cast<const C*>(a.cf3)[o];
+      }
+      r1 = r1 - a.temp4 * ik(kxq, cp);
+      r3 = r3 - a.temp4 * ik(kzk, cp);
+      if (mean) r1.x -= a.temp4 * a.force;
+      r1 = r1 + a.temp5 * f1;
+      r3 = r3 + a.temp5 * f3;
+      reinterpret_cast<C*>(a.cf1)[o] = f1;
+      reinterpret_cast<C*>(a.cf3)[o] = f3;
+      smem[0 * g.nz + nat] = r1;
+      smem[1 * g.nz + nat] = r3;
+    }
+    if (in2) {
+      C cu2 = reinterpret_cast<const C*>(a.cu2)[o];
+      C cpj = reinterpret_cast<const C*>(a.cp)[o];
+      C cpm = reinterpret_cast<const C*>(a.cp)[o - g.sp];
+      C f2 = (-a.nu * visc) * cu2;
+      if (!mean) {
+        for (int n = 0; n < a.nth; ++n) {
+          C th = reinterpret_cast<const C*>(a.cth[n])[o];
+          C thm = reinterpret_cast<const C*>(a.cth[n])[o - g.sp];
+          f2 = f2 + (a.grav_y / (2.0 * cdy)) * (cdyfm * th + cdyf * thm);
+        }
+      }
+      f2 = f2 - (Z(4) + ik(kzk, Z(5)));
+      C r2 = cu2;
+      if (a.rk_step > 1) r2 = r2 + a.temp3 * reinterpret_cast<const C*>(a.cf2)[o];
+      r2 = r2 - (a.temp4 / cdy) * (cpj - cpm);
+      r2 = r2 + a.temp5 * f2;
+      reinterpret_cast<C*>(a.cf2)[o] = f2;
+      smem[2 * g.nz + nat] = r2;
+    }
+    for (int n = 0; n < a.nth; ++n) {
+      if (!inT[n]) continue;
+      C th = reinterpret_cast<const C*>(a.cth[n])[o];
+      C ft = (-(a.nu / a.pr[n]) * visc) * th;
+      ft = ft - a.dthdx[n] * cu1 - a.dthdz[n] * cu3;
+      ft = ft - (Z(6 + 2 * n) + ik(kzk, Z(7 + 2 * n)));
+      C rt = th;
+      if (a.rk_step > 1) rt = rt + a.temp3 * reinterpret_cast<const C*>(a.cfth[n])[o];
+      rt = rt + a.temp5 * ft;
+      reinterpret_cast<C*>(a.cfth[n])[o] = ft;
+      smem[(3 + n) * g.nz + nat] = rt;
+    }
+  }
+  // zero the dealiased gap of the cr slots (fft.f90:336-340)
+  const int ncr = 3 + a.nth;
+  for (int c = 0; c < ncr; ++c)
+    for (int k = g.nkz + 1 + threadIdx.x; k < g.nz - g.nkz; k += blockDim.x)
+      smem[c * g.nz + k] = C{0.0, 0.0};
+  // --- backward z transform of cr
+  auto cr_used = [&](int c) -> bool {
+    if (c < 2) return in1;
+    if (c == 2) return in2;
+    return inT[c - 3];
+  };
+  for (int c0 = 0; c0 < ncr; c0 += ngroups) {
+    const int c = c0 + grp;
+    const bool act = c < ncr && cr_used(c);
+    C* buf = smem + (long long)c * g.nz;
+    lds_fft<double, 1>(buf, g.nz, g.lognz, tw, gt);
+    if (act) {
+      C* out = reinterpret_cast<C*>(a.Tcr[c]) + to;
+      for (int k = gt; k < g.nz; k += G) out[k] = buf[k];
+    }
+  }
+}
+
+// ------------------------------------------------------ tridiagonal solves --
+// Real Crank-Nicolson solve of channel.f90:656-707 / :716-877 for one field,
+// one (x,z) column per lane, rows j = 0..Ny+1.  Without LES the matrix rows
+// depend only on j (dy, dyf, temp1, nu_v_scale, Pr), so the LU factors of the
+// reference's Thomas sweep (thomas_forward_real_mpi / thomas_backward_real_mpi,
+// :1335-1405, :1460-1496) are computed once on the host.  Each column's rows
+// are split over S segments of L rows (one wavefront per segment); a segment
+// runs its part of both sweeps in registers and the segments are joined by
+// the lane-independent carry products (linear first-order recurrences).
+struct SegRow {
+  double m;    // forward multiplier  -a(j)/b'(j-1)            (0 at j = 0)
+  double pm;   // prod of m from the segment's first row to j
+  double e;    // 1/b'(j)
+  double f;    // -c(j)/b'(j)
+  double qf;   // prod of f from j to the segment's last row
+  double wp, wm;  // explicit stencil weights (assembled rows)
+  int kind;    // 1 = assembled row (vec = r + explicit), 0 = fixed row (vec = vb),
+               // 2 = fixed row with vec = vb * (current time-dependent BC value)
+  double vb;   // fixed-row right-hand side
+};
+
+template <int L>
+__global__ void __launch_bounds__(512) k_thomas_seg(const double* __restrict__ r,
+                                                     double* __restrict__ u,
+                                                     const SegRow* __restrict__ rows, int n, int S,
+                                                     long long pp, int ca, int cb, double kscale) {
+  __shared__ double carry_f[8][64];
+  __shared__ double carry_b[8][64];
+  const int lane = threadIdx.x, seg = threadIdx.y;
+  const long long col = (long long)blockIdx.x * 64 + lane;
+  const int s0 = seg * L;
+  double v[L];
+  double w[L + 2];
+#pragma unroll
+  for (int t = 0; t < L + 2; ++t) {
+    const int jj = s0 - 1 + t;
+    w[t] = (jj >= 0 && jj < n) ? u[(long long)jj * pp + col] : 0.0;
+  }
+#pragma unroll
+  for (int t = 0; t < L; ++t) {
+    const int jj = s0 + t;
+    double val = 0.0;
+    if (jj < n) {
+      const SegRow rw = rows[jj];
+      if (rw.kind == 1)
+        val = r[(long long)jj * pp + col] + rw.wp * (w[t + 2] - w[t + 1]) - rw.wm * (w[t + 1] - w[t]);
+      else if (rw.kind == 2)
+        val = rw.vb * kscale;
+      else
+        val = rw.vb;
+    }
+    v[t] = val;
+  }
+  // forward sweep, local part
+  double h = 0.0;
+#pragma unroll
+  for (int t = 0; t < L; ++t) {
+    const int jj = s0 + t;
+    if (jj < n) {
+      h = v[t] + rows[jj].m * h;
+      v[t] = h;
+    }
+  }
+  const int s1 = min(s0 + L, n) - 1;  // last row of this segment
+  {
+    double last = 0.0;
+#pragma unroll
+    for (int t = 0; t < L; ++t)
+      if (s0 + t == s1) last = v[t];
+    carry_f[seg][lane] = last;
+  }
+  __syncthreads();
+  double gin = 0.0;
+  for (int sp = 0; sp < seg; ++sp) {
+    const int l1 = min(sp * L + L, n) - 1;
+    gin = carry_f[sp][lane] + rows[l1].pm * gin;
+  }
+#pragma unroll
+  for (int t = 0; t < L; ++t) {
+    const int jj = s0 + t;
+    if (jj < n) v[t] += rows[jj].pm * gin;
+  }
+  // backward sweep, local part
+  double x = 0.0;
+#pragma unroll
+  for (int t = L - 1; t >= 0; --t) {
+    const int jj = s0 + t;
+    if (jj < n) {
+      x = v[t] * rows[jj].e + rows[jj].f * x;
+      v[t] = x;
+    }
+  }
+  carry_b[seg][lane] = v[0];
+  __syncthreads();
+  double xin = 0.0;
+  for (int sp = S - 1; sp > seg; --sp) xin = carry_b[sp][lane] + rows[sp * L].qf * xin;
+#pragma unroll
+  for (int t = 0; t < L; ++t) {
+    const int jj = s0 + t;
+    if (jj < n && jj >= ca && jj <= cb) u[(long long)jj * pp + col] = v[t] + rows[jj].qf * xin;
+  }
+}
+
+// ----------------------------------------------------------- projection --
+// rem_div_chan (channel.f90:946-1023) with apply_BC_rem_div_mpi (:1974-2015)
+// and the complex Thomas sweeps (:1408-1457, :1499-1540); optionally the
+// pressure update cp += phi/h (:905-911).  One (q, kk) mode per lane.
+struct RemDivArgs {
+  double2* cu1;
+  double2* cu2;
+  double2* cu3;
+  double2* cp;
+  double* sb;    // scratch b' (ny2 x sp)
+  double2* sg;   // scratch g' (ny2 x sp)
+  const double* kx;
+  const double* kz;
+  const double* dy;
+  const double* dyf;
+  double inv_h;  // 1/temp4
+  int update_cp;
+  int js, je;
+};
+
+__global__ void __launch_bounds__(256) k_remdiv(RemDivArgs a, Geo g) {
+  typedef cpx<double> C;
+  const long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= g.sp) return;
+  const int q = (int)(m / g.nkk), kk = (int)(m % g.nkk);
+  const double kxq = a.kx[q], kzk = a.kz[kk];
+  const double lam = kxq * kxq + kzk * kzk;
+  const bool mean = (q == 0 && kk == 0);
+  const int ny = g.ny;
+  C* cu1 = reinterpret_cast<C*>(a.cu1);
+  C* cu2 = reinterpret_cast<C*>(a.cu2);
+  C* cu3 = reinterpret_cast<C*>(a.cu3);
+  C* cp = reinterpret_cast<C*>(a.cp);
+  C* sg = reinterpret_cast<C*>(a.sg);
+  // row 1: Neumann phi1 = phi2, or Dirichlet phi1 = 0 for the (0,0) mode
+  double bprev = 1.0;
+  double muprev = mean ? 0.0 : -1.0;
+  C gprev = {0.0, 0.0};
+  a.sb[g.sp + m] = bprev;
+  sg[g.sp + m] = gprev;
+  C cu2j = cu2[(long long)2 * g.sp + m];  // cu2 at row j+1 for j = 1
+  for (int j = 2; j <= ny; ++j) {
+    const long long o = (long long)j * g.sp + m;
+    double ml, md, mu;
+    C rhs;
+    if (j < ny) {
+      ml = 1.0 / (a.dy[j] * a.dyf[j]);
+      md = -lam - 1.0 / (a.dy[j + 1] * a.dyf[j]) - 1.0 / (a.dy[j] * a.dyf[j]);
+      mu = 1.0 / (a.dy[j + 1] * a.dyf[j]);
+      C cu2n = cu2[o + g.sp];
+      rhs = ik(kxq, cu1[o]) + (1.0 / a.dyf[j]) * (cu2n - cu2j) + ik(kzk, cu3[o]);
+      cu2j = cu2n;
+    } else {
+      ml = 1.0;
+      md = -1.0;
+      mu = 0.0;
+      rhs = C{0.0, 0.0};
+    }
+    const double mlt = -ml / bprev;
+    const double b = md + mlt * muprev;
+    const C gg = rhs + mlt * gprev;
+    a.sb[o] = b;
+    sg[o] = gg;
+    bprev = b;
+    gprev = gg;
+    muprev = mu;
+  }
+  // backward sweep + projection + pressure
+  C phi_up = {0.0, 0.0};
+  for (int j = ny; j >= 1; --j) {
+    const long long o = (long long)j * g.sp + m;
+    double mu;
+    if (j == ny)
+      mu = 0.0;
+    else if (j == 1)
+      mu = mean ? 0.0 : -1.0;
+    else
+      mu = 1.0 / (a.dy[j + 1] * a.dyf[j]);
+    const C gg = sg[o];
+    const double b = a.sb[o];
+    C phi;
+    if (j == ny)
+      phi = (1.0 / b) * gg;
+    else
+      phi = (1.0 / b) * (gg - mu * phi_up);
+    cu1[o] = cu1[o] - ik(kxq, phi);
+    cu3[o] = cu3[o] - ik(kzk, phi);
+    if (j + 1 <= ny && j + 1 >= 2) {
+      const long long ou = o + g.sp;
+      cu2[ou] = cu2[ou] - (1.0 / a.dy[j + 1]) * (phi_up - phi);
+    }
+    if (a.update_cp && j >= a.js && j <= a.je) cp[o] = cp[o] + a.inv_h * phi;
+    phi_up = phi;
+  }
+}
+
+// ---------------------------------------------------------- wall BCs ----
+// apply_BC_vel_lower_post / apply_BC_vel_upper_post (boundary.f90:299-498).
+struct BcPostArgs {
+  double2* cu1;
+  double2* cu2;
+  double2* cu3;
+  const double* dy;
+  const double* dyf;
+  int u_lo, v_lo, w_lo, u_hi, v_hi, w_hi;
+  double u_lo_c1, v_lo_c1, w_lo_c1, u_hi_c1, v_hi_c1, w_hi_c1;
+  double w_hi_c1_transient;
+};
+
+__global__ void k_bc_post(BcPostArgs a, Geo g) {
+  typedef cpx<double> C;
+  const long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= g.sp) return;
+  const bool mean = (m == 0);
+  const long long P = g.sp;
+  const int ny = g.ny;
+  C* cu1 = reinterpret_cast<C*>(a.cu1);
+  C* cu2 = reinterpret_cast<C*>(a.cu2);
+  C* cu3 = reinterpret_cast<C*>(a.cu3);
+  const C Z = {0.0, 0.0};
+  // lower wall
+  if (a.u_lo == 0) {
+    cu1[m] = Z;
+    cu1[P + m] = Z;
+    if (mean) cu1[P + m] = cu1[m] = C{a.u_lo_c1, 0.0};
+  } else {
+    cu1[P + m] = cu1[2 * P + m];
+    cu1[m] = cu1[P + m];
+    if (mean) {
+      cu1[P + m].x = cu1[2 * P + m].x - a.dy[2] * a.u_lo_c1;
+      cu1[m] = C{cu1[P + m].x - a.dy[1] * a.u_lo_c1, cu1[P + m].y};
+    }
+  }
+  if (a.w_lo == 0) {
+    cu3[m] = Z;
+    cu3[P + m] = Z;
+    if (mean) cu3[P + m] = cu3[m] = C{a.w_lo_c1, 0.0};
+  } else {
+    cu3[P + m] = cu3[2 * P + m];
+    cu3[m] = cu3[P + m];
+    if (mean) {
+      cu3[P + m].x = cu3[2 * P + m].x - a.dy[2] * a.w_lo_c1;
+      cu3[m] = C{cu3[P + m].x - a.dy[1] * a.w_lo_c1, cu3[P + m].y};
+    }
+  }
+  if (a.v_lo == 0) {
+    C t = cu2[2 * P + m];
+    cu2[P + m] = C{2.0 * a.v_lo_c1 - t.x, -t.y};
+    cu2[m] = cu2[P + m];
+  } else {
+    cu2[P + m] = cu2[2 * P + m];
+    cu2[m] = cu2[P + m];
+    if (mean) {
+      cu2[P + m].x = cu2[2 * P + m].x - a.dyf[1] * a.v_lo_c1;
+      cu2[m] = C{cu2[P + m].x - a.dyf[1] * a.v_lo_c1, cu2[P + m].y};
+    }
+  }
+  // upper wall
+  const long long T0 = (long long)ny * P + m, T1 = T0 + P, Tm = T0 - P;
+  if (a.u_hi == 0) {
+    cu1[T0] = Z;
+    cu1[T1] = Z;
+    if (mean) cu1[T0] = cu1[T1] = C{a.u_hi_c1, 0.0};
+  } else {
+    cu1[T0] = cu1[Tm];
+    cu1[T1] = cu1[T0];
+    if (mean) {
+      cu1[T0].x = cu1[Tm].x + a.dy[ny] * a.u_hi_c1;
+      cu1[T1] = C{cu1[T0].x + a.dy[ny] * a.u_hi_c1, cu1[T0].y};
+    }
+  }
+  if (a.w_hi == 0) {
+    cu3[T0] = Z;
+    cu3[T1] = Z;
+    if (mean) {
+      cu3[T0] = C{a.w_hi_c1, 0.0};
+      cu3[T1] = Z;  // boundary.f90:454 overwrites the ghost
+    }
+  } else {
+    cu3[T0] = cu3[Tm];
+    cu3[T1] = cu3[T0];
+    if (mean) {
+      cu3[T0].x = cu3[Tm].x + a.dy[ny] * a.w_hi_c1_transient;
+      cu3[T1] = C{cu3[T0].x + a.dy[ny] * a.w_hi_c1_transient, cu3[T0].y};
+    }
+  }
+  if (a.v_hi == 0) {
+    if (mean) {
+      C t = cu2[T0];
+      cu2[T1] = C{2.0 * a.v_hi_c1 - t.x, -t.y};
+    }
+  } else {
+    cu2[T0] = cu2[Tm];
+    cu2[T1] = cu2[T0];
+    if (mean) cu2[T1].x = cu2[T0].x + a.dy[ny] * a.v_hi_c1;
+  }
+}
+
+}  // namespace dgpu

@@ -135,6 +135,28 @@ int dgpu_step(dgpu_handle* h, int nsteps, double* time, int32_t* time_step);
 /* Wait for all work queued on the handle's stream. */
 int dgpu_synchronize(dgpu_handle* h);
 
+/*
+ * Per-kernel-class timing (no counterpart in the reference, whose only timer
+ * is the wall clock of control.f90:16-76).  While enabled, every kernel
+ * launch is bracketed by HIP events on the handle's stream; dgpu_timing_read
+ * returns, per class, the summed kernel time (ms), the summed algorithmic HBM
+ * bytes of those launches and the launch count.  Enabling resets the sums.
+ */
+#define DGPU_K_XFWD 0     /* x-direction forward transform pass */
+#define DGPU_K_XINV 1     /* x-direction inverse transform pass */
+#define DGPU_K_ZFWD 2     /* z-direction forward transform pass */
+#define DGPU_K_ZINV 3     /* z-direction inverse transform pass */
+#define DGPU_K_XPROD 4    /* nonlinear products + x transform */
+#define DGPU_K_RHS 5      /* z transform of products + explicit RK terms */
+#define DGPU_K_THOMAS 6   /* Crank-Nicolson tridiagonal solve */
+#define DGPU_K_REMDIV 7   /* pressure Poisson solve + projection */
+#define DGPU_K_BCPOST 8   /* wall boundary values */
+#define DGPU_NCLASS 9
+int dgpu_timing_enable(dgpu_handle* h, int enable);
+int dgpu_timing_read(dgpu_handle* h, double* ms /*[DGPU_NCLASS]*/, double* bytes /*[DGPU_NCLASS]*/,
+                     int64_t* count /*[DGPU_NCLASS]*/);
+const char* dgpu_kernel_class_name(int cls);
+
 /* Sizes of the reference-layout buffers (in doubles) for a field id. */
 int64_t dgpu_field_doubles(const dgpu_handle* h, int field);
 
