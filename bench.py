@@ -51,6 +51,29 @@ def cpu_baseline(p, gy, st, npts):
     }
 
 
+def copy_probe(nbytes):
+    """Device-to-device hipMemcpy of nbytes (a plain copy kernel of known
+    size): calibrates the HBM byte counters and gives the copy bandwidth."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    hip.hipMemset(a, 1, ctypes.c_size_t(nbytes))
+    hip.hipDeviceSynchronize()
+    best = None
+    for _ in range(3):
+        t0 = time.perf_counter()
+        assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0  # DeviceToDevice
+        hip.hipDeviceSynchronize()
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    hip.hipFree(a)
+    hip.hipFree(b)
+    return {"bytes": nbytes, "seconds": best, "GBps_read_plus_write": 2 * nbytes / best / 1e9}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -60,6 +83,8 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--timing-steps", type=int, default=2)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="also run a device-to-device copy of this size first")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU "
@@ -68,6 +93,10 @@ def main():
 
     from diablo3_test_amd import workloads
     from diablo3_test_amd.binding import ChannelSolver
+
+    probe = copy_probe(args.copy_probe_mib << 20) if args.copy_probe_mib else None
+    if probe:
+        log(f"bench: copy probe {probe}")
 
     p, gy, ic, seed, amp = workloads.workload(args.workload)
     npts = p["nx"] * p["ny"] * p["nz"]
@@ -147,6 +176,8 @@ def main():
         "algorithmic_GB_per_step": step_bytes / 1e9,
         "kernels": kernels,
     }
+    if probe:
+        line["copy_probe"] = probe
     g.close()
     if not args.no_cpu_baseline:
         log("bench: timing the CPU oracle on one step of the same state")

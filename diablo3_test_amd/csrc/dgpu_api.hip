@@ -6,6 +6,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <new>
@@ -50,14 +51,17 @@ struct DevBuf {
   }
 };
 
-// Rows per segment of the CN solver: a multiple of 8, at most 56 (the rows
-// live in registers), with at most 8 segments (512 threads per block).
+// Rows per segment of the CN solver: a multiple of 8 (the rows live in
+// registers), with at most 16 segments (1024 threads per block).  The target
+// length trades registers per thread against segments per column.
+int seg_target = 24;
 int seg_len_for(int n) {
-  int S = (n + 47) / 48;
-  if (S > 8) S = (n + 55) / 56;
-  if (S > 8) return -1;
+  int S = (n + seg_target - 1) / seg_target;
+  if (S > 16) S = 16;
   int L = (n + S - 1) / S;
-  return (L + 7) / 8 * 8;
+  L = (L + 7) / 8 * 8;
+  if (L > 56 || (n + L - 1) / L > 16) return -1;
+  return L;
 }
 
 }  // namespace
@@ -370,40 +374,44 @@ void collect_timing(dgpu_handle* h) {
 }
 
 // ------------------------------------------------------- launch helpers ----
-// Transform groups per workgroup for the x passes: as many as fit 256
-// threads, the jobs of one tile and the LDS limit (transform buffers plus
-// `nout` staging tiles).
-int x_ngroups(const dgpu_handle* h, int jobs, int nout) {
+// Transform groups per workgroup for the x passes: a divisor of `jobs` (the
+// transforms one KT-row tile needs), at most 1024 threads, within the LDS
+// limit.  All of a tile's transforms in one round when they fit.
+int x_ngroups(const dgpu_handle* h, int jobs, int max_threads = 512) {
   const Geo& g = h->g;
-  int G = g.nx / 8;
-  int ng = 256 / G;
-  if (ng < 1) ng = 1;
-  if (ng > jobs) ng = jobs;
-  auto lds = [&](int n) { return (size_t)(n * g.nx + nout * g.nq * KT) * sizeof(double2); };
-  while (ng > 1 && lds(ng) > h->lds_limit) --ng;
-  if (lds(ng) > h->lds_limit) throw ApiError{DGPU_EUNSUP, "Nx too large for the x-transform pass"};
+  const int G = g.nx / 8;
+  int ng = jobs;
+  auto lds = [&](int n) { return (size_t)n * fft_lds_len(g.nx) * sizeof(double2); };
+  while (ng > 1 && (ng * G > max_threads || lds(ng) > h->lds_limit)) ng /= 2;
+  if (ng * G > max_threads || lds(ng) > h->lds_limit)
+    throw ApiError{DGPU_EUNSUP, "Nx too large for the x-transform pass"};
   return ng;
 }
 
+size_t x_lds(const dgpu_handle* h, int ng) { return (size_t)ng * fft_lds_len(h->g.nx) * sizeof(double2); }
+
 int z_ngroups(const dgpu_handle* h) {
   const Geo& g = h->g;
-  int G = g.nz / 8;
+  const int G = g.nz / 8;
+  if (G > 512) throw ApiError{DGPU_EUNSUP, "Nz too large for the z-transform pass"};
   int ng = 256 / G;
   if (ng < 1) ng = 1;
-  while (ng > 1 && (size_t)ng * g.nz * sizeof(double2) > h->lds_limit) --ng;
-  if ((size_t)ng * g.nz * sizeof(double2) > h->lds_limit)
-    throw ApiError{DGPU_EUNSUP, "Nz too large for the z-transform pass"};
+  auto lds = [&](int n) { return (size_t)n * fft_lds_len(g.nz) * sizeof(double2); };
+  while (ng > 1 && lds(ng) > h->lds_limit) --ng;
+  if (lds(ng) > h->lds_limit) throw ApiError{DGPU_EUNSUP, "Nz too large for the z-transform pass"};
   return ng;
 }
+
+size_t z_lds(const dgpu_handle* h, int ng) { return (size_t)ng * fft_lds_len(h->g.nz) * sizeof(double2); }
 
 void launch_xfwd(dgpu_handle* h, const double* u, double2* T, int j0, int j1) {
   if (j1 < j0) return;
   const Geo& g = h->g;
-  const int ng = x_ngroups(h, KT / 2, 1);
-  const size_t lds = (size_t)(ng * g.nx + g.nq * KT) * sizeof(double2);
+  const int ng = x_ngroups(h, KT / 2);
+  const size_t lds = x_lds(h, ng);
   dim3 grid(g.nz / KT, j1 - j0 + 1), block(ng * (g.nx / 8));
   Timed tm(h, DGPU_K_XFWD, (8.0 * g.pp + 16.0 * g.tp) * (j1 - j0 + 1));
-  hipLaunchKernelGGL(k_xfwd<double>, grid, block, lds, h->stream, u,
+  hipLaunchKernelGGL((g.nx <= 512 ? k_xfwd<double, true> : k_xfwd<double, false>), grid, block, lds, h->stream, u,
                      reinterpret_cast<cpx<double>*>(T), g,
                      reinterpret_cast<const cpx<double>*>(h->twx.p), j0);
   HIPCHECK(hipGetLastError());
@@ -412,11 +420,11 @@ void launch_xfwd(dgpu_handle* h, const double* u, double2* T, int j0, int j1) {
 void launch_xinv(dgpu_handle* h, const double2* T, double* u, int j0, int j1) {
   if (j1 < j0) return;
   const Geo& g = h->g;
-  const int ng = x_ngroups(h, KT / 2, 1);
-  const size_t lds = (size_t)(ng * g.nx + g.nq * KT) * sizeof(double2);
+  const int ng = x_ngroups(h, KT / 2);
+  const size_t lds = x_lds(h, ng);
   dim3 grid(g.nz / KT, j1 - j0 + 1), block(ng * (g.nx / 8));
   Timed tm(h, DGPU_K_XINV, (8.0 * g.pp + 16.0 * g.tp) * (j1 - j0 + 1));
-  hipLaunchKernelGGL(k_xinv<double>, grid, block, lds, h->stream,
+  hipLaunchKernelGGL((g.nx <= 512 ? k_xinv<double, true> : k_xinv<double, false>), grid, block, lds, h->stream,
                      reinterpret_cast<const cpx<double>*>(T), u, g,
                      reinterpret_cast<const cpx<double>*>(h->twx.p), j0);
   HIPCHECK(hipGetLastError());
@@ -426,10 +434,10 @@ void launch_zfwd(dgpu_handle* h, const double2* T, double2* C, int j0, int j1) {
   if (j1 < j0) return;
   const Geo& g = h->g;
   const int ng = z_ngroups(h);
-  const size_t lds = (size_t)ng * g.nz * sizeof(double2);
+  const size_t lds = z_lds(h, ng);
   dim3 grid((g.nq + ng - 1) / ng, j1 - j0 + 1), block(ng * (g.nz / 8));
   Timed tm(h, DGPU_K_ZFWD, (16.0 * g.tp + 16.0 * g.sp) * (j1 - j0 + 1));
-  hipLaunchKernelGGL(k_zfwd<double>, grid, block, lds, h->stream,
+  hipLaunchKernelGGL((g.nz <= 512 ? k_zfwd<double, true> : k_zfwd<double, false>), grid, block, lds, h->stream,
                      reinterpret_cast<const cpx<double>*>(T), reinterpret_cast<cpx<double>*>(C), g,
                      reinterpret_cast<const cpx<double>*>(h->twz.p), j0);
   HIPCHECK(hipGetLastError());
@@ -439,10 +447,10 @@ void launch_zinv(dgpu_handle* h, const double2* C, double2* T, int j0, int j1) {
   if (j1 < j0) return;
   const Geo& g = h->g;
   const int ng = z_ngroups(h);
-  const size_t lds = (size_t)ng * g.nz * sizeof(double2);
+  const size_t lds = z_lds(h, ng);
   dim3 grid((g.nq + ng - 1) / ng, j1 - j0 + 1), block(ng * (g.nz / 8));
   Timed tm(h, DGPU_K_ZINV, (16.0 * g.tp + 16.0 * g.sp) * (j1 - j0 + 1));
-  hipLaunchKernelGGL(k_zinv<double>, grid, block, lds, h->stream,
+  hipLaunchKernelGGL((g.nz <= 512 ? k_zinv<double, true> : k_zinv<double, false>), grid, block, lds, h->stream,
                      reinterpret_cast<const cpx<double>*>(C), reinterpret_cast<cpx<double>*>(T), g,
                      reinterpret_cast<const cpx<double>*>(h->twz.p), j0);
   HIPCHECK(hipGetLastError());
@@ -561,46 +569,35 @@ void rk_substep(dgpu_handle* h, int rk_step, double time, int time_step) {
   for (int n = 0; n < nth; ++n) to_physical(h, h->cth[n].p, h->th[n].p, h->tpool[3 + n].p);
 
   // 2. products, x-transformed (channel.f90:299-459, 557-608)
-  ProdArgs pa;
-  pa.u1 = h->u[0].p;
-  pa.u2 = h->u[1].p;
-  pa.u3 = h->u[2].p;
-  pa.th = nullptr;
-  pa.dy = h->d_dy.p;
-  pa.dyf = h->d_dyf.p;
-  pa.kx = h->d_kx.p;
-  pa.js = h->jstart;
-  pa.je = h->jend;
-  pa.jst = 1 << 30;
-  pa.jet = -1;
   {
-    const int ngx = x_ngroups(h, KT, 2);
-    const size_t lds = (size_t)(ngx * g.nx + 2 * g.nq * KT) * sizeof(double2);
-    dim3 grid(g.nz / KT, Ny), block(ngx * (g.nx / 8));
-    auto slotp = [&](int s) { return reinterpret_cast<cpx<double>*>(h->tpool[s].p); };
-    const int outs[4][2] = {{0, -1}, {1, 2}, {3, 5}, {4, -1}};
-    for (int pair = 0; pair < 4; ++pair) {
-      cpx<double>* o0 = slotp(outs[pair][0]);
-      cpx<double>* o1 = outs[pair][1] >= 0 ? slotp(outs[pair][1]) : nullptr;
-      const int nread[4] = {2, 3, 2, 2};
-      Timed tm(h, DGPU_K_XPROD, (8.0 * g.pp * nread[pair] + 16.0 * g.tp * (o1 ? 2 : 1)) * Ny);
-      hipLaunchKernelGGL(k_xprod<double>, grid, block, lds, h->stream, pa, g, pair, o0, o1,
-                         reinterpret_cast<const cpx<double>*>(h->twx.p), 1);
-      HIPCHECK(hipGetLastError());
-    }
+    ProdAllArgs pa;
+    std::memset(&pa, 0, sizeof(pa));
+    pa.base.u1 = h->u[0].p;
+    pa.base.u2 = h->u[1].p;
+    pa.base.u3 = h->u[2].p;
+    pa.base.th = nullptr;
+    pa.base.dy = h->d_dy.p;
+    pa.base.dyf = h->d_dyf.p;
+    pa.base.kx = h->d_kx.p;
+    pa.base.js = h->jstart;
+    pa.base.je = h->jend;
+    pa.base.jst = 1 << 30;
+    pa.base.jet = -1;
+    pa.nth = nth;
     for (int n = 0; n < nth; ++n) {
-      ProdArgs pt = pa;
-      pt.th = h->th[n].p;
-      pt.jst = h->jstart_th[n];
-      pt.jet = h->jend_th[n];
-      for (int pair = 4; pair < 6; ++pair) {
-        cpx<double>* o0 = slotp(6 + 2 * n + (pair - 4));
-        Timed tm(h, DGPU_K_XPROD, (8.0 * g.pp * (pair == 4 ? 3 : 2) + 16.0 * g.tp) * Ny);
-        hipLaunchKernelGGL(k_xprod<double>, grid, block, lds, h->stream, pt, g, pair, o0,
-                           (cpx<double>*)nullptr, reinterpret_cast<const cpx<double>*>(h->twx.p), 1);
-        HIPCHECK(hipGetLastError());
-      }
+      pa.th[n] = h->th[n].p;
+      pa.jst[n] = h->jstart_th[n];
+      pa.jet[n] = h->jend_th[n];
     }
+    for (int s = 0; s < nslot; ++s) pa.out[s] = h->tpool[s].p;
+    const int ngx = x_ngroups(h, KT, 512);
+    const size_t lds = x_lds(h, ngx);
+    dim3 grid(g.nz / KT, Ny), block(ngx * (g.nx / 8));
+    // reads u1, u2, u3, th once (neighbour planes from cache); writes the product x-spectra
+    Timed tm(h, DGPU_K_XPROD, (8.0 * g.pp * (3 + nth) + 16.0 * g.tp * nslot) * Ny);
+    hipLaunchKernelGGL((g.nx <= 512 ? k_xprod_all<double, true> : k_xprod_all<double, false>), grid, block, lds, h->stream, pa, g,
+                       reinterpret_cast<const cpx<double>*>(h->twx.p), 1);
+    HIPCHECK(hipGetLastError());
   }
 
   // 3. explicit spectral terms, cf, cr, backward z transform of cr
@@ -649,18 +646,18 @@ void rk_substep(dgpu_handle* h, int rk_step, double time, int time_step) {
     a.js = h->jstart;
     a.je = h->jend;
     const int G = g.nz / 8;
-    int ng = 1024 / G;
+    int ng = 512 / G;
     if (ng < 1) ng = 1;
     if (ng > nslot) ng = nslot;
     const int rounds = (nslot + ng - 1) / ng;
-    const size_t lds = (size_t)rounds * ng * g.nz * sizeof(double2);
+    const size_t lds = (size_t)rounds * ng * fft_lds_len(g.nz) * sizeof(double2);
     if (lds > h->lds_limit) throw ApiError{DGPU_EUNSUP, "Nz too large for the spectral RHS pass"};
     dim3 grid(g.nq, Ny), block(ng * G);
     // x-spectra of the products in, cr x-spectra out; cu1..3, cp, cth, cf1..3,
     // cfth read and cf1..3, cfth written (neighbour planes come from cache)
     Timed tm(h, DGPU_K_RHS,
              (16.0 * g.tp * (nslot + ncr) + 16.0 * g.sp * (4 + 2 * nth + 3 + nth + 3 + nth)) * Ny);
-    hipLaunchKernelGGL(k_rhs, grid, block, lds, h->stream, a, g, h->twz.p, 1);
+    hipLaunchKernelGGL((g.nz <= 512 ? k_rhs<true> : k_rhs<false>), grid, block, lds, h->stream, a, g, h->twz.p, 1);
     HIPCHECK(hipGetLastError());
   }
 
@@ -847,9 +844,13 @@ void create_impl(dgpu_handle* h, const double* gy) {
   HIPCHECK(hipGetDeviceProperties(&prop, dev));
   h->lds_limit = prop.sharedMemPerBlock;
   if (h->lds_limit > 65536) {
-    const void* kernels[] = {(const void*)k_rhs, (const void*)k_xfwd<double>,
-                             (const void*)k_xinv<double>, (const void*)k_zfwd<double>,
-                             (const void*)k_zinv<double>, (const void*)k_xprod<double>};
+    const void* kernels[] = {
+        (const void*)k_rhs<true>,           (const void*)k_rhs<false>,
+        (const void*)k_xfwd<double, true>,  (const void*)k_xfwd<double, false>,
+        (const void*)k_xinv<double, true>,  (const void*)k_xinv<double, false>,
+        (const void*)k_zfwd<double, true>,  (const void*)k_zfwd<double, false>,
+        (const void*)k_zinv<double, true>,  (const void*)k_zinv<double, false>,
+        (const void*)k_xprod_all<double, true>, (const void*)k_xprod_all<double, false>};
     for (const void* k : kernels)
       HIPCHECK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds_limit));
   }
@@ -899,7 +900,8 @@ void create_impl(dgpu_handle* h, const double* gy) {
   upload(h->d_dy, h->dy);
   upload(h->d_dyf, h->dyf);
 
-  // CN factor tables
+  // CN factor tables (DGPU_SEG_TARGET overrides the target segment length)
+  if (const char* e = std::getenv("DGPU_SEG_TARGET")) seg_target = std::atoi(e);
   const int n = g.ny2;
   h->seg_L = seg_len_for(n);
   if (h->seg_L < 0) throw ApiError{DGPU_EUNSUP, "ny too large for the segmented solver"};

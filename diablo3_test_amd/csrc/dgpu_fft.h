@@ -47,6 +47,12 @@ __device__ __forceinline__ cpx<R> ik(R k, cpx<R> a) {
   return {-k * a.y, k * a.x};
 }
 
+// Store of a result that the next kernel, not this one, reads back.
+template <typename R>
+__device__ __forceinline__ void st_out(cpx<R>* p, cpx<R> v) {
+  *p = v;
+}
+
 // Small DFTs in registers.  S = -1 forward, +1 backward.
 template <typename R, int S>
 __device__ __forceinline__ void dft2(cpx<R>* v) {
@@ -86,10 +92,32 @@ __device__ __forceinline__ void dft8(cpx<R>* v) {
   v[7] = e3 - t3;
 }
 
+// LDS index of element i of a transform buffer: one pad slot after every 8
+// elements, so that the stride-8 stores of a radix-8 pass (and the strided
+// loads that follow) fall in distinct LDS banks.  A buffer of N elements
+// occupies fft_lds_len(N) slots.
+__device__ __forceinline__ int pidx(int i) { return i + (i >> 3); }
+__host__ __device__ constexpr int fft_lds_len(int n) { return n + (n >> 3); }
+
+// Synchronisation between the passes of one transform.  When the G = N/8
+// lanes of a transform lie inside one wavefront (G <= 64, groups aligned to
+// G), LDS operations of the wavefront execute in program order, so a
+// wavefront-scope fence plus a scheduling barrier is enough; larger
+// transforms span wavefronts and need the workgroup barrier.
+template <bool WAVE>
+__device__ __forceinline__ void fft_sync() {
+  if constexpr (WAVE) {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  } else {
+    __syncthreads();
+  }
+}
+
 // One Stockham pass of radix 2^LOGR over a length-N transform in LDS, after
 // the radices already applied multiply to Ns = 2^logNs.  Each of the G = N/8
 // threads owns 8 complex values (8 >> LOGR butterflies).
-template <typename R, int S, int LOGR>
+template <typename R, int S, int LOGR, bool WAVE>
 __device__ __forceinline__ void stockham_pass(cpx<R>* buf, int N, int logNs,
                                               const cpx<R>* __restrict__ tw, int gt) {
   constexpr int RDX = 1 << LOGR, NB = 8 >> LOGR;
@@ -101,8 +129,8 @@ __device__ __forceinline__ void stockham_pass(cpx<R>* buf, int N, int logNs,
 #pragma unroll
   for (int bb = 0; bb < NB; ++bb)
 #pragma unroll
-    for (int r = 0; r < RDX; ++r) v[bb * RDX + r] = buf[gt + bb * G + r * stride];
-  __syncthreads();
+    for (int r = 0; r < RDX; ++r) v[bb * RDX + r] = buf[pidx(gt + bb * G + r * stride)];
+  fft_sync<WAVE>();
 #pragma unroll
   for (int bb = 0; bb < NB; ++bb) {
     const int j = gt + bb * G;
@@ -123,33 +151,52 @@ __device__ __forceinline__ void stockham_pass(cpx<R>* buf, int N, int logNs,
       dft2<R, S>(v + bb * 2);
     const int idxD = ((j >> logNs) << (logNs + LOGR)) + k;
 #pragma unroll
-    for (int r = 0; r < RDX; ++r) buf[idxD + r * Ns] = v[bb * RDX + r];
+    for (int r = 0; r < RDX; ++r) buf[pidx(idxD + r * Ns)] = v[bb * RDX + r];
   }
-  __syncthreads();
+  fft_sync<WAVE>();
 }
 
-// In-place complex FFT of length N (power of two, 8 <= N <= 4096) held in LDS
-// at `buf`, executed by the G = N/8 threads of one group (gt = 0..G-1).
-// tw[t] = exp(-2 pi i t / N), t = 0..N-1.  Every thread of the workgroup must
-// call this the same number of times with the same N: it synchronises the
-// whole workgroup (one barrier on entry, two per pass).
-template <typename R, int S>
-__device__ __forceinline__ void lds_fft(cpx<R>* buf, int N, int logN, const cpx<R>* __restrict__ tw, int gt) {
-  __syncthreads();
-  int logNs = 0;
+// Complex FFT of length N (power of two, 8 <= N <= 4096) by the G = N/8
+// threads of one group (gt = 0..G-1).  Input in registers: v[r] = x[gt + G r],
+// the operands of the first radix-8 butterfly, which therefore runs without
+// touching LDS.  Output in LDS at `buf` in natural order (padded, see pidx).
+// tw[t] = exp(-2 pi i t / N).  The caller must not touch `buf` of this group
+// between its last use and this call returning, except through the returned
+// data.  Every thread of the workgroup calls this the same number of times
+// with the same N when WAVE is false (workgroup barriers inside).
+template <typename R, int S, bool WAVE>
+__device__ __forceinline__ void fft_regs(cpx<R> (&v)[8], cpx<R>* buf, int N, int logN,
+                                         const cpx<R>* __restrict__ tw, int gt) {
+  dft8<R, S>(v);  // Ns = 1: no twiddles; outputs y[8 gt + r]
+  fft_sync<WAVE>();  // the previous use of buf is finished
+#pragma unroll
+  for (int r = 0; r < 8; ++r) buf[pidx(gt * 8 + r)] = v[r];
+  fft_sync<WAVE>();
+  int logNs = 3;
   while (logNs < logN) {
     const int rem = logN - logNs;
     if (rem >= 3) {
-      stockham_pass<R, S, 3>(buf, N, logNs, tw, gt);
+      stockham_pass<R, S, 3, WAVE>(buf, N, logNs, tw, gt);
       logNs += 3;
     } else if (rem == 2) {
-      stockham_pass<R, S, 2>(buf, N, logNs, tw, gt);
+      stockham_pass<R, S, 2, WAVE>(buf, N, logNs, tw, gt);
       logNs += 2;
     } else {
-      stockham_pass<R, S, 1>(buf, N, logNs, tw, gt);
+      stockham_pass<R, S, 1, WAVE>(buf, N, logNs, tw, gt);
       logNs += 1;
     }
   }
+}
+
+// fft_regs with the wave/workgroup choice made from N: transforms of at most
+// 512 points (G <= 64 lanes) synchronise within their wavefront.
+template <typename R, int S>
+__device__ __forceinline__ void fft_regs_auto(cpx<R> (&v)[8], cpx<R>* buf, int N, int logN,
+                                              const cpx<R>* __restrict__ tw, int gt) {
+  if (N <= 512)
+    fft_regs<R, S, true>(v, buf, N, logN, tw, gt);
+  else
+    fft_regs<R, S, false>(v, buf, N, logN, tw, gt);
 }
 
 }  // namespace dgpu

@@ -39,160 +39,160 @@ __device__ __forceinline__ int kz_nat(const Geo& g, int kk) {
 template <typename R>
 __device__ __forceinline__ void split_pair(const cpx<R>* f, int n, int q, R scale, cpx<R>& A,
                                            cpx<R>& B) {
-  cpx<R> z = f[q], zc = conjg(f[(n - q) & (n - 1)]);
+  cpx<R> z = f[pidx(q)], zc = conjg(f[pidx((n - q) & (n - 1))]);
   R h = (R)0.5 * scale;
   A = {h * (z.x + zc.x), h * (z.y + zc.y)};
   B = {h * (z.y - zc.y), -h * (z.x - zc.x)};
 }
 
 // Forward x transform of one field on planes j0.. : the r2c of fft.f90:258
-// with the 1/Nx scale and dealias of :259-268.  Two rows share one complex FFT.
-// grid (Nz/KT, nplanes), block 256, dynamic LDS (ngroups*Nx + nq*KT) cpx.
-template <typename R>
-__global__ void __launch_bounds__(256) k_xfwd(const R* __restrict__ u, cpx<R>* __restrict__ T,
-                                              Geo g, const cpx<R>* __restrict__ tw, int j0) {
+// with the 1/Nx scale and dealias of :259-268.  Two rows share one complex
+// FFT; a workgroup owns KT rows of one plane and writes their kx <= Nkx
+// spectra straight from the transform buffers as KT-long runs of T[j][q][k].
+// grid (Nz/KT, nplanes), block ngroups*Nx/8 (ngroups divides KT/2),
+// LDS ngroups*fft_lds_len(Nx) cpx.
+template <typename R, bool WAVE>
+__global__ void __launch_bounds__(512) k_xfwd(const R* __restrict__ u, cpx<R>* __restrict__ T,
+                                               Geo g, const cpx<R>* __restrict__ tw, int j0) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<R>* smem = reinterpret_cast<cpx<R>*>(smem_raw);
   const int G = g.nx >> 3;
+  const int L = fft_lds_len(g.nx);
   const int ngroups = blockDim.x / G;
   const int grp = threadIdx.x / G, gt = threadIdx.x % G;
-  cpx<R>* fbuf = smem + grp * g.nx;
-  cpx<R>* stage = smem + ngroups * g.nx;
+  cpx<R>* fbuf = smem + grp * L;
   const int j = j0 + blockIdx.y, k0 = blockIdx.x * KT;
   const R* plane = u + (long long)j * g.pp;
+  cpx<R>* out = T + (long long)j * g.tp + k0;
   const R inv = (R)1 / (R)g.nx;
+  const int nr = 2 * ngroups;  // rows per round
   for (int m0 = 0; m0 < KT / 2; m0 += ngroups) {
     const int m = m0 + grp;
-    const bool act = grp < ngroups && m < KT / 2;
-    if (act) {
-      const R* ra = plane + (long long)(k0 + 2 * m) * g.nx;
-      const R* rb = ra + g.nx;
-      for (int i = gt; i < g.nx; i += G) fbuf[i] = {ra[i], rb[i]};
-    }
-    lds_fft<R, -1>(fbuf, g.nx, g.lognx, tw, gt);
-    if (act) {
-      for (int q = gt; q < g.nq; q += G) {
-        cpx<R> A, B;
-        split_pair(fbuf, g.nx, q, inv, A, B);
-        stage[q * KT + 2 * m] = A;
-        stage[q * KT + 2 * m + 1] = B;
-      }
+    const R* ra = plane + (long long)(k0 + 2 * m) * g.nx;
+    const R* rb = ra + g.nx;
+    cpx<R> v[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) v[r] = {ra[gt + G * r], rb[gt + G * r]};
+    fft_regs<R, -1, WAVE>(v, fbuf, g.nx, g.lognx, tw, gt);
+    __syncthreads();  // every group's spectra are read below
+    for (int idx = threadIdx.x; idx < g.nq * nr; idx += blockDim.x) {
+      const int q = idx / nr, kk = idx - q * nr;
+      cpx<R> A, B;
+      split_pair(smem + (kk >> 1) * L, g.nx, q, inv, A, B);
+      st_out(out + (long long)q * g.nz + 2 * m0 + kk, (kk & 1) ? B : A);
     }
     __syncthreads();
-  }
-  cpx<R>* out = T + (long long)j * g.tp + k0;
-  for (int idx = threadIdx.x; idx < g.nq * KT; idx += blockDim.x) {
-    const int q = idx / KT, kk = idx % KT;
-    out[(long long)q * g.nz + kk] = stage[idx];
   }
 }
 
 // Inverse x transform: dealias (fft.f90:372-378) and c2r (:380), unnormalised.
-// FFTW's c2r reads only the real part of the kx = 0 input.
-template <typename R>
-__global__ void __launch_bounds__(256) k_xinv(const cpx<R>* __restrict__ T, R* __restrict__ u,
-                                              Geo g, const cpx<R>* __restrict__ tw, int j0) {
+// FFTW's c2r reads only the real part of the kx = 0 input.  Rows 2m and 2m+1
+// are synthesised together: Z = A + iB with A, B extended Hermitian.
+template <typename R, bool WAVE>
+__global__ void __launch_bounds__(512) k_xinv(const cpx<R>* __restrict__ T, R* __restrict__ u,
+                                               Geo g, const cpx<R>* __restrict__ tw, int j0) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<R>* smem = reinterpret_cast<cpx<R>*>(smem_raw);
   const int G = g.nx >> 3;
+  const int L = fft_lds_len(g.nx);
   const int ngroups = blockDim.x / G;
   const int grp = threadIdx.x / G, gt = threadIdx.x % G;
-  cpx<R>* fbuf = smem + grp * g.nx;
-  cpx<R>* stage = smem + ngroups * g.nx;
+  cpx<R>* fbuf = smem + grp * L;
   const int j = j0 + blockIdx.y, k0 = blockIdx.x * KT;
   const cpx<R>* in = T + (long long)j * g.tp + k0;
-  for (int idx = threadIdx.x; idx < g.nq * KT; idx += blockDim.x) {
-    const int q = idx / KT, kk = idx % KT;
-    stage[idx] = in[(long long)q * g.nz + kk];
-  }
   R* plane = u + (long long)j * g.pp;
   for (int m0 = 0; m0 < KT / 2; m0 += ngroups) {
-    const int m = m0 + grp;
-    const bool act = grp < ngroups && m < KT / 2;
-    __syncthreads();  // stage complete / previous round's reads of fbuf done
-    if (act) {
-      for (int qq = gt; qq < g.nx; qq += G) {
-        cpx<R> z = {(R)0, (R)0};
-        if (qq == 0) {
-          z = {stage[2 * m].x, stage[2 * m + 1].x};
-        } else if (qq <= g.nkx) {
-          cpx<R> A = stage[qq * KT + 2 * m], B = stage[qq * KT + 2 * m + 1];
-          z = {A.x - B.y, A.y + B.x};
-        } else if (qq >= g.nx - g.nkx) {
-          const int q = g.nx - qq;
-          cpx<R> A = stage[q * KT + 2 * m], B = stage[q * KT + 2 * m + 1];
-          z = {A.x + B.y, B.x - A.y};
-        }
-        fbuf[qq] = z;
+    __syncthreads();  // previous round's reads of the buffers are done
+    for (int qq = g.nkx + 1 + gt; qq < g.nx - g.nkx; qq += G) fbuf[pidx(qq)] = {(R)0, (R)0};
+    for (int idx = threadIdx.x; idx < g.nq * ngroups; idx += blockDim.x) {
+      const int q = idx / ngroups, mm = idx - q * ngroups;
+      const cpx<R>* src = in + (long long)q * g.nz + 2 * (m0 + mm);
+      const cpx<R> A = src[0], B = src[1];
+      cpx<R>* f = smem + mm * L;
+      if (q == 0) {
+        f[pidx(0)] = {A.x, B.x};
+      } else {
+        f[pidx(q)] = {A.x - B.y, A.y + B.x};
+        f[pidx(g.nx - q)] = {A.x + B.y, B.x - A.y};
       }
     }
-    lds_fft<R, 1>(fbuf, g.nx, g.lognx, tw, gt);
-    if (act) {
-      R* ra = plane + (long long)(k0 + 2 * m) * g.nx;
-      R* rb = ra + g.nx;
-      for (int i = gt; i < g.nx; i += G) {
-        cpx<R> z = fbuf[i];
-        ra[i] = z.x;
-        rb[i] = z.y;
-      }
+    __syncthreads();  // the spectra of every group are in place
+    cpx<R> v[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) v[r] = fbuf[pidx(gt + G * r)];
+    fft_regs<R, 1, WAVE>(v, fbuf, g.nx, g.lognx, tw, gt);
+    R* ra = plane + (long long)(k0 + 2 * (m0 + grp)) * g.nx;
+    R* rb = ra + g.nx;
+    for (int i = gt; i < g.nx; i += G) {
+      const cpx<R> z = fbuf[pidx(i)];
+      ra[i] = z.x;
+      rb[i] = z.y;
     }
   }
 }
 
 // ---------------------------------------------------------------- z pass --
 // Forward z transform + 1/Nz + pack (fft.f90:298-313).  One column (j, q) per
-// group; grid (ceil(nq/ngroups), nplanes), block 256.
-template <typename R>
-__global__ void __launch_bounds__(256) k_zfwd(const cpx<R>* __restrict__ T, cpx<R>* __restrict__ C,
-                                              Geo g, const cpx<R>* __restrict__ tw, int j0) {
+// group; grid (ceil(nq/ngroups), nplanes), block ngroups*Nz/8.
+template <typename R, bool WAVE>
+__global__ void __launch_bounds__(512) k_zfwd(const cpx<R>* __restrict__ T, cpx<R>* __restrict__ C,
+                                               Geo g, const cpx<R>* __restrict__ tw, int j0) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<R>* smem = reinterpret_cast<cpx<R>*>(smem_raw);
   const int G = g.nz >> 3;
   const int ngroups = blockDim.x / G;
   const int grp = threadIdx.x / G, gt = threadIdx.x % G;
-  cpx<R>* fbuf = smem + grp * g.nz;
+  cpx<R>* fbuf = smem + grp * fft_lds_len(g.nz);
   const int j = j0 + blockIdx.y, q = blockIdx.x * ngroups + grp;
-  const bool act = grp < ngroups && q < g.nq;
+  const bool act = q < g.nq;
+  cpx<R> v[8];
   if (act) {
     const cpx<R>* col = T + (long long)j * g.tp + (long long)q * g.nz;
-    for (int k = gt; k < g.nz; k += G) fbuf[k] = col[k];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) v[r] = col[gt + G * r];
+  } else {
+#pragma unroll
+    for (int r = 0; r < 8; ++r) v[r] = {(R)0, (R)0};
   }
-  lds_fft<R, -1>(fbuf, g.nz, g.lognz, tw, gt);
+  fft_regs<R, -1, WAVE>(v, fbuf, g.nz, g.lognz, tw, gt);
   if (act) {
     const R inv = (R)1 / (R)g.nz;
     cpx<R>* out = C + (long long)j * g.sp + (long long)q * g.nkk;
-    for (int kk = gt; kk < g.nkk; kk += G) out[kk] = inv * fbuf[kz_nat(g, kk)];
+    for (int kk = gt; kk < g.nkk; kk += G) out[kk] = inv * fbuf[pidx(kz_nat(g, kk))];
   }
 }
 
 // Unpack + zero (fft.f90:330-341) + backward z transform (:345), unnormalised.
-template <typename R>
-__global__ void __launch_bounds__(256) k_zinv(const cpx<R>* __restrict__ C, cpx<R>* __restrict__ T,
-                                              Geo g, const cpx<R>* __restrict__ tw, int j0) {
+template <typename R, bool WAVE>
+__global__ void __launch_bounds__(512) k_zinv(const cpx<R>* __restrict__ C, cpx<R>* __restrict__ T,
+                                               Geo g, const cpx<R>* __restrict__ tw, int j0) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<R>* smem = reinterpret_cast<cpx<R>*>(smem_raw);
   const int G = g.nz >> 3;
   const int ngroups = blockDim.x / G;
   const int grp = threadIdx.x / G, gt = threadIdx.x % G;
-  cpx<R>* fbuf = smem + grp * g.nz;
+  cpx<R>* fbuf = smem + grp * fft_lds_len(g.nz);
   const int j = j0 + blockIdx.y, q = blockIdx.x * ngroups + grp;
-  const bool act = grp < ngroups && q < g.nq;
-  if (act) {
-    const cpx<R>* col = C + (long long)j * g.sp + (long long)q * g.nkk;
-    const int off = g.nz - 2 * g.nkz - 1;
-    for (int k = gt; k < g.nz; k += G) {
-      cpx<R> z = {(R)0, (R)0};
+  const bool act = q < g.nq;
+  cpx<R> v[8];
+  const cpx<R>* col = C + (long long)j * g.sp + (long long)(act ? q : 0) * g.nkk;
+  const int off = g.nz - 2 * g.nkz - 1;
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const int k = gt + G * r;
+    cpx<R> z = {(R)0, (R)0};
+    if (act) {
       if (k <= g.nkz)
         z = col[k];
       else if (k >= g.nz - g.nkz)
         z = col[k - off];
-      fbuf[k] = z;
     }
+    v[r] = z;
   }
-  lds_fft<R, 1>(fbuf, g.nz, g.lognz, tw, gt);
+  fft_regs<R, 1, WAVE>(v, fbuf, g.nz, g.lognz, tw, gt);
   if (act) {
     cpx<R>* out = T + (long long)j * g.tp + (long long)q * g.nz;
-    for (int k = gt; k < g.nz; k += G) out[k] = fbuf[k];
+    for (int k = gt; k < g.nz; k += G) st_out(out + k, fbuf[pidx(k)]);
   }
 }
 
@@ -208,16 +208,30 @@ __global__ void __launch_bounds__(256) k_zinv(const cpx<R>* __restrict__ C, cpx<
 //   pair 3: (W1, V2)     -> X2  = ikx F(W1) + F(V2)              (W1 = interp(u1) u2)
 //   pair 4: (th u1, Vt)  -> Xt  = ikx F(th u1) + F(Vt)
 //   pair 5: (th u3, 0)   -> Xt3 = F(th u3)
+constexpr int MAXTH = 4;
+constexpr int MAXSLOT = 6 + 2 * MAXTH;
+
 struct ProdArgs {
   const double* u1;
   const double* u2;
   const double* u3;
-  const double* th;
+  const double* th;  // scalar of pairs 4 and 5 (set per scalar by the kernel)
   const double* dy;
   const double* dyf;
   const double* kx;  // kx[q]
   int js, je;        // jstart, jend
   int jst, jet;      // jstart_th(n), jend_th(n)
+};
+
+// All product pairs of one substep in one pass (the slot numbering of k_rhs):
+// pair 0 -> slot 0, pair 1 -> slots 1, 2, pair 2 -> slots 3, 5, pair 3 -> slot 4,
+// scalar n: pair 4 -> slot 6+2n, pair 5 -> slot 7+2n.
+struct ProdAllArgs {
+  ProdArgs base;
+  const double* th[MAXTH];
+  int jst[MAXTH], jet[MAXTH];
+  int nth;
+  double2* out[MAXSLOT];
 };
 
 template <typename R>
@@ -278,60 +292,165 @@ __device__ __forceinline__ void pair_ranges(const ProdArgs& a, const Geo& g, int
   }
 }
 
-// grid (Nz/KT, nplanes), block 256, LDS (ngroups*Nx + 2*nq*KT) cpx.
+// All product pairs for a KT-row tile of plane j in one workgroup.  Each
+// thread reads its slice of the input rows (planes j-1, j, j+1) once into
+// registers, in three phases -- u1 u2 (pairs 0, 3), u3 (pairs 1, 2), then
+// each scalar (pairs 4, 5) -- and every product is formed from registers, so
+// each input row crosses the memory system once per workgroup.
 template <typename R>
-__global__ void __launch_bounds__(256) k_xprod(ProdArgs a, Geo g, int pair, cpx<R>* __restrict__ O0,
-                                               cpx<R>* __restrict__ O1, const cpx<R>* __restrict__ tw,
-                                               int j0) {
+__device__ __forceinline__ void emit_pair(cpx<R>* smem, int L, const Geo& g, int ngroups, int pair,
+                                          R inv, const double* kx, cpx<R>* out0, cpx<R>* out1,
+                                          int m0) {
+  for (int idx = threadIdx.x; idx < g.nq * ngroups; idx += blockDim.x) {
+    const int q = idx / ngroups, mm = idx - q * ngroups;
+    cpx<R> A, B;
+    split_pair(smem + mm * L, g.nx, q, inv, A, B);
+    const cpx<R> ikA = ik((R)kx[q], A);
+    const long long o = (long long)q * g.nz + m0 + mm;
+    switch (pair) {
+      case 0: st_out(out0 + o, ikA + B); break;
+      case 1: st_out(out0 + o, A); st_out(out1 + o, ikA + B); break;
+      case 2: st_out(out0 + o, A); st_out(out1 + o, B); break;
+      case 3:
+      case 4: st_out(out0 + o, ikA + B); break;
+      default: st_out(out0 + o, A); break;
+    }
+  }
+}
+
+template <typename R, bool WAVE>
+__global__ void __launch_bounds__(512) k_xprod_all(ProdAllArgs a, Geo g,
+                                                    const cpx<R>* __restrict__ tw, int j0) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<R>* smem = reinterpret_cast<cpx<R>*>(smem_raw);
-  const int G = g.nx >> 3;
+  const int G = g.nx >> 3;  // 8 values of a row per thread
+  const int L = fft_lds_len(g.nx);
   const int ngroups = blockDim.x / G;
   const int grp = threadIdx.x / G, gt = threadIdx.x % G;
-  cpx<R>* fbuf = smem + grp * g.nx;
-  cpx<R>* st0 = smem + ngroups * g.nx;
-  cpx<R>* st1 = st0 + g.nq * KT;
+  cpx<R>* fbuf = smem + grp * L;
   const int j = j0 + blockIdx.y, k0 = blockIdx.x * KT;
-  bool inA, inB;
-  pair_ranges(a, g, pair, j, inA, inB);
   const R inv = (R)1 / (R)g.nx;
-  for (int m0 = 0; m0 < KT; m0 += ngroups) {
-    const int m = m0 + grp;
-    const bool act = grp < ngroups && m < KT;
-    if (act) {
-      const long long rowo = (long long)j * g.pp + (long long)(k0 + m) * g.nx;
-      for (int i = gt; i < g.nx; i += G) {
-        R pa, pb;
-        prod_pair<R>(a, g, pair, j, rowo + i, inA, inB, pa, pb);
-        fbuf[i] = {pa, pb};
-      }
-    }
-    lds_fft<R, -1>(fbuf, g.nx, g.lognx, tw, gt);
-    if (act) {
-      for (int q = gt; q < g.nq; q += G) {
-        cpx<R> A, B;
-        split_pair(fbuf, g.nx, q, inv, A, B);
-        const R kxq = (R)a.kx[q];
-        cpx<R> ikA = ik(kxq, A);
-        switch (pair) {
-          case 0: st0[q * KT + m] = ikA + B; break;
-          case 1: st0[q * KT + m] = A; st1[q * KT + m] = ikA + B; break;
-          case 2: st0[q * KT + m] = A; st1[q * KT + m] = B; break;
-          case 3:
-          case 4: st0[q * KT + m] = ikA + B; break;
-          default: st0[q * KT + m] = A; break;
-        }
-      }
-    }
+  const ProdArgs& b = a.base;
+  const double dyj = b.dy[j], dyfj = b.dyf[j], dyfm = b.dyf[j - 1];
+  const double c2dyf = 1.0 / (2.0 * dyfj);
+  const long long pp = g.pp;
+  auto outp = [&](int s) {
+    return reinterpret_cast<cpx<R>*>(a.out[s]) + (long long)j * g.tp + k0;
+  };
+  // transform one product pair held as this thread's 8 values, then make
+  // every group's spectra visible to the cross-group writes of emit_pair
+  auto fft_pair = [&](const double* va, const double* vb) {
+    cpx<R> v[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) v[r] = {(R)va[r], (R)vb[r]};
+    fft_regs<R, -1, WAVE>(v, fbuf, g.nx, g.lognx, tw, gt);
     __syncthreads();
+  };
+  bool r1, r2;
+  {
+    bool x;
+    pair_ranges(b, g, 0, j, r1, x);
+    pair_ranges(b, g, 3, j, r2, x);
   }
-  const bool two = (pair == 1 || pair == 2);
-  cpx<R>* out0 = O0 + (long long)j * g.tp + k0;
-  cpx<R>* out1 = two ? O1 + (long long)j * g.tp + k0 : nullptr;
-  for (int idx = threadIdx.x; idx < g.nq * KT; idx += blockDim.x) {
-    const int q = idx / KT, kk = idx % KT;
-    out0[(long long)q * g.nz + kk] = st0[idx];
-    if (two) out1[(long long)q * g.nz + kk] = st1[idx];
+  for (int m0 = 0; m0 < KT; m0 += ngroups) {
+    const long long rowc = (long long)j * pp + (long long)(k0 + m0 + grp) * g.nx + gt;
+    double u1m[8], u1c[8], u1p[8], u2m[8], u2c[8], u2p[8];
+    double va[8], vb[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const long long o = rowc + G * r;
+      u1m[r] = b.u1[o - pp];
+      u1c[r] = b.u1[o];
+      u1p[r] = b.u1[o + pp];
+      u2m[r] = b.u2[o - pp];
+      u2c[r] = b.u2[o];
+      u2p[r] = b.u2[o + pp];
+    }
+    // pair 0: (u1u1, V1) -> slot 0 = ikx F(u1u1) + F(V1)              (channel.f90:319-325, 399-407)
+    if (r1) {
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        va[r] = u1c[r] * u1c[r];
+        vb[r] = (u1p[r] * u2p[r] + u1c[r] * u2p[r] - u1c[r] * u2c[r] - u1m[r] * u2c[r]) * c2dyf;
+      }
+      fft_pair(va, vb);
+      emit_pair(smem, L, g, ngroups, 0, inv, b.kx, outp(0), (cpx<R>*)nullptr, m0);
+      __syncthreads();
+    }
+    // pair 3: (W1, V2) -> slot 4 = ikx F(W1) + F(V2)                   (:356-365, 440-449)
+    if (r2) {
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        va[r] = ((dyfj * u1c[r] + dyfm * u1m[r]) / (2.0 * dyj)) * u2c[r];
+        const double sp = u2c[r] + u2p[r], sm = u2c[r] + u2m[r];
+        vb[r] = (0.25 * (sp * sp) - 0.25 * (sm * sm)) / dyj;
+      }
+      fft_pair(va, vb);
+      emit_pair(smem, L, g, ngroups, 3, inv, b.kx, outp(4), (cpx<R>*)nullptr, m0);
+      __syncthreads();
+    }
+    // u3 at j-1, j, j+1 replaces u1 at j-1, j+1 and u2 at j-1
+    double* u3m = u1m;
+    double* u3p = u1p;
+    double* u3c = u2m;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const long long o = rowc + G * r;
+      u3m[r] = b.u3[o - pp];
+      u3c[r] = b.u3[o];
+      u3p[r] = b.u3[o + pp];
+    }
+    // pair 1: (u3u1, V3) -> slot 1 = F(u3u1), slot 2 = ikx F(u3u1) + F(V3)   (:298-305, 419-428)
+    if (r1) {
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        va[r] = u3c[r] * u1c[r];
+        vb[r] = (u3p[r] * u2p[r] + u3c[r] * u2p[r] - u3c[r] * u2c[r] - u3m[r] * u2c[r]) * c2dyf;
+      }
+      fft_pair(va, vb);
+      emit_pair(smem, L, g, ngroups, 1, inv, b.kx, outp(1), outp(2), m0);
+      __syncthreads();
+    }
+    // pair 2: (u3u3, W3) -> slot 3 = F(u3u3), slot 5 = F(W3)            (:337-344, 377-386)
+    if (r1 || r2) {
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        va[r] = r1 ? u3c[r] * u3c[r] : 0.0;
+        vb[r] = r2 ? ((dyfj * u3c[r] + dyfm * u3m[r]) / (2.0 * dyj)) * u2c[r] : 0.0;
+      }
+      fft_pair(va, vb);
+      emit_pair(smem, L, g, ngroups, 2, inv, b.kx, outp(3), outp(5), m0);
+      __syncthreads();
+    }
+    // scalars: (th u1, Vt) -> slot 6+2n, (th u3, 0) -> slot 7+2n       (:557-600)
+    for (int n = 0; n < a.nth; ++n) {
+      if (j < a.jst[n] || j > a.jet[n]) continue;  // uniform across the block
+      const double* th = a.th[n];
+      double thm[8], thc[8], thp[8];
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        const long long o = rowc + G * r;
+        thm[r] = th[o - pp];
+        thc[r] = th[o];
+        thp[r] = th[o + pp];
+      }
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        va[r] = thc[r] * u1c[r];
+        vb[r] = (thp[r] * u2p[r] + thc[r] * u2p[r] - thc[r] * u2c[r] - thm[r] * u2c[r]) * c2dyf;
+      }
+      fft_pair(va, vb);
+      emit_pair(smem, L, g, ngroups, 4, inv, b.kx, outp(6 + 2 * n), (cpx<R>*)nullptr, m0);
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        va[r] = thc[r] * u3c[r];
+        vb[r] = 0.0;
+      }
+      fft_pair(va, vb);
+      emit_pair(smem, L, g, ngroups, 5, inv, b.kx, outp(7 + 2 * n), (cpx<R>*)nullptr, m0);
+      __syncthreads();
+    }
   }
 }
 
@@ -342,9 +461,6 @@ __global__ void __launch_bounds__(256) k_xprod(ProdArgs a, Geo g, int pair, cpx<
 // through the backward z transform (first half of fft_xz_to_physical, :485-487,
 // :621).  Slots: 0 X1, 1 X13, 2 X3, 3 X33, 4 X2, 5 X32, 6+2n Xt_n, 7+2n Xt3_n.
 // cr goes to slot 0 (cr1), 1 (cr3), 2 (cr2), 3+n (crth_n) once read.
-constexpr int MAXTH = 4;
-constexpr int MAXSLOT = 6 + 2 * MAXTH;
-
 struct RhsArgs {
   const double2* X[MAXSLOT];      // x-spectra of the products
   double2* Tcr[3 + MAXTH];        // out: x-spectra of cr1, cr3, cr2, crth_n (slot order)
@@ -370,7 +486,8 @@ struct RhsArgs {
   int jst[MAXTH], jet[MAXTH];
 };
 
-__global__ void __launch_bounds__(1024) k_rhs(RhsArgs a, Geo g, const double2* __restrict__ twd,
+template <bool WAVE>
+__global__ void __launch_bounds__(512) k_rhs(RhsArgs a, Geo g, const double2* __restrict__ twd,
                                               int j0) {
   typedef cpx<double> C;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -380,6 +497,7 @@ __global__ void __launch_bounds__(1024) k_rhs(RhsArgs a, Geo g, const double2* _
   const int ngroups = blockDim.x / G;
   const int grp = threadIdx.x / G, gt = threadIdx.x % G;
   const int j = j0 + blockIdx.y, q = blockIdx.x;
+  const int L = fft_lds_len(g.nz);
   const int nslot = 6 + 2 * a.nth;
   const bool in1 = j >= a.js && j <= a.je;
   const bool in2 = j >= 2 && j <= g.ny;
@@ -397,13 +515,19 @@ __global__ void __launch_bounds__(1024) k_rhs(RhsArgs a, Geo g, const double2* _
   for (int s0 = 0; s0 < nslot; s0 += ngroups) {
     const int s = s0 + grp;
     const bool act = s < nslot && slot_used(s);
-    C* buf = smem + (long long)s * g.nz;
+    C* buf = smem + (long long)s * L;
+    C v[8];
     if (act) {
       const C* col = reinterpret_cast<const C*>(a.X[s]) + to;
-      for (int k = gt; k < g.nz; k += G) buf[k] = col[k];
+#pragma unroll
+      for (int r = 0; r < 8; ++r) v[r] = col[gt + G * r];
+    } else {
+#pragma unroll
+      for (int r = 0; r < 8; ++r) v[r] = C{0.0, 0.0};
     }
-    lds_fft<double, -1>(buf, g.nz, g.lognz, tw, gt);
+    fft_regs<double, -1, WAVE>(v, buf, g.nz, g.lognz, tw, gt);
   }
+  __syncthreads();  // the combine below reads every slot
   // --- explicit terms, mode by mode
   const double invnz = 1.0 / g.nz;
   const double kxq = a.kx[q];
@@ -417,7 +541,8 @@ __global__ void __launch_bounds__(1024) k_rhs(RhsArgs a, Geo g, const double2* _
     const bool mean = (q == 0 && kk == 0);
     // slot s at this mode; a slot is read before the cr slot that reuses it
     // is written (cr slot 3+n < 6+2n, and slots 0..3 belong to u1/u3)
-    auto Z = [&](int s) -> C { return invnz * smem[(long long)s * g.nz + nat]; };
+    const int pn = pidx(nat);
+    auto Z = [&](int s) -> C { return invnz * smem[(long long)s * L + pn]; };
     C cu1 = {0, 0}, cu3 = {0, 0};
     if (in1 || a.nth > 0) {
       cu1 = reinterpret_cast<const C*>(a.cu1)[o];
@@ -446,8 +571,8 @@ __global__ void __launch_bounds__(1024) k_rhs(RhsArgs a, Geo g, const double2* _
       r3 = r3 + a.temp5 * f3;
       reinterpret_cast<C*>(a.cf1)[o] = f1;
       reinterpret_cast<C*>(a.cf3)[o] = f3;
-      smem[0 * g.nz + nat] = r1;
-      smem[1 * g.nz + nat] = r3;
+      smem[0 * L + pn] = r1;
+      smem[1 * L + pn] = r3;
     }
     if (in2) {
       C cu2 = reinterpret_cast<const C*>(a.cu2)[o];
@@ -467,7 +592,7 @@ __global__ void __launch_bounds__(1024) k_rhs(RhsArgs a, Geo g, const double2* _
       r2 = r2 - (a.temp4 / cdy) * (cpj - cpm);
       r2 = r2 + a.temp5 * f2;
       reinterpret_cast<C*>(a.cf2)[o] = f2;
-      smem[2 * g.nz + nat] = r2;
+      smem[2 * L + pn] = r2;
     }
     for (int n = 0; n < a.nth; ++n) {
       if (!inT[n]) continue;
@@ -479,14 +604,14 @@ __global__ void __launch_bounds__(1024) k_rhs(RhsArgs a, Geo g, const double2* _
       if (a.rk_step > 1) rt = rt + a.temp3 * reinterpret_cast<const C*>(a.cfth[n])[o];
       rt = rt + a.temp5 * ft;
       reinterpret_cast<C*>(a.cfth[n])[o] = ft;
-      smem[(3 + n) * g.nz + nat] = rt;
+      smem[(3 + n) * L + pn] = rt;
     }
   }
   // zero the dealiased gap of the cr slots (fft.f90:336-340)
   const int ncr = 3 + a.nth;
   for (int c = 0; c < ncr; ++c)
     for (int k = g.nkz + 1 + threadIdx.x; k < g.nz - g.nkz; k += blockDim.x)
-      smem[c * g.nz + k] = C{0.0, 0.0};
+      smem[c * L + pidx(k)] = C{0.0, 0.0};
   // --- backward z transform of cr
   auto cr_used = [&](int c) -> bool {
     if (c < 2) return in1;
@@ -496,11 +621,15 @@ __global__ void __launch_bounds__(1024) k_rhs(RhsArgs a, Geo g, const double2* _
   for (int c0 = 0; c0 < ncr; c0 += ngroups) {
     const int c = c0 + grp;
     const bool act = c < ncr && cr_used(c);
-    C* buf = smem + (long long)c * g.nz;
-    lds_fft<double, 1>(buf, g.nz, g.lognz, tw, gt);
+    C* buf = smem + (long long)c * L;
+    __syncthreads();  // the cr slots were written by every thread of the block
+    C v[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) v[r] = buf[pidx(gt + G * r)];
+    fft_regs<double, 1, WAVE>(v, buf, g.nz, g.lognz, tw, gt);
     if (act) {
       C* out = reinterpret_cast<C*>(a.Tcr[c]) + to;
-      for (int k = gt; k < g.nz; k += G) out[k] = buf[k];
+      for (int k = gt; k < g.nz; k += G) st_out(out + k, buf[pidx(k)]);
     }
   }
 }
@@ -527,13 +656,15 @@ struct SegRow {
 };
 
 template <int L>
-__global__ void __launch_bounds__(512) k_thomas_seg(const double* __restrict__ r,
+__global__ void __launch_bounds__(1024) k_thomas_seg(const double* __restrict__ r,
                                                      double* __restrict__ u,
                                                      const SegRow* __restrict__ rows, int n, int S,
                                                      long long pp, int ca, int cb, double kscale) {
-  __shared__ double carry_f[8][64];
-  __shared__ double carry_b[8][64];
-  const int lane = threadIdx.x, seg = threadIdx.y;
+  __shared__ double carry_f[16][64];
+  __shared__ double carry_b[16][64];
+  // blockDim.x == 64: a wavefront is one segment, so seg is wave-uniform and
+  // the row-table reads below become scalar loads
+  const int lane = threadIdx.x, seg = __builtin_amdgcn_readfirstlane(threadIdx.y);
   const long long col = (long long)blockIdx.x * 64 + lane;
   const int s0 = seg * L;
   double v[L];
