@@ -178,7 +178,7 @@ def load_library(path=None):
     return _LIB[path]
 
 
-NCLASS = 9
+NCLASS = 10
 
 
 def spectral_shape(p):

@@ -14,7 +14,7 @@
 #include <vector>
 
 #include "../../include/diablo_gpu.h"
-#include "dgpu_kernels.h"
+#include "dgpu_les.h"
 
 using namespace dgpu;
 
@@ -90,6 +90,18 @@ struct dgpu_handle {
   DevBuf<double> d_kx, d_kz, d_dy, d_dyf;
   // Crank-Nicolson factor tables: [field][rk] rows, field 0 u1, 1 u2, 2 u3, 3+n th_n
   DevBuf<SegRow> rows;
+  // LES (allocated when use_les): velocity/scalar gradients, model outputs,
+  // their spectra, the subgrid forcing, the pre-BC state copy, CN scratch
+  DevBuf<TriRow> trirows;              // raw CN rows [field][rk][j]
+  DevBuf<double> grad[9];
+  DevBuf<double> dth[3][DGPU_MAX_TH];
+  DevBuf<double> tau[7];               // t11 t33 t13 t12 t23 c1 c3
+  DevBuf<double> thf[2][DGPU_MAX_TH];
+  DevBuf<double2> ctau[7];
+  DevBuf<double2> cthf[2][DGPU_MAX_TH];
+  DevBuf<double2> lesf[3], lesfth[DGPU_MAX_TH];
+  DevBuf<double2> cu_pre[3], cth_pre[DGPU_MAX_TH];
+  DevBuf<double> sbl;
   int seg_L = 0, seg_S = 0;
   int nfields = 0;
   size_t lds_limit = 65536;
@@ -298,6 +310,36 @@ void factor_rows(const Tri& t, int L, SegRow* out) {
   }
 }
 
+// Raw CN rows with the LES weights of channel.f90:676-689 (th), 736-749 (u2),
+// 792-805 (u1), 848-861 (u3): temp2 = h/2 times the face metrics.
+void raw_rows(const dgpu_handle* h, const Tri& t, int field, int rs, TriRow* out) {
+  const int n = (int)t.a.size();
+  const double temp2 = 0.5 * h->h_bar[rs];
+  const auto& dy = h->dy;
+  const auto& dyf = h->dyf;
+  for (int j = 0; j < n; ++j) {
+    TriRow& r = out[j];
+    r.a = t.a[j];
+    r.b = t.b[j];
+    r.c = t.c[j];
+    r.vb = t.vb[j];
+    r.wp = t.wp[j];
+    r.wm = t.wm[j];
+    r.kind = t.kind[j];
+    r.vtype = (field == 1);
+    r.lp = r.lm = 0.0;
+    if (t.kind[j] == 1) {
+      if (field == 1) {
+        r.lp = temp2 / (dyf[j] * dy[j]);
+        r.lm = temp2 / (dyf[j - 1] * dy[j]);
+      } else {
+        r.lp = temp2 / (dy[j + 1] * dyf[j]);
+        r.lm = temp2 / (dy[j] * dyf[j]);
+      }
+    }
+  }
+}
+
 void twiddles(int n, std::vector<double2>& w) {
   w.resize(n);
   for (int t = 0; t < n; ++t) {
@@ -456,6 +498,34 @@ void launch_zinv(dgpu_handle* h, const double2* C, double2* T, int j0, int j1) {
   HIPCHECK(hipGetLastError());
 }
 
+void launch_zinv_op(dgpu_handle* h, const double2* C, double2* T, int j0, int j1, int op) {
+  if (j1 < j0) return;
+  const Geo& g = h->g;
+  const int ng = z_ngroups(h);
+  const size_t lds = z_lds(h, ng);
+  dim3 grid((g.nq + ng - 1) / ng, j1 - j0 + 1), block(ng * (g.nz / 8));
+  Timed tm(h, DGPU_K_ZINV, (16.0 * g.tp + 16.0 * g.sp) * (j1 - j0 + 1));
+  hipLaunchKernelGGL((g.nz <= 512 ? k_zinv_op<double, true> : k_zinv_op<double, false>), grid, block,
+                     lds, h->stream, reinterpret_cast<const cpx<double>*>(C),
+                     reinterpret_cast<cpx<double>*>(T), g,
+                     reinterpret_cast<const cpx<double>*>(h->twz.p), j0, op, h->d_kx.p, h->d_kz.p,
+                     h->d_dy.p, h->d_dyf.p);
+  HIPCHECK(hipGetLastError());
+}
+
+void launch_thomas_les(dgpu_handle* h, int field, int rs, const double* r, double* u, const double* nt,
+                       int ca, int cb, double transient) {
+  const Geo& g = h->g;
+  const int n = g.ny2;
+  const TriRow* rows = h->trirows.p + ((size_t)field * 3 + rs) * n;
+  dim3 grid((unsigned)((g.pp + 255) / 256)), block(256);
+  // reads r, u, nu_t; writes b' and g' and reads them back; writes u
+  Timed tm(h, DGPU_K_THOMAS, 56.0 * g.pp * n);
+  hipLaunchKernelGGL(k_thomas_les, grid, block, 0, h->stream, r, u, nt, h->sbl.p, rows, n, g.pp, ca,
+                     cb, transient);
+  HIPCHECK(hipGetLastError());
+}
+
 // spectral -> physical for one field, all planes (fft_xz_to_physical, fft.f90:319-382)
 void to_physical(dgpu_handle* h, const double2* C, double* u, double2* T) {
   const int n2 = h->g.ny2;
@@ -549,6 +619,154 @@ void launch_bc_post(dgpu_handle* h, double time) {
   HIPCHECK(hipGetLastError());
 }
 
+// ------------------------------------------------------------------ LES --
+// les_chan + les_chan_th (les.f90:20-1455) for the substep: wall and ghost
+// values on the spectral state, the nine velocity gradients (and three per
+// scalar) to physical space, the model kernels, and the forward transforms
+// and divergence that give the subgrid forcing of cf1..3 and cfth.  Leaves
+// u1..3 and th in physical space, as the reference's les_chan does.
+void les_stage(dgpu_handle* h, double time) {
+  const dgpu_params& p = h->p;
+  const Geo& g = h->g;
+  const int nth = h->nth, Ny = g.ny, n2 = g.ny2;
+  launch_bc_post(h, time);  // apply_BC_vel_mpi_post (les.f90:80)
+  {
+    LesBcArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.cu1 = h->cu[0].p;
+    a.cu3 = h->cu[2].p;
+    a.dy = h->d_dy.p;
+    a.nth = nth;
+    a.u_lo = p.u_bc_ymin;
+    a.u_hi = p.u_bc_ymax;
+    a.w_lo = p.w_bc_ymin;
+    a.w_hi = p.w_bc_ymax;
+    for (int n = 0; n < nth; ++n) {
+      a.cth[n] = h->cth[n].p;
+      a.th_lo[n] = p.th_bc_ymin[n];
+      a.th_hi[n] = p.th_bc_ymax[n];
+      a.th_lo_c1[n] = p.th_bc_ymin_c1[n];
+      a.th_hi_c1[n] = p.th_bc_ymax_c1[n];
+    }
+    dim3 grid((unsigned)((g.sp + 255) / 256)), block(256);
+    Timed tm(h, DGPU_K_BCPOST, 16.0 * 12 * g.sp * (1 + nth));
+    hipLaunchKernelGGL(k_les_bc, grid, block, 0, h->stream, a, g);
+    HIPCHECK(hipGetLastError());
+  }
+  // gradients to physical space
+  double2* T = h->tpool[0].p;
+  const double2* cu1 = h->cu[0].p;
+  const double2* cu2 = h->cu[1].p;
+  const double2* cu3 = h->cu[2].p;
+  const struct {
+    const double2* c;
+    int op;
+  } gsrc[9] = {{cu1, OP_DX}, {cu1, OP_DY_GY}, {cu1, OP_DZ},  {cu2, OP_DX}, {cu2, OP_DY_GYF},
+               {cu2, OP_DZ}, {cu3, OP_DX},    {cu3, OP_DY_GY}, {cu3, OP_DZ}};
+  for (int m = 0; m < 9; ++m) {
+    launch_zinv_op(h, gsrc[m].c, T, 0, n2 - 1, gsrc[m].op);
+    launch_xinv(h, T, h->grad[m].p, 0, n2 - 1);
+  }
+  for (int n = 0; n < nth; ++n) {
+    const int ops[3] = {OP_DX, OP_DY_GY, OP_DZ};
+    for (int w = 0; w < 3; ++w) {
+      if (w == 1 && p.les_model_type != 5) continue;
+      launch_zinv_op(h, h->cth[n].p, T, 0, n2 - 1, ops[w]);
+      launch_xinv(h, T, h->dth[w][n].p, 0, n2 - 1);
+    }
+  }
+  // the state itself (les.f90:586-588, 1207-1209, 1229-1231)
+  for (int f = 0; f < 3; ++f) to_physical(h, h->cu[f].p, h->u[f].p, h->tpool[f].p);
+  for (int n = 0; n < nth; ++n) to_physical(h, h->cth[n].p, h->th[n].p, h->tpool[3 + n].p);
+  // velocity model
+  {
+    LesVelArgs a;
+    for (int m = 0; m < 9; ++m) a.G[m] = h->grad[m].p;
+    a.u1 = h->u[0].p;
+    a.u3 = h->u[2].p;
+    a.tau11 = h->tau[0].p;
+    a.tau33 = h->tau[1].p;
+    a.tau13 = h->tau[2].p;
+    a.tau12 = h->tau[3].p;
+    a.tau23 = h->tau[4].p;
+    a.cr1 = h->tau[5].p;
+    a.cr3 = h->tau[6].p;
+    a.nu_t = h->nu_t.p;
+    a.dy = h->d_dy.p;
+    a.dyf = h->d_dyf.p;
+    a.dx1 = p.lx / p.nx;
+    a.dz1 = p.lz / p.nz;
+    a.type = p.les_model_type;
+    a.js = h->jstart;
+    a.je = h->jend;
+    const long long npt = g.pp * (Ny + 1);
+    dim3 grid((unsigned)((npt + 255) / 256)), block(256);
+    Timed tm(h, DGPU_K_LES, (9.0 + 2.0 + 8.0) * 8.0 * npt);
+    hipLaunchKernelGGL(k_les_vel, grid, block, 0, h->stream, a, g);
+    HIPCHECK(hipGetLastError());
+  }
+  for (int m = 0; m < 7; ++m) to_fourier(h, h->tau[m].p, h->ctau[m].p, h->tpool[0].p, 1, Ny);
+  // scalar model
+  for (int n = 0; n < nth; ++n) {
+    LesThArgs a;
+    for (int m = 0; m < 9; ++m) a.G[m] = h->grad[m].p;
+    a.T1 = h->dth[0][n].p;
+    a.T2 = (p.les_model_type == 5) ? h->dth[1][n].p : h->dth[0][n].p;
+    a.T3 = h->dth[2][n].p;
+    a.nu_t = h->nu_t.p;
+    a.f1 = h->thf[0][n].p;
+    a.f3 = h->thf[1][n].p;
+    a.kappa_t = h->kappa_t[n].p;
+    a.dy = h->d_dy.p;
+    a.dyf = h->d_dyf.p;
+    a.dx1 = p.lx / p.nx;
+    a.dz1 = p.lz / p.nz;
+    a.type = p.les_model_type;
+    a.jst = h->jstart_th[n];
+    a.jet = h->jend_th[n];
+    const long long npt = g.pp * (Ny + 1);
+    dim3 grid((unsigned)((npt + 255) / 256)), block(256);
+    Timed tm(h, DGPU_K_LES, (9.0 + 3.0 + 4.0) * 8.0 * npt);
+    hipLaunchKernelGGL(k_les_th, grid, block, 0, h->stream, a, g);
+    HIPCHECK(hipGetLastError());
+    for (int w = 0; w < 2; ++w) to_fourier(h, h->thf[w][n].p, h->cthf[w][n].p, h->tpool[0].p, 1, Ny);
+  }
+  // divergence into the subgrid forcing
+  {
+    LesDivArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.t11 = h->ctau[0].p;
+    a.t33 = h->ctau[1].p;
+    a.t13 = h->ctau[2].p;
+    a.t12 = h->ctau[3].p;
+    a.t23 = h->ctau[4].p;
+    a.c1 = h->ctau[5].p;
+    a.c3 = h->ctau[6].p;
+    a.f1 = h->lesf[0].p;
+    a.f2 = h->lesf[1].p;
+    a.f3 = h->lesf[2].p;
+    for (int n = 0; n < nth; ++n) {
+      a.th_f1[n] = h->cthf[0][n].p;
+      a.th_f3[n] = h->cthf[1][n].p;
+      a.fth[n] = h->lesfth[n].p;
+      a.jst[n] = h->jstart_th[n];
+      a.jet[n] = h->jend_th[n];
+    }
+    a.kx = h->d_kx.p;
+    a.kz = h->d_kz.p;
+    a.dyf = h->d_dyf.p;
+    a.js = h->jstart;
+    a.j2 = Ny - 1;
+    a.nth = nth;
+    a.th_sign = (p.les_model_type == 5) ? -1 : 1;
+    const long long nm = g.sp * n2;
+    dim3 grid((unsigned)((nm + 255) / 256)), block(256);
+    Timed tm(h, DGPU_K_LES, 16.0 * (7 + 3 + 3 * nth) * nm);
+    hipLaunchKernelGGL(k_les_div, grid, block, 0, h->stream, a, g);
+    HIPCHECK(hipGetLastError());
+  }
+}
+
 // ----------------------------------------------------------- the substep --
 void rk_substep(dgpu_handle* h, int rk_step, double time, int time_step) {
   const dgpu_params& p = h->p;
@@ -559,14 +777,31 @@ void rk_substep(dgpu_handle* h, int rk_step, double time, int time_step) {
   const double temp3 = h->zeta_bar[rs] * h->h_bar[rs];  // channel.f90:29-33
   const double temp4 = h->h_bar[rs];
   const double temp5 = h->beta_bar[rs] * h->h_bar[rs];
-  (void)time_step;
-  if (p.use_les) throw ApiError{DGPU_EUNSUP, "LES path not available in this build"};
   const int nslot = 6 + 2 * nth;
   const int ncr = 3 + nth;
-
-  // 1. state to physical space (channel.f90:279-287)
-  for (int f = 0; f < 3; ++f) to_physical(h, h->cu[f].p, h->u[f].p, h->tpool[f].p);
-  for (int n = 0; n < nth; ++n) to_physical(h, h->cth[n].p, h->th[n].p, h->tpool[3 + n].p);
+  // LES gate (channel.f90:259)
+  const bool les = p.use_les && (!p.create_new_flow || time_step > 100);
+  // The explicit spectral terms (channel.f90:39-253) read the state before
+  // les_chan applies its wall values, so the LES path keeps that copy.
+  const double2* lin_cu[3] = {h->cu[0].p, h->cu[1].p, h->cu[2].p};
+  const double2* lin_cth[DGPU_MAX_TH] = {nullptr, nullptr, nullptr, nullptr};
+  for (int n = 0; n < nth; ++n) lin_cth[n] = h->cth[n].p;
+  if (les) {
+    const size_t bytes = sizeof(double2) * (size_t)g.ny2 * g.sp;
+    for (int f = 0; f < 3; ++f) {
+      HIPCHECK(hipMemcpyAsync(h->cu_pre[f].p, h->cu[f].p, bytes, hipMemcpyDeviceToDevice, h->stream));
+      lin_cu[f] = h->cu_pre[f].p;
+    }
+    for (int n = 0; n < nth; ++n) {
+      HIPCHECK(hipMemcpyAsync(h->cth_pre[n].p, h->cth[n].p, bytes, hipMemcpyDeviceToDevice, h->stream));
+      lin_cth[n] = h->cth_pre[n].p;
+    }
+    les_stage(h, time);
+  } else {
+    // 1. state to physical space (channel.f90:279-287)
+    for (int f = 0; f < 3; ++f) to_physical(h, h->cu[f].p, h->u[f].p, h->tpool[f].p);
+    for (int n = 0; n < nth; ++n) to_physical(h, h->cth[n].p, h->th[n].p, h->tpool[3 + n].p);
+  }
 
   // 2. products, x-transformed (channel.f90:299-459, 557-608)
   {
@@ -606,15 +841,21 @@ void rk_substep(dgpu_handle* h, int rk_step, double time, int time_step) {
     std::memset(&a, 0, sizeof(a));
     for (int s = 0; s < nslot; ++s) a.X[s] = h->tpool[s].p;
     for (int c = 0; c < ncr; ++c) a.Tcr[c] = h->tpool[nslot + c].p;
-    a.cu1 = h->cu[0].p;
-    a.cu2 = h->cu[1].p;
-    a.cu3 = h->cu[2].p;
+    a.cu1 = lin_cu[0];
+    a.cu2 = lin_cu[1];
+    a.cu3 = lin_cu[2];
     a.cp = h->cp.p;
+    if (les) {
+      a.lesf1 = h->lesf[0].p;
+      a.lesf2 = h->lesf[1].p;
+      a.lesf3 = h->lesf[2].p;
+      for (int n = 0; n < nth; ++n) a.lesfth[n] = h->lesfth[n].p;
+    }
     a.cf1 = h->cf[0].p;
     a.cf2 = h->cf[1].p;
     a.cf3 = h->cf[2].p;
     for (int n = 0; n < nth; ++n) {
-      a.cth[n] = h->cth[n].p;
+      a.cth[n] = lin_cth[n];
       a.cfth[n] = h->cfth[n].p;
       a.pr[n] = p.pr[n];
       a.dthdx[n] = p.dthdx[n];
@@ -674,12 +915,22 @@ void rk_substep(dgpu_handle* h, int rk_step, double time, int time_step) {
     transient = p.w_bc_ymax_c1 + p.amp_omega0 * std::sin(p.omega0 * (p.ro_inv / p.delta * time - p.force_start));
     h->w_transient = transient;
   }
-  for (int n = 0; n < nth; ++n)
-    launch_thomas(h, 3 + n, rs, h->rth[n].p, h->th[n].p, h->jstart_th[n] - 1, h->jend_th[n] + 1,
-                  transient);
-  launch_thomas(h, 1, rs, h->r[1].p, h->u[1].p, 1, Ny + 1, transient);
-  launch_thomas(h, 0, rs, h->r[0].p, h->u[0].p, h->jstart - 1, h->jend + 1, transient);
-  launch_thomas(h, 2, rs, h->r[2].p, h->u[2].p, h->jstart - 1, h->jend + 1, transient);
+  if (p.use_les) {
+    // with use_LES the eddy viscosity enters every CN row (channel.f90:519-547, 636-689)
+    for (int n = 0; n < nth; ++n)
+      launch_thomas_les(h, 3 + n, rs, h->rth[n].p, h->th[n].p, h->kappa_t[n].p, h->jstart_th[n] - 1,
+                        h->jend_th[n] + 1, transient);
+    launch_thomas_les(h, 1, rs, h->r[1].p, h->u[1].p, h->nu_t.p, 1, Ny + 1, transient);
+    launch_thomas_les(h, 0, rs, h->r[0].p, h->u[0].p, h->nu_t.p, h->jstart - 1, h->jend + 1, transient);
+    launch_thomas_les(h, 2, rs, h->r[2].p, h->u[2].p, h->nu_t.p, h->jstart - 1, h->jend + 1, transient);
+  } else {
+    for (int n = 0; n < nth; ++n)
+      launch_thomas(h, 3 + n, rs, h->rth[n].p, h->th[n].p, h->jstart_th[n] - 1, h->jend_th[n] + 1,
+                    transient);
+    launch_thomas(h, 1, rs, h->r[1].p, h->u[1].p, 1, Ny + 1, transient);
+    launch_thomas(h, 0, rs, h->r[0].p, h->u[0].p, h->jstart - 1, h->jend + 1, transient);
+    launch_thomas(h, 2, rs, h->r[2].p, h->u[2].p, h->jstart - 1, h->jend + 1, transient);
+  }
 
   // 6. back to Fourier space on the solved planes (channel.f90:891-896)
   to_fourier(h, h->u[0].p, h->cu[0].p, h->tpool[0].p, h->jstart - 1, h->jend + 1);
@@ -773,6 +1024,10 @@ int check_params(const dgpu_params* p, std::string& why) {
   if (p->delta == 0.0 && p->ro_inv != 0.0) { why = "delta must be nonzero when ro_inv is"; return -1; }
   for (int n = 0; n < p->n_th; ++n)
     if (!(p->pr[n] > 0)) { why = "pr must be positive"; return -1; }
+  if (p->use_les && p->les_model_type != 1 && p->les_model_type != 4 && p->les_model_type != 5) {
+    why = "les_model_type must be 1 (Smagorinsky), 4 (AMD-Rozema) or 5 (AMD-Verstappen)";
+    return -1;
+  }
   auto bc01 = [](int b) { return b == 0 || b == 1; };
   if (!bc01(p->u_bc_ymin) || !bc01(p->u_bc_ymax) || !bc01(p->v_bc_ymin) || !bc01(p->v_bc_ymax) ||
       !bc01(p->w_bc_ymin) || !bc01(p->w_bc_ymax)) {
@@ -814,6 +1069,22 @@ void free_all(dgpu_handle* h) {
   h->d_dy.release();
   h->d_dyf.release();
   h->rows.release();
+  h->trirows.release();
+  for (auto& b : h->grad) b.release();
+  for (auto& b : h->tau) b.release();
+  for (auto& b : h->ctau) b.release();
+  for (auto& b : h->lesf) b.release();
+  for (auto& b : h->cu_pre) b.release();
+  for (int n = 0; n < DGPU_MAX_TH; ++n) {
+    for (int w = 0; w < 3; ++w) h->dth[w][n].release();
+    for (int w = 0; w < 2; ++w) {
+      h->thf[w][n].release();
+      h->cthf[w][n].release();
+    }
+    h->lesfth[n].release();
+    h->cth_pre[n].release();
+  }
+  h->sbl.release();
 }
 
 void create_impl(dgpu_handle* h, const double* gy) {
@@ -850,6 +1121,7 @@ void create_impl(dgpu_handle* h, const double* gy) {
         (const void*)k_xinv<double, true>,  (const void*)k_xinv<double, false>,
         (const void*)k_zfwd<double, true>,  (const void*)k_zfwd<double, false>,
         (const void*)k_zinv<double, true>,  (const void*)k_zinv<double, false>,
+        (const void*)k_zinv_op<double, true>, (const void*)k_zinv_op<double, false>,
         (const void*)k_xprod_all<double, true>, (const void*)k_xprod_all<double, false>};
     for (const void* k : kernels)
       HIPCHECK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds_limit));
@@ -914,6 +1186,34 @@ void create_impl(dgpu_handle* h, const double* gy) {
       factor_rows(t, h->seg_L, rows.data() + ((size_t)f * 3 + rs) * n);
     }
   upload(h->rows, rows);
+  if (p.use_les) {
+    std::vector<TriRow> tr((size_t)h->nfields * 3 * n);
+    for (int f = 0; f < h->nfields; ++f)
+      for (int rs = 0; rs < 3; ++rs) {
+        Tri t = build_tri(h, f, rs);
+        raw_rows(h, t, f, rs, tr.data() + ((size_t)f * 3 + rs) * n);
+      }
+    upload(h->trirows, tr);
+    for (int m = 0; m < 9; ++m) h->grad[m].alloc(np);
+    for (int m = 0; m < 7; ++m) {
+      h->tau[m].alloc(np);
+      h->ctau[m].alloc(ns);
+    }
+    for (int f = 0; f < 3; ++f) {
+      h->lesf[f].alloc(ns);
+      h->cu_pre[f].alloc(ns);
+    }
+    for (int nn = 0; nn < h->nth; ++nn) {
+      for (int w = 0; w < 3; ++w) h->dth[w][nn].alloc(np);
+      for (int w = 0; w < 2; ++w) {
+        h->thf[w][nn].alloc(np);
+        h->cthf[w][nn].alloc(ns);
+      }
+      h->lesfth[nn].alloc(ns);
+      h->cth_pre[nn].alloc(ns);
+    }
+    h->sbl.alloc(np);
+  }
   HIPCHECK(hipDeviceSynchronize());
 }
 
@@ -1102,7 +1402,7 @@ int dgpu_timing_read(dgpu_handle* h, double* ms, double* bytes, int64_t* count) 
 const char* dgpu_kernel_class_name(int cls) {
   static const char* names[DGPU_NCLASS] = {"x_forward", "x_inverse", "z_forward", "z_inverse",
                                            "x_products", "spectral_rhs", "cn_solve",
-                                           "projection", "wall_bc"};
+                                           "projection", "wall_bc", "les_model"};
   return (cls >= 0 && cls < DGPU_NCLASS) ? names[cls] : "";
 }
 

@@ -480,6 +480,10 @@ struct RhsArgs {
   double nu, beta, coriolis;  // coriolis = Ro_inv/delta
   double grav_x, grav_y, grav_z;
   double pr[MAXTH], dthdx[MAXTH], dthdz[MAXTH];
+  const double2* lesf1;             // subgrid forcing of cf1..3, cfth (null without LES)
+  const double2* lesf2;
+  const double2* lesf3;
+  const double2* lesfth[MAXTH];
   double temp3, temp4, temp5, force;  // force = px0 (+ oscillation), 0 if none
   int rk_step, nth;
   int js, je;
@@ -557,6 +561,10 @@ __global__ void __launch_bounds__(512) k_rhs(RhsArgs a, Geo g, const double2* __
         f1 = f1 + a.grav_x * th;
         f3 = f3 + a.grav_z * th;
       }
+      if (a.lesf1) {
+        f1 = f1 + reinterpret_cast<const C*>(a.lesf1)[o];
+        f3 = f3 + reinterpret_cast<const C*>(a.lesf3)[o];
+      }
       f1 = f1 - (Z(0) + ik(kzk, Z(1)));
       f3 = f3 - (Z(2) + ik(kzk, Z(3)));
       C r1 = cu1, r3 = cu3;
@@ -586,6 +594,7 @@ __global__ void __launch_bounds__(512) k_rhs(RhsArgs a, Geo g, const double2* __
           f2 = f2 + (a.grav_y / (2.0 * cdy)) * (cdyfm * th + cdyf * thm);
         }
       }
+      if (a.lesf2) f2 = f2 + reinterpret_cast<const C*>(a.lesf2)[o];
       f2 = f2 - (Z(4) + ik(kzk, Z(5)));
       C r2 = cu2;
       if (a.rk_step > 1) r2 = r2 + a.temp3 * reinterpret_cast<const C*>(a.cf2)[o];
@@ -599,6 +608,7 @@ __global__ void __launch_bounds__(512) k_rhs(RhsArgs a, Geo g, const double2* __
       C th = reinterpret_cast<const C*>(a.cth[n])[o];
       C ft = (-(a.nu / a.pr[n]) * visc) * th;
       ft = ft - a.dthdx[n] * cu1 - a.dthdz[n] * cu3;
+      if (a.lesfth[n]) ft = ft + reinterpret_cast<const C*>(a.lesfth[n])[o];
       ft = ft - (Z(6 + 2 * n) + ik(kzk, Z(7 + 2 * n)));
       C rt = th;
       if (a.rk_step > 1) rt = rt + a.temp3 * reinterpret_cast<const C*>(a.cfth[n])[o];

@@ -151,7 +151,8 @@ int dgpu_synchronize(dgpu_handle* h);
 #define DGPU_K_THOMAS 6   /* Crank-Nicolson tridiagonal solve */
 #define DGPU_K_REMDIV 7   /* pressure Poisson solve + projection */
 #define DGPU_K_BCPOST 8   /* wall boundary values */
-#define DGPU_NCLASS 9
+#define DGPU_K_LES 9      /* subgrid model pointwise kernels and forcing */
+#define DGPU_NCLASS 10
 int dgpu_timing_enable(dgpu_handle* h, int enable);
 int dgpu_timing_read(dgpu_handle* h, double* ms /*[DGPU_NCLASS]*/, double* bytes /*[DGPU_NCLASS]*/,
                      int64_t* count /*[DGPU_NCLASS]*/);

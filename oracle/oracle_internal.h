@@ -65,9 +65,15 @@ struct Solver {
   void rk_chan_1(int rk_step, double time, int time_step);
   // LES (oracle_les.cpp)
   std::vector<double> les_buf;  // lazily allocated LES work buffers
+  bool les_ready = false;
+  double* lb(int id);           // LES work buffer id (physical/Fourier views)
+  void apply_bc_th_post();
+  void apply_bc_les();
+  void compute_strain(bool invariant, double beta_sgs);
+  void compute_rotation(bool invariant, double beta_sgs);
+  void compute_all_gradients();
   int les_chan(double time);
   int les_chan_th(double time);
-  bool les_ready = false;
 };
 
 }  // namespace orc

@@ -392,6 +392,10 @@ static int check_params(const dgpu_params* p, std::string& why) {
   if (!pow2(p->nx) || !pow2(p->nz)) { why = "nx and nz must be powers of two >= 8"; return -1; }
   if (p->ny < 5) { why = "ny must be >= 5"; return -1; }
   if (p->n_th < 0 || p->n_th > DGPU_MAX_TH) { why = "n_th out of range"; return -1; }
+  if (p->use_les && p->les_model_type != 1 && p->les_model_type != 4 && p->les_model_type != 5) {
+    why = "les_model_type must be 1, 4 or 5";
+    return -1;
+  }
   if (!(p->re > 0) || !(p->delta_t > 0)) { why = "re and delta_t must be positive"; return -1; }
   return 0;
 }

@@ -13,7 +13,8 @@ import pytest
 from diablo3_test_amd import workloads
 from diablo3_test_amd.binding import DgpuError
 from diablo3_test_amd.grid import metrics
-from helpers import active, field_errors, gpu_solver, oracle_solver, prepared, rel_l2
+from helpers import (active, check_against_floor, field_errors, gpu_solver, oracle_solver, prepared,
+                     rel_l2, report, sensitivity)
 
 pytestmark = pytest.mark.gpu
 
@@ -24,10 +25,6 @@ def pair(p, gy, st):
     g = prepared(gpu_solver(p, gy), st)
     o = prepared(oracle_solver(p, gy), st)
     return g, o
-
-
-def report(tag, errs):
-    print(tag, " ".join(f"{k}={v:.2e}" for k, v in errs.items()), flush=True)
 
 
 def test_set_get_roundtrip():
@@ -107,34 +104,6 @@ VARIANTS = {
     "no_scalar": dict(n_th=0, pr=[], th_bc_ymin=[], th_bc_ymax=[], th_bc_ymin_c1=[],
                       th_bc_ymax_c1=[], dthdx=[], dthdz=[]),
 }
-
-
-def sensitivity(p, gy, st, nsteps):
-    """Per-field relative change of the oracle's result when its input is
-    perturbed at the 1e-16 level: the round-off floor that any two correct
-    FP64 implementations differ by (the pressure, a small field built from
-    O(amplitude^2) terms divided by h, sits far above the velocities)."""
-    rng = np.random.default_rng(99)
-    st2 = {k: v * (1 + 1e-16 * rng.standard_normal(v.shape)) for k, v in st.items()}
-    a = prepared(oracle_solver(p, gy), st)
-    b = prepared(oracle_solver(p, gy), st2)
-    a.step(nsteps)
-    b.step(nsteps)
-    return field_errors(a.get_state(), b.get_state(), p)
-
-
-def check_against_floor(tag, errs, floor, bar=None):
-    """GPU-vs-oracle error within 50x the oracle's own round-off floor, and
-    (where given) within the stated absolute bar.  The absolute bar is only
-    meaningful for well-conditioned cases: with a Neumann v wall, for
-    example, the oracle's 1e-16 sensitivity already reaches 2e-11 in three
-    steps."""
-    report(tag, errs)
-    report(tag + " floor", floor)
-    for k, e in errs.items():
-        if bar is not None:
-            assert e <= bar, (k, e)
-        assert e <= 50 * floor[k] + 1e-15, (k, e, floor[k])
 
 
 @pytest.mark.parametrize("name", sorted(VARIANTS))
