@@ -87,6 +87,7 @@ struct dgpu_handle {
   DevBuf<double> sb;
   DevBuf<double2> sg;
   DevBuf<double2> twx, twz;
+  DevBuf<float2> twx32, twz32;  // FP32 twiddles of the mixed form
   DevBuf<double> d_kx, d_kz, d_dy, d_dyf;
   // Crank-Nicolson factor tables: [field][rk] rows, field 0 u1, 1 u2, 2 u3, 3+n th_n
   DevBuf<SegRow> rows;
@@ -101,7 +102,7 @@ struct dgpu_handle {
   DevBuf<double2> cthf[2][DGPU_MAX_TH];
   DevBuf<double2> lesf[3], lesfth[DGPU_MAX_TH];
   DevBuf<double2> cu_pre[3], cth_pre[DGPU_MAX_TH];
-  DevBuf<double> sbl;
+  DevBuf<double> sbl, sgl;              // b', g' of the per-column CN solve
   int seg_L = 0, seg_S = 0;
   int nfields = 0;
   size_t lds_limit = 65536;
@@ -294,6 +295,9 @@ void factor_rows(const Tri& t, int L, SegRow* out) {
     r.wm = t.wm[j];
     r.kind = t.kind[j];
     r.vb = t.vb[j];
+    r.ra = t.a[j];
+    r.rb = t.b[j];
+    r.rc = t.c[j];
   }
   for (int s0 = 0; s0 < n; s0 += L) {
     const int s1 = std::min(s0 + L, n) - 1;
@@ -416,141 +420,192 @@ void collect_timing(dgpu_handle* h) {
 }
 
 // ------------------------------------------------------- launch helpers ----
+// R is the transform precision: double for the FP64 path, float for the mixed
+// form.  Physical fields and x-spectra are stored in R (the buffers are sized
+// for FP64 and a float view uses their first half); spectral fields are FP64.
+template <typename R>
+R* phys(const DevBuf<double>& b) {
+  return reinterpret_cast<R*>(b.p);
+}
+template <typename R>
+cpx<R>* xspec(const DevBuf<double2>& b) {
+  return reinterpret_cast<cpx<R>*>(b.p);
+}
+template <typename R>
+const cpx<R>* twx_of(const dgpu_handle* h);
+template <>
+const cpx<double>* twx_of<double>(const dgpu_handle* h) {
+  return reinterpret_cast<const cpx<double>*>(h->twx.p);
+}
+template <>
+const cpx<float>* twx_of<float>(const dgpu_handle* h) {
+  return reinterpret_cast<const cpx<float>*>(h->twx32.p);
+}
+template <typename R>
+const cpx<R>* twz_of(const dgpu_handle* h);
+template <>
+const cpx<double>* twz_of<double>(const dgpu_handle* h) {
+  return reinterpret_cast<const cpx<double>*>(h->twz.p);
+}
+template <>
+const cpx<float>* twz_of<float>(const dgpu_handle* h) {
+  return reinterpret_cast<const cpx<float>*>(h->twz32.p);
+}
+
 // Transform groups per workgroup for the x passes: a divisor of `jobs` (the
-// transforms one KT-row tile needs), at most 1024 threads, within the LDS
-// limit.  All of a tile's transforms in one round when they fit.
+// transforms one KT-row tile needs), at most max_threads threads, within the
+// LDS limit.  All of a tile's transforms in one round when they fit.
+template <typename R>
 int x_ngroups(const dgpu_handle* h, int jobs, int max_threads = 512) {
   const Geo& g = h->g;
   const int G = g.nx / 8;
   int ng = jobs;
-  auto lds = [&](int n) { return (size_t)n * fft_lds_len(g.nx) * sizeof(double2); };
+  auto lds = [&](int n) { return (size_t)n * fft_lds_len(g.nx) * sizeof(cpx<R>); };
   while (ng > 1 && (ng * G > max_threads || lds(ng) > h->lds_limit)) ng /= 2;
   if (ng * G > max_threads || lds(ng) > h->lds_limit)
     throw ApiError{DGPU_EUNSUP, "Nx too large for the x-transform pass"};
   return ng;
 }
 
-size_t x_lds(const dgpu_handle* h, int ng) { return (size_t)ng * fft_lds_len(h->g.nx) * sizeof(double2); }
+template <typename R>
+size_t x_lds(const dgpu_handle* h, int ng) {
+  return (size_t)ng * fft_lds_len(h->g.nx) * sizeof(cpx<R>);
+}
 
+template <typename R>
 int z_ngroups(const dgpu_handle* h) {
   const Geo& g = h->g;
   const int G = g.nz / 8;
   if (G > 512) throw ApiError{DGPU_EUNSUP, "Nz too large for the z-transform pass"};
   int ng = 256 / G;
   if (ng < 1) ng = 1;
-  auto lds = [&](int n) { return (size_t)n * fft_lds_len(g.nz) * sizeof(double2); };
+  auto lds = [&](int n) { return (size_t)n * fft_lds_len(g.nz) * sizeof(cpx<R>); };
   while (ng > 1 && lds(ng) > h->lds_limit) --ng;
   if (lds(ng) > h->lds_limit) throw ApiError{DGPU_EUNSUP, "Nz too large for the z-transform pass"};
   return ng;
 }
 
-size_t z_lds(const dgpu_handle* h, int ng) { return (size_t)ng * fft_lds_len(h->g.nz) * sizeof(double2); }
+template <typename R>
+size_t z_lds(const dgpu_handle* h, int ng) {
+  return (size_t)ng * fft_lds_len(h->g.nz) * sizeof(cpx<R>);
+}
 
-void launch_xfwd(dgpu_handle* h, const double* u, double2* T, int j0, int j1) {
+template <typename R>
+void launch_xfwd(dgpu_handle* h, const R* u, cpx<R>* T, int j0, int j1) {
   if (j1 < j0) return;
   const Geo& g = h->g;
-  const int ng = x_ngroups(h, KT / 2);
-  const size_t lds = x_lds(h, ng);
+  constexpr int KT = XTile<R>::KT;
+  const int ng = x_ngroups<R>(h, KT / 2);
+  const size_t lds = x_lds<R>(h, ng);
   dim3 grid(g.nz / KT, j1 - j0 + 1), block(ng * (g.nx / 8));
-  Timed tm(h, DGPU_K_XFWD, (8.0 * g.pp + 16.0 * g.tp) * (j1 - j0 + 1));
-  hipLaunchKernelGGL((g.nx <= 512 ? k_xfwd<double, true> : k_xfwd<double, false>), grid, block, lds, h->stream, u,
-                     reinterpret_cast<cpx<double>*>(T), g,
-                     reinterpret_cast<const cpx<double>*>(h->twx.p), j0);
+  Timed tm(h, DGPU_K_XFWD, (sizeof(R) * g.pp + 2.0 * sizeof(R) * g.tp) * (j1 - j0 + 1));
+  hipLaunchKernelGGL((g.nx <= 512 ? k_xfwd<R, true> : k_xfwd<R, false>), grid, block, lds, h->stream, u, T, g,
+                     twx_of<R>(h), j0);
   HIPCHECK(hipGetLastError());
 }
 
-void launch_xinv(dgpu_handle* h, const double2* T, double* u, int j0, int j1) {
+template <typename R>
+void launch_xinv(dgpu_handle* h, const cpx<R>* T, R* u, int j0, int j1) {
   if (j1 < j0) return;
   const Geo& g = h->g;
-  const int ng = x_ngroups(h, KT / 2);
-  const size_t lds = x_lds(h, ng);
+  constexpr int KT = XTile<R>::KT;
+  const int ng = x_ngroups<R>(h, KT / 2);
+  const size_t lds = x_lds<R>(h, ng);
   dim3 grid(g.nz / KT, j1 - j0 + 1), block(ng * (g.nx / 8));
-  Timed tm(h, DGPU_K_XINV, (8.0 * g.pp + 16.0 * g.tp) * (j1 - j0 + 1));
-  hipLaunchKernelGGL((g.nx <= 512 ? k_xinv<double, true> : k_xinv<double, false>), grid, block, lds, h->stream,
-                     reinterpret_cast<const cpx<double>*>(T), u, g,
-                     reinterpret_cast<const cpx<double>*>(h->twx.p), j0);
+  Timed tm(h, DGPU_K_XINV, (sizeof(R) * g.pp + 2.0 * sizeof(R) * g.tp) * (j1 - j0 + 1));
+  hipLaunchKernelGGL((g.nx <= 512 ? k_xinv<R, true> : k_xinv<R, false>), grid, block, lds, h->stream, T, u, g,
+                     twx_of<R>(h), j0);
   HIPCHECK(hipGetLastError());
 }
 
-void launch_zfwd(dgpu_handle* h, const double2* T, double2* C, int j0, int j1) {
+template <typename R>
+void launch_zfwd(dgpu_handle* h, const cpx<R>* T, double2* C, int j0, int j1, bool accum) {
   if (j1 < j0) return;
   const Geo& g = h->g;
-  const int ng = z_ngroups(h);
-  const size_t lds = z_lds(h, ng);
+  const int ng = z_ngroups<R>(h);
+  const size_t lds = z_lds<R>(h, ng);
   dim3 grid((g.nq + ng - 1) / ng, j1 - j0 + 1), block(ng * (g.nz / 8));
-  Timed tm(h, DGPU_K_ZFWD, (16.0 * g.tp + 16.0 * g.sp) * (j1 - j0 + 1));
-  hipLaunchKernelGGL((g.nz <= 512 ? k_zfwd<double, true> : k_zfwd<double, false>), grid, block, lds, h->stream,
-                     reinterpret_cast<const cpx<double>*>(T), reinterpret_cast<cpx<double>*>(C), g,
-                     reinterpret_cast<const cpx<double>*>(h->twz.p), j0);
+  Timed tm(h, DGPU_K_ZFWD, (2.0 * sizeof(R) * g.tp + (accum ? 32.0 : 16.0) * g.sp) * (j1 - j0 + 1));
+  hipLaunchKernelGGL((g.nz <= 512 ? k_zfwd<R, true> : k_zfwd<R, false>), grid, block, lds, h->stream, T,
+                     reinterpret_cast<cpx<double>*>(C), g, twz_of<R>(h), j0, accum ? 1 : 0);
   HIPCHECK(hipGetLastError());
 }
 
-void launch_zinv(dgpu_handle* h, const double2* C, double2* T, int j0, int j1) {
+template <typename R>
+void launch_zinv(dgpu_handle* h, const double2* C, cpx<R>* T, int j0, int j1) {
   if (j1 < j0) return;
   const Geo& g = h->g;
-  const int ng = z_ngroups(h);
-  const size_t lds = z_lds(h, ng);
+  const int ng = z_ngroups<R>(h);
+  const size_t lds = z_lds<R>(h, ng);
   dim3 grid((g.nq + ng - 1) / ng, j1 - j0 + 1), block(ng * (g.nz / 8));
-  Timed tm(h, DGPU_K_ZINV, (16.0 * g.tp + 16.0 * g.sp) * (j1 - j0 + 1));
-  hipLaunchKernelGGL((g.nz <= 512 ? k_zinv<double, true> : k_zinv<double, false>), grid, block, lds, h->stream,
-                     reinterpret_cast<const cpx<double>*>(C), reinterpret_cast<cpx<double>*>(T), g,
-                     reinterpret_cast<const cpx<double>*>(h->twz.p), j0);
+  Timed tm(h, DGPU_K_ZINV, (2.0 * sizeof(R) * g.tp + 16.0 * g.sp) * (j1 - j0 + 1));
+  hipLaunchKernelGGL((g.nz <= 512 ? k_zinv<R, true> : k_zinv<R, false>), grid, block, lds, h->stream,
+                     reinterpret_cast<const cpx<double>*>(C), T, g, twz_of<R>(h), j0);
   HIPCHECK(hipGetLastError());
 }
 
-void launch_zinv_op(dgpu_handle* h, const double2* C, double2* T, int j0, int j1, int op) {
+template <typename R>
+void launch_zinv_op(dgpu_handle* h, const double2* C, cpx<R>* T, int j0, int j1, int op) {
   if (j1 < j0) return;
   const Geo& g = h->g;
-  const int ng = z_ngroups(h);
-  const size_t lds = z_lds(h, ng);
+  const int ng = z_ngroups<R>(h);
+  const size_t lds = z_lds<R>(h, ng);
   dim3 grid((g.nq + ng - 1) / ng, j1 - j0 + 1), block(ng * (g.nz / 8));
-  Timed tm(h, DGPU_K_ZINV, (16.0 * g.tp + 16.0 * g.sp) * (j1 - j0 + 1));
-  hipLaunchKernelGGL((g.nz <= 512 ? k_zinv_op<double, true> : k_zinv_op<double, false>), grid, block,
-                     lds, h->stream, reinterpret_cast<const cpx<double>*>(C),
-                     reinterpret_cast<cpx<double>*>(T), g,
-                     reinterpret_cast<const cpx<double>*>(h->twz.p), j0, op, h->d_kx.p, h->d_kz.p,
-                     h->d_dy.p, h->d_dyf.p);
+  Timed tm(h, DGPU_K_ZINV, (2.0 * sizeof(R) * g.tp + 16.0 * g.sp) * (j1 - j0 + 1));
+  hipLaunchKernelGGL((g.nz <= 512 ? k_zinv_op<R, true> : k_zinv_op<R, false>), grid, block, lds, h->stream,
+                     reinterpret_cast<const cpx<double>*>(C), T, g, twz_of<R>(h), j0, op, h->d_kx.p,
+                     h->d_kz.p, h->d_dy.p, h->d_dyf.p);
   HIPCHECK(hipGetLastError());
 }
 
-void launch_thomas_les(dgpu_handle* h, int field, int rs, const double* r, double* u, const double* nt,
-                       int ca, int cb, double transient) {
+// The CN solve when the rows depend on nu_t / kappa_t per column.
+template <typename R>
+void launch_thomas_les(dgpu_handle* h, int field, int rs, const R* r, R* u, const double* nt, int ca,
+                       int cb, double transient, bool incr) {
   const Geo& g = h->g;
   const int n = g.ny2;
   const TriRow* rows = h->trirows.p + ((size_t)field * 3 + rs) * n;
   dim3 grid((unsigned)((g.pp + 255) / 256)), block(256);
   // reads r, u, nu_t; writes b' and g' and reads them back; writes u
-  Timed tm(h, DGPU_K_THOMAS, 56.0 * g.pp * n);
-  hipLaunchKernelGGL(k_thomas_les, grid, block, 0, h->stream, r, u, nt, h->sbl.p, rows, n, g.pp, ca,
-                     cb, transient);
+  Timed tm(h, DGPU_K_THOMAS, (3.0 * sizeof(R) + 8.0 + 32.0) * g.pp * n);
+  hipLaunchKernelGGL(k_thomas_les<R>, grid, block, 0, h->stream, r, u, nt, h->sbl.p, h->sgl.p, rows, n, g.pp,
+                     ca, cb, transient, incr ? 1 : 0);
   HIPCHECK(hipGetLastError());
 }
 
 // spectral -> physical for one field, all planes (fft_xz_to_physical, fft.f90:319-382)
-void to_physical(dgpu_handle* h, const double2* C, double* u, double2* T) {
+template <typename R>
+void to_physical(dgpu_handle* h, const double2* C, R* u, cpx<R>* T) {
   const int n2 = h->g.ny2;
-  launch_zinv(h, C, T, 0, n2 - 1);
-  launch_xinv(h, T, u, 0, n2 - 1);
+  launch_zinv<R>(h, C, T, 0, n2 - 1);
+  launch_xinv<R>(h, T, u, 0, n2 - 1);
 }
 
-// physical -> spectral for one field on planes [j0, j1] (fft_xz_to_fourier, fft.f90:247-315)
-void to_fourier(dgpu_handle* h, const double* u, double2* C, double2* T, int j0, int j1) {
-  launch_xfwd(h, u, T, j0, j1);
-  launch_zfwd(h, T, C, j0, j1);
+// physical -> spectral for one field on planes [j0, j1] (fft_xz_to_fourier,
+// fft.f90:247-315); with accum the transform is added to C
+template <typename R>
+void to_fourier(dgpu_handle* h, const R* u, double2* C, cpx<R>* T, int j0, int j1, bool accum = false) {
+  launch_xfwd<R>(h, u, T, j0, j1);
+  launch_zfwd<R>(h, T, C, j0, j1, accum);
 }
 
-void launch_thomas(dgpu_handle* h, int field, int rs, const double* r, double* u, int ca, int cb,
-                   double transient) {
+template <typename R>
+void launch_thomas(dgpu_handle* h, int field, int rs, const R* r, R* u, int ca, int cb, double transient,
+                   bool incr) {
   const Geo& g = h->g;
   const int n = g.ny2;
   const SegRow* rows = h->rows.p + ((size_t)field * 3 + rs) * n;
   dim3 grid((unsigned)(g.pp / 64)), block(64, h->seg_S);
-  Timed tm(h, DGPU_K_THOMAS, 24.0 * g.pp * n);
+  Timed tm(h, DGPU_K_THOMAS, 3.0 * sizeof(R) * g.pp * n);
   switch (h->seg_L) {
-#define TCASE(L)                                                                              \
-  case L:                                                                                     \
-    hipLaunchKernelGGL(k_thomas_seg<L>, grid, block, 0, h->stream, r, u, rows, n, h->seg_S, \
-                       g.pp, ca, cb, transient);                                             \
+#define TCASE(L)                                                                                         \
+  case L:                                                                                                \
+    if (incr)                                                                                            \
+      hipLaunchKernelGGL((k_thomas_seg<L, R, true>), grid, block, 0, h->stream, r, u, rows, n, h->seg_S,  \
+                         g.pp, ca, cb, transient);                                                       \
+    else                                                                                                 \
+      hipLaunchKernelGGL((k_thomas_seg<L, R, false>), grid, block, 0, h->stream, r, u, rows, n, h->seg_S, \
+                         g.pp, ca, cb, transient);                                                       \
     break;
     TCASE(8)
     TCASE(16)
@@ -625,6 +680,7 @@ void launch_bc_post(dgpu_handle* h, double time) {
 // scalar) to physical space, the model kernels, and the forward transforms
 // and divergence that give the subgrid forcing of cf1..3 and cfth.  Leaves
 // u1..3 and th in physical space, as the reference's les_chan does.
+template <typename R>
 void les_stage(dgpu_handle* h, double time) {
   const dgpu_params& p = h->p;
   const Geo& g = h->g;
@@ -654,7 +710,7 @@ void les_stage(dgpu_handle* h, double time) {
     HIPCHECK(hipGetLastError());
   }
   // gradients to physical space
-  double2* T = h->tpool[0].p;
+  cpx<R>* T = xspec<R>(h->tpool[0]);
   const double2* cu1 = h->cu[0].p;
   const double2* cu2 = h->cu[1].p;
   const double2* cu3 = h->cu[2].p;
@@ -664,33 +720,33 @@ void les_stage(dgpu_handle* h, double time) {
   } gsrc[9] = {{cu1, OP_DX}, {cu1, OP_DY_GY}, {cu1, OP_DZ},  {cu2, OP_DX}, {cu2, OP_DY_GYF},
                {cu2, OP_DZ}, {cu3, OP_DX},    {cu3, OP_DY_GY}, {cu3, OP_DZ}};
   for (int m = 0; m < 9; ++m) {
-    launch_zinv_op(h, gsrc[m].c, T, 0, n2 - 1, gsrc[m].op);
-    launch_xinv(h, T, h->grad[m].p, 0, n2 - 1);
+    launch_zinv_op<R>(h, gsrc[m].c, T, 0, n2 - 1, gsrc[m].op);
+    launch_xinv<R>(h, T, phys<R>(h->grad[m]), 0, n2 - 1);
   }
   for (int n = 0; n < nth; ++n) {
     const int ops[3] = {OP_DX, OP_DY_GY, OP_DZ};
     for (int w = 0; w < 3; ++w) {
       if (w == 1 && p.les_model_type != 5) continue;
-      launch_zinv_op(h, h->cth[n].p, T, 0, n2 - 1, ops[w]);
-      launch_xinv(h, T, h->dth[w][n].p, 0, n2 - 1);
+      launch_zinv_op<R>(h, h->cth[n].p, T, 0, n2 - 1, ops[w]);
+      launch_xinv<R>(h, T, phys<R>(h->dth[w][n]), 0, n2 - 1);
     }
   }
   // the state itself (les.f90:586-588, 1207-1209, 1229-1231)
-  for (int f = 0; f < 3; ++f) to_physical(h, h->cu[f].p, h->u[f].p, h->tpool[f].p);
-  for (int n = 0; n < nth; ++n) to_physical(h, h->cth[n].p, h->th[n].p, h->tpool[3 + n].p);
+  for (int f = 0; f < 3; ++f) to_physical<R>(h, h->cu[f].p, phys<R>(h->u[f]), xspec<R>(h->tpool[f]));
+  for (int n = 0; n < nth; ++n) to_physical<R>(h, h->cth[n].p, phys<R>(h->th[n]), xspec<R>(h->tpool[3 + n]));
   // velocity model
   {
-    LesVelArgs a;
-    for (int m = 0; m < 9; ++m) a.G[m] = h->grad[m].p;
-    a.u1 = h->u[0].p;
-    a.u3 = h->u[2].p;
-    a.tau11 = h->tau[0].p;
-    a.tau33 = h->tau[1].p;
-    a.tau13 = h->tau[2].p;
-    a.tau12 = h->tau[3].p;
-    a.tau23 = h->tau[4].p;
-    a.cr1 = h->tau[5].p;
-    a.cr3 = h->tau[6].p;
+    LesVelArgs<R> a;
+    for (int m = 0; m < 9; ++m) a.G[m] = phys<R>(h->grad[m]);
+    a.u1 = phys<R>(h->u[0]);
+    a.u3 = phys<R>(h->u[2]);
+    a.tau11 = phys<R>(h->tau[0]);
+    a.tau33 = phys<R>(h->tau[1]);
+    a.tau13 = phys<R>(h->tau[2]);
+    a.tau12 = phys<R>(h->tau[3]);
+    a.tau23 = phys<R>(h->tau[4]);
+    a.cr1 = phys<R>(h->tau[5]);
+    a.cr3 = phys<R>(h->tau[6]);
     a.nu_t = h->nu_t.p;
     a.dy = h->d_dy.p;
     a.dyf = h->d_dyf.p;
@@ -701,21 +757,22 @@ void les_stage(dgpu_handle* h, double time) {
     a.je = h->jend;
     const long long npt = g.pp * (Ny + 1);
     dim3 grid((unsigned)((npt + 255) / 256)), block(256);
-    Timed tm(h, DGPU_K_LES, (9.0 + 2.0 + 8.0) * 8.0 * npt);
-    hipLaunchKernelGGL(k_les_vel, grid, block, 0, h->stream, a, g);
+    Timed tm(h, DGPU_K_LES, ((9.0 + 2.0 + 7.0) * sizeof(R) + 8.0) * npt);
+    hipLaunchKernelGGL(k_les_vel<R>, grid, block, 0, h->stream, a, g);
     HIPCHECK(hipGetLastError());
   }
-  for (int m = 0; m < 7; ++m) to_fourier(h, h->tau[m].p, h->ctau[m].p, h->tpool[0].p, 1, Ny);
+  for (int m = 0; m < 7; ++m)
+    to_fourier<R>(h, phys<R>(h->tau[m]), h->ctau[m].p, xspec<R>(h->tpool[0]), 1, Ny);
   // scalar model
   for (int n = 0; n < nth; ++n) {
-    LesThArgs a;
-    for (int m = 0; m < 9; ++m) a.G[m] = h->grad[m].p;
-    a.T1 = h->dth[0][n].p;
-    a.T2 = (p.les_model_type == 5) ? h->dth[1][n].p : h->dth[0][n].p;
-    a.T3 = h->dth[2][n].p;
+    LesThArgs<R> a;
+    for (int m = 0; m < 9; ++m) a.G[m] = phys<R>(h->grad[m]);
+    a.T1 = phys<R>(h->dth[0][n]);
+    a.T2 = (p.les_model_type == 5) ? phys<R>(h->dth[1][n]) : phys<R>(h->dth[0][n]);
+    a.T3 = phys<R>(h->dth[2][n]);
     a.nu_t = h->nu_t.p;
-    a.f1 = h->thf[0][n].p;
-    a.f3 = h->thf[1][n].p;
+    a.f1 = phys<R>(h->thf[0][n]);
+    a.f3 = phys<R>(h->thf[1][n]);
     a.kappa_t = h->kappa_t[n].p;
     a.dy = h->d_dy.p;
     a.dyf = h->d_dyf.p;
@@ -726,10 +783,11 @@ void les_stage(dgpu_handle* h, double time) {
     a.jet = h->jend_th[n];
     const long long npt = g.pp * (Ny + 1);
     dim3 grid((unsigned)((npt + 255) / 256)), block(256);
-    Timed tm(h, DGPU_K_LES, (9.0 + 3.0 + 4.0) * 8.0 * npt);
-    hipLaunchKernelGGL(k_les_th, grid, block, 0, h->stream, a, g);
+    Timed tm(h, DGPU_K_LES, ((9.0 + 3.0 + 2.0) * sizeof(R) + 16.0) * npt);
+    hipLaunchKernelGGL(k_les_th<R>, grid, block, 0, h->stream, a, g);
     HIPCHECK(hipGetLastError());
-    for (int w = 0; w < 2; ++w) to_fourier(h, h->thf[w][n].p, h->cthf[w][n].p, h->tpool[0].p, 1, Ny);
+    for (int w = 0; w < 2; ++w)
+      to_fourier<R>(h, phys<R>(h->thf[w][n]), h->cthf[w][n].p, xspec<R>(h->tpool[0]), 1, Ny);
   }
   // divergence into the subgrid forcing
   {
@@ -768,12 +826,19 @@ void les_stage(dgpu_handle* h, double time) {
 }
 
 // ----------------------------------------------------------- the substep --
-void rk_substep(dgpu_handle* h, int rk_step, double time, int time_step) {
+// One rk_chan_1 (channel.f90:10-917).  With R = float (the mixed form) the
+// transforms, products and LES run in FP32 while the spectral state, the
+// explicit spectral terms, the CN solves and the projection stay FP64; the
+// RHS pass and the solves then work on increments (RHS less state, new state
+// less old), so the FP64 state is never rounded to FP32 as a whole.
+template <typename R>
+void rk_substep_t(dgpu_handle* h, int rk_step, double time, int time_step) {
   const dgpu_params& p = h->p;
   const Geo& g = h->g;
   const int rs = rk_step - 1;
   const int nth = h->nth;
   const int Ny = g.ny;
+  const bool incr = sizeof(R) < sizeof(double);
   const double temp3 = h->zeta_bar[rs] * h->h_bar[rs];  // channel.f90:29-33
   const double temp4 = h->h_bar[rs];
   const double temp5 = h->beta_bar[rs] * h->h_bar[rs];
@@ -796,21 +861,21 @@ void rk_substep(dgpu_handle* h, int rk_step, double time, int time_step) {
       HIPCHECK(hipMemcpyAsync(h->cth_pre[n].p, h->cth[n].p, bytes, hipMemcpyDeviceToDevice, h->stream));
       lin_cth[n] = h->cth_pre[n].p;
     }
-    les_stage(h, time);
+    les_stage<R>(h, time);
   } else {
     // 1. state to physical space (channel.f90:279-287)
-    for (int f = 0; f < 3; ++f) to_physical(h, h->cu[f].p, h->u[f].p, h->tpool[f].p);
-    for (int n = 0; n < nth; ++n) to_physical(h, h->cth[n].p, h->th[n].p, h->tpool[3 + n].p);
+    for (int f = 0; f < 3; ++f) to_physical<R>(h, h->cu[f].p, phys<R>(h->u[f]), xspec<R>(h->tpool[f]));
+    for (int n = 0; n < nth; ++n)
+      to_physical<R>(h, h->cth[n].p, phys<R>(h->th[n]), xspec<R>(h->tpool[3 + n]));
   }
 
   // 2. products, x-transformed (channel.f90:299-459, 557-608)
   {
     ProdAllArgs pa;
     std::memset(&pa, 0, sizeof(pa));
-    pa.base.u1 = h->u[0].p;
-    pa.base.u2 = h->u[1].p;
-    pa.base.u3 = h->u[2].p;
-    pa.base.th = nullptr;
+    pa.base.u1 = phys<R>(h->u[0]);
+    pa.base.u2 = phys<R>(h->u[1]);
+    pa.base.u3 = phys<R>(h->u[2]);
     pa.base.dy = h->d_dy.p;
     pa.base.dyf = h->d_dyf.p;
     pa.base.kx = h->d_kx.p;
@@ -820,18 +885,19 @@ void rk_substep(dgpu_handle* h, int rk_step, double time, int time_step) {
     pa.base.jet = -1;
     pa.nth = nth;
     for (int n = 0; n < nth; ++n) {
-      pa.th[n] = h->th[n].p;
+      pa.th[n] = phys<R>(h->th[n]);
       pa.jst[n] = h->jstart_th[n];
       pa.jet[n] = h->jend_th[n];
     }
     for (int s = 0; s < nslot; ++s) pa.out[s] = h->tpool[s].p;
-    const int ngx = x_ngroups(h, KT, 512);
-    const size_t lds = x_lds(h, ngx);
+    constexpr int KT = XTile<R>::KT;
+    const int ngx = x_ngroups<R>(h, KT, 512);
+    const size_t lds = x_lds<R>(h, ngx);
     dim3 grid(g.nz / KT, Ny), block(ngx * (g.nx / 8));
     // reads u1, u2, u3, th once (neighbour planes from cache); writes the product x-spectra
-    Timed tm(h, DGPU_K_XPROD, (8.0 * g.pp * (3 + nth) + 16.0 * g.tp * nslot) * Ny);
-    hipLaunchKernelGGL((g.nx <= 512 ? k_xprod_all<double, true> : k_xprod_all<double, false>), grid, block, lds, h->stream, pa, g,
-                       reinterpret_cast<const cpx<double>*>(h->twx.p), 1);
+    Timed tm(h, DGPU_K_XPROD, (sizeof(R) * g.pp * (3 + nth) + 2.0 * sizeof(R) * g.tp * nslot) * Ny);
+    hipLaunchKernelGGL((g.nx <= 512 ? k_xprod_all<R, true> : k_xprod_all<R, false>), grid, block, lds, h->stream,
+                       pa, g, twx_of<R>(h), 1);
     HIPCHECK(hipGetLastError());
   }
 
@@ -845,6 +911,10 @@ void rk_substep(dgpu_handle* h, int rk_step, double time, int time_step) {
     a.cu2 = lin_cu[1];
     a.cu3 = lin_cu[2];
     a.cp = h->cp.p;
+    a.incr = !incr ? 0 : (les ? 1 : 2);
+    a.cub1 = h->cu[0].p;
+    a.cub2 = h->cu[1].p;
+    a.cub3 = h->cu[2].p;
     if (les) {
       a.lesf1 = h->lesf[0].p;
       a.lesf2 = h->lesf[1].p;
@@ -856,6 +926,7 @@ void rk_substep(dgpu_handle* h, int rk_step, double time, int time_step) {
     a.cf3 = h->cf[2].p;
     for (int n = 0; n < nth; ++n) {
       a.cth[n] = lin_cth[n];
+      a.cthb[n] = h->cth[n].p;
       a.cfth[n] = h->cfth[n].p;
       a.pr[n] = p.pr[n];
       a.dthdx[n] = p.dthdx[n];
@@ -891,23 +962,24 @@ void rk_substep(dgpu_handle* h, int rk_step, double time, int time_step) {
     if (ng < 1) ng = 1;
     if (ng > nslot) ng = nslot;
     const int rounds = (nslot + ng - 1) / ng;
-    const size_t lds = (size_t)rounds * ng * fft_lds_len(g.nz) * sizeof(double2);
+    const size_t lds = (size_t)rounds * ng * fft_lds_len(g.nz) * sizeof(cpx<R>);
     if (lds > h->lds_limit) throw ApiError{DGPU_EUNSUP, "Nz too large for the spectral RHS pass"};
     dim3 grid(g.nq, Ny), block(ng * G);
     // x-spectra of the products in, cr x-spectra out; cu1..3, cp, cth, cf1..3,
     // cfth read and cf1..3, cfth written (neighbour planes come from cache)
     Timed tm(h, DGPU_K_RHS,
-             (16.0 * g.tp * (nslot + ncr) + 16.0 * g.sp * (4 + 2 * nth + 3 + nth + 3 + nth)) * Ny);
-    hipLaunchKernelGGL((g.nz <= 512 ? k_rhs<true> : k_rhs<false>), grid, block, lds, h->stream, a, g, h->twz.p, 1);
+             (2.0 * sizeof(R) * g.tp * (nslot + ncr) + 16.0 * g.sp * (4 + 2 * nth + 3 + nth + 3 + nth)) * Ny);
+    hipLaunchKernelGGL((g.nz <= 512 ? k_rhs<R, true> : k_rhs<R, false>), grid, block, lds, h->stream, a, g,
+                       twz_of<R>(h), 1);
     HIPCHECK(hipGetLastError());
   }
 
   // 4. cr to physical space (second half of fft_xz_to_physical, :485-487, :621)
   // slot order of cr: 0 cr1, 1 cr3, 2 cr2, 3+n crth_n
-  launch_xinv(h, h->tpool[nslot + 0].p, h->r[0].p, 1, Ny);
-  launch_xinv(h, h->tpool[nslot + 1].p, h->r[2].p, 1, Ny);
-  launch_xinv(h, h->tpool[nslot + 2].p, h->r[1].p, 1, Ny);
-  for (int n = 0; n < nth; ++n) launch_xinv(h, h->tpool[nslot + 3 + n].p, h->rth[n].p, 1, Ny);
+  launch_xinv<R>(h, xspec<R>(h->tpool[nslot + 0]), phys<R>(h->r[0]), 1, Ny);
+  launch_xinv<R>(h, xspec<R>(h->tpool[nslot + 1]), phys<R>(h->r[2]), 1, Ny);
+  launch_xinv<R>(h, xspec<R>(h->tpool[nslot + 2]), phys<R>(h->r[1]), 1, Ny);
+  for (int n = 0; n < nth; ++n) launch_xinv<R>(h, xspec<R>(h->tpool[nslot + 3 + n]), phys<R>(h->rth[n]), 1, Ny);
 
   // 5. implicit vertical solves (channel.f90:656-877)
   double transient = h->w_transient;
@@ -918,29 +990,40 @@ void rk_substep(dgpu_handle* h, int rk_step, double time, int time_step) {
   if (p.use_les) {
     // with use_LES the eddy viscosity enters every CN row (channel.f90:519-547, 636-689)
     for (int n = 0; n < nth; ++n)
-      launch_thomas_les(h, 3 + n, rs, h->rth[n].p, h->th[n].p, h->kappa_t[n].p, h->jstart_th[n] - 1,
-                        h->jend_th[n] + 1, transient);
-    launch_thomas_les(h, 1, rs, h->r[1].p, h->u[1].p, h->nu_t.p, 1, Ny + 1, transient);
-    launch_thomas_les(h, 0, rs, h->r[0].p, h->u[0].p, h->nu_t.p, h->jstart - 1, h->jend + 1, transient);
-    launch_thomas_les(h, 2, rs, h->r[2].p, h->u[2].p, h->nu_t.p, h->jstart - 1, h->jend + 1, transient);
+      launch_thomas_les<R>(h, 3 + n, rs, phys<R>(h->rth[n]), phys<R>(h->th[n]), h->kappa_t[n].p,
+                           h->jstart_th[n] - 1, h->jend_th[n] + 1, transient, incr);
+    launch_thomas_les<R>(h, 1, rs, phys<R>(h->r[1]), phys<R>(h->u[1]), h->nu_t.p, 1, Ny + 1, transient, incr);
+    launch_thomas_les<R>(h, 0, rs, phys<R>(h->r[0]), phys<R>(h->u[0]), h->nu_t.p, h->jstart - 1, h->jend + 1,
+                         transient, incr);
+    launch_thomas_les<R>(h, 2, rs, phys<R>(h->r[2]), phys<R>(h->u[2]), h->nu_t.p, h->jstart - 1, h->jend + 1,
+                         transient, incr);
   } else {
     for (int n = 0; n < nth; ++n)
-      launch_thomas(h, 3 + n, rs, h->rth[n].p, h->th[n].p, h->jstart_th[n] - 1, h->jend_th[n] + 1,
-                    transient);
-    launch_thomas(h, 1, rs, h->r[1].p, h->u[1].p, 1, Ny + 1, transient);
-    launch_thomas(h, 0, rs, h->r[0].p, h->u[0].p, h->jstart - 1, h->jend + 1, transient);
-    launch_thomas(h, 2, rs, h->r[2].p, h->u[2].p, h->jstart - 1, h->jend + 1, transient);
+      launch_thomas<R>(h, 3 + n, rs, phys<R>(h->rth[n]), phys<R>(h->th[n]), h->jstart_th[n] - 1,
+                       h->jend_th[n] + 1, transient, incr);
+    launch_thomas<R>(h, 1, rs, phys<R>(h->r[1]), phys<R>(h->u[1]), 1, Ny + 1, transient, incr);
+    launch_thomas<R>(h, 0, rs, phys<R>(h->r[0]), phys<R>(h->u[0]), h->jstart - 1, h->jend + 1, transient, incr);
+    launch_thomas<R>(h, 2, rs, phys<R>(h->r[2]), phys<R>(h->u[2]), h->jstart - 1, h->jend + 1, transient, incr);
   }
 
-  // 6. back to Fourier space on the solved planes (channel.f90:891-896)
-  to_fourier(h, h->u[0].p, h->cu[0].p, h->tpool[0].p, h->jstart - 1, h->jend + 1);
-  to_fourier(h, h->u[1].p, h->cu[1].p, h->tpool[1].p, 1, Ny + 1);
-  to_fourier(h, h->u[2].p, h->cu[2].p, h->tpool[2].p, h->jstart - 1, h->jend + 1);
+  // 6. back to Fourier space on the solved planes (channel.f90:891-896); the
+  // mixed form adds the transformed increment to the FP64 state
+  to_fourier<R>(h, phys<R>(h->u[0]), h->cu[0].p, xspec<R>(h->tpool[0]), h->jstart - 1, h->jend + 1, incr);
+  to_fourier<R>(h, phys<R>(h->u[1]), h->cu[1].p, xspec<R>(h->tpool[1]), 1, Ny + 1, incr);
+  to_fourier<R>(h, phys<R>(h->u[2]), h->cu[2].p, xspec<R>(h->tpool[2]), h->jstart - 1, h->jend + 1, incr);
   for (int n = 0; n < nth; ++n)
-    to_fourier(h, h->th[n].p, h->cth[n].p, h->tpool[3 + n].p, h->jstart_th[n] - 1, h->jend_th[n] + 1);
+    to_fourier<R>(h, phys<R>(h->th[n]), h->cth[n].p, xspec<R>(h->tpool[3 + n]), h->jstart_th[n] - 1,
+                  h->jend_th[n] + 1, incr);
 
   // 7. projection and pressure (channel.f90:901-911)
   launch_remdiv(h, true, temp4);
+}
+
+void rk_substep(dgpu_handle* h, int rk_step, double time, int time_step) {
+  if (h->precision == DGPU_MIXED)
+    rk_substep_t<float>(h, rk_step, time, time_step);
+  else
+    rk_substep_t<double>(h, rk_step, time, time_step);
 }
 
 // -------------------------------------------------- reference layouts ----
@@ -1064,6 +1147,8 @@ void free_all(dgpu_handle* h) {
   h->sg.release();
   h->twx.release();
   h->twz.release();
+  h->twx32.release();
+  h->twz32.release();
   h->d_kx.release();
   h->d_kz.release();
   h->d_dy.release();
@@ -1085,6 +1170,7 @@ void free_all(dgpu_handle* h) {
     h->cth_pre[n].release();
   }
   h->sbl.release();
+  h->sgl.release();
 }
 
 void create_impl(dgpu_handle* h, const double* gy) {
@@ -1116,13 +1202,20 @@ void create_impl(dgpu_handle* h, const double* gy) {
   h->lds_limit = prop.sharedMemPerBlock;
   if (h->lds_limit > 65536) {
     const void* kernels[] = {
-        (const void*)k_rhs<true>,           (const void*)k_rhs<false>,
-        (const void*)k_xfwd<double, true>,  (const void*)k_xfwd<double, false>,
-        (const void*)k_xinv<double, true>,  (const void*)k_xinv<double, false>,
-        (const void*)k_zfwd<double, true>,  (const void*)k_zfwd<double, false>,
-        (const void*)k_zinv<double, true>,  (const void*)k_zinv<double, false>,
+        (const void*)k_rhs<double, true>,     (const void*)k_rhs<double, false>,
+        (const void*)k_xfwd<double, true>,    (const void*)k_xfwd<double, false>,
+        (const void*)k_xinv<double, true>,    (const void*)k_xinv<double, false>,
+        (const void*)k_zfwd<double, true>,    (const void*)k_zfwd<double, false>,
+        (const void*)k_zinv<double, true>,    (const void*)k_zinv<double, false>,
         (const void*)k_zinv_op<double, true>, (const void*)k_zinv_op<double, false>,
-        (const void*)k_xprod_all<double, true>, (const void*)k_xprod_all<double, false>};
+        (const void*)k_xprod_all<double, true>, (const void*)k_xprod_all<double, false>,
+        (const void*)k_rhs<float, true>,      (const void*)k_rhs<float, false>,
+        (const void*)k_xfwd<float, true>,     (const void*)k_xfwd<float, false>,
+        (const void*)k_xinv<float, true>,     (const void*)k_xinv<float, false>,
+        (const void*)k_zfwd<float, true>,     (const void*)k_zfwd<float, false>,
+        (const void*)k_zinv<float, true>,     (const void*)k_zinv<float, false>,
+        (const void*)k_zinv_op<float, true>,  (const void*)k_zinv_op<float, false>,
+        (const void*)k_xprod_all<float, true>, (const void*)k_xprod_all<float, false>};
     for (const void* k : kernels)
       HIPCHECK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds_limit));
   }
@@ -1163,10 +1256,19 @@ void create_impl(dgpu_handle* h, const double* gy) {
   HIPCHECK(hipMemset(h->nu_t.p, 0, sizeof(double) * np));
 
   std::vector<double2> w;
+  std::vector<float2> w32;
+  auto to32 = [&]() {
+    w32.resize(w.size());
+    for (size_t t = 0; t < w.size(); ++t) w32[t] = make_float2((float)w[t].x, (float)w[t].y);
+  };
   twiddles(g.nx, w);
   upload(h->twx, w);
+  to32();
+  upload(h->twx32, w32);
   twiddles(g.nz, w);
   upload(h->twz, w);
+  to32();
+  upload(h->twz32, w32);
   upload(h->d_kx, h->kx);
   upload(h->d_kz, h->kz);
   upload(h->d_dy, h->dy);
@@ -1213,6 +1315,7 @@ void create_impl(dgpu_handle* h, const double* gy) {
       h->cth_pre[nn].alloc(ns);
     }
     h->sbl.alloc(np);
+    h->sgl.alloc(np);
   }
   HIPCHECK(hipDeviceSynchronize());
 }
@@ -1249,7 +1352,8 @@ int dgpu_create(const dgpu_params* p, const double* gy, int precision, dgpu_hand
   *out = nullptr;
   std::string why;
   if (check_params(p, why)) return DGPU_EINVAL;
-  if (precision != DGPU_FP64) return DGPU_EUNSUP;
+  if (precision != DGPU_FP64 && precision != DGPU_MIXED) return DGPU_EINVAL;
+  if (precision == DGPU_MIXED && p->nz < XTile<float>::KT) return DGPU_EUNSUP;
   dgpu_handle* h = new (std::nothrow) dgpu_handle();
   if (!h) return DGPU_ENOMEM;
   h->p = *p;

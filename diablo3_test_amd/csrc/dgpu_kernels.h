@@ -16,7 +16,12 @@
 
 namespace dgpu {
 
-constexpr int KT = 8;  // rows (z lines) per workgroup in the x passes
+// Rows (z lines) per workgroup in the x passes: each workgroup reads and
+// writes its x-spectra as runs of KT complex values, one 128-byte line.
+template <typename R>
+struct XTile {
+  static constexpr int KT = 128 / (2 * (int)sizeof(R));
+};
 
 struct Geo {
   int nx, ny, nz;     // Nx, Ny, Nz
@@ -54,6 +59,7 @@ __device__ __forceinline__ void split_pair(const cpx<R>* f, int n, int q, R scal
 template <typename R, bool WAVE>
 __global__ void __launch_bounds__(512) k_xfwd(const R* __restrict__ u, cpx<R>* __restrict__ T,
                                                Geo g, const cpx<R>* __restrict__ tw, int j0) {
+  constexpr int KT = XTile<R>::KT;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<R>* smem = reinterpret_cast<cpx<R>*>(smem_raw);
   const int G = g.nx >> 3;
@@ -91,6 +97,7 @@ __global__ void __launch_bounds__(512) k_xfwd(const R* __restrict__ u, cpx<R>* _
 template <typename R, bool WAVE>
 __global__ void __launch_bounds__(512) k_xinv(const cpx<R>* __restrict__ T, R* __restrict__ u,
                                                Geo g, const cpx<R>* __restrict__ tw, int j0) {
+  constexpr int KT = XTile<R>::KT;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<R>* smem = reinterpret_cast<cpx<R>*>(smem_raw);
   const int G = g.nx >> 3;
@@ -133,10 +140,12 @@ __global__ void __launch_bounds__(512) k_xinv(const cpx<R>* __restrict__ T, R* _
 
 // ---------------------------------------------------------------- z pass --
 // Forward z transform + 1/Nz + pack (fft.f90:298-313).  One column (j, q) per
-// group; grid (ceil(nq/ngroups), nplanes), block ngroups*Nz/8.
+// group; grid (ceil(nq/ngroups), nplanes), block ngroups*Nz/8.  The spectral
+// array is FP64 whatever the transform precision R; with accum the result is
+// added to it (the FP32 transform of an increment of an FP64 state).
 template <typename R, bool WAVE>
-__global__ void __launch_bounds__(512) k_zfwd(const cpx<R>* __restrict__ T, cpx<R>* __restrict__ C,
-                                               Geo g, const cpx<R>* __restrict__ tw, int j0) {
+__global__ void __launch_bounds__(512) k_zfwd(const cpx<R>* __restrict__ T, cpx<double>* __restrict__ C,
+                                               Geo g, const cpx<R>* __restrict__ tw, int j0, int accum) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<R>* smem = reinterpret_cast<cpx<R>*>(smem_raw);
   const int G = g.nz >> 3;
@@ -157,14 +166,19 @@ __global__ void __launch_bounds__(512) k_zfwd(const cpx<R>* __restrict__ T, cpx<
   fft_regs<R, -1, WAVE>(v, fbuf, g.nz, g.lognz, tw, gt);
   if (act) {
     const R inv = (R)1 / (R)g.nz;
-    cpx<R>* out = C + (long long)j * g.sp + (long long)q * g.nkk;
-    for (int kk = gt; kk < g.nkk; kk += G) out[kk] = inv * fbuf[pidx(kz_nat(g, kk))];
+    cpx<double>* out = C + (long long)j * g.sp + (long long)q * g.nkk;
+    for (int kk = gt; kk < g.nkk; kk += G) {
+      const cpx<R> z = inv * fbuf[pidx(kz_nat(g, kk))];
+      cpx<double> zd = {(double)z.x, (double)z.y};
+      if (accum) zd = out[kk] + zd;
+      out[kk] = zd;
+    }
   }
 }
 
 // Unpack + zero (fft.f90:330-341) + backward z transform (:345), unnormalised.
 template <typename R, bool WAVE>
-__global__ void __launch_bounds__(512) k_zinv(const cpx<R>* __restrict__ C, cpx<R>* __restrict__ T,
+__global__ void __launch_bounds__(512) k_zinv(const cpx<double>* __restrict__ C, cpx<R>* __restrict__ T,
                                                Geo g, const cpx<R>* __restrict__ tw, int j0) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<R>* smem = reinterpret_cast<cpx<R>*>(smem_raw);
@@ -175,19 +189,19 @@ __global__ void __launch_bounds__(512) k_zinv(const cpx<R>* __restrict__ C, cpx<
   const int j = j0 + blockIdx.y, q = blockIdx.x * ngroups + grp;
   const bool act = q < g.nq;
   cpx<R> v[8];
-  const cpx<R>* col = C + (long long)j * g.sp + (long long)(act ? q : 0) * g.nkk;
+  const cpx<double>* col = C + (long long)j * g.sp + (long long)(act ? q : 0) * g.nkk;
   const int off = g.nz - 2 * g.nkz - 1;
 #pragma unroll
   for (int r = 0; r < 8; ++r) {
     const int k = gt + G * r;
-    cpx<R> z = {(R)0, (R)0};
+    cpx<double> z = {0.0, 0.0};
     if (act) {
       if (k <= g.nkz)
         z = col[k];
       else if (k >= g.nz - g.nkz)
         z = col[k - off];
     }
-    v[r] = z;
+    v[r] = {(R)z.x, (R)z.y};
   }
   fft_regs<R, 1, WAVE>(v, fbuf, g.nz, g.lognz, tw, gt);
   if (act) {
@@ -212,10 +226,9 @@ constexpr int MAXTH = 4;
 constexpr int MAXSLOT = 6 + 2 * MAXTH;
 
 struct ProdArgs {
-  const double* u1;
-  const double* u2;
-  const double* u3;
-  const double* th;  // scalar of pairs 4 and 5 (set per scalar by the kernel)
+  const void* u1;    // physical fields in the transform precision R
+  const void* u2;
+  const void* u3;
   const double* dy;
   const double* dyf;
   const double* kx;  // kx[q]
@@ -228,54 +241,11 @@ struct ProdArgs {
 // scalar n: pair 4 -> slot 6+2n, pair 5 -> slot 7+2n.
 struct ProdAllArgs {
   ProdArgs base;
-  const double* th[MAXTH];
+  const void* th[MAXTH];
   int jst[MAXTH], jet[MAXTH];
   int nth;
   double2* out[MAXSLOT];
 };
-
-template <typename R>
-__device__ __forceinline__ void prod_pair(const ProdArgs& a, const Geo& g, int pair, int j,
-                                          long long o, bool inA, bool inB, R& pa, R& pb) {
-  const long long up = o + g.pp, dn = o - g.pp;
-  double va = 0.0, vb = 0.0;
-  switch (pair) {
-    case 0:
-      if (inA) va = a.u1[o] * a.u1[o];
-      if (inB)
-        vb = (a.u1[up] * a.u2[up] + a.u1[o] * a.u2[up] - a.u1[o] * a.u2[o] - a.u1[dn] * a.u2[o]) /
-             (2.0 * a.dyf[j]);
-      break;
-    case 1:
-      if (inA) va = a.u3[o] * a.u1[o];
-      if (inB)
-        vb = (a.u3[up] * a.u2[up] + a.u3[o] * a.u2[up] - a.u3[o] * a.u2[o] - a.u3[dn] * a.u2[o]) /
-             (2.0 * a.dyf[j]);
-      break;
-    case 2:
-      if (inA) va = a.u3[o] * a.u3[o];
-      if (inB) vb = ((a.dyf[j] * a.u3[o] + a.dyf[j - 1] * a.u3[dn]) / (2.0 * a.dy[j])) * a.u2[o];
-      break;
-    case 3:
-      if (inA) va = ((a.dyf[j] * a.u1[o] + a.dyf[j - 1] * a.u1[dn]) / (2.0 * a.dy[j])) * a.u2[o];
-      if (inB) {
-        double s = a.u2[o] + a.u2[up], t = a.u2[o] + a.u2[dn];
-        vb = (0.25 * (s * s) - 0.25 * (t * t)) / a.dy[j];
-      }
-      break;
-    case 4:
-      if (inA) va = a.th[o] * a.u1[o];
-      if (inB)
-        vb = (a.th[up] * a.u2[up] + a.th[o] * a.u2[up] - a.th[o] * a.u2[o] - a.th[dn] * a.u2[o]) /
-             (2.0 * a.dyf[j]);
-      break;
-    default:
-      if (inA) va = a.th[o] * a.u3[o];
-      break;
-  }
-  pa = (R)va;
-  pb = (R)vb;
-}
 
 __device__ __forceinline__ void pair_ranges(const ProdArgs& a, const Geo& g, int pair, int j,
                                             bool& inA, bool& inB) {
@@ -321,6 +291,7 @@ __device__ __forceinline__ void emit_pair(cpx<R>* smem, int L, const Geo& g, int
 template <typename R, bool WAVE>
 __global__ void __launch_bounds__(512) k_xprod_all(ProdAllArgs a, Geo g,
                                                     const cpx<R>* __restrict__ tw, int j0) {
+  constexpr int KT = XTile<R>::KT;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<R>* smem = reinterpret_cast<cpx<R>*>(smem_raw);
   const int G = g.nx >> 3;  // 8 values of a row per thread
@@ -331,18 +302,22 @@ __global__ void __launch_bounds__(512) k_xprod_all(ProdAllArgs a, Geo g,
   const int j = j0 + blockIdx.y, k0 = blockIdx.x * KT;
   const R inv = (R)1 / (R)g.nx;
   const ProdArgs& b = a.base;
-  const double dyj = b.dy[j], dyfj = b.dyf[j], dyfm = b.dyf[j - 1];
-  const double c2dyf = 1.0 / (2.0 * dyfj);
+  const R* U1 = reinterpret_cast<const R*>(b.u1);
+  const R* U2 = reinterpret_cast<const R*>(b.u2);
+  const R* U3 = reinterpret_cast<const R*>(b.u3);
+  const R dyj = (R)b.dy[j], dyfj = (R)b.dyf[j], dyfm = (R)b.dyf[j - 1];
+  const R c2dyf = (R)1 / ((R)2 * dyfj);
+  const R quarter = (R)0.25, zero = (R)0;
   const long long pp = g.pp;
   auto outp = [&](int s) {
     return reinterpret_cast<cpx<R>*>(a.out[s]) + (long long)j * g.tp + k0;
   };
   // transform one product pair held as this thread's 8 values, then make
   // every group's spectra visible to the cross-group writes of emit_pair
-  auto fft_pair = [&](const double* va, const double* vb) {
+  auto fft_pair = [&](const R* va, const R* vb) {
     cpx<R> v[8];
 #pragma unroll
-    for (int r = 0; r < 8; ++r) v[r] = {(R)va[r], (R)vb[r]};
+    for (int r = 0; r < 8; ++r) v[r] = {va[r], vb[r]};
     fft_regs<R, -1, WAVE>(v, fbuf, g.nx, g.lognx, tw, gt);
     __syncthreads();
   };
@@ -354,17 +329,17 @@ __global__ void __launch_bounds__(512) k_xprod_all(ProdAllArgs a, Geo g,
   }
   for (int m0 = 0; m0 < KT; m0 += ngroups) {
     const long long rowc = (long long)j * pp + (long long)(k0 + m0 + grp) * g.nx + gt;
-    double u1m[8], u1c[8], u1p[8], u2m[8], u2c[8], u2p[8];
-    double va[8], vb[8];
+    R u1m[8], u1c[8], u1p[8], u2m[8], u2c[8], u2p[8];
+    R va[8], vb[8];
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
       const long long o = rowc + G * r;
-      u1m[r] = b.u1[o - pp];
-      u1c[r] = b.u1[o];
-      u1p[r] = b.u1[o + pp];
-      u2m[r] = b.u2[o - pp];
-      u2c[r] = b.u2[o];
-      u2p[r] = b.u2[o + pp];
+      u1m[r] = U1[o - pp];
+      u1c[r] = U1[o];
+      u1p[r] = U1[o + pp];
+      u2m[r] = U2[o - pp];
+      u2c[r] = U2[o];
+      u2p[r] = U2[o + pp];
     }
     // pair 0: (u1u1, V1) -> slot 0 = ikx F(u1u1) + F(V1)              (channel.f90:319-325, 399-407)
     if (r1) {
@@ -381,24 +356,24 @@ __global__ void __launch_bounds__(512) k_xprod_all(ProdAllArgs a, Geo g,
     if (r2) {
 #pragma unroll
       for (int r = 0; r < 8; ++r) {
-        va[r] = ((dyfj * u1c[r] + dyfm * u1m[r]) / (2.0 * dyj)) * u2c[r];
-        const double sp = u2c[r] + u2p[r], sm = u2c[r] + u2m[r];
-        vb[r] = (0.25 * (sp * sp) - 0.25 * (sm * sm)) / dyj;
+        va[r] = ((dyfj * u1c[r] + dyfm * u1m[r]) / ((R)2 * dyj)) * u2c[r];
+        const R sp = u2c[r] + u2p[r], sm = u2c[r] + u2m[r];
+        vb[r] = (quarter * (sp * sp) - quarter * (sm * sm)) / dyj;
       }
       fft_pair(va, vb);
       emit_pair(smem, L, g, ngroups, 3, inv, b.kx, outp(4), (cpx<R>*)nullptr, m0);
       __syncthreads();
     }
     // u3 at j-1, j, j+1 replaces u1 at j-1, j+1 and u2 at j-1
-    double* u3m = u1m;
-    double* u3p = u1p;
-    double* u3c = u2m;
+    R* u3m = u1m;
+    R* u3p = u1p;
+    R* u3c = u2m;
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
       const long long o = rowc + G * r;
-      u3m[r] = b.u3[o - pp];
-      u3c[r] = b.u3[o];
-      u3p[r] = b.u3[o + pp];
+      u3m[r] = U3[o - pp];
+      u3c[r] = U3[o];
+      u3p[r] = U3[o + pp];
     }
     // pair 1: (u3u1, V3) -> slot 1 = F(u3u1), slot 2 = ikx F(u3u1) + F(V3)   (:298-305, 419-428)
     if (r1) {
@@ -415,8 +390,8 @@ __global__ void __launch_bounds__(512) k_xprod_all(ProdAllArgs a, Geo g,
     if (r1 || r2) {
 #pragma unroll
       for (int r = 0; r < 8; ++r) {
-        va[r] = r1 ? u3c[r] * u3c[r] : 0.0;
-        vb[r] = r2 ? ((dyfj * u3c[r] + dyfm * u3m[r]) / (2.0 * dyj)) * u2c[r] : 0.0;
+        va[r] = r1 ? u3c[r] * u3c[r] : zero;
+        vb[r] = r2 ? ((dyfj * u3c[r] + dyfm * u3m[r]) / ((R)2 * dyj)) * u2c[r] : zero;
       }
       fft_pair(va, vb);
       emit_pair(smem, L, g, ngroups, 2, inv, b.kx, outp(3), outp(5), m0);
@@ -425,8 +400,8 @@ __global__ void __launch_bounds__(512) k_xprod_all(ProdAllArgs a, Geo g,
     // scalars: (th u1, Vt) -> slot 6+2n, (th u3, 0) -> slot 7+2n       (:557-600)
     for (int n = 0; n < a.nth; ++n) {
       if (j < a.jst[n] || j > a.jet[n]) continue;  // uniform across the block
-      const double* th = a.th[n];
-      double thm[8], thc[8], thp[8];
+      const R* th = reinterpret_cast<const R*>(a.th[n]);
+      R thm[8], thc[8], thp[8];
 #pragma unroll
       for (int r = 0; r < 8; ++r) {
         const long long o = rowc + G * r;
@@ -445,7 +420,7 @@ __global__ void __launch_bounds__(512) k_xprod_all(ProdAllArgs a, Geo g,
 #pragma unroll
       for (int r = 0; r < 8; ++r) {
         va[r] = thc[r] * u3c[r];
-        vb[r] = 0.0;
+        vb[r] = zero;
       }
       fft_pair(va, vb);
       emit_pair(smem, L, g, ngroups, 5, inv, b.kx, outp(7 + 2 * n), (cpx<R>*)nullptr, m0);
@@ -462,13 +437,20 @@ __global__ void __launch_bounds__(512) k_xprod_all(ProdAllArgs a, Geo g,
 // :621).  Slots: 0 X1, 1 X13, 2 X3, 3 X33, 4 X2, 5 X32, 6+2n Xt_n, 7+2n Xt3_n.
 // cr goes to slot 0 (cr1), 1 (cr3), 2 (cr2), 3+n (crth_n) once read.
 struct RhsArgs {
-  const double2* X[MAXSLOT];      // x-spectra of the products
-  double2* Tcr[3 + MAXTH];        // out: x-spectra of cr1, cr3, cr2, crth_n (slot order)
-  const double2* cu1;
+  const void* X[MAXSLOT];         // x-spectra of the products (transform precision)
+  void* Tcr[3 + MAXTH];           // out: x-spectra of cr1, cr3, cr2, crth_n (slot order)
+  const double2* cu1;             // state the explicit terms read
   const double2* cu2;
   const double2* cu3;
   const double2* cp;
   const double2* cth[MAXTH];
+  // incr: cr is written less the state that was transformed to physical space
+  // (cub*), so that only the O(h) increment passes through the transform
+  int incr;  // 0: cr; 1: cr less cub*; 2: cr less the state read (cub* == cu*)
+  const double2* cub1;
+  const double2* cub2;
+  const double2* cub3;
+  const double2* cthb[MAXTH];
   double2* cf1;
   double2* cf2;
   double2* cf3;
@@ -490,13 +472,13 @@ struct RhsArgs {
   int jst[MAXTH], jet[MAXTH];
 };
 
-template <bool WAVE>
-__global__ void __launch_bounds__(512) k_rhs(RhsArgs a, Geo g, const double2* __restrict__ twd,
+template <typename R, bool WAVE>
+__global__ void __launch_bounds__(512) k_rhs(RhsArgs a, Geo g, const cpx<R>* __restrict__ tw,
                                               int j0) {
   typedef cpx<double> C;
+  typedef cpx<R> CR;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  C* smem = reinterpret_cast<C*>(smem_raw);
-  const C* tw = reinterpret_cast<const C*>(twd);
+  CR* smem = reinterpret_cast<CR*>(smem_raw);
   const int G = g.nz >> 3;
   const int ngroups = blockDim.x / G;
   const int grp = threadIdx.x / G, gt = threadIdx.x % G;
@@ -519,17 +501,17 @@ __global__ void __launch_bounds__(512) k_rhs(RhsArgs a, Geo g, const double2* __
   for (int s0 = 0; s0 < nslot; s0 += ngroups) {
     const int s = s0 + grp;
     const bool act = s < nslot && slot_used(s);
-    C* buf = smem + (long long)s * L;
-    C v[8];
+    CR* buf = smem + (long long)s * L;
+    CR v[8];
     if (act) {
-      const C* col = reinterpret_cast<const C*>(a.X[s]) + to;
+      const CR* col = reinterpret_cast<const CR*>(a.X[s]) + to;
 #pragma unroll
       for (int r = 0; r < 8; ++r) v[r] = col[gt + G * r];
     } else {
 #pragma unroll
-      for (int r = 0; r < 8; ++r) v[r] = C{0.0, 0.0};
+      for (int r = 0; r < 8; ++r) v[r] = CR{(R)0, (R)0};
     }
-    fft_regs<double, -1, WAVE>(v, buf, g.nz, g.lognz, tw, gt);
+    fft_regs<R, -1, WAVE>(v, buf, g.nz, g.lognz, tw, gt);
   }
   __syncthreads();  // the combine below reads every slot
   // --- explicit terms, mode by mode
@@ -546,7 +528,12 @@ __global__ void __launch_bounds__(512) k_rhs(RhsArgs a, Geo g, const double2* __
     // slot s at this mode; a slot is read before the cr slot that reuses it
     // is written (cr slot 3+n < 6+2n, and slots 0..3 belong to u1/u3)
     const int pn = pidx(nat);
-    auto Z = [&](int s) -> C { return invnz * smem[(long long)s * L + pn]; };
+    auto Z = [&](int s) -> C {
+      const CR z = smem[(long long)s * L + pn];
+      return invnz * C{(double)z.x, (double)z.y};
+    };
+    auto put = [&](int s, C v) { smem[(long long)s * L + pn] = CR{(R)v.x, (R)v.y}; };
+    auto ld = [&](const double2* p) -> C { return reinterpret_cast<const C*>(p)[o]; };
     C cu1 = {0, 0}, cu3 = {0, 0};
     if (in1 || a.nth > 0) {
       cu1 = reinterpret_cast<const C*>(a.cu1)[o];
@@ -568,6 +555,13 @@ __global__ void __launch_bounds__(512) k_rhs(RhsArgs a, Geo g, const double2* __
       f1 = f1 - (Z(0) + ik(kzk, Z(1)));
       f3 = f3 - (Z(2) + ik(kzk, Z(3)));
       C r1 = cu1, r3 = cu3;
+      if (a.incr == 1) {
+        r1 = r1 - ld(a.cub1);
+        r3 = r3 - ld(a.cub3);
+      } else if (a.incr == 2) {
+        r1 = C{0.0, 0.0};
+        r3 = C{0.0, 0.0};
+      }
       if (a.rk_step > 1) {
         r1 = r1 + a.temp3 * reinterpret_cast<const C*>(a.cf1)[o];
         r3 = r3 + a.temp3 * reinterpret_cast<const C*>(a.cf3)[o];
@@ -579,8 +573,8 @@ __global__ void __launch_bounds__(512) k_rhs(RhsArgs a, Geo g, const double2* __
       r3 = r3 + a.temp5 * f3;
       reinterpret_cast<C*>(a.cf1)[o] = f1;
       reinterpret_cast<C*>(a.cf3)[o] = f3;
-      smem[0 * L + pn] = r1;
-      smem[1 * L + pn] = r3;
+      put(0, r1);
+      put(1, r3);
     }
     if (in2) {
       C cu2 = reinterpret_cast<const C*>(a.cu2)[o];
@@ -597,11 +591,13 @@ __global__ void __launch_bounds__(512) k_rhs(RhsArgs a, Geo g, const double2* __
       if (a.lesf2) f2 = f2 + reinterpret_cast<const C*>(a.lesf2)[o];
       f2 = f2 - (Z(4) + ik(kzk, Z(5)));
       C r2 = cu2;
+      if (a.incr == 1) r2 = r2 - ld(a.cub2);
+      if (a.incr == 2) r2 = C{0.0, 0.0};
       if (a.rk_step > 1) r2 = r2 + a.temp3 * reinterpret_cast<const C*>(a.cf2)[o];
       r2 = r2 - (a.temp4 / cdy) * (cpj - cpm);
       r2 = r2 + a.temp5 * f2;
       reinterpret_cast<C*>(a.cf2)[o] = f2;
-      smem[2 * L + pn] = r2;
+      put(2, r2);
     }
     for (int n = 0; n < a.nth; ++n) {
       if (!inT[n]) continue;
@@ -611,17 +607,19 @@ __global__ void __launch_bounds__(512) k_rhs(RhsArgs a, Geo g, const double2* __
       if (a.lesfth[n]) ft = ft + reinterpret_cast<const C*>(a.lesfth[n])[o];
       ft = ft - (Z(6 + 2 * n) + ik(kzk, Z(7 + 2 * n)));
       C rt = th;
+      if (a.incr == 1) rt = rt - ld(a.cthb[n]);
+      if (a.incr == 2) rt = C{0.0, 0.0};
       if (a.rk_step > 1) rt = rt + a.temp3 * reinterpret_cast<const C*>(a.cfth[n])[o];
       rt = rt + a.temp5 * ft;
       reinterpret_cast<C*>(a.cfth[n])[o] = ft;
-      smem[(3 + n) * L + pn] = rt;
+      put(3 + n, rt);
     }
   }
   // zero the dealiased gap of the cr slots (fft.f90:336-340)
   const int ncr = 3 + a.nth;
   for (int c = 0; c < ncr; ++c)
     for (int k = g.nkz + 1 + threadIdx.x; k < g.nz - g.nkz; k += blockDim.x)
-      smem[c * L + pidx(k)] = C{0.0, 0.0};
+      smem[c * L + pidx(k)] = CR{(R)0, (R)0};
   // --- backward z transform of cr
   auto cr_used = [&](int c) -> bool {
     if (c < 2) return in1;
@@ -631,14 +629,14 @@ __global__ void __launch_bounds__(512) k_rhs(RhsArgs a, Geo g, const double2* __
   for (int c0 = 0; c0 < ncr; c0 += ngroups) {
     const int c = c0 + grp;
     const bool act = c < ncr && cr_used(c);
-    C* buf = smem + (long long)c * L;
+    CR* buf = smem + (long long)c * L;
     __syncthreads();  // the cr slots were written by every thread of the block
-    C v[8];
+    CR v[8];
 #pragma unroll
     for (int r = 0; r < 8; ++r) v[r] = buf[pidx(gt + G * r)];
-    fft_regs<double, 1, WAVE>(v, buf, g.nz, g.lognz, tw, gt);
+    fft_regs<R, 1, WAVE>(v, buf, g.nz, g.lognz, tw, gt);
     if (act) {
-      C* out = reinterpret_cast<C*>(a.Tcr[c]) + to;
+      CR* out = reinterpret_cast<CR*>(a.Tcr[c]) + to;
       for (int k = gt; k < g.nz; k += G) st_out(out + k, buf[pidx(k)]);
     }
   }
@@ -663,11 +661,17 @@ struct SegRow {
   int kind;    // 1 = assembled row (vec = r + explicit), 0 = fixed row (vec = vb),
                // 2 = fixed row with vec = vb * (current time-dependent BC value)
   double vb;   // fixed-row right-hand side
+  double ra, rb, rc;  // the matrix row itself (used for fixed rows by the increment form)
 };
 
-template <int L>
-__global__ void __launch_bounds__(1024) k_thomas_seg(const double* __restrict__ r,
-                                                     double* __restrict__ u,
+// T is the storage type of r and u (the arithmetic is FP64).  INCR is the
+// increment form: r holds the right-hand side less the state (RHS pass with
+// incr), and the solve is for d = x - u with the same factors, A d = vec - A u:
+// assembled rows (A = I - E, vec = r + u + E u) give r + 2 E u, fixed rows
+// vec - (a u(j-1) + b u(j) + c u(j+1)).  u is overwritten with d.
+template <int L, typename T, bool INCR>
+__global__ void __launch_bounds__(1024) k_thomas_seg(const T* __restrict__ r,
+                                                     T* __restrict__ u,
                                                      const SegRow* __restrict__ rows, int n, int S,
                                                      long long pp, int ca, int cb, double kscale) {
   __shared__ double carry_f[16][64];
@@ -682,7 +686,7 @@ __global__ void __launch_bounds__(1024) k_thomas_seg(const double* __restrict__ 
 #pragma unroll
   for (int t = 0; t < L + 2; ++t) {
     const int jj = s0 - 1 + t;
-    w[t] = (jj >= 0 && jj < n) ? u[(long long)jj * pp + col] : 0.0;
+    w[t] = (jj >= 0 && jj < n) ? (double)u[(long long)jj * pp + col] : 0.0;
   }
 #pragma unroll
   for (int t = 0; t < L; ++t) {
@@ -690,12 +694,12 @@ __global__ void __launch_bounds__(1024) k_thomas_seg(const double* __restrict__ 
     double val = 0.0;
     if (jj < n) {
       const SegRow rw = rows[jj];
+      const double ex = rw.wp * (w[t + 2] - w[t + 1]) - rw.wm * (w[t + 1] - w[t]);
       if (rw.kind == 1)
-        val = r[(long long)jj * pp + col] + rw.wp * (w[t + 2] - w[t + 1]) - rw.wm * (w[t + 1] - w[t]);
-      else if (rw.kind == 2)
-        val = rw.vb * kscale;
+        val = (double)r[(long long)jj * pp + col] + (INCR ? 2.0 * ex : ex);
       else
-        val = rw.vb;
+        val = (rw.kind == 2) ? rw.vb * kscale : rw.vb;
+      if (INCR && rw.kind != 1) val -= rw.ra * w[t] + rw.rb * w[t + 1] + rw.rc * w[t + 2];
     }
     v[t] = val;
   }
@@ -745,7 +749,8 @@ __global__ void __launch_bounds__(1024) k_thomas_seg(const double* __restrict__ 
 #pragma unroll
   for (int t = 0; t < L; ++t) {
     const int jj = s0 + t;
-    if (jj < n && jj >= ca && jj <= cb) u[(long long)jj * pp + col] = v[t] + rows[jj].qf * xin;
+    if (jj < n && jj >= ca && jj <= cb)
+      u[(long long)jj * pp + col] = (T)(v[t] + rows[jj].qf * xin);
   }
 }
 
@@ -769,6 +774,11 @@ struct RemDivArgs {
   int js, je;
 };
 
+// The sweeps run over batches of RB rows: a batch's loads are issued
+// together before its part of the recurrence, so each lane keeps several rows
+// in flight (one lane per mode gives fewer than one wavefront per SIMD).
+constexpr int RB = 8;
+
 __global__ void __launch_bounds__(256) k_remdiv(RemDivArgs a, Geo g) {
   typedef cpx<double> C;
   const long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -778,6 +788,7 @@ __global__ void __launch_bounds__(256) k_remdiv(RemDivArgs a, Geo g) {
   const double lam = kxq * kxq + kzk * kzk;
   const bool mean = (q == 0 && kk == 0);
   const int ny = g.ny;
+  const long long sp = g.sp;
   C* cu1 = reinterpret_cast<C*>(a.cu1);
   C* cu2 = reinterpret_cast<C*>(a.cu2);
   C* cu3 = reinterpret_cast<C*>(a.cu3);
@@ -787,61 +798,92 @@ __global__ void __launch_bounds__(256) k_remdiv(RemDivArgs a, Geo g) {
   double bprev = 1.0;
   double muprev = mean ? 0.0 : -1.0;
   C gprev = {0.0, 0.0};
-  a.sb[g.sp + m] = bprev;
-  sg[g.sp + m] = gprev;
-  C cu2j = cu2[(long long)2 * g.sp + m];  // cu2 at row j+1 for j = 1
-  for (int j = 2; j <= ny; ++j) {
-    const long long o = (long long)j * g.sp + m;
-    double ml, md, mu;
-    C rhs;
-    if (j < ny) {
-      ml = 1.0 / (a.dy[j] * a.dyf[j]);
-      md = -lam - 1.0 / (a.dy[j + 1] * a.dyf[j]) - 1.0 / (a.dy[j] * a.dyf[j]);
-      mu = 1.0 / (a.dy[j + 1] * a.dyf[j]);
-      C cu2n = cu2[o + g.sp];
-      rhs = ik(kxq, cu1[o]) + (1.0 / a.dyf[j]) * (cu2n - cu2j) + ik(kzk, cu3[o]);
-      cu2j = cu2n;
-    } else {
-      ml = 1.0;
-      md = -1.0;
-      mu = 0.0;
-      rhs = C{0.0, 0.0};
+  a.sb[sp + m] = bprev;
+  sg[sp + m] = gprev;
+  C cu2j = cu2[2 * sp + m];  // cu2 at row j+1 for j = 1
+  // forward sweep over rows 2..ny; row ny is phi(ny) = phi(ny-1)
+  for (int j0 = 2; j0 <= ny; j0 += RB) {
+    C d1[RB], d3[RB], d2[RB];
+#pragma unroll
+    for (int t = 0; t < RB; ++t) {
+      const int j = j0 + t;
+      const long long o = (long long)j * sp + m;
+      if (j < ny) {
+        d1[t] = cu1[o];
+        d3[t] = cu3[o];
+        d2[t] = cu2[o + sp];
+      }
     }
-    const double mlt = -ml / bprev;
-    const double b = md + mlt * muprev;
-    const C gg = rhs + mlt * gprev;
-    a.sb[o] = b;
-    sg[o] = gg;
-    bprev = b;
-    gprev = gg;
-    muprev = mu;
+#pragma unroll
+    for (int t = 0; t < RB; ++t) {
+      const int j = j0 + t;
+      if (j > ny) break;
+      const long long o = (long long)j * sp + m;
+      double ml, md, mu;
+      C rhs;
+      if (j < ny) {
+        ml = 1.0 / (a.dy[j] * a.dyf[j]);
+        md = -lam - 1.0 / (a.dy[j + 1] * a.dyf[j]) - 1.0 / (a.dy[j] * a.dyf[j]);
+        mu = 1.0 / (a.dy[j + 1] * a.dyf[j]);
+        rhs = ik(kxq, d1[t]) + (1.0 / a.dyf[j]) * (d2[t] - cu2j) + ik(kzk, d3[t]);
+        cu2j = d2[t];
+      } else {
+        ml = 1.0;
+        md = -1.0;
+        mu = 0.0;
+        rhs = C{0.0, 0.0};
+      }
+      const double mlt = -ml / bprev;
+      const double b = md + mlt * muprev;
+      const C gg = rhs + mlt * gprev;
+      a.sb[o] = b;
+      sg[o] = gg;
+      bprev = b;
+      gprev = gg;
+      muprev = mu;
+    }
   }
-  // backward sweep + projection + pressure
+  // backward sweep + projection + pressure, rows ny..1 in batches
   C phi_up = {0.0, 0.0};
-  for (int j = ny; j >= 1; --j) {
-    const long long o = (long long)j * g.sp + m;
-    double mu;
-    if (j == ny)
-      mu = 0.0;
-    else if (j == 1)
-      mu = mean ? 0.0 : -1.0;
-    else
-      mu = 1.0 / (a.dy[j + 1] * a.dyf[j]);
-    const C gg = sg[o];
-    const double b = a.sb[o];
-    C phi;
-    if (j == ny)
-      phi = (1.0 / b) * gg;
-    else
-      phi = (1.0 / b) * (gg - mu * phi_up);
-    cu1[o] = cu1[o] - ik(kxq, phi);
-    cu3[o] = cu3[o] - ik(kzk, phi);
-    if (j + 1 <= ny && j + 1 >= 2) {
-      const long long ou = o + g.sp;
-      cu2[ou] = cu2[ou] - (1.0 / a.dy[j + 1]) * (phi_up - phi);
+  for (int j0 = ny; j0 >= 1; j0 -= RB) {
+    C gg[RB], v1[RB], v3[RB], v2[RB], vp[RB];
+    double bb[RB];
+#pragma unroll
+    for (int t = 0; t < RB; ++t) {
+      const int j = j0 - t;
+      const long long o = (long long)j * sp + m;
+      if (j >= 1) {
+        gg[t] = sg[o];
+        bb[t] = a.sb[o];
+        v1[t] = cu1[o];
+        v3[t] = cu3[o];
+        if (j + 1 <= ny && j + 1 >= 2) v2[t] = cu2[o + sp];
+        if (a.update_cp && j >= a.js && j <= a.je) vp[t] = cp[o];
+      }
     }
-    if (a.update_cp && j >= a.js && j <= a.je) cp[o] = cp[o] + a.inv_h * phi;
-    phi_up = phi;
+#pragma unroll
+    for (int t = 0; t < RB; ++t) {
+      const int j = j0 - t;
+      if (j < 1) break;
+      const long long o = (long long)j * sp + m;
+      double mu;
+      if (j == ny)
+        mu = 0.0;
+      else if (j == 1)
+        mu = mean ? 0.0 : -1.0;
+      else
+        mu = 1.0 / (a.dy[j + 1] * a.dyf[j]);
+      C phi;
+      if (j == ny)
+        phi = (1.0 / bb[t]) * gg[t];
+      else
+        phi = (1.0 / bb[t]) * (gg[t] - mu * phi_up);
+      cu1[o] = v1[t] - ik(kxq, phi);
+      cu3[o] = v3[t] - ik(kzk, phi);
+      if (j + 1 <= ny && j + 1 >= 2) cu2[o + sp] = v2[t] - (1.0 / a.dy[j + 1]) * (phi_up - phi);
+      if (a.update_cp && j >= a.js && j <= a.je) cp[o] = vp[t] + a.inv_h * phi;
+      phi_up = phi;
+    }
   }
 }
 

@@ -31,7 +31,7 @@ enum DerivOp { OP_ID = 0, OP_DX = 1, OP_DZ = 2, OP_DY_GY = 3, OP_DY_GYF = 4 };
 // where the derivative is not defined (j = 0 for OP_DY_GY, j = Ny+1 for
 // OP_DY_GYF) are written as zero.
 template <typename R, bool WAVE>
-__global__ void __launch_bounds__(512) k_zinv_op(const cpx<R>* __restrict__ C, cpx<R>* __restrict__ T,
+__global__ void __launch_bounds__(512) k_zinv_op(const cpx<double>* __restrict__ C, cpx<R>* __restrict__ T,
                                                  Geo g, const cpx<R>* __restrict__ tw, int j0, int op,
                                                  const double* __restrict__ kx,
                                                  const double* __restrict__ kz,
@@ -48,33 +48,33 @@ __global__ void __launch_bounds__(512) k_zinv_op(const cpx<R>* __restrict__ C, c
   if (op == OP_DY_GY && j == 0) act = false;
   if (op == OP_DY_GYF && j == g.ny + 1) act = false;
   const long long base = (long long)j * g.sp + (long long)(q < g.nq ? q : 0) * g.nkk;
-  const cpx<R>* col = C + base;
-  const R kxq = (R)kx[q < g.nq ? q : 0];
-  R sdy = (R)0;
-  if (act && op == OP_DY_GY) sdy = (R)(1.0 / dy[j]);
-  if (act && op == OP_DY_GYF) sdy = (R)(1.0 / dyf[j]);
+  const cpx<double>* col = C + base;
+  const double kxq = kx[q < g.nq ? q : 0];
+  double sdy = 0.0;
+  if (act && op == OP_DY_GY) sdy = 1.0 / dy[j];
+  if (act && op == OP_DY_GYF) sdy = 1.0 / dyf[j];
   const int off = g.nz - 2 * g.nkz - 1;
   cpx<R> v[8];
 #pragma unroll
   for (int r = 0; r < 8; ++r) {
     const int k = gt + G * r;
-    cpx<R> z = {(R)0, (R)0};
+    cpx<double> z = {0.0, 0.0};
     int kk = -1;
     if (k <= g.nkz)
       kk = k;
     else if (k >= g.nz - g.nkz)
       kk = k - off;
     if (act && kk >= 0) {
-      const cpx<R> c = col[kk];
+      const cpx<double> c = col[kk];
       switch (op) {
         case OP_DX: z = ik(kxq, c); break;
-        case OP_DZ: z = ik((R)kz[kk], c); break;
+        case OP_DZ: z = ik(kz[kk], c); break;
         case OP_DY_GY: z = sdy * (c - col[kk - g.sp]); break;
         case OP_DY_GYF: z = sdy * (col[kk + g.sp] - c); break;
         default: z = c; break;
       }
     }
-    v[r] = z;
+    v[r] = {(R)z.x, (R)z.y};
   }
   fft_regs<R, 1, WAVE>(v, fbuf, g.nz, g.lognz, tw, gt);
   if (q < g.nq) {
@@ -84,17 +84,20 @@ __global__ void __launch_bounds__(512) k_zinv_op(const cpx<R>* __restrict__ C, c
 }
 
 // ------------------------------------------------------- velocity model --
+// R is the precision of the physical fields and of the model arithmetic;
+// nu_t and kappa_t are stored in FP64 for the CN solve.
+template <typename R>
 struct LesVelArgs {
-  const double* G[9];   // G11 G12 G13 G21 G22 G23 G31 G32 G33 (see above)
-  const double* u1;
-  const double* u3;
-  double* tau11;        // out: delta_yf * s1 * S11 etc. (GYF), delta_y * temp * (S12 - ...) (GY)
-  double* tau33;
-  double* tau13;
-  double* tau12;
-  double* tau23;
-  double* cr1;          // out: nu_t (u1(j) - u1(j-1))/dy(j)  (les.f90:893)
-  double* cr3;          // out: nu_t (u3(j) - u3(j-1))/dy(j)  (les.f90:908)
+  const R* G[9];        // G11 G12 G13 G21 G22 G23 G31 G32 G33 (see above)
+  const R* u1;
+  const R* u3;
+  R* tau11;             // out: delta_yf * s1 * S11 etc. (GYF), delta_y * temp * (S12 - ...) (GY)
+  R* tau33;
+  R* tau13;
+  R* tau12;
+  R* tau23;
+  R* cr1;               // out: nu_t (u1(j) - u1(j-1))/dy(j)  (les.f90:893)
+  R* cr3;               // out: nu_t (u3(j) - u3(j-1))/dy(j)  (les.f90:908)
   double* nu_t;         // out
   const double* dy;
   const double* dyf;
@@ -108,18 +111,18 @@ constexpr double kCSmag = 0.17;    // les.f90:41
 constexpr double kCAmd = 0.2887;   // les.f90:42
 
 // GYF <-> GY interpolation weights of les.f90:150-165 and 316-327.
-__device__ __forceinline__ double gy_of(const double* a, long long o, long long pp, double dyfm,
-                                        double dyfj, double dyj) {
-  return (a[o] * dyfm + a[o - pp] * dyfj) / (2.0 * dyj);
+template <typename R>
+__device__ __forceinline__ R gy_of(const R* a, long long o, long long pp, R dyfm, R dyfj, R dyj) {
+  return (a[o] * dyfm + a[o - pp] * dyfj) / ((R)2 * dyj);
 }
 
 // -(I3 - I4) / (I1 - I2) clipped at 0 for a strain s and rotation w (AMD
 // numerator of les.f90:597-637 with unit filter weights, or :311-361 with
 // the Rozema weights dxq, dyq, dzq).
-__device__ __forceinline__ double amd_value(double a1, double a2, double a3, double a4, double a5,
-                                            double a6, double w4, double w5, double w6, double dxq,
-                                            double dyq, double dzq, bool rozema) {
-  double v;
+template <typename R>
+__device__ __forceinline__ R amd_value(R a1, R a2, R a3, R a4, R a5, R a6, R w4, R w5, R w6, R dxq,
+                                       R dyq, R dzq, bool rozema) {
+  R v;
   if (rozema) {
     v = dxq * a1 * a1 * a1 + dyq * a2 * a2 * a2 + dzq * a3 * a3 * a3 + (2 * dxq + dyq) * a1 * a4 * a4 +
         (2 * dxq + dzq) * a1 * a5 * a5 + (2 * dyq + dxq) * a2 * a4 * a4 + (2 * dyq + dzq) * a2 * a6 * a6 +
@@ -134,7 +137,7 @@ __device__ __forceinline__ double amd_value(double a1, double a2, double a3, dou
     v = -v - a1 * w4 * w4 - a1 * w5 * w5 - a2 * w4 * w4 - a2 * w6 * w6 - a3 * w5 * w5 - a3 * w6 * w6 -
         2 * a4 * w5 * w6 + 2 * a5 * w4 * w6 - 2 * a6 * w4 * w5;
   }
-  if (v <= 0.0) return 0.0;
+  if (v <= (R)0) return (R)0;
   return v / (a1 * a1 + a2 * a2 + a3 * a3 + 2 * a4 * a4 + 2 * w4 * w4 + 2 * a5 * a5 + 2 * w5 * w5 +
               2 * a6 * a6 + 2 * w6 * w6);
 }
@@ -145,31 +148,33 @@ __device__ __forceinline__ double amd_value(double a1, double a2, double a3, dou
 // and the explicit cross-term fluxes.  Values at points the substep never
 // consumes (GY plane 1, GYF plane Ny+1, the zeroed nu_t planes) follow the
 // reference where they are cheap and are otherwise left as the raw strain.
-__global__ void __launch_bounds__(256) k_les_vel(LesVelArgs a, Geo g) {
+template <typename R>
+__global__ void __launch_bounds__(256) k_les_vel(LesVelArgs<R> a, Geo g) {
   const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long pp = g.pp;
   const int j = 1 + (int)(idx / pp);
   if (j > g.ny + 1) return;
   const long long o = (long long)j * pp + idx % pp;
   const int ny = g.ny;
-  const double* G11 = a.G[0];
-  const double* G12 = a.G[1];
-  const double* G13 = a.G[2];
-  const double* G21 = a.G[3];
-  const double* G22 = a.G[4];
-  const double* G23 = a.G[5];
-  const double* G31 = a.G[6];
-  const double* G32 = a.G[7];
-  const double* G33 = a.G[8];
-  const double dyj = a.dy[j], dyfj = a.dyf[j], dyfm = a.dyf[j - 1];
-  const double dxs = a.dx1 * kBetaSgs, dzs = a.dz1 * kBetaSgs;
+  const R h5 = (R)0.5, two = (R)2, four = (R)4, one = (R)1, zero = (R)0;
+  const R* G11 = a.G[0];
+  const R* G12 = a.G[1];
+  const R* G13 = a.G[2];
+  const R* G21 = a.G[3];
+  const R* G22 = a.G[4];
+  const R* G23 = a.G[5];
+  const R* G31 = a.G[6];
+  const R* G32 = a.G[7];
+  const R* G33 = a.G[8];
+  const R dyj = (R)a.dy[j], dyfj = (R)a.dyf[j], dyfm = (R)a.dyf[j - 1];
+  const R dxs = (R)(a.dx1 * kBetaSgs), dzs = (R)(a.dz1 * kBetaSgs);
   // strain at this plane: GYF components S11 S22 S33 S13, GY components S12 S23
-  const double S11 = G11[o], S22 = G22[o], S33 = G33[o];
-  const double S13 = 0.5 * (G13[o] + G31[o]);
-  const double S12 = 0.5 * (G12[o] + G21[o]);
-  const double S23 = 0.5 * (G23[o] + G32[o]);
+  const R S11 = G11[o], S22 = G22[o], S33 = G33[o];
+  const R S13 = h5 * (G13[o] + G31[o]);
+  const R S12 = h5 * (G12[o] + G21[o]);
+  const R S23 = h5 * (G23[o] + G32[o]);
   // ---- GYF point (plane j)
-  double s1 = 0.0;
+  R s1 = zero;
   bool gyf_in;
   if (a.type == 5)
     gyf_in = (j >= a.js && j <= a.je);
@@ -179,38 +184,38 @@ __global__ void __launch_bounds__(256) k_les_vel(LesVelArgs a, Geo g) {
   if (gyf_in && have_up) {
     const long long u = o + pp;
     if (a.type == 1) {
-      const double s12 = 0.5 * (0.5 * (G12[u] + G21[u]) + S12);
-      const double s23 = 0.5 * (0.5 * (G23[u] + G32[u]) + S23);
-      s1 = sqrt(2.0 * S11 * S11 + 4.0 * s12 * s12 + 4.0 * S13 * S13 + 2.0 * S22 * S22 + 4.0 * s23 * s23 +
-                2.0 * S33 * S33);
+      const R s12 = h5 * (h5 * (G12[u] + G21[u]) + S12);
+      const R s23 = h5 * (h5 * (G23[u] + G32[u]) + S23);
+      s1 = sqrt(two * S11 * S11 + four * s12 * s12 + four * S13 * S13 + two * S22 * S22 + four * s23 * s23 +
+                two * S33 * S33);
     } else if (a.type == 4) {
-      const double dxq = dxs * dxs, dzq = dzs * dzs, dyq = (dyj * 2.0) * (dyj * 2.0);
-      const double a4 = 0.5 * (0.5 * (G12[u] + G21[u]) + S12);
-      const double a6 = 0.5 * (0.5 * (G23[u] + G32[u]) + S23);
-      const double w4 = 0.5 * (0.5 * (G12[u] - G21[u]) + 0.5 * (G12[o] - G21[o]));
-      const double w5 = 0.5 * (G13[o] - G31[o]);
-      const double w6 = 0.5 * (0.5 * (G23[u] - G32[u]) + 0.5 * (G23[o] - G32[o]));
+      const R dxq = dxs * dxs, dzq = dzs * dzs, dyq = (dyj * two) * (dyj * two);
+      const R a4 = h5 * (h5 * (G12[u] + G21[u]) + S12);
+      const R a6 = h5 * (h5 * (G23[u] + G32[u]) + S23);
+      const R w4 = h5 * (h5 * (G12[u] - G21[u]) + h5 * (G12[o] - G21[o]));
+      const R w5 = h5 * (G13[o] - G31[o]);
+      const R w6 = h5 * (h5 * (G23[u] - G32[u]) + h5 * (G23[o] - G32[o]));
       s1 = amd_value(S11, S22, S33, a4, S13, a6, w4, w5, w6, dxq, dyq, dzq, true);
     } else {
       // invariant strain/rotation: GY components scaled with dys = 2 dy at
       // their own plane, GYF ones with dxs, dzs (les.f90:1527-1559, 1633-1662)
-      const double dysu = a.dy[j + 1] * 2.0, dysj = dyj * 2.0;
-      const double si4u = 0.5 * (G12[u] * (dysu / dxs) + G21[u] * (dxs / dysu));
-      const double si4j = 0.5 * (G12[o] * (dysj / dxs) + G21[o] * (dxs / dysj));
-      const double si6u = 0.5 * (G23[u] * (dzs / dysu) + G32[u] * (dysu / dzs));
-      const double si6j = 0.5 * (G23[o] * (dzs / dysj) + G32[o] * (dysj / dzs));
-      const double o4u = 0.5 * (G12[u] * (dysu / dxs) - G21[u] * (dxs / dysu));
-      const double o4j = 0.5 * (G12[o] * (dysj / dxs) - G21[o] * (dxs / dysj));
-      const double o6u = 0.5 * (G23[u] * (dzs / dysu) - G32[u] * (dysu / dzs));
-      const double o6j = 0.5 * (G23[o] * (dzs / dysj) - G32[o] * (dysj / dzs));
-      const double si5 = 0.5 * (G13[o] * (dzs / dxs) + G31[o] * (dxs / dzs));
-      const double o5 = 0.5 * (G13[o] * (dzs / dxs) - G31[o] * (dxs / dzs));
-      s1 = amd_value(S11, S22, S33, 0.5 * (si4u + si4j), si5, 0.5 * (si6u + si6j), 0.5 * (o4u + o4j), o5,
-                     0.5 * (o6u + o6j), 1.0, 1.0, 1.0, false);
+      const R dysu = (R)a.dy[j + 1] * two, dysj = dyj * two;
+      const R si4u = h5 * (G12[u] * (dysu / dxs) + G21[u] * (dxs / dysu));
+      const R si4j = h5 * (G12[o] * (dysj / dxs) + G21[o] * (dxs / dysj));
+      const R si6u = h5 * (G23[u] * (dzs / dysu) + G32[u] * (dysu / dzs));
+      const R si6j = h5 * (G23[o] * (dzs / dysj) + G32[o] * (dysj / dzs));
+      const R o4u = h5 * (G12[u] * (dysu / dxs) - G21[u] * (dxs / dysu));
+      const R o4j = h5 * (G12[o] * (dysj / dxs) - G21[o] * (dxs / dysj));
+      const R o6u = h5 * (G23[u] * (dzs / dysu) - G32[u] * (dysu / dzs));
+      const R o6j = h5 * (G23[o] * (dzs / dysj) - G32[o] * (dysj / dzs));
+      const R si5 = h5 * (G13[o] * (dzs / dxs) + G31[o] * (dxs / dzs));
+      const R o5 = h5 * (G13[o] * (dzs / dxs) - G31[o] * (dxs / dzs));
+      s1 = amd_value(S11, S22, S33, h5 * (si4u + si4j), si5, h5 * (si6u + si6j), h5 * (o4u + o4j), o5,
+                     h5 * (o6u + o6j), one, one, one, false);
     }
   }
   // ---- GY point (plane j)
-  double temp = 0.0;
+  R temp = zero;
   bool gy_in;
   if (a.type == 1)
     gy_in = true;                      // les.f90:146-168, 185-197: planes 1..Ny+1
@@ -220,79 +225,85 @@ __global__ void __launch_bounds__(256) k_les_vel(LesVelArgs a, Geo g) {
     gy_in = (j >= 2 && j <= ny);       // :647-738 computes 2..Ny (Ny+1 never consumed)
   if (gy_in) {
     const long long m = o - pp;
-    const double i11 = gy_of(G11, o, pp, dyfm, dyfj, dyj);
-    const double i22 = gy_of(G22, o, pp, dyfm, dyfj, dyj);
-    const double i33 = gy_of(G33, o, pp, dyfm, dyfj, dyj);
+    const R i11 = gy_of(G11, o, pp, dyfm, dyfj, dyj);
+    const R i22 = gy_of(G22, o, pp, dyfm, dyfj, dyj);
+    const R i33 = gy_of(G33, o, pp, dyfm, dyfj, dyj);
     if (a.type == 1) {
-      const double i13 = 0.5 * (gy_of(G13, o, pp, dyfm, dyfj, dyj) + gy_of(G31, o, pp, dyfm, dyfj, dyj));
-      temp = sqrt(2.0 * i11 * i11 + 4.0 * S12 * S12 + 4.0 * i13 * i13 + 2.0 * i22 * i22 + 4.0 * S23 * S23 +
-                  2.0 * i33 * i33);
+      const R i13 = h5 * (gy_of(G13, o, pp, dyfm, dyfj, dyj) + gy_of(G31, o, pp, dyfm, dyfj, dyj));
+      temp = sqrt(two * i11 * i11 + four * S12 * S12 + four * i13 * i13 + two * i22 * i22 + four * S23 * S23 +
+                  two * i33 * i33);
     } else if (a.type == 4) {
-      const double dxq = dxs * dxs, dzq = dzs * dzs, dyq = (dyj * 2.0) * (dyj * 2.0);
-      const double i13 = 0.5 * (gy_of(G13, o, pp, dyfm, dyfj, dyj) + gy_of(G31, o, pp, dyfm, dyfj, dyj));
-      const double w4 = 0.5 * (G12[o] - G21[o]);
-      const double w5 = (0.5 * (G13[o] - G31[o]) * dyfm + 0.5 * (G13[m] - G31[m]) * dyfj) / (2.0 * dyj);
-      const double w6 = 0.5 * (G23[o] - G32[o]);
+      const R dxq = dxs * dxs, dzq = dzs * dzs, dyq = (dyj * two) * (dyj * two);
+      const R i13 = h5 * (gy_of(G13, o, pp, dyfm, dyfj, dyj) + gy_of(G31, o, pp, dyfm, dyfj, dyj));
+      const R w4 = h5 * (G12[o] - G21[o]);
+      const R w5 = (h5 * (G13[o] - G31[o]) * dyfm + h5 * (G13[m] - G31[m]) * dyfj) / (two * dyj);
+      const R w6 = h5 * (G23[o] - G32[o]);
       temp = amd_value(i11, i22, i33, S12, i13, S23, w4, w5, w6, dxq, dyq, dzq, true);
     } else {
-      const double dysj = dyj * 2.0;
-      const double si4 = 0.5 * (G12[o] * (dysj / dxs) + G21[o] * (dxs / dysj));
-      const double si6 = 0.5 * (G23[o] * (dzs / dysj) + G32[o] * (dysj / dzs));
-      const double o4 = 0.5 * (G12[o] * (dysj / dxs) - G21[o] * (dxs / dysj));
-      const double o6 = 0.5 * (G23[o] * (dzs / dysj) - G32[o] * (dysj / dzs));
-      const double si5o = 0.5 * (G13[o] * (dzs / dxs) + G31[o] * (dxs / dzs));
-      const double si5m = 0.5 * (G13[m] * (dzs / dxs) + G31[m] * (dxs / dzs));
-      const double o5o = 0.5 * (G13[o] * (dzs / dxs) - G31[o] * (dxs / dzs));
-      const double o5m = 0.5 * (G13[m] * (dzs / dxs) - G31[m] * (dxs / dzs));
-      const double si5 = (si5o * dyfm + si5m * dyfj) / (2.0 * dyj);
-      const double o5 = (o5o * dyfm + o5m * dyfj) / (2.0 * dyj);
-      temp = amd_value(i11, i22, i33, si4, si5, si6, o4, o5, o6, 1.0, 1.0, 1.0, false);
+      const R dysj = dyj * two;
+      const R si4 = h5 * (G12[o] * (dysj / dxs) + G21[o] * (dxs / dysj));
+      const R si6 = h5 * (G23[o] * (dzs / dysj) + G32[o] * (dysj / dzs));
+      const R o4 = h5 * (G12[o] * (dysj / dxs) - G21[o] * (dxs / dysj));
+      const R o6 = h5 * (G23[o] * (dzs / dysj) - G32[o] * (dysj / dzs));
+      const R si5o = h5 * (G13[o] * (dzs / dxs) + G31[o] * (dxs / dzs));
+      const R si5m = h5 * (G13[m] * (dzs / dxs) + G31[m] * (dxs / dzs));
+      const R o5o = h5 * (G13[o] * (dzs / dxs) - G31[o] * (dxs / dzs));
+      const R o5m = h5 * (G13[m] * (dzs / dxs) - G31[m] * (dxs / dzs));
+      const R si5 = (si5o * dyfm + si5m * dyfj) / (two * dyj);
+      const R o5 = (o5o * dyfm + o5m * dyfj) / (two * dyj);
+      temp = amd_value(i11, i22, i33, si4, si5, si6, o4, o5, o6, one, one, one, false);
     }
   }
   // ---- filter factors (les.f90:203-222, 499-518, 775-799)
-  double dyf_fac, dy_fac;
-  if (a.type == 1) {
-    const double c2 = -2.0 * kCSmag * kCSmag;
-    dyf_fac = c2 * pow(a.dx1 * kBetaSgs * dyfj * 1.0 * a.dz1 * kBetaSgs, 2.0 / 3.0);
-    dy_fac = c2 * pow(a.dx1 * kBetaSgs * dyj * 2.0 * a.dz1 * kBetaSgs, 2.0 / 3.0);
-  } else if (a.type == 4) {
-    dyf_fac = dy_fac = -2.0 * kCAmd * kCAmd;
-  } else {
-    const double c2 = -2.0 * kCAmd * kCAmd * 3.0;
-    dyf_fac = c2 / (1.0 / (dxs * dxs) + 1.0 / ((dyfj * 2.0) * (dyfj * 2.0)) + 1.0 / (dzs * dzs));
-    dy_fac = c2 / (1.0 / (dxs * dxs) + 1.0 / ((dyj * 2.0) * (dyj * 2.0)) + 1.0 / (dzs * dzs));
+  // per-plane factors, formed in FP64 from the FP64 metrics
+  double dyf_f, dy_f;
+  {
+    const double ddxs = a.dx1 * kBetaSgs, ddzs = a.dz1 * kBetaSgs, ddy = a.dy[j], ddyf = a.dyf[j];
+    if (a.type == 1) {
+      const double c2 = -2.0 * kCSmag * kCSmag;
+      dyf_f = c2 * pow(ddxs * ddyf * 1.0 * ddzs, 2.0 / 3.0);
+      dy_f = c2 * pow(ddxs * ddy * 2.0 * ddzs, 2.0 / 3.0);
+    } else if (a.type == 4) {
+      dyf_f = dy_f = -2.0 * kCAmd * kCAmd;
+    } else {
+      const double c2 = -2.0 * kCAmd * kCAmd * 3.0;
+      dyf_f = c2 / (1.0 / (ddxs * ddxs) + 1.0 / ((ddyf * 2.0) * (ddyf * 2.0)) + 1.0 / (ddzs * ddzs));
+      dy_f = c2 / (1.0 / (ddxs * ddxs) + 1.0 / ((ddy * 2.0) * (ddy * 2.0)) + 1.0 / (ddzs * ddzs));
+    }
   }
+  const R dyf_fac = (R)dyf_f, dy_fac = (R)dy_f;
   // GYF stresses (Sij2 is handled implicitly through nu_t)
-  const double f1 = (gyf_in && have_up) ? s1 : 1.0;
+  const R f1 = (gyf_in && have_up) ? s1 : one;
   a.tau11[o] = dyf_fac * (f1 * S11);
   a.tau33[o] = dyf_fac * (f1 * S33);
   a.tau13[o] = dyf_fac * (f1 * S13);
   // GY stresses, less the parts treated implicitly in the CN solve
-  const double du1 = (a.u1[o] - a.u1[o - pp]) / dyj, du3 = (a.u3[o] - a.u3[o - pp]) / dyj;
+  const R du1 = (a.u1[o] - a.u1[o - pp]) / dyj, du3 = (a.u3[o] - a.u3[o - pp]) / dyj;
   if (gy_in) {
-    a.tau12[o] = dy_fac * (temp * (S12 - 0.5 * du1));
-    a.tau23[o] = dy_fac * (temp * (S23 - 0.5 * du3));
+    a.tau12[o] = dy_fac * (temp * (S12 - h5 * du1));
+    a.tau23[o] = dy_fac * (temp * (S23 - h5 * du3));
   } else {
     a.tau12[o] = dy_fac * S12;
     a.tau23[o] = dy_fac * S23;
   }
   // eddy viscosity at GY points, zero at planes 1, 2, Ny, Ny+1 (ghost_les_mpi)
-  double nt = 0.0;
-  if (j >= 3 && j <= ny - 1 && gy_in) nt = -0.5 * dy_fac * temp;
-  a.nu_t[o] = nt;
-  a.cr1[o] = (j >= 2 && j <= ny) ? nt * du1 : 0.0;
-  a.cr3[o] = (j >= 2 && j <= ny) ? nt * du3 : 0.0;
+  R nt = zero;
+  if (j >= 3 && j <= ny - 1 && gy_in) nt = -h5 * dy_fac * temp;
+  a.nu_t[o] = (double)nt;
+  a.cr1[o] = (j >= 2 && j <= ny) ? nt * du1 : zero;
+  a.cr3[o] = (j >= 2 && j <= ny) ? nt * du3 : zero;
 }
 
 // --------------------------------------------------------- scalar model --
+template <typename R>
 struct LesThArgs {
-  const double* G[9];
-  const double* T1;   // d1 th (GYF)
-  const double* T2;   // (th(j)-th(j-1))/dy(j) (GY)
-  const double* T3;   // d3 th (GYF)
+  const R* G[9];
+  const R* T1;        // d1 th (GYF)
+  const R* T2;        // (th(j)-th(j-1))/dy(j) (GY)
+  const R* T3;        // d3 th (GYF)
   const double* nu_t;
-  double* f1;         // out: flux to be x-differentiated (GYF)
-  double* f3;         // out: flux to be z-differentiated (GYF)
+  R* f1;              // out: flux to be x-differentiated (GYF)
+  R* f3;              // out: flux to be z-differentiated (GYF)
   double* kappa_t;    // out
   const double* dy;
   const double* dyf;
@@ -306,75 +317,79 @@ struct LesThArgs {
 // model at GYF (:1238-1275) and GY points (:1278-1330), kappa_t (:1373-1384)
 // and the fluxes 0.5 delta_yf s1_th d_i th (:1334-1342, 1392-1401), with the
 // filter factor applied before the forward transform.
-__global__ void __launch_bounds__(256) k_les_th(LesThArgs a, Geo g) {
+template <typename R>
+__global__ void __launch_bounds__(256) k_les_th(LesThArgs<R> a, Geo g) {
   const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long pp = g.pp;
   const int j = 1 + (int)(idx / pp);
   if (j > g.ny + 1) return;
   const long long o = (long long)j * pp + idx % pp;
   const int ny = g.ny;
+  const R h5 = (R)0.5, two = (R)2, zero = (R)0;
   if (a.type != 5) {
     const double kt = a.nu_t[o];
     a.kappa_t[o] = kt;
-    a.f1[o] = kt * a.T1[o];
-    a.f3[o] = kt * a.T3[o];
+    a.f1[o] = (R)kt * a.T1[o];
+    a.f3[o] = (R)kt * a.T3[o];
     return;
   }
-  const double* G11 = a.G[0];
-  const double* G12 = a.G[1];
-  const double* G13 = a.G[2];
-  const double* G21 = a.G[3];
-  const double* G22 = a.G[4];
-  const double* G23 = a.G[5];
-  const double* G31 = a.G[6];
-  const double* G32 = a.G[7];
-  const double* G33 = a.G[8];
-  const double dxs = a.dx1 * kBetaSgs, dzs = a.dz1 * kBetaSgs;
-  const double dyj = a.dy[j], dyfj = a.dyf[j], dyfm = a.dyf[j - 1];
+  const R* G11 = a.G[0];
+  const R* G12 = a.G[1];
+  const R* G13 = a.G[2];
+  const R* G21 = a.G[3];
+  const R* G22 = a.G[4];
+  const R* G23 = a.G[5];
+  const R* G31 = a.G[6];
+  const R* G32 = a.G[7];
+  const R* G33 = a.G[8];
+  const double ddxs = a.dx1 * kBetaSgs, ddzs = a.dz1 * kBetaSgs;
+  const R dxs = (R)ddxs, dzs = (R)ddzs;
+  const R dyj = (R)a.dy[j], dyfj = (R)a.dyf[j], dyfm = (R)a.dyf[j - 1];
+  const double c2 = -2.0 * kCAmd * kCAmd * 3.0;
   // GYF point: du2dy(j) of the reference is G22(j-1)
-  double f1 = 0.0, f3 = 0.0;
+  R f1 = zero, f3 = zero;
   if (j >= a.jst && j <= a.jet && j <= ny) {
     const long long u = o + pp, m = o - pp;
-    const double dly = dyfj * 2.0;
-    const double dx2 = dxs * dxs, dy2 = dly * dly, dz2 = dzs * dzs;
-    const double tx = a.T1[o], tz = a.T3[o], ty = 0.5 * (a.T2[u] + a.T2[o]);
-    double v = -(dx2 * G11[o] * tx * tx + dy2 * G22[m] * ty * ty + dz2 * G33[o] * tz * tz +
-                 (dy2 * 0.5 * (G12[u] + G12[o]) + dx2 * 0.5 * (G21[u] + G21[o])) * tx * ty +
-                 (dz2 * G13[o] + dx2 * G31[o]) * tx * tz +
-                 (dz2 * 0.5 * (G23[u] + G23[o]) + dy2 * 0.5 * (G32[u] + G32[o])) * ty * tz);
-    if (v <= 0.0)
-      v = 0.0;
+    const R dly = dyfj * two;
+    const R dx2 = dxs * dxs, dy2 = dly * dly, dz2 = dzs * dzs;
+    const R tx = a.T1[o], tz = a.T3[o], ty = h5 * (a.T2[u] + a.T2[o]);
+    R v = -(dx2 * G11[o] * tx * tx + dy2 * G22[m] * ty * ty + dz2 * G33[o] * tz * tz +
+            (dy2 * h5 * (G12[u] + G12[o]) + dx2 * h5 * (G21[u] + G21[o])) * tx * ty +
+            (dz2 * G13[o] + dx2 * G31[o]) * tx * tz +
+            (dz2 * h5 * (G23[u] + G23[o]) + dy2 * h5 * (G32[u] + G32[o])) * ty * tz);
+    if (v <= zero)
+      v = zero;
     else
       v = v / ((dxs * tx) * (dxs * tx) + (dly * ty) * (dly * ty) + (dzs * tz) * (dzs * tz));
-    const double c2 = -2.0 * kCAmd * kCAmd * 3.0;
-    const double dyf_fac = c2 / (1.0 / (dxs * dxs) + 1.0 / (dly * dly) + 1.0 / (dzs * dzs));
-    f1 = 0.5 * dyf_fac * (v * tx);
-    f3 = 0.5 * dyf_fac * (v * tz);
+    const double ddly = a.dyf[j] * 2.0;
+    const R dyf_fac = (R)(c2 / (1.0 / (ddxs * ddxs) + 1.0 / (ddly * ddly) + 1.0 / (ddzs * ddzs)));
+    f1 = h5 * dyf_fac * (v * tx);
+    f3 = h5 * dyf_fac * (v * tz);
   }
   a.f1[o] = f1;
   a.f3[o] = f3;
   // GY point
-  double kt = 0.0;
+  R kt = zero;
   if (j >= 3 && j <= ny - 1) {
     const long long m = o - pp;
-    const double dly = dyj * 2.0;
-    const double dx2 = dxs * dxs, dy2 = dly * dly, dz2 = dzs * dzs;
-    auto gy = [&](const double* f) { return (f[o] * dyfm + f[m] * dyfj) / (2.0 * dyj); };
+    const R dly = dyj * two;
+    const R dx2 = dxs * dxs, dy2 = dly * dly, dz2 = dzs * dzs;
+    auto gy = [&](const R* f) { return (f[o] * dyfm + f[m] * dyfj) / (two * dyj); };
     // du2dy at GY interpolates du2dy(j) = G22(j-1) and du2dy(j-1) = G22(j-2)
-    const double d22 = (G22[m] * dyfm + G22[m - pp] * dyfj) / (2.0 * dyj);
-    const double tx = gy(a.T1), tz = gy(a.T3), ty = a.T2[o];
-    double v = -(dx2 * gy(G11) * tx * tx + dy2 * d22 * ty * ty + dz2 * gy(G33) * tz * tz +
-                 (dy2 * G12[o] + dx2 * G21[o]) * tx * ty + (dz2 * gy(G13) + dx2 * gy(G31)) * tx * tz +
-                 (dz2 * G23[o] + dy2 * G32[o]) * ty * tz);
-    if (v <= 0.0)
-      v = 0.0;
+    const R d22 = (G22[m] * dyfm + G22[m - pp] * dyfj) / (two * dyj);
+    const R tx = gy(a.T1), tz = gy(a.T3), ty = a.T2[o];
+    R v = -(dx2 * gy(G11) * tx * tx + dy2 * d22 * ty * ty + dz2 * gy(G33) * tz * tz +
+            (dy2 * G12[o] + dx2 * G21[o]) * tx * ty + (dz2 * gy(G13) + dx2 * gy(G31)) * tx * tz +
+            (dz2 * G23[o] + dy2 * G32[o]) * ty * tz);
+    if (v <= zero)
+      v = zero;
     else
       v = v / ((dxs * tx) * (dxs * tx) + (dly * ty) * (dly * ty) + (dzs * tz) * (dzs * tz));
-    const double c2 = -2.0 * kCAmd * kCAmd * 3.0;
-    const double dy_fac = c2 / (1.0 / (dxs * dxs) + 1.0 / (dly * dly) + 1.0 / (dzs * dzs));
-    kt = -0.5 * dy_fac * v;
+    const double ddly = a.dy[j] * 2.0;
+    const R dy_fac = (R)(c2 / (1.0 / (ddxs * ddxs) + 1.0 / (ddly * ddly) + 1.0 / (ddzs * ddzs)));
+    kt = -h5 * dy_fac * v;
   }
-  a.kappa_t[o] = kt;
+  a.kappa_t[o] = (double)kt;
 }
 
 // ---------------------------------------------- stress divergence into cf --
@@ -529,13 +544,17 @@ struct TriRow {
   int vtype;        // 1 for u2 (GY rows: nu_t averaged to GYF)
 };
 
-__global__ void __launch_bounds__(256) k_thomas_les(const double* __restrict__ r, double* __restrict__ u,
+// T is the storage type of r and u; the sweeps run in FP64 with b' and g' in
+// FP64 scratch.  incr has the meaning of k_thomas_seg's flag.
+template <typename T>
+__global__ void __launch_bounds__(256) k_thomas_les(const T* __restrict__ r, T* __restrict__ u,
                                                     const double* __restrict__ nt, double* __restrict__ sb,
-                                                    const TriRow* __restrict__ rows, int n, long long pp,
-                                                    int ca, int cb, double kscale) {
+                                                    double* __restrict__ sg, const TriRow* __restrict__ rows,
+                                                    int n, long long pp, int ca, int cb, double kscale,
+                                                    int incr) {
   const long long col = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (col >= pp) return;
-  auto U = [&](int j) { return u[(long long)j * pp + col]; };
+  auto U = [&](int j) { return (double)u[(long long)j * pp + col]; };
   auto N = [&](int j) { return nt[(long long)j * pp + col]; };
   // row coefficients of this lane
   auto row = [&](int j, double& a, double& b, double& c, double& vec, double um, double uc, double up) {
@@ -555,7 +574,7 @@ __global__ void __launch_bounds__(256) k_thomas_les(const double* __restrict__ r
       a -= rw.lm * nl;
       b += rw.lp * nu + rw.lm * nl;
       c -= rw.lp * nu;
-      vec = r[(long long)j * pp + col] + rw.wp * (up - uc) - rw.wm * (uc - um) +
+      vec = (double)r[(long long)j * pp + col] + (incr ? uc : 0.0) + rw.wp * (up - uc) - rw.wm * (uc - um) +
             (rw.lp * nu * (up - uc) - rw.lm * nl * (uc - um));
     } else {
       vec = (rw.kind == 2) ? rw.vb * kscale : rw.vb;
@@ -566,7 +585,7 @@ __global__ void __launch_bounds__(256) k_thomas_les(const double* __restrict__ r
   row(0, a, b, c, vec, um, uc, up);
   double bp = b, gp = vec, cprev = c;
   sb[col] = bp;
-  if (0 >= ca) u[col] = gp;
+  sg[col] = gp;
   for (int j = 1; j < n; ++j) {
     um = uc;
     uc = up;
@@ -578,11 +597,14 @@ __global__ void __launch_bounds__(256) k_thomas_les(const double* __restrict__ r
     cprev = c;
     const long long o = (long long)j * pp + col;
     sb[o] = bp;
-    if (j >= ca) u[o] = gp;
+    sg[o] = gp;
   }
   // backward: x(n-1) = g'/b', x(j) = (g'(j) - c(j) x(j+1)) / b'(j)
   double x = gp / bp;
-  u[(long long)(n - 1) * pp + col] = x;
+  if (n - 1 >= ca && n - 1 <= cb) {
+    const long long o = (long long)(n - 1) * pp + col;
+    u[o] = (T)(x - (incr ? (double)u[o] : 0.0));
+  }
   for (int j = n - 2; j >= ca; --j) {
     const TriRow& rw = rows[j];
     double c = rw.c;
@@ -591,10 +613,9 @@ __global__ void __launch_bounds__(256) k_thomas_les(const double* __restrict__ r
       c -= rw.lp * nu;
     }
     const long long o = (long long)j * pp + col;
-    x = (u[o] - c * x) / sb[o];
-    u[o] = x;
+    x = (sg[o] - c * x) / sb[o];
+    if (j <= cb) u[o] = (T)(x - (incr ? (double)u[o] : 0.0));
   }
-  (void)cb;
 }
 
 }  // namespace dgpu

@@ -296,3 +296,23 @@ def test_oracle_metrics_match_python():
     np.testing.assert_array_equal(kz, kz2)
     # pi rounded to single precision (fft.f90:152): kx(1) = 2*3.14159274.../Lx
     assert kx[1] == 2.0 * float(np.float32(np.pi)) / p["lx"]
+
+
+def test_wall_shear_and_bulk_of_poiseuille():
+    """stats.py on u1 = 1 - y^2 sampled at GYF: tau = 2 nu on both walls to the
+    first-order one-sided difference, bulk velocity 2/3 to the trapezoid rule."""
+    from diablo3_test_amd import stats
+    ny = 65
+    p = small_params(nx=16, ny=ny, nz=16)
+    gy = workloads.channel_grid(ny)
+    gyf, _, _ = metrics(gy)
+    cu1 = np.zeros((ny + 2, p["nz"] + 2, p["nx"] // 2 + 1), dtype=complex)
+    cu1[:, 0, 0] = 1 - gyf ** 2
+    nu = 1.0 / p["re"]
+    lo, hi = stats.wall_shear(cu1, gy, nu)
+    h1 = gyf[2] - gyf[1]
+    # (U(2) - U(1)) / h with U = 1 - y^2 at y = -1, -1 + h: exactly 2 - h
+    np.testing.assert_allclose(lo, nu * (2 - h1), rtol=1e-12)
+    np.testing.assert_allclose(hi, lo, rtol=1e-12)  # symmetric grid
+    ub = stats.bulk_velocity(cu1, gy)
+    assert abs(ub - 2.0 / 3.0) < 2e-3
