@@ -74,13 +74,62 @@ def copy_probe(nbytes):
     return {"bytes": nbytes, "seconds": best, "GBps_read_plus_write": 2 * nbytes / best / 1e9}
 
 
+def run_form(p, gy, st, precision, steps, warmup, timing_steps):
+    """Time `steps` RK3 steps of one form with the state resident, then the
+    per-kernel-class split over `timing_steps` more (outside the timed region)."""
+    from diablo3_test_amd.binding import ChannelSolver
+
+    npts = p["nx"] * p["ny"] * p["nz"]
+    g = ChannelSolver(p, gy, precision=precision)
+    g.set_state(st)
+    g.pre_first_step()
+    g.synchronize()
+    log(f"bench: {precision} state resident, warming up")
+    g.step(warmup)
+    g.synchronize()
+    t0 = time.perf_counter()
+    g.step(steps)
+    g.synchronize()
+    sec = time.perf_counter() - t0
+    ms_per_step = sec / steps * 1e3
+    log(f"bench: {precision} {steps} steps in {sec:.3f} s, {ms_per_step:.2f} ms/step")
+    g.timing_enable(True)
+    g.step(timing_steps)
+    kt = g.timing_read()
+    g.timing_enable(False)
+    g.close()
+    total_ms = sum(v[0] for v in kt.values())
+    total_bytes = sum(v[1] for v in kt.values())
+    kernels = {}
+    for name, (ms, by, ct) in kt.items():
+        if ct:
+            kernels[name] = {
+                "ms_per_step": ms / timing_steps,
+                "launches_per_step": ct / timing_steps,
+                "GBps": by / (ms * 1e-3) / 1e9 if ms > 0 else None,
+                "share": ms / total_ms if total_ms > 0 else None,
+            }
+    step_bytes = total_bytes / timing_steps
+    return {
+        "value": npts * steps / sec / 1e6,
+        "ms_per_step": ms_per_step,
+        "kt": kt,
+        "kernels": kernels,
+        "step_hbm_frac": step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "algorithmic_GB_per_step": step_bytes / 1e9,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="dns_512x257x512")
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"],
+                    help="the form whose throughput is `value`")
+    ap.add_argument("--forms", default="fp64,mixed",
+                    help="forms also measured and reported under `forms` (comma list, or 'none')")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--timing-steps", type=int, default=2)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -92,7 +141,6 @@ def main():
         sys.exit(2)
 
     from diablo3_test_amd import workloads
-    from diablo3_test_amd.binding import ChannelSolver
 
     probe = copy_probe(args.copy_probe_mib << 20) if args.copy_probe_mib else None
     if probe:
@@ -100,64 +148,37 @@ def main():
 
     p, gy, ic, seed, amp = workloads.workload(args.workload)
     npts = p["nx"] * p["ny"] * p["nz"]
-    log(f"bench: {args.workload} {p['nx']}x{p['ny']}x{p['nz']} precision={args.precision}")
+    log(f"bench: {args.workload} {p['nx']}x{p['ny']}x{p['nz']}")
     st = workloads.initial_state(p, gy, ic, seed, amp)
-    g = ChannelSolver(p, gy, precision=args.precision)
-    g.set_state(st)
-    g.pre_first_step()
-    g.synchronize()
-    log("bench: state resident, warming up")
-    g.step(args.warmup)
-    g.synchronize()
-    t0 = time.perf_counter()
-    g.step(args.steps)
-    g.synchronize()
-    t1 = time.perf_counter()
-    sec = t1 - t0
-    ms_per_step = sec / args.steps * 1e3
-    value = npts * args.steps / sec / 1e6
-    log(f"bench: {args.steps} steps in {sec:.3f} s, {ms_per_step:.2f} ms/step")
+    main_form = run_form(p, gy, st, args.precision, args.steps, args.warmup, args.timing_steps)
+    forms = {args.precision: main_form}
+    extra = [] if args.forms == "none" else [f for f in args.forms.split(",") if f and f != args.precision]
+    for f in extra:
+        forms[f] = run_form(p, gy, st, f, args.steps, args.warmup, args.timing_steps)
 
-    # per-kernel-class timing on extra steps (not part of the timed region)
-    g.timing_enable(True)
-    g.step(args.timing_steps)
-    kt = g.timing_read()
-    g.timing_enable(False)
-    total_ms = sum(v[0] for v in kt.values())
-    total_bytes = sum(v[1] for v in kt.values())
-    kernels = {}
-    for name, (ms, by, ct) in kt.items():
-        if ct:
-            kernels[name] = {
-                "ms_per_step": ms / args.timing_steps,
-                "launches_per_step": ct / args.timing_steps,
-                "GBps": by / (ms * 1e-3) / 1e9 if ms > 0 else None,
-                "share": ms / total_ms if total_ms > 0 else None,
-            }
-    dom = max(kt.items(), key=lambda kv: kv[1][0])
-    dname, (dms, dbytes, dct) = dom
+    kt = main_form["kt"]
+    dname, (dms, dbytes, dct) = max(kt.items(), key=lambda kv: kv[1][0])
     achieved = dbytes / (dms * 1e-3) / 1e9
-    traffic = None
     roofline = {
         "bound": "hbm",
         "achieved": achieved,
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS,
-        "traffic": traffic,
+        "traffic": None,
         "kernel": dname,
         "bytes_per_launch": dbytes / dct,
         "avg_launch_ms": dms / dct,
     }
-    step_bytes = total_bytes / args.timing_steps
+    base = forms.get("fp64", main_form)
     line = {
         "metric": METRIC,
-        "value": value,
+        "value": main_form["value"],
         "unit": "Mpts·RK3-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": main_form["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -172,13 +193,23 @@ def main():
             "n_th": p["n_th"],
         },
         "roofline": roofline,
-        "step_hbm_frac": step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
-        "algorithmic_GB_per_step": step_bytes / 1e9,
-        "kernels": kernels,
+        "step_hbm_frac": main_form["step_hbm_frac"],
+        "algorithmic_GB_per_step": main_form["algorithmic_GB_per_step"],
+        "kernels": main_form["kernels"],
+        "forms": {
+            name: {
+                "value": f["value"],
+                "ms_per_step": f["ms_per_step"],
+                "vs_fp64": f["value"] / base["value"],
+                "step_hbm_frac": f["step_hbm_frac"],
+                "algorithmic_GB_per_step": f["algorithmic_GB_per_step"],
+                "kernels_ms_per_step": {k: v["ms_per_step"] for k, v in f["kernels"].items()},
+            }
+            for name, f in forms.items()
+        },
     }
     if probe:
         line["copy_probe"] = probe
-    g.close()
     if not args.no_cpu_baseline:
         log("bench: timing the CPU oracle on one step of the same state")
         line["cpu_baseline"] = cpu_baseline(p, gy, st, npts)

@@ -1,0 +1,120 @@
+"""Gradients through the channel substep: checkpointed adjoint runs on the
+HIP adjoint kernels, and a torch.autograd.Function on top of them.
+
+The loss is the time average of the wall shear stress on both walls
+(statistics.f90:143-146, see stats.py) over N steps of the driver loop
+(for/diablo.f90:98-108):
+
+    L = (1/N) sum_{n=1..N} (tau_lower(S_n) + tau_upper(S_n)) / 2,
+
+a linear function of the mean profile Re cu1(0, 0, :) of each step's state.
+Its gradient with respect to the initial state S_0 comes from the adjoint
+substeps run backwards.  The forward run stores the state every
+`checkpoint_every` steps; the backward sweep restores each checkpoint,
+recomputes that segment storing every substep's input, and runs the adjoint
+substeps over it.
+"""
+import numpy as np
+import torch
+
+from .binding import ChannelSolver, FIELD_IDS, PER_SCALAR
+from .grid import metrics
+
+STATE = ("cu1", "cu2", "cu3", "cp", "cth", "cf1", "cf2", "cf3", "cfth")
+
+
+def wall_shear_weights(p, gy):
+    """w with (tau_lower + tau_upper)/2 = w . Re cu1(0,0,:)."""
+    gyf, _, _ = metrics(gy)
+    ny = p["ny"]
+    nu = 1.0 / p["re"]
+    w = np.zeros(ny + 2)
+    h1 = gyf[2] - gyf[1]
+    h2 = gyf[ny] - gyf[ny - 1]
+    w[2] += 0.5 * nu / h1
+    w[1] -= 0.5 * nu / h1
+    w[ny] -= 0.5 * nu / h2
+    w[ny - 1] += 0.5 * nu / h2
+    return w
+
+
+class WallShearRun:
+    """N steps from a given initial state on one solver handle (FP64 DNS)."""
+
+    def __init__(self, solver, nsteps, checkpoint_every=10, time0=0.0, time_step0=0):
+        if not isinstance(solver, ChannelSolver):
+            raise TypeError("solver must be a ChannelSolver")
+        self.s = solver
+        self.n = int(nsteps)
+        self.k = max(1, int(checkpoint_every))
+        self.t0 = float(time0)
+        self.ts0 = int(time_step0)
+        p = solver.p
+        self.dt = p.delta_t
+        pd = {"ny": p.ny, "re": p.re}
+        self.w = wall_shear_weights(pd, solver.gy)
+
+    def _step(self, n):
+        """Step n (0-based) of the driver loop: time_step = ts0 + n + 1, time fixed over the substeps."""
+        t = self.t0 + n * self.dt
+        ts = self.ts0 + n + 1
+        for rk in (1, 2, 3):
+            self.s.rk_chan_1(rk, t, ts)
+
+    def loss(self, state, save_checkpoints=False):
+        """Set the initial state, run N steps, return L."""
+        self.s.set_state(state)
+        total = 0.0
+        for n in range(self.n):
+            if save_checkpoints and n % self.k == 0:
+                self.s.checkpoint(n // self.k, save=True)
+            self._step(n)
+            total += float(self.w @ self.s.mean_profile("cu1"))
+        return total / self.n
+
+    def gradient(self, state):
+        """(L, dL/dS_0) with dL/dS_0 a dict of reference-layout complex arrays."""
+        L = self.loss(state, save_checkpoints=True)
+        nseg = (self.n + self.k - 1) // self.k
+        base = nseg  # substep snapshots follow the segment checkpoints
+        s = self.s
+        s.adj_zero()
+        wN = self.w / self.n
+        for seg in reversed(range(nseg)):
+            n0, n1 = seg * self.k, min(self.n, (seg + 1) * self.k)
+            s.checkpoint(seg, save=False)
+            for n in range(n0, n1):
+                t = self.t0 + n * self.dt
+                ts = self.ts0 + n + 1
+                for rk in (1, 2, 3):
+                    s.checkpoint(base + 3 * (n - n0) + rk - 1, save=True)
+                    s.rk_chan_1(rk, t, ts)
+            for n in reversed(range(n0, n1)):
+                t = self.t0 + n * self.dt
+                ts = self.ts0 + n + 1
+                s.adj_add_mean("cu1", wN)  # the loss term of step n's output
+                for rk in (3, 2, 1):
+                    s.checkpoint(base + 3 * (n - n0) + rk - 1, save=False)
+                    s.adj_rk_chan_1(rk, t, ts)
+        return L, s.adj_get_state(STATE)
+
+
+class WallShearLoss(torch.autograd.Function):
+    """L(cu1, cu2, cu3, cth) for a WallShearRun; the other state fields (cp,
+    the RK carry) are held fixed at `rest`.  Tensors are complex128 in the
+    reference layout; backward runs the checkpointed HIP adjoint."""
+
+    @staticmethod
+    def forward(ctx, cu1, cu2, cu3, cth, run, rest):
+        state = dict(rest)
+        for k, v in (("cu1", cu1), ("cu2", cu2), ("cu3", cu3), ("cth", cth)):
+            state[k] = v.detach().cpu().numpy()
+        ctx.run = run
+        ctx.state = state
+        return torch.tensor(run.loss(state), dtype=torch.float64)
+
+    @staticmethod
+    def backward(ctx, g):
+        _, grad = ctx.run.gradient(ctx.state)
+        out = [g * torch.from_numpy(grad[k]) for k in ("cu1", "cu2", "cu3", "cth")]
+        return out[0], out[1], out[2], out[3], None, None

@@ -174,11 +174,25 @@ def load_library(path=None):
         lib.dgpu_timing_read.restype = ctypes.c_int
         lib.dgpu_kernel_class_name.argtypes = [ctypes.c_int]
         lib.dgpu_kernel_class_name.restype = ctypes.c_char_p
+        h = ctypes.c_void_p
+        d = P(ctypes.c_double)
+        for name, args in [
+            ("adj_set_field", [h, ctypes.c_int, d]),
+            ("adj_get_field", [h, ctypes.c_int, d]),
+            ("adj_zero", [h]),
+            ("adj_rk_substep", [h, ctypes.c_int, ctypes.c_double, ctypes.c_int]),
+            ("adj_add_mean", [h, ctypes.c_int, d]),
+            ("mean_profile", [h, ctypes.c_int, d]),
+            ("checkpoint", [h, ctypes.c_int, ctypes.c_int]),
+        ]:
+            fn = getattr(lib, "dgpu_" + name)
+            fn.argtypes = args
+            fn.restype = ctypes.c_int
         _LIB[path] = lib
     return _LIB[path]
 
 
-NCLASS = 10
+NCLASS = 11
 
 
 def spectral_shape(p):
@@ -309,6 +323,53 @@ class ChannelSolver:
 
     def synchronize(self):
         self._check(self._fn("synchronize")(self._h), "synchronize")
+
+    # ---- adjoint of the FP64 DNS substep (HIP library only)
+    def adj_set_field(self, name, arr):
+        fid = FIELD_IDS[name]
+        a = np.ascontiguousarray(np.asarray(arr, dtype=np.complex128).reshape(self._shape(name)))
+        self._check(self.lib.dgpu_adj_set_field(self._h, fid, a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+                    f"adj_set_field({name})")
+
+    def adj_get_field(self, name):
+        fid = FIELD_IDS[name]
+        a = np.zeros(self._shape(name), dtype=np.complex128)
+        self._check(self.lib.dgpu_adj_get_field(self._h, fid, a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+                    f"adj_get_field({name})")
+        return a
+
+    def adj_set_state(self, state):
+        for k, v in state.items():
+            self.adj_set_field(k, v)
+
+    def adj_get_state(self, names=("cu1", "cu2", "cu3", "cp", "cth", "cf1", "cf2", "cf3", "cfth")):
+        return {k: self.adj_get_field(k) for k in names if not (k in PER_SCALAR and self.p.n_th == 0)}
+
+    def adj_zero(self):
+        self._check(self.lib.dgpu_adj_zero(self._h), "adj_zero")
+
+    def adj_rk_chan_1(self, rk_step, time=None, time_step=None):
+        """Adjoint of rk_chan_1: forward state = substep input, adjoint state =
+        gradient w.r.t. the substep output -> gradient w.r.t. the input."""
+        t = self.time if time is None else time
+        ts = self.time_step if time_step is None else time_step
+        self._check(self.lib.dgpu_adj_rk_substep(self._h, int(rk_step), float(t), int(ts)), "adj_rk_substep")
+
+    def adj_add_mean(self, name, w):
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        assert w.size == self.p.ny + 2
+        self._check(self.lib.dgpu_adj_add_mean(self._h, FIELD_IDS[name],
+                                               w.ctypes.data_as(ctypes.POINTER(ctypes.c_double))), "adj_add_mean")
+
+    def mean_profile(self, name="cu1"):
+        out = np.zeros(self.p.ny + 2)
+        self._check(self.lib.dgpu_mean_profile(self._h, FIELD_IDS[name],
+                                               out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))), "mean_profile")
+        return out
+
+    def checkpoint(self, slot, save=True):
+        """Save (or restore) the forward state in device slot `slot`."""
+        self._check(self.lib.dgpu_checkpoint(self._h, int(slot), 1 if save else 0), "checkpoint")
 
     def close(self):
         if self._h:

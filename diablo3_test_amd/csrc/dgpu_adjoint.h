@@ -1,0 +1,414 @@
+// dgpu_adjoint.h -- adjoint (vector-Jacobian product) of the FP64 DNS
+// substep rk_chan_1 (for/channel.f90:10-917) and rem_div_chan (:946-1023).
+//
+// Conventions.  Complex arrays are real pairs; the adjoint of a complex array
+// c is lam = dL/dRe c + i dL/dIm c.  A complex-linear map y = m x then has the
+// adjoint xbar += conj(m) ybar; ik products become -ik.  The transforms
+// (fft.f90:247-382) as real-linear maps have the transposes
+//   (forward F)^T lam = P(W lam) / (Nx Nz),     W = 1 at kx = 0, 1/2 at kx > 0,
+//   (inverse P)^T ubar = Nx Nz W' F(ubar),       W' = 1 at kx = 0, 2 at kx > 0,
+// so the adjoint reuses the forward passes with per-kx scale factors.  The
+// Thomas solves become solves with the transposed tridiagonal matrices
+// ("reverse" sweeps), and the products' Jacobians are gathered over the
+// three-plane stencils of channel.f90:299-459, 557-608.
+#pragma once
+#include "dgpu_kernels.h"
+
+namespace dgpu {
+
+// ------------------------------------------------- projection adjoint ----
+// rem_div_chan's map (cu1, cu2, cu3, cp) -> (cu1 - ikx phi, cu2 - d phi/dy,
+// cu3 - ikz phi, cp + phi/h) with phi = T^-1 div(cu), per (kx, kz) mode:
+// phibar from the outputs, y = T^-T phibar, then the divergence's transpose.
+struct AdjRemDivArgs {
+  double2* a1;   // in/out: adjoints of cu1, cu2, cu3
+  double2* a2;
+  double2* a3;
+  const double2* ap;  // adjoint of cp (the map passes it through)
+  double* sb;    // scratch b'
+  double2* sg;   // scratch g'
+  const double* kx;
+  const double* kz;
+  const double* dy;
+  const double* dyf;
+  double inv_h;
+  int js, je;
+};
+
+__global__ void __launch_bounds__(256) k_adj_remdiv(AdjRemDivArgs a, Geo g) {
+  typedef cpx<double> C;
+  const long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= g.sp) return;
+  const int q = (int)(m / g.nkk), kk = (int)(m % g.nkk);
+  const double kxq = a.kx[q], kzk = a.kz[kk];
+  const double lam = kxq * kxq + kzk * kzk;
+  const bool mean = (q == 0 && kk == 0);
+  const int ny = g.ny;
+  const long long sp = g.sp;
+  C* a1 = reinterpret_cast<C*>(a.a1);
+  C* a2 = reinterpret_cast<C*>(a.a2);
+  C* a3 = reinterpret_cast<C*>(a.a3);
+  const C* ap = reinterpret_cast<const C*>(a.ap);
+  C* sg = reinterpret_cast<C*>(a.sg);
+  // rows of T (forward rem_div): row 1 (1, mu1), rows 2..ny-1 (ml, md, mu), row ny (1, -1)
+  auto ml_of = [&](int j) -> double { return (j == ny) ? 1.0 : ((j == 1) ? 0.0 : 1.0 / (a.dy[j] * a.dyf[j])); };
+  auto md_of = [&](int j) -> double {
+    if (j == 1) return 1.0;
+    if (j == ny) return -1.0;
+    return -lam - 1.0 / (a.dy[j + 1] * a.dyf[j]) - 1.0 / (a.dy[j] * a.dyf[j]);
+  };
+  auto mu_of = [&](int j) -> double {
+    if (j == 1) return mean ? 0.0 : -1.0;
+    if (j == ny) return 0.0;
+    return 1.0 / (a.dy[j + 1] * a.dyf[j]);
+  };
+  auto phibar = [&](int j) -> C {
+    const long long o = (long long)j * sp + m;
+    C v = ik(kxq, a1[o]) + ik(kzk, a3[o]);
+    if (j >= 2) v = v - (1.0 / a.dy[j]) * a2[o];
+    if (j + 1 <= ny) v = v + (1.0 / a.dy[j + 1]) * a2[o + sp];
+    if (j >= a.js && j <= a.je) v = v + a.inv_h * ap[o];
+    return v;
+  };
+  // forward elimination of T^T y = phibar: sub(j) = mu(j-1), sup(j) = ml(j+1)
+  double bprev = md_of(1);
+  C gprev = phibar(1);
+  a.sb[sp + m] = bprev;
+  sg[sp + m] = gprev;
+  for (int j = 2; j <= ny; ++j) {
+    const long long o = (long long)j * sp + m;
+    const double mlt = -mu_of(j - 1) / bprev;
+    const double b = md_of(j) + mlt * ml_of(j);
+    const C gg = phibar(j) + mlt * gprev;
+    a.sb[o] = b;
+    sg[o] = gg;
+    bprev = b;
+    gprev = gg;
+  }
+  // back substitution; rows 2..ny-1 carry the divergence: rhs_j = ikx cu1 +
+  // (cu2(j+1) - cu2(j))/dyf(j) + ikz cu3, so cu1bar -= ikx y, cu3bar -= ikz y,
+  // cu2bar(j+1) += y/dyf(j), cu2bar(j) -= y/dyf(j)
+  C yup = {0.0, 0.0};
+  for (int j = ny; j >= 1; --j) {
+    const long long o = (long long)j * sp + m;
+    C y;
+    if (j == ny)
+      y = (1.0 / a.sb[o]) * sg[o];
+    else
+      y = (1.0 / a.sb[o]) * (sg[o] - ml_of(j + 1) * yup);
+    const bool div_row = (j >= 2 && j <= ny - 1);
+    const bool div_up = (j + 1 >= 2 && j + 1 <= ny - 1);
+    if (div_row) {
+      a1[o] = a1[o] - ik(kxq, y);
+      a3[o] = a3[o] - ik(kzk, y);
+    }
+    // cu2bar at row j+1: the rhs rows j and j+1 touch it
+    if (j + 1 <= ny + 1) {
+      C d = {0.0, 0.0};
+      if (div_row) d = d + (1.0 / a.dyf[j]) * y;
+      if (div_up) d = d - (1.0 / a.dyf[j + 1]) * yup;
+      if (div_row || div_up) a2[o + sp] = a2[o + sp] + d;
+    }
+    yup = y;
+  }
+  // cu2bar at row 1 is not touched (rhs row 1 is a BC row)
+}
+
+// ------------------------------------------------------ CN adjoint ----
+// After the transposed solve vbar = A^-T xbar (k_thomas_seg with the
+// transposed factor table), the explicit half vec = r + E u of the assembled
+// rows gives rbar = vbar on those rows and ubar = E^T vbar, with
+// E u = wp (u(j+1) - u(j)) - wm (u(j) - u(j-1)).
+__global__ void __launch_bounds__(256) k_adj_cnpost(const double* __restrict__ vbar, double* __restrict__ rbar,
+                                                    double* __restrict__ ubar, const SegRow* __restrict__ rows,
+                                                    Geo g) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long pp = g.pp;
+  const int n = g.ny2;
+  if (idx >= pp * n) return;
+  const int j = (int)(idx / pp);
+  const SegRow rj = rows[j];
+  const double vj = vbar[idx];
+  double ub = 0.0;
+  if (rj.kind == 1) ub -= (rj.wp + rj.wm) * vj;
+  if (j >= 1 && rows[j - 1].kind == 1) ub += rows[j - 1].wp * vbar[idx - pp];
+  if (j + 1 < n && rows[j + 1].kind == 1) ub += rows[j + 1].wm * vbar[idx + pp];
+  rbar[idx] = (rj.kind == 1) ? vj : 0.0;
+  ubar[idx] = ub;
+}
+
+// ------------------------------------------------- spectral RHS adjoint ----
+// Part a (per spectral entry): fbar = cfbar_new + temp5 crbar on the rows
+// where the forward wrote cf (0 elsewhere), and cfbar_old = temp3 crbar there
+// (rk_step > 1; cf_old is read only through cr), cfbar_new elsewhere.
+struct AdjRhsArgs {
+  const double2* acr[3 + MAXTH];  // crbar: cr1, cr2, cr3, crth_n (natural order)
+  double2* acf[3 + MAXTH];        // in: cfbar of the output; out: cfbar of the input
+  double2* fbar[3 + MAXTH];       // out (part a) / in (part b)
+  double2* acu[3];                // in/out: cubar
+  double2* acp;                   // in/out: cpbar
+  double2* acth[MAXTH];           // in/out: cthbar
+  const double* kx;
+  const double* kz;
+  const double* dy;
+  const double* dyf;
+  double nu, beta, coriolis;
+  double grav_x, grav_y, grav_z;
+  double pr[MAXTH], dthdx[MAXTH], dthdz[MAXTH];
+  double temp3, temp4, temp5;
+  int rk_step, nth;
+  int js, je;
+  int jst[MAXTH], jet[MAXTH];
+};
+
+__device__ __forceinline__ bool adj_in_range(const AdjRhsArgs& a, const Geo& g, int c, int j) {
+  if (c == 0 || c == 2) return j >= a.js && j <= a.je;
+  if (c == 1) return j >= 2 && j <= g.ny;
+  return j >= a.jst[c - 3] && j <= a.jet[c - 3];
+}
+
+__global__ void __launch_bounds__(256) k_adj_rhs_a(AdjRhsArgs a, Geo g) {
+  typedef cpx<double> C;
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= g.sp * g.ny2) return;
+  const int j = (int)(idx / g.sp);
+  const int nc = 3 + a.nth;
+  for (int c = 0; c < nc; ++c) {
+    C* cf = reinterpret_cast<C*>(a.acf[c]);
+    C* fb = reinterpret_cast<C*>(a.fbar[c]);
+    if (adj_in_range(a, g, c, j)) {
+      const C cr = reinterpret_cast<const C*>(a.acr[c])[idx];
+      fb[idx] = cf[idx] + a.temp5 * cr;
+      cf[idx] = (a.rk_step > 1) ? a.temp3 * cr : C{0.0, 0.0};
+    } else {
+      fb[idx] = C{0.0, 0.0};
+    }
+  }
+}
+
+// Part b: the linear explicit terms of channel.f90:39-253 read cu1..3, cp and
+// th; their transposes, gathered per entry (the buoyancy and -temp4 dcp/dy
+// terms couple planes j and j+1).
+__global__ void __launch_bounds__(256) k_adj_rhs_b(AdjRhsArgs a, Geo g) {
+  typedef cpx<double> C;
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= g.sp * g.ny2) return;
+  const int j = (int)(idx / g.sp);
+  const long long mm = idx % g.sp;
+  const int q = (int)(mm / g.nkk), kk = (int)(mm % g.nkk);
+  const double kxq = a.kx[q], kzk = a.kz[kk];
+  const double lap = kxq * kxq + kzk * kzk;
+  const double visc = (a.beta == 1.0) ? lap : pow(lap, a.beta);
+  const bool mean = (q == 0 && kk == 0);
+  const long long sp = g.sp;
+  const int ny = g.ny;
+  auto L = [&](const double2* p, long long o) { return reinterpret_cast<const C*>(p)[o]; };
+  auto inr = [&](int c, int jj) { return jj >= 0 && jj < g.ny2 && adj_in_range(a, g, c, jj); };
+  const C zero = {0.0, 0.0};
+  const C f1 = L(a.fbar[0], idx), f2 = L(a.fbar[1], idx), f3 = L(a.fbar[2], idx);
+  const C r1 = inr(0, j) ? L(a.acr[0], idx) : zero;
+  const C r2 = inr(1, j) ? L(a.acr[1], idx) : zero;
+  const C r3 = inr(2, j) ? L(a.acr[2], idx) : zero;
+  C b1 = r1 + (-a.nu * visc) * f1 - a.coriolis * f3;
+  C b3 = r3 + (-a.nu * visc) * f3 + a.coriolis * f1;
+  C b2 = r2 + (-a.nu * visc) * f2;
+  // cr1 -= temp4 ikx cp, cr3 -= temp4 ikz cp, cr2 -= temp4 (cp(j) - cp(j-1))/dy(j)
+  C bp = a.temp4 * (ik(kxq, r1) + ik(kzk, r3));
+  bp = bp - (a.temp4 / a.dy[j > 0 ? j : 1]) * r2;
+  if (j + 1 <= ny && inr(1, j + 1)) bp = bp + (a.temp4 / a.dy[j + 1]) * L(a.acr[1], idx + sp);
+  const C f2up = (j + 1 < g.ny2) ? L(a.fbar[1], idx + sp) : zero;
+  for (int n = 0; n < a.nth; ++n) {
+    const C ft = L(a.fbar[3 + n], idx);
+    const C rt = inr(3 + n, j) ? L(a.acr[3 + n], idx) : zero;
+    b1 = b1 - a.dthdx[n] * ft;
+    b3 = b3 - a.dthdz[n] * ft;
+    C bt = rt + (-(a.nu / a.pr[n]) * visc) * ft + a.grav_x * f1 + a.grav_z * f3;
+    if (!mean) {
+      // f2(j) += grav_y (dyf(j-1) th(j) + dyf(j) th(j-1)) / (2 dy(j)) on rows 2..Ny
+      if (j >= 2 && j <= ny) bt = bt + (a.grav_y * a.dyf[j - 1] / (2.0 * a.dy[j])) * f2;
+      if (j + 1 >= 2 && j + 1 <= ny) bt = bt + (a.grav_y * a.dyf[j + 1] / (2.0 * a.dy[j + 1])) * f2up;
+    }
+    C* th = reinterpret_cast<C*>(a.acth[n]);
+    th[idx] = th[idx] + bt;
+  }
+  C* u1 = reinterpret_cast<C*>(a.acu[0]);
+  C* u2 = reinterpret_cast<C*>(a.acu[1]);
+  C* u3 = reinterpret_cast<C*>(a.acu[2]);
+  C* cp = reinterpret_cast<C*>(a.acp);
+  u1[idx] = u1[idx] + b1;
+  u2[idx] = u2[idx] + b2;
+  u3[idx] = u3[idx] + b3;
+  cp[idx] = cp[idx] + bp;
+}
+
+// -------------------------------------------- products adjoint, x pass ----
+// From the x-spectra adjoints of the product slots to the physical adjoints
+// of the products.  A slot X = ikx Fx(A) + Fx(B) gives Abar = Fx^T(-ikx Xbar),
+// Bbar = Fx^T(Xbar); Fx^T lam = c2r(W lam) / Nx.  Each pair's (Abar, Bbar)
+// comes out of one complex transform (real part Abar, imaginary part Bbar).
+//   pair 0: (-ikx X0, X0)          -> u1u1, V1
+//   pair 1: (X1 - ikx X2, X2)      -> u3u1, V3
+//   pair 2: (X3, X5)               -> u3u3, W3
+//   pair 3: (-ikx X4, X4)          -> W1, V2
+//   pair 4+2n: (-ikx X6, X6)       -> th u1, Vt
+//   pair 5+2n: (X7, 0)             -> th u3
+struct AdjPairArgs {
+  const double2* X[MAXSLOT];  // slot adjoints (x-spectra)
+  double* outA[4 + 2 * MAXTH];
+  double* outB[4 + 2 * MAXTH];
+  const double* kx;
+  int npairs;
+};
+
+template <bool WAVE>
+__global__ void __launch_bounds__(512) k_adj_xpairs(AdjPairArgs a, Geo g, const cpx<double>* __restrict__ tw,
+                                                    int j0) {
+  typedef cpx<double> C;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C* smem = reinterpret_cast<C*>(smem_raw);
+  const int G = g.nx >> 3;
+  const int L = fft_lds_len(g.nx);
+  const int ngroups = blockDim.x / G;
+  const int grp = threadIdx.x / G, gt = threadIdx.x % G;
+  C* fbuf = smem + grp * L;
+  const int j = j0 + blockIdx.y, k = blockIdx.x;  // one z line per workgroup
+  const double inv = 1.0 / g.nx;
+  const long long to = (long long)j * g.tp + k;
+  for (int p0 = 0; p0 < a.npairs; p0 += ngroups) {
+    const int pr = p0 + grp;
+    const bool act = pr < a.npairs;
+    __syncthreads();
+    for (int qq = g.nkx + 1 + gt; qq < g.nx - g.nkx; qq += G) fbuf[pidx(qq)] = C{0.0, 0.0};
+    if (act) {
+      for (int q = gt; q < g.nq; q += G) {
+        const long long o = to + (long long)q * g.nz;
+        auto X = [&](int s) { return reinterpret_cast<const C*>(a.X[s])[o]; };
+        const double kxq = a.kx[q];
+        C al, be;
+        const int base = (pr >= 4) ? 6 + 2 * ((pr - 4) >> 1) : 0;
+        switch (pr < 4 ? pr : 4 + ((pr - 4) & 1)) {
+          case 0: al = ik(-kxq, X(0)); be = X(0); break;
+          case 1: al = X(1) + ik(-kxq, X(2)); be = X(2); break;
+          case 2: al = X(3); be = X(5); break;
+          case 3: al = ik(-kxq, X(4)); be = X(4); break;
+          case 4: al = ik(-kxq, X(base)); be = X(base); break;
+          default: al = X(base + 1); be = C{0.0, 0.0}; break;
+        }
+        const double w = (q == 0 ? 1.0 : 0.5) * inv;
+        const C A = w * al, B = w * be;
+        if (q == 0) {
+          fbuf[pidx(0)] = C{A.x, B.x};
+        } else {
+          fbuf[pidx(q)] = C{A.x - B.y, A.y + B.x};
+          fbuf[pidx(g.nx - q)] = C{A.x + B.y, B.x - A.y};
+        }
+      }
+    }
+    __syncthreads();
+    C v[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) v[r] = fbuf[pidx(gt + G * r)];
+    fft_regs<double, 1, WAVE>(v, fbuf, g.nx, g.lognx, tw, gt);
+    if (act) {
+      double* oa = a.outA[pr] + (long long)j * g.pp + (long long)k * g.nx;
+      double* ob = a.outB[pr] + (long long)j * g.pp + (long long)k * g.nx;
+      for (int i = gt; i < g.nx; i += G) {
+        const C z = fbuf[pidx(i)];
+        oa[i] = z.x;
+        ob[i] = z.y;
+      }
+    }
+  }
+}
+
+// ------------------------------------------ products adjoint, gather ----
+// ubar(j) += sum of dP/du(j) Pbar over the products P at rows j-1, j, j+1
+// (channel.f90:299-459, 557-608); Pbar is zero on rows the forward skipped.
+struct AdjProdArgs {
+  const double* u1;
+  const double* u2;
+  const double* u3;
+  const double* th[MAXTH];
+  const double* A[4 + 2 * MAXTH];  // pair order of k_adj_xpairs
+  const double* B[4 + 2 * MAXTH];
+  double* ub1;
+  double* ub2;
+  double* ub3;
+  double* ubth[MAXTH];
+  const double* dy;
+  const double* dyf;
+  int nth;
+};
+
+__global__ void __launch_bounds__(256) k_adj_prod(AdjProdArgs a, Geo g) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long pp = g.pp;
+  const int n = g.ny2;
+  if (idx >= pp * n) return;
+  const int j = (int)(idx / pp);
+  const long long o = idx, up = idx + pp, dn = idx - pp;
+  const bool hu = j + 1 < n, hd = j >= 1;
+  auto V = [&](const double* f, long long i, bool ok) { return ok ? f[i] : 0.0; };
+  const double u1c = a.u1[o], u2c = a.u2[o], u3c = a.u3[o];
+  const double u1p = V(a.u1, up, hu), u2p = V(a.u2, up, hu), u3p = V(a.u3, up, hu);
+  const double u1m = V(a.u1, dn, hd), u2m = V(a.u2, dn, hd), u3m = V(a.u3, dn, hd);
+  auto c_of = [&](int jj) { return 1.0 / (2.0 * a.dyf[jj]); };
+  const double cj = c_of(j), cm = hd ? c_of(j - 1) : 0.0, cp = hu ? c_of(j + 1) : 0.0;
+  // pair Pbar values at j-1, j, j+1
+  auto A = [&](int pr, long long i, bool ok) { return ok ? a.A[pr][i] : 0.0; };
+  auto B = [&](int pr, long long i, bool ok) { return ok ? a.B[pr][i] : 0.0; };
+  double b1 = 0.0, b2 = 0.0, b3 = 0.0;
+  // pair 0: A0 = u1 u1; B0 = c (u1p u2p + u1 u2p - u1 u2 - u1m u2)
+  b1 += 2.0 * u1c * A(0, o, true);
+  b1 += cj * (u2p - u2c) * B(0, o, true) + cm * u2c * B(0, dn, hd) - cp * u2p * B(0, up, hu);
+  b2 += -cj * (u1c + u1m) * B(0, o, true) + cm * (u1c + u1m) * B(0, dn, hd);
+  // pair 1: A1 = u3 u1; B1 as B0 with u3
+  b1 += u3c * A(1, o, true);
+  b3 += u1c * A(1, o, true);
+  b3 += cj * (u2p - u2c) * B(1, o, true) + cm * u2c * B(1, dn, hd) - cp * u2p * B(1, up, hu);
+  b2 += -cj * (u3c + u3m) * B(1, o, true) + cm * (u3c + u3m) * B(1, dn, hd);
+  // pair 2: A2 = u3 u3; B2 = I3 u2, I3(j) = (dyf(j) u3(j) + dyf(j-1) u3(j-1)) / (2 dy(j))
+  b3 += 2.0 * u3c * A(2, o, true);
+  {
+    const double i3 = hd ? (a.dyf[j] * u3c + a.dyf[j - 1] * u3m) / (2.0 * a.dy[j]) : 0.0;
+    b2 += i3 * B(2, o, true);
+    b3 += (hd ? a.dyf[j] / (2.0 * a.dy[j]) : 0.0) * u2c * B(2, o, true);
+    if (hu) b3 += (a.dyf[j] / (2.0 * a.dy[j + 1])) * u2p * B(2, up, true);
+  }
+  // pair 3: A3 = I1 u2; B3 = (1/4 (u2 + u2p)^2 - 1/4 (u2 + u2m)^2) / dy
+  {
+    const double i1 = hd ? (a.dyf[j] * u1c + a.dyf[j - 1] * u1m) / (2.0 * a.dy[j]) : 0.0;
+    b2 += i1 * A(3, o, true);
+    b1 += (hd ? a.dyf[j] / (2.0 * a.dy[j]) : 0.0) * u2c * A(3, o, true);
+    if (hu) b1 += (a.dyf[j] / (2.0 * a.dy[j + 1])) * u2p * A(3, up, true);
+    if (hd) b2 += (0.5 * (u2p - u2m) / a.dy[j]) * B(3, o, true);
+    if (hd) b2 += (0.5 * (u2m + u2c) / a.dy[j - 1 > 0 ? j - 1 : 1]) * B(3, dn, true);
+    if (hu) b2 += (-0.5 * (u2p + u2c) / a.dy[j + 1]) * B(3, up, true);
+  }
+  // scalars: A4 = th u1, B4 = c (thp u2p + th u2p - th u2 - thm u2), A5 = th u3
+  for (int s = 0; s < a.nth; ++s) {
+    const double* th = a.th[s];
+    const double tc = th[o], tp = V(th, up, hu), tm = V(th, dn, hd);
+    const int p4 = 4 + 2 * s, p5 = p4 + 1;
+    double bt = 0.0;
+    b1 += tc * A(p4, o, true);
+    bt += u1c * A(p4, o, true);
+    bt += cj * (u2p - u2c) * B(p4, o, true) + cm * u2c * B(p4, dn, hd) - cp * u2p * B(p4, up, hu);
+    b2 += -cj * (tc + tm) * B(p4, o, true) + cm * (tc + tm) * B(p4, dn, hd);
+    b3 += tc * A(p5, o, true);
+    bt += u3c * A(p5, o, true);
+    a.ubth[s][o] += bt;
+  }
+  a.ub1[o] += b1;
+  a.ub2[o] += b2;
+  a.ub3[o] += b3;
+}
+
+// The (0,0)-mode entries of a spectral field: a[j] (real part) over planes.
+__global__ void k_add_mean(double2* __restrict__ c, const double* __restrict__ w, Geo g) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= g.ny2) return;
+  c[(long long)j * g.sp].x += w[j];
+}
+
+}  // namespace dgpu

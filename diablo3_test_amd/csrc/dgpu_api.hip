@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "../../include/diablo_gpu.h"
+#include "dgpu_adjoint.h"
 #include "dgpu_les.h"
 
 using namespace dgpu;
@@ -103,6 +104,20 @@ struct dgpu_handle {
   DevBuf<double2> lesf[3], lesfth[DGPU_MAX_TH];
   DevBuf<double2> cu_pre[3], cth_pre[DGPU_MAX_TH];
   DevBuf<double> sbl, sgl;              // b', g' of the per-column CN solve
+  // adjoint (allocated by the first adjoint call): the adjoint state, its
+  // scratch, the transposed CN factor tables, and forward-state checkpoints
+  bool adj_ready = false;
+  DevBuf<double2> a_cu[3], a_cp, a_cth[DGPU_MAX_TH], a_cf[3], a_cfth[DGPU_MAX_TH];
+  DevBuf<double2> a_cr[3 + DGPU_MAX_TH], a_fb[3 + DGPU_MAX_TH];
+  DevBuf<double> a_rb[3 + DGPU_MAX_TH], a_ub[3 + DGPU_MAX_TH];
+  DevBuf<double> a_pA[4 + 2 * DGPU_MAX_TH], a_pB[4 + 2 * DGPU_MAX_TH];
+  DevBuf<SegRow> rowsT;
+  DevBuf<double> d_w;  // Ny+2 weights of dgpu_adj_add_mean
+  struct Snap {
+    std::vector<DevBuf<double2>> f;
+    double w_transient = 0.0;
+  };
+  std::vector<Snap> snaps;
   int seg_L = 0, seg_S = 0;
   int nfields = 0;
   size_t lds_limit = 65536;
@@ -519,7 +534,8 @@ void launch_xinv(dgpu_handle* h, const cpx<R>* T, R* u, int j0, int j1) {
 }
 
 template <typename R>
-void launch_zfwd(dgpu_handle* h, const cpx<R>* T, double2* C, int j0, int j1, bool accum) {
+void launch_zfwd(dgpu_handle* h, const cpx<R>* T, double2* C, int j0, int j1, bool accum, double s0 = 1.0,
+                 double s1 = 1.0) {
   if (j1 < j0) return;
   const Geo& g = h->g;
   const int ng = z_ngroups<R>(h);
@@ -527,12 +543,13 @@ void launch_zfwd(dgpu_handle* h, const cpx<R>* T, double2* C, int j0, int j1, bo
   dim3 grid((g.nq + ng - 1) / ng, j1 - j0 + 1), block(ng * (g.nz / 8));
   Timed tm(h, DGPU_K_ZFWD, (2.0 * sizeof(R) * g.tp + (accum ? 32.0 : 16.0) * g.sp) * (j1 - j0 + 1));
   hipLaunchKernelGGL((g.nz <= 512 ? k_zfwd<R, true> : k_zfwd<R, false>), grid, block, lds, h->stream, T,
-                     reinterpret_cast<cpx<double>*>(C), g, twz_of<R>(h), j0, accum ? 1 : 0);
+                     reinterpret_cast<cpx<double>*>(C), g, twz_of<R>(h), j0, accum ? 1 : 0, (R)s0, (R)s1);
   HIPCHECK(hipGetLastError());
 }
 
 template <typename R>
-void launch_zinv(dgpu_handle* h, const double2* C, cpx<R>* T, int j0, int j1) {
+void launch_zinv(dgpu_handle* h, const double2* C, cpx<R>* T, int j0, int j1, double s0 = 1.0,
+                 double s1 = 1.0) {
   if (j1 < j0) return;
   const Geo& g = h->g;
   const int ng = z_ngroups<R>(h);
@@ -540,12 +557,13 @@ void launch_zinv(dgpu_handle* h, const double2* C, cpx<R>* T, int j0, int j1) {
   dim3 grid((g.nq + ng - 1) / ng, j1 - j0 + 1), block(ng * (g.nz / 8));
   Timed tm(h, DGPU_K_ZINV, (2.0 * sizeof(R) * g.tp + 16.0 * g.sp) * (j1 - j0 + 1));
   hipLaunchKernelGGL((g.nz <= 512 ? k_zinv<R, true> : k_zinv<R, false>), grid, block, lds, h->stream,
-                     reinterpret_cast<const cpx<double>*>(C), T, g, twz_of<R>(h), j0);
+                     reinterpret_cast<const cpx<double>*>(C), T, g, twz_of<R>(h), j0, s0, s1);
   HIPCHECK(hipGetLastError());
 }
 
 template <typename R>
-void launch_zinv_op(dgpu_handle* h, const double2* C, cpx<R>* T, int j0, int j1, int op) {
+void launch_zinv_op(dgpu_handle* h, const double2* C, cpx<R>* T, int j0, int j1, int op, double s0 = 1.0,
+                    double s1 = 1.0) {
   if (j1 < j0) return;
   const Geo& g = h->g;
   const int ng = z_ngroups<R>(h);
@@ -554,7 +572,7 @@ void launch_zinv_op(dgpu_handle* h, const double2* C, cpx<R>* T, int j0, int j1,
   Timed tm(h, DGPU_K_ZINV, (2.0 * sizeof(R) * g.tp + 16.0 * g.sp) * (j1 - j0 + 1));
   hipLaunchKernelGGL((g.nz <= 512 ? k_zinv_op<R, true> : k_zinv_op<R, false>), grid, block, lds, h->stream,
                      reinterpret_cast<const cpx<double>*>(C), T, g, twz_of<R>(h), j0, op, h->d_kx.p,
-                     h->d_kz.p, h->d_dy.p, h->d_dyf.p);
+                     h->d_kz.p, h->d_dy.p, h->d_dyf.p, s0, s1);
   HIPCHECK(hipGetLastError());
 }
 
@@ -591,10 +609,10 @@ void to_fourier(dgpu_handle* h, const R* u, double2* C, cpx<R>* T, int j0, int j
 
 template <typename R>
 void launch_thomas(dgpu_handle* h, int field, int rs, const R* r, R* u, int ca, int cb, double transient,
-                   bool incr) {
+                   bool incr, const SegRow* table = nullptr) {
   const Geo& g = h->g;
   const int n = g.ny2;
-  const SegRow* rows = h->rows.p + ((size_t)field * 3 + rs) * n;
+  const SegRow* rows = (table ? table : h->rows.p) + ((size_t)field * 3 + rs) * n;
   dim3 grid((unsigned)(g.pp / 64)), block(64, h->seg_S);
   Timed tm(h, DGPU_K_THOMAS, 3.0 * sizeof(R) * g.pp * n);
   switch (h->seg_L) {
@@ -1026,6 +1044,298 @@ void rk_substep(dgpu_handle* h, int rk_step, double time, int time_step) {
     rk_substep_t<double>(h, rk_step, time, time_step);
 }
 
+// ------------------------------------------------------------- adjoint --
+// Fields of one substep: 0 u1, 1 u2, 2 u3, 3+n th_n; their solved planes.
+void solve_range(const dgpu_handle* h, int f, int* ca, int* cb) {
+  if (f == 1) {
+    *ca = 1;
+    *cb = h->g.ny + 1;
+  } else if (f < 3) {
+    *ca = h->jstart - 1;
+    *cb = h->jend + 1;
+  } else {
+    *ca = h->jstart_th[f - 3] - 1;
+    *cb = h->jend_th[f - 3] + 1;
+  }
+}
+
+void adj_alloc(dgpu_handle* h) {
+  if (h->adj_ready) return;
+  const Geo& g = h->g;
+  const int nth = h->nth, nf = 3 + nth, np_ = 4 + 2 * nth;
+  const size_t ns = (size_t)g.ny2 * g.sp, npx = (size_t)g.ny2 * g.pp;
+  auto zs = [&](DevBuf<double2>& b) {
+    b.alloc(ns);
+    HIPCHECK(hipMemset(b.p, 0, sizeof(double2) * ns));
+  };
+  auto zp = [&](DevBuf<double>& b) {
+    b.alloc(npx);
+    HIPCHECK(hipMemset(b.p, 0, sizeof(double) * npx));
+  };
+  for (int f = 0; f < 3; ++f) {
+    zs(h->a_cu[f]);
+    zs(h->a_cf[f]);
+  }
+  zs(h->a_cp);
+  for (int n = 0; n < nth; ++n) {
+    zs(h->a_cth[n]);
+    zs(h->a_cfth[n]);
+  }
+  for (int f = 0; f < nf; ++f) {
+    zs(h->a_cr[f]);
+    zs(h->a_fb[f]);
+    zp(h->a_rb[f]);
+    zp(h->a_ub[f]);
+  }
+  for (int q = 0; q < np_; ++q) {  // planes 0 and Ny+1 stay zero
+    zp(h->a_pA[q]);
+    zp(h->a_pB[q]);
+  }
+  // transposed CN matrices, all rows plain: (A^T) sub(j) = c(j-1), sup(j) = a(j+1)
+  const int n = g.ny2;
+  std::vector<SegRow> rt((size_t)h->nfields * 3 * n);
+  for (int f = 0; f < h->nfields; ++f)
+    for (int rs = 0; rs < 3; ++rs) {
+      Tri t = build_tri(h, f, rs);
+      Tri tt = t;
+      for (int j = 0; j < n; ++j) {
+        tt.a[j] = (j >= 1) ? t.c[j - 1] : 0.0;
+        tt.c[j] = (j + 1 < n) ? t.a[j + 1] : 0.0;
+        tt.kind[j] = 3;
+        tt.wp[j] = tt.wm[j] = tt.vb[j] = 0.0;
+      }
+      factor_rows(tt, h->seg_L, rt.data() + ((size_t)f * 3 + rs) * n);
+    }
+  upload(h->rowsT, rt);
+  h->d_w.alloc(n);
+  h->adj_ready = true;
+}
+
+double2* adj_spec(dgpu_handle* h, int f) { return f < 3 ? h->a_cu[f].p : h->a_cth[f - 3].p; }
+const double2* fwd_spec(dgpu_handle* h, int f) { return f < 3 ? h->cu[f].p : h->cth[f - 3].p; }
+
+// The vector-Jacobian product of rk_chan_1 (FP64 DNS): on entry the forward
+// state is the substep's input and the adjoint state is the gradient of the
+// loss with respect to the substep's output; on exit the adjoint state is the
+// gradient with respect to the input.  The forward state is not changed.
+void adj_substep(dgpu_handle* h, int rk_step, double time, int time_step) {
+  (void)time;
+  (void)time_step;
+  const dgpu_params& p = h->p;
+  const Geo& g = h->g;
+  if (h->precision != DGPU_FP64 || p.use_les)
+    throw ApiError{DGPU_EUNSUP, "the adjoint covers the FP64 DNS substep"};
+  adj_alloc(h);
+  const int rs = rk_step - 1;
+  const int nth = h->nth, nf = 3 + nth, Ny = g.ny, n2 = g.ny2;
+  const double temp3 = h->zeta_bar[rs] * h->h_bar[rs];
+  const double temp4 = h->h_bar[rs];
+  const double temp5 = h->beta_bar[rs] * h->h_bar[rs];
+  const double NxNz = (double)g.nx * g.nz;
+  const size_t ns = (size_t)n2 * g.sp;
+  const size_t npx = (size_t)n2 * g.pp;
+  cpx<double>* T0 = xspec<double>(h->tpool[0]);
+  auto grid1 = [](long long n) { return dim3((unsigned)((n + 255) / 256)); };
+
+  // 1. projection and pressure update (channel.f90:901-911, 946-1023)
+  {
+    AdjRemDivArgs a;
+    a.a1 = h->a_cu[0].p;
+    a.a2 = h->a_cu[1].p;
+    a.a3 = h->a_cu[2].p;
+    a.ap = h->a_cp.p;
+    a.sb = h->sb.p;
+    a.sg = h->sg.p;
+    a.kx = h->d_kx.p;
+    a.kz = h->d_kz.p;
+    a.dy = h->d_dy.p;
+    a.dyf = h->d_dyf.p;
+    a.inv_h = 1.0 / temp4;
+    a.js = h->jstart;
+    a.je = h->jend;
+    Timed tm(h, DGPU_K_ADJOINT, 128.0 * g.sp * Ny);
+    hipLaunchKernelGGL(k_adj_remdiv, grid1(g.sp), dim3(256), 0, h->stream, a, g);
+    HIPCHECK(hipGetLastError());
+  }
+  // 2. the forward transform of the solved planes: xbar = F^T cubar there, and
+  //    those planes of cubar are consumed (the other planes pass through)
+  for (int f = 0; f < nf; ++f) {
+    int ca, cb;
+    solve_range(h, f, &ca, &cb);
+    double* xb = (f < 3) ? h->r[f].p : h->rth[f - 3].p;
+    HIPCHECK(hipMemsetAsync(xb, 0, sizeof(double) * npx, h->stream));
+    launch_zinv<double>(h, adj_spec(h, f), T0, ca, cb, 1.0 / NxNz, 0.5 / NxNz);
+    launch_xinv<double>(h, T0, xb, ca, cb);
+    HIPCHECK(hipMemsetAsync(adj_spec(h, f) + (size_t)ca * g.sp, 0, sizeof(double2) * g.sp * (cb - ca + 1),
+                            h->stream));
+  }
+  // 3. CN solves: vbar = A^-T xbar (into the physical state buffer, which
+  //    step 7 refills), then rbar and ubar = E^T vbar
+  for (int f = 0; f < nf; ++f) {
+    double* xb = (f < 3) ? h->r[f].p : h->rth[f - 3].p;
+    double* vb = (f < 3) ? h->u[f].p : h->th[f - 3].p;
+    launch_thomas<double>(h, f, rs, xb, vb, 0, n2 - 1, 0.0, false, h->rowsT.p);
+    const SegRow* rows = h->rows.p + ((size_t)f * 3 + rs) * n2;
+    Timed tm(h, DGPU_K_ADJOINT, 32.0 * npx);
+    hipLaunchKernelGGL(k_adj_cnpost, grid1((long long)npx), dim3(256), 0, h->stream, vb, h->a_rb[f].p,
+                       h->a_ub[f].p, rows, g);
+    HIPCHECK(hipGetLastError());
+  }
+  // 4. crbar = P^T rbar on planes 1..Ny (the inverse transform of cr)
+  for (int f = 0; f < nf; ++f) {
+    launch_xfwd<double>(h, h->a_rb[f].p, T0, 1, Ny);
+    launch_zfwd<double>(h, T0, h->a_cr[f].p, 1, Ny, false, NxNz, 2.0 * NxNz);
+  }
+  // 5. the explicit spectral terms
+  {
+    AdjRhsArgs a;
+    std::memset(&a, 0, sizeof(a));
+    // natural order u1, u2, u3, th_n
+    for (int f = 0; f < nf; ++f) {
+      a.acr[f] = h->a_cr[f].p;
+      a.fbar[f] = h->a_fb[f].p;
+    }
+    a.acf[0] = h->a_cf[0].p;
+    a.acf[1] = h->a_cf[1].p;
+    a.acf[2] = h->a_cf[2].p;
+    for (int n = 0; n < nth; ++n) {
+      a.acf[3 + n] = h->a_cfth[n].p;
+      a.acth[n] = h->a_cth[n].p;
+      a.pr[n] = p.pr[n];
+      a.dthdx[n] = p.dthdx[n];
+      a.dthdz[n] = p.dthdz[n];
+      a.jst[n] = h->jstart_th[n];
+      a.jet[n] = h->jend_th[n];
+    }
+    a.acu[0] = h->a_cu[0].p;
+    a.acu[1] = h->a_cu[1].p;
+    a.acu[2] = h->a_cu[2].p;
+    a.acp = h->a_cp.p;
+    a.kx = h->d_kx.p;
+    a.kz = h->d_kz.p;
+    a.dy = h->d_dy.p;
+    a.dyf = h->d_dyf.p;
+    a.nu = h->nu;
+    a.beta = p.beta;
+    a.coriolis = p.ro_inv / p.delta;
+    a.grav_x = p.grav_x;
+    a.grav_y = p.grav_y;
+    a.grav_z = p.grav_z;
+    a.temp3 = temp3;
+    a.temp4 = temp4;
+    a.temp5 = temp5;
+    a.rk_step = rk_step;
+    a.nth = nth;
+    a.js = h->jstart;
+    a.je = h->jend;
+    Timed tm(h, DGPU_K_ADJOINT, 16.0 * (5.0 * nf + 8) * ns);
+    hipLaunchKernelGGL(k_adj_rhs_a, grid1((long long)ns), dim3(256), 0, h->stream, a, g);
+    HIPCHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_adj_rhs_b, grid1((long long)ns), dim3(256), 0, h->stream, a, g);
+    HIPCHECK(hipGetLastError());
+  }
+  // 6. product slots: Xbar = Fz^T Zbar, Zbar_s = -fbar or ikz fbar (the
+  //    combinations of k_rhs: f1 -= Z0 + ikz Z1, f3 -= Z2 + ikz Z3,
+  //    f2 -= Z4 + ikz Z5, ft -= Z6 + ikz Z7)
+  const int nslot = 6 + 2 * nth;
+  for (int s = 0; s < nslot; ++s) {
+    int src;
+    if (s < 6)
+      src = (s < 2) ? 0 : (s < 4 ? 2 : 1);
+    else
+      src = 3 + (s - 6) / 2;
+    const bool dz = (s & 1) != 0;
+    const double sc = (dz ? 1.0 : -1.0) / g.nz;
+    launch_zinv_op<double>(h, h->a_fb[src].p, xspec<double>(h->tpool[s]), 0, n2 - 1, dz ? OP_DZ : OP_ID, sc, sc);
+  }
+  // 7. the forward fields at the substep's input
+  for (int f = 0; f < 3; ++f) to_physical<double>(h, h->cu[f].p, h->u[f].p, xspec<double>(h->tpool[nslot]));
+  for (int n = 0; n < nth; ++n) to_physical<double>(h, h->cth[n].p, h->th[n].p, xspec<double>(h->tpool[nslot]));
+  // 8. product adjoints (x pass) and their Jacobians
+  {
+    AdjPairArgs a;
+    std::memset(&a, 0, sizeof(a));
+    for (int s = 0; s < nslot; ++s) a.X[s] = h->tpool[s].p;
+    const int npairs = 4 + 2 * nth;
+    for (int q = 0; q < npairs; ++q) {
+      a.outA[q] = h->a_pA[q].p;
+      a.outB[q] = h->a_pB[q].p;
+    }
+    a.kx = h->d_kx.p;
+    a.npairs = npairs;
+    const int G = g.nx / 8;
+    int ng = npairs;
+    while (ng > 1 && (ng * G > 512 || (size_t)ng * fft_lds_len(g.nx) * sizeof(double2) > h->lds_limit)) --ng;
+    const size_t lds = (size_t)ng * fft_lds_len(g.nx) * sizeof(double2);
+    dim3 grid(g.nz, Ny), block(ng * G);
+    Timed tm(h, DGPU_K_ADJOINT, (16.0 * g.tp * nslot + 16.0 * g.pp * npairs) * Ny);
+    hipLaunchKernelGGL((g.nx <= 512 ? k_adj_xpairs<true> : k_adj_xpairs<false>), grid, block, lds, h->stream, a,
+                       g, twx_of<double>(h), 1);
+    HIPCHECK(hipGetLastError());
+  }
+  {
+    AdjProdArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.u1 = h->u[0].p;
+    a.u2 = h->u[1].p;
+    a.u3 = h->u[2].p;
+    for (int n = 0; n < nth; ++n) {
+      a.th[n] = h->th[n].p;
+      a.ubth[n] = h->a_ub[3 + n].p;
+    }
+    for (int q = 0; q < 4 + 2 * nth; ++q) {
+      a.A[q] = h->a_pA[q].p;
+      a.B[q] = h->a_pB[q].p;
+    }
+    a.ub1 = h->a_ub[0].p;
+    a.ub2 = h->a_ub[1].p;
+    a.ub3 = h->a_ub[2].p;
+    a.dy = h->d_dy.p;
+    a.dyf = h->d_dyf.p;
+    a.nth = nth;
+    Timed tm(h, DGPU_K_ADJOINT, 8.0 * npx * (3 + nth + 2 * (4 + 2 * nth) + 2 * (3 + nth)));
+    hipLaunchKernelGGL(k_adj_prod, grid1((long long)npx), dim3(256), 0, h->stream, a, g);
+    HIPCHECK(hipGetLastError());
+  }
+  // 9. cubar += P^T ubar (the inverse transforms of the state)
+  for (int f = 0; f < nf; ++f) {
+    launch_xfwd<double>(h, h->a_ub[f].p, T0, 0, n2 - 1);
+    launch_zfwd<double>(h, T0, adj_spec(h, f), 0, n2 - 1, true, NxNz, 2.0 * NxNz);
+  }
+}
+
+// Checkpoint slot `slot` of the forward state (save or restore).
+void checkpoint(dgpu_handle* h, int slot, bool save) {
+  const size_t ns = (size_t)h->g.ny2 * h->g.sp;
+  const int nth = h->nth;
+  std::vector<double2*> live = {h->cu[0].p, h->cu[1].p, h->cu[2].p, h->cp.p, h->cf[0].p, h->cf[1].p, h->cf[2].p};
+  for (int n = 0; n < nth; ++n) {
+    live.push_back(h->cth[n].p);
+    live.push_back(h->cfth[n].p);
+  }
+  if (slot < 0 || slot > 4096) throw ApiError{DGPU_EINVAL, "checkpoint slot out of range"};
+  if ((size_t)slot >= h->snaps.size()) {
+    if (!save) throw ApiError{DGPU_EINVAL, "checkpoint slot was never saved"};
+    h->snaps.resize(slot + 1);
+  }
+  auto& sn = h->snaps[slot];
+  if (sn.f.empty()) {
+    if (!save) throw ApiError{DGPU_EINVAL, "checkpoint slot was never saved"};
+    sn.f.resize(live.size());
+    for (auto& b : sn.f) b.alloc(ns);
+  }
+  for (size_t i = 0; i < live.size(); ++i) {
+    if (save)
+      HIPCHECK(hipMemcpyAsync(sn.f[i].p, live[i], sizeof(double2) * ns, hipMemcpyDeviceToDevice, h->stream));
+    else
+      HIPCHECK(hipMemcpyAsync(live[i], sn.f[i].p, sizeof(double2) * ns, hipMemcpyDeviceToDevice, h->stream));
+  }
+  if (save)
+    sn.w_transient = h->w_transient;
+  else
+    h->w_transient = sn.w_transient;
+}
+
 // -------------------------------------------------- reference layouts ----
 size_t ref_spec_doubles(const Geo& g) { return (size_t)(g.nx + 2) * (g.nz + 2) * (g.ny + 2); }
 
@@ -1076,6 +1386,21 @@ void phys_to_ref(const dgpu_handle* h, const std::vector<double>& in, double* re
 }
 
 // (buffer, is_spectral) for field id f, scalar n
+bool adj_field_buf(dgpu_handle* h, int f, int n, void** dev) {
+  switch (f) {
+    case DGPU_CU1: *dev = h->a_cu[0].p; return true;
+    case DGPU_CU2: *dev = h->a_cu[1].p; return true;
+    case DGPU_CU3: *dev = h->a_cu[2].p; return true;
+    case DGPU_CP: *dev = h->a_cp.p; return true;
+    case DGPU_CTH: *dev = h->a_cth[n].p; return true;
+    case DGPU_CF1: *dev = h->a_cf[0].p; return true;
+    case DGPU_CF2: *dev = h->a_cf[1].p; return true;
+    case DGPU_CF3: *dev = h->a_cf[2].p; return true;
+    case DGPU_CFTH: *dev = h->a_cfth[n].p; return true;
+  }
+  return false;
+}
+
 bool field_buf(dgpu_handle* h, int f, int n, void** dev, bool* spec) {
   *spec = true;
   switch (f) {
@@ -1171,6 +1496,22 @@ void free_all(dgpu_handle* h) {
   }
   h->sbl.release();
   h->sgl.release();
+  for (auto& b : h->a_cu) b.release();
+  for (auto& b : h->a_cf) b.release();
+  for (auto& b : h->a_cth) b.release();
+  for (auto& b : h->a_cfth) b.release();
+  for (auto& b : h->a_cr) b.release();
+  for (auto& b : h->a_fb) b.release();
+  for (auto& b : h->a_rb) b.release();
+  for (auto& b : h->a_ub) b.release();
+  for (auto& b : h->a_pA) b.release();
+  for (auto& b : h->a_pB) b.release();
+  h->a_cp.release();
+  h->rowsT.release();
+  h->d_w.release();
+  for (auto& sn : h->snaps)
+    for (auto& b : sn.f) b.release();
+  h->snaps.clear();
 }
 
 void create_impl(dgpu_handle* h, const double* gy) {
@@ -1215,7 +1556,8 @@ void create_impl(dgpu_handle* h, const double* gy) {
         (const void*)k_zfwd<float, true>,     (const void*)k_zfwd<float, false>,
         (const void*)k_zinv<float, true>,     (const void*)k_zinv<float, false>,
         (const void*)k_zinv_op<float, true>,  (const void*)k_zinv_op<float, false>,
-        (const void*)k_xprod_all<float, true>, (const void*)k_xprod_all<float, false>};
+        (const void*)k_xprod_all<float, true>, (const void*)k_xprod_all<float, false>,
+        (const void*)k_adj_xpairs<true>,      (const void*)k_adj_xpairs<false>};
     for (const void* k : kernels)
       HIPCHECK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds_limit));
   }
@@ -1506,8 +1848,111 @@ int dgpu_timing_read(dgpu_handle* h, double* ms, double* bytes, int64_t* count) 
 const char* dgpu_kernel_class_name(int cls) {
   static const char* names[DGPU_NCLASS] = {"x_forward", "x_inverse", "z_forward", "z_inverse",
                                            "x_products", "spectral_rhs", "cn_solve",
-                                           "projection", "wall_bc", "les_model"};
+                                           "projection", "wall_bc", "les_model", "adjoint"};
   return (cls >= 0 && cls < DGPU_NCLASS) ? names[cls] : "";
+}
+
+int dgpu_adj_set_field(dgpu_handle* h, int field, const double* host) {
+  if (!h || !host) return DGPU_EINVAL;
+  if (field < 0 || field > DGPU_CFTH) {
+    h->err = "bad adjoint field id";
+    return DGPU_EINVAL;
+  }
+  return run_guarded(h, [&] {
+    adj_alloc(h);
+    const size_t per = ref_spec_doubles(h->g);
+    for (int n = 0; n < nfield_copies(h, field); ++n) {
+      void* dev;
+      adj_field_buf(h, field, n, &dev);
+      std::vector<double2> v;
+      spec_from_ref(h, host + n * per, v);
+      HIPCHECK(hipMemcpyAsync(dev, v.data(), sizeof(double2) * v.size(), hipMemcpyHostToDevice, h->stream));
+      HIPCHECK(hipStreamSynchronize(h->stream));
+    }
+  });
+}
+
+int dgpu_adj_get_field(dgpu_handle* h, int field, double* host) {
+  if (!h || !host) return DGPU_EINVAL;
+  if (field < 0 || field > DGPU_CFTH) {
+    h->err = "bad adjoint field id";
+    return DGPU_EINVAL;
+  }
+  return run_guarded(h, [&] {
+    adj_alloc(h);
+    const size_t per = ref_spec_doubles(h->g);
+    for (int n = 0; n < nfield_copies(h, field); ++n) {
+      void* dev;
+      adj_field_buf(h, field, n, &dev);
+      std::vector<double2> v((size_t)h->g.ny2 * h->g.sp);
+      HIPCHECK(hipMemcpyAsync(v.data(), dev, sizeof(double2) * v.size(), hipMemcpyDeviceToHost, h->stream));
+      HIPCHECK(hipStreamSynchronize(h->stream));
+      spec_to_ref(h, v, host + n * per);
+    }
+  });
+}
+
+int dgpu_adj_zero(dgpu_handle* h) {
+  if (!h) return DGPU_EINVAL;
+  return run_guarded(h, [&] {
+    adj_alloc(h);
+    const size_t ns = (size_t)h->g.ny2 * h->g.sp;
+    for (int f = DGPU_CU1; f <= DGPU_CFTH; ++f)
+      for (int n = 0; n < nfield_copies(h, f); ++n) {
+        void* dev;
+        adj_field_buf(h, f, n, &dev);
+        HIPCHECK(hipMemsetAsync(dev, 0, sizeof(double2) * ns, h->stream));
+      }
+  });
+}
+
+int dgpu_adj_rk_substep(dgpu_handle* h, int rk_step, double time, int time_step) {
+  if (!h) return DGPU_EINVAL;
+  if (rk_step < 1 || rk_step > 3) {
+    h->err = "rk_step must be 1, 2 or 3";
+    return DGPU_EINVAL;
+  }
+  return run_guarded(h, [&] { adj_substep(h, rk_step, time, time_step); });
+}
+
+int dgpu_adj_add_mean(dgpu_handle* h, int field, const double* w) {
+  if (!h || !w) return DGPU_EINVAL;
+  if (field != DGPU_CU1 && field != DGPU_CU2 && field != DGPU_CU3 && field != DGPU_CP) {
+    h->err = "adj_add_mean takes cu1, cu2, cu3 or cp";
+    return DGPU_EINVAL;
+  }
+  return run_guarded(h, [&] {
+    adj_alloc(h);
+    void* dev;
+    adj_field_buf(h, field, 0, &dev);
+    HIPCHECK(hipMemcpyAsync(h->d_w.p, w, sizeof(double) * h->g.ny2, hipMemcpyHostToDevice, h->stream));
+    hipLaunchKernelGGL(k_add_mean, dim3((h->g.ny2 + 63) / 64), dim3(64), 0, h->stream,
+                       reinterpret_cast<double2*>(dev), h->d_w.p, h->g);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(h->stream));
+  });
+}
+
+int dgpu_mean_profile(dgpu_handle* h, int field, double* out) {
+  if (!h || !out) return DGPU_EINVAL;
+  if (field != DGPU_CU1 && field != DGPU_CU2 && field != DGPU_CU3 && field != DGPU_CP) {
+    h->err = "mean_profile takes cu1, cu2, cu3 or cp";
+    return DGPU_EINVAL;
+  }
+  return run_guarded(h, [&] {
+    void* dev;
+    bool spec;
+    field_buf(h, field, 0, &dev, &spec);
+    // the (0,0) entry of every plane: a strided copy of one double per plane
+    HIPCHECK(hipMemcpy2DAsync(out, sizeof(double), dev, sizeof(double2) * h->g.sp, sizeof(double), h->g.ny2,
+                              hipMemcpyDeviceToHost, h->stream));
+    HIPCHECK(hipStreamSynchronize(h->stream));
+  });
+}
+
+int dgpu_checkpoint(dgpu_handle* h, int slot, int save) {
+  if (!h) return DGPU_EINVAL;
+  return run_guarded(h, [&] { checkpoint(h, slot, save != 0); });
 }
 
 int dgpu_synchronize(dgpu_handle* h) {

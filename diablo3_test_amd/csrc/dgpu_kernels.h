@@ -145,7 +145,8 @@ __global__ void __launch_bounds__(512) k_xinv(const cpx<R>* __restrict__ T, R* _
 // added to it (the FP32 transform of an increment of an FP64 state).
 template <typename R, bool WAVE>
 __global__ void __launch_bounds__(512) k_zfwd(const cpx<R>* __restrict__ T, cpx<double>* __restrict__ C,
-                                               Geo g, const cpx<R>* __restrict__ tw, int j0, int accum) {
+                                               Geo g, const cpx<R>* __restrict__ tw, int j0, int accum,
+                                               R s0, R s1) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<R>* smem = reinterpret_cast<cpx<R>*>(smem_raw);
   const int G = g.nz >> 3;
@@ -165,7 +166,9 @@ __global__ void __launch_bounds__(512) k_zfwd(const cpx<R>* __restrict__ T, cpx<
   }
   fft_regs<R, -1, WAVE>(v, fbuf, g.nz, g.lognz, tw, gt);
   if (act) {
-    const R inv = (R)1 / (R)g.nz;
+    // 1/Nz, times s0 (kx = 0) or s1 (kx > 0): 1 for the transform itself,
+    // Nx Nz W' for the transpose of the inverse transform
+    const R inv = ((q == 0) ? s0 : s1) / (R)g.nz;
     cpx<double>* out = C + (long long)j * g.sp + (long long)q * g.nkk;
     for (int kk = gt; kk < g.nkk; kk += G) {
       const cpx<R> z = inv * fbuf[pidx(kz_nat(g, kk))];
@@ -177,9 +180,12 @@ __global__ void __launch_bounds__(512) k_zfwd(const cpx<R>* __restrict__ T, cpx<
 }
 
 // Unpack + zero (fft.f90:330-341) + backward z transform (:345), unnormalised.
+// s0 (kx = 0) and s1 (kx > 0) scale the input: 1 for the transform, W/(Nx Nz)
+// for the transpose of the forward transform.
 template <typename R, bool WAVE>
 __global__ void __launch_bounds__(512) k_zinv(const cpx<double>* __restrict__ C, cpx<R>* __restrict__ T,
-                                               Geo g, const cpx<R>* __restrict__ tw, int j0) {
+                                               Geo g, const cpx<R>* __restrict__ tw, int j0, double s0,
+                                               double s1) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<R>* smem = reinterpret_cast<cpx<R>*>(smem_raw);
   const int G = g.nz >> 3;
@@ -191,6 +197,7 @@ __global__ void __launch_bounds__(512) k_zinv(const cpx<double>* __restrict__ C,
   cpx<R> v[8];
   const cpx<double>* col = C + (long long)j * g.sp + (long long)(act ? q : 0) * g.nkk;
   const int off = g.nz - 2 * g.nkz - 1;
+  const double sc = (q == 0) ? s0 : s1;
 #pragma unroll
   for (int r = 0; r < 8; ++r) {
     const int k = gt + G * r;
@@ -201,7 +208,7 @@ __global__ void __launch_bounds__(512) k_zinv(const cpx<double>* __restrict__ C,
       else if (k >= g.nz - g.nkz)
         z = col[k - off];
     }
-    v[r] = {(R)z.x, (R)z.y};
+    v[r] = {(R)(sc * z.x), (R)(sc * z.y)};
   }
   fft_regs<R, 1, WAVE>(v, fbuf, g.nz, g.lognz, tw, gt);
   if (act) {
@@ -659,7 +666,8 @@ struct SegRow {
   double qf;   // prod of f from j to the segment's last row
   double wp, wm;  // explicit stencil weights (assembled rows)
   int kind;    // 1 = assembled row (vec = r + explicit), 0 = fixed row (vec = vb),
-               // 2 = fixed row with vec = vb * (current time-dependent BC value)
+               // 2 = fixed row with vec = vb * (current time-dependent BC value),
+               // 3 = plain row (vec = r; the transposed solves of the adjoint)
   double vb;   // fixed-row right-hand side
   double ra, rb, rc;  // the matrix row itself (used for fixed rows by the increment form)
 };
@@ -697,6 +705,8 @@ __global__ void __launch_bounds__(1024) k_thomas_seg(const T* __restrict__ r,
       const double ex = rw.wp * (w[t + 2] - w[t + 1]) - rw.wm * (w[t + 1] - w[t]);
       if (rw.kind == 1)
         val = (double)r[(long long)jj * pp + col] + (INCR ? 2.0 * ex : ex);
+      else if (rw.kind == 3)
+        val = (double)r[(long long)jj * pp + col];
       else
         val = (rw.kind == 2) ? rw.vb * kscale : rw.vb;
       if (INCR && rw.kind != 1) val -= rw.ra * w[t] + rw.rb * w[t + 1] + rw.rc * w[t + 2];

@@ -36,7 +36,7 @@ __global__ void __launch_bounds__(512) k_zinv_op(const cpx<double>* __restrict__
                                                  const double* __restrict__ kx,
                                                  const double* __restrict__ kz,
                                                  const double* __restrict__ dy,
-                                                 const double* __restrict__ dyf) {
+                                                 const double* __restrict__ dyf, double s0, double s1) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<R>* smem = reinterpret_cast<cpx<R>*>(smem_raw);
   const int G = g.nz >> 3;
@@ -53,6 +53,7 @@ __global__ void __launch_bounds__(512) k_zinv_op(const cpx<double>* __restrict__
   double sdy = 0.0;
   if (act && op == OP_DY_GY) sdy = 1.0 / dy[j];
   if (act && op == OP_DY_GYF) sdy = 1.0 / dyf[j];
+  const double sc = (q == 0) ? s0 : s1;
   const int off = g.nz - 2 * g.nkz - 1;
   cpx<R> v[8];
 #pragma unroll
@@ -74,7 +75,7 @@ __global__ void __launch_bounds__(512) k_zinv_op(const cpx<double>* __restrict__
         default: z = c; break;
       }
     }
-    v[r] = {(R)z.x, (R)z.y};
+    v[r] = {(R)(sc * z.x), (R)(sc * z.y)};
   }
   fft_regs<R, 1, WAVE>(v, fbuf, g.nz, g.lognz, tw, gt);
   if (q < g.nq) {

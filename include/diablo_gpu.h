@@ -136,6 +136,36 @@ int dgpu_step(dgpu_handle* h, int nsteps, double* time, int32_t* time_step);
 int dgpu_synchronize(dgpu_handle* h);
 
 /*
+ * Adjoint of the FP64 DNS substep (no counterpart in the reference, which has
+ * no differentiation; it is the added "differentiation" form of the substep
+ * for gradient-based studies).  The handle holds an adjoint state with the
+ * spectral fields of the forward state (ids DGPU_CU1..DGPU_CFTH, reference
+ * layout): for a real loss L of a complex field c it holds
+ * dL/dRe c + i dL/dIm c.
+ *
+ * dgpu_adj_rk_substep: with the forward state equal to the INPUT of substep
+ * rk_step (as rk_chan_1 would receive it) and the adjoint state equal to the
+ * gradient with respect to that substep's OUTPUT, replace the adjoint state
+ * by the gradient with respect to the input.  The forward state is left
+ * unchanged.  Returns DGPU_EUNSUP for the mixed form and for use_les.
+ */
+int dgpu_adj_set_field(dgpu_handle* h, int field, const double* host);
+int dgpu_adj_get_field(dgpu_handle* h, int field, double* host);
+int dgpu_adj_zero(dgpu_handle* h);
+int dgpu_adj_rk_substep(dgpu_handle* h, int rk_step, double time, int time_step);
+/* Adds w[j] (j = 0..Ny+1) to the real part of the (0,0) mode of the adjoint
+   of field (cu1, cu2, cu3 or cp): the gradient of a loss of the mean profile,
+   such as the wall shear stress of statistics.f90:143-146. */
+int dgpu_adj_add_mean(dgpu_handle* h, int field, const double* w);
+/* Real part of the (0,0) mode of a forward field on planes 0..Ny+1 (the mean
+   profile ume of statistics.f90:728), Ny+2 doubles. */
+int dgpu_mean_profile(dgpu_handle* h, int field, double* out);
+/* Save (save != 0) or restore the forward state (cu1..3, cp, cth, cf1..3,
+   cfth and the f_type 4 wall value) in device checkpoint slot `slot`
+   (0..4096, allocated on first save): recomputation for the adjoint. */
+int dgpu_checkpoint(dgpu_handle* h, int slot, int save);
+
+/*
  * Per-kernel-class timing (no counterpart in the reference, whose only timer
  * is the wall clock of control.f90:16-76).  While enabled, every kernel
  * launch is bracketed by HIP events on the handle's stream; dgpu_timing_read
@@ -152,7 +182,8 @@ int dgpu_synchronize(dgpu_handle* h);
 #define DGPU_K_REMDIV 7   /* pressure Poisson solve + projection */
 #define DGPU_K_BCPOST 8   /* wall boundary values */
 #define DGPU_K_LES 9      /* subgrid model pointwise kernels and forcing */
-#define DGPU_NCLASS 10
+#define DGPU_K_ADJOINT 10 /* adjoint-only kernels (projection, CN post, RHS, products) */
+#define DGPU_NCLASS 11
 int dgpu_timing_enable(dgpu_handle* h, int enable);
 int dgpu_timing_read(dgpu_handle* h, double* ms /*[DGPU_NCLASS]*/, double* bytes /*[DGPU_NCLASS]*/,
                      int64_t* count /*[DGPU_NCLASS]*/);

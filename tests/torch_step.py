@@ -1,0 +1,351 @@
+"""A small FP64 PyTorch reimplementation of the channel substep, for autograd.
+
+It restates rk_chan_1 (for/channel.f90:10-917, DNS branch) and rem_div_chan
+(:946-1023) with torch operations, so torch.autograd gives the exact
+vector-Jacobian product of the discrete step: the reference the hand-written
+HIP adjoint is checked against.  It is test infrastructure, sized for small
+grids (the Thomas sweeps loop over y in Python).
+
+State arrays are the dealiased spectral block in the reference's packed kz
+order: complex128 of shape (Ny+2, 2*Nkz+1, Nkx+1) (planes, kz, kx); per-scalar
+arrays carry a leading N_th axis.  Planes outside a solve's range keep their
+spectral values (the reference round-trips them through its transforms,
+which is the identity on a real field's spectrum).
+"""
+import math
+
+import numpy as np
+import torch
+
+from diablo3_test_amd import workloads
+from diablo3_test_amd.grid import metrics
+
+STATE = ("cu1", "cu2", "cu3", "cp", "cth", "cf1", "cf2", "cf3", "cfth")
+
+
+class TorchChannel:
+    def __init__(self, p, gy):
+        self.p = p
+        self.nx, self.ny, self.nz = p["nx"], p["ny"], p["nz"]
+        self.nth = p["n_th"]
+        self.nkx, self.nkz = self.nx // 3, self.nz // 3
+        self.nq, self.nkk = self.nkx + 1, 2 * self.nkz + 1
+        gyf, dy, dyf = metrics(gy)
+        self.dy = torch.tensor(dy, dtype=torch.float64)
+        self.dyf = torch.tensor(dyf, dtype=torch.float64)
+        kx, kz = workloads.wavenumbers(p)
+        self.kx = torch.tensor(kx, dtype=torch.float64)[None, :]
+        self.kz = torch.tensor(kz, dtype=torch.float64)[:, None]
+        self.ikx = 1j * self.kx
+        self.ikz = 1j * self.kz
+        lap = self.kx ** 2 + self.kz ** 2
+        self.visc = lap if p["beta"] == 1.0 else lap ** p["beta"]
+        self.nu = 1.0 / p["re"]
+        dt = p["delta_t"]
+        self.h_bar = [dt * 8.0 / 15.0, dt * 2.0 / 15.0, dt * 5.0 / 15.0]
+        self.beta_bar = [1.0, 25.0 / 8.0, 9.0 / 4.0]
+        self.zeta_bar = [0.0, -17.0 / 8.0, -5.0 / 4.0]
+        self.js = 1 if p["u_bc_ymin"] == 1 else 2
+        self.je = self.ny
+        self.jst = [1 if p["th_bc_ymin"][n] == 1 else 2 for n in range(self.nth)]
+        self.jet = [self.ny] * self.nth
+        self.w_transient = p["w_bc_ymax_c1"]
+
+    # ------------------------------------------------------------ transforms
+    def to_phys(self, c):
+        """fft_xz_to_physical (fft.f90:319-382) of (n, nkk, nq) -> (n, Nz, Nx)."""
+        n = c.shape[0]
+        nz, nkz, nq = self.nz, self.nkz, self.nq
+        gap = torch.zeros((n, nz - 2 * nkz - 1, nq), dtype=c.dtype)
+        full = torch.cat([c[:, : nkz + 1], gap, c[:, nkz + 1:]], dim=1)
+        y = torch.fft.ifft(full, dim=1) * nz
+        pad = torch.zeros((n, nz, self.nx // 2 + 1 - nq), dtype=c.dtype)
+        y = torch.cat([y, pad], dim=2)
+        # FFTW's c2r reads only the real part of the kx = 0 column
+        y = torch.cat([y[:, :, :1].real.to(y.dtype), y[:, :, 1:]], dim=2)
+        return torch.fft.irfft(y, n=self.nx, dim=2) * self.nx
+
+    def to_spec(self, u):
+        """fft_xz_to_fourier (fft.f90:247-315) of (n, Nz, Nx) -> (n, nkk, nq)."""
+        f = torch.fft.rfft(u, dim=2)[:, :, : self.nq] / self.nx
+        f = torch.fft.fft(f, dim=1) / self.nz
+        return torch.cat([f[:, : self.nkz + 1], f[:, self.nz - self.nkz:]], dim=1)
+
+    # ------------------------------------------------------------ helpers
+    def rows(self, a, lo, hi):
+        """Mask of planes lo..hi as a broadcastable (n, 1, 1) float tensor."""
+        m = torch.zeros((a.shape[0], 1, 1), dtype=torch.float64)
+        m[lo: hi + 1] = 1.0
+        return m
+
+    def thomas(self, ml, md, mu, vec):
+        """Real tridiagonal solve over axis 0 (thomas_forward/backward_real_mpi)."""
+        n = vec.shape[0]
+        b = [md[0]]
+        g = [vec[0]]
+        for j in range(1, n):
+            mlt = -ml[j] / b[j - 1]
+            b.append(md[j] + mlt * mu[j - 1])
+            g.append(vec[j] + mlt * g[j - 1])
+        x = [None] * n
+        x[n - 1] = g[n - 1] / b[n - 1]
+        for j in range(n - 2, -1, -1):
+            x[j] = (g[j] - mu[j] * x[j + 1]) / b[j]
+        return torch.stack(x)
+
+    def bc_rows(self, kind, time):
+        """apply_BC_{1,2,3,th}_{lower,upper} row values: {j: (l, d, u, v)}."""
+        p, ny, dy, dyf = self.p, self.ny, self.dy, self.dyf
+        out = {}
+        if kind in ("u1", "u3") or kind.startswith("th"):
+            if kind == "u1":
+                bcl, bcu, c1l, c1u = p["u_bc_ymin"], p["u_bc_ymax"], p["u_bc_ymin_c1"], p["u_bc_ymax_c1"]
+            elif kind == "u3":
+                bcl, bcu, c1l, c1u = p["w_bc_ymin"], p["w_bc_ymax"], p["w_bc_ymin_c1"], p["w_bc_ymax_c1"]
+                if bcu != 0:
+                    if p["f_type"] == 4:
+                        self.w_transient = p["w_bc_ymax_c1"] + p["amp_omega0"] * math.sin(
+                            p["omega0"] * (p["ro_inv"] / p["delta"] * time - p["force_start"]))
+                    c1u = self.w_transient
+            else:
+                n = int(kind[2:])
+                bcl, bcu = p["th_bc_ymin"][n], p["th_bc_ymax"][n]
+                c1l, c1u = p["th_bc_ymin_c1"][n], p["th_bc_ymax_c1"][n]
+            out[0] = (0.0, 1.0, 0.0, 0.0)
+            out[1] = (0.0, 1.0, 0.0, c1l) if bcl == 0 else (0.0, -1.0, 1.0, float(dy[2]) * c1l)
+            if bcu == 0:
+                out[ny + 1] = (0.0, 1.0, 0.0, 0.0)
+                out[ny] = (0.0, 1.0, 0.0, c1u)
+            else:
+                out[ny] = (-1.0, 1.0, 0.0, float(dy[ny]) * c1u)
+                out[ny + 1] = (0.0, 1.0, 0.0, 0.0)
+        else:  # u2
+            if p["v_bc_ymin"] == 0:
+                out[1] = (0.0, 1.0, 0.0, p["v_bc_ymin_c1"])
+                out[2] = (0.0, 1.0, 0.0, p["v_bc_ymin_c1"])
+            else:
+                out[1] = (0.0, -1.0, 1.0, float(dyf[1]) * p["v_bc_ymin_c1"])
+            out[0] = (0.0, 1.0, 0.0, 0.0)
+            if p["v_bc_ymax"] == 0:
+                out[ny + 1] = (0.0, 1.0, 0.0, p["v_bc_ymax_c1"])
+                out[ny] = (0.0, 1.0, 0.0, p["v_bc_ymax_c1"])
+            else:
+                out[ny + 1] = (-1.0, 1.0, 0.0, float(dyf[ny]) * p["v_bc_ymax_c1"])
+        return out
+
+    def cn_solve(self, kind, u, r, coef, time):
+        """Explicit half + implicit CN solve of one physical field (channel.f90:494-547, 656-877)."""
+        ny, dy, dyf = self.ny, self.dy, self.dyf
+        n = ny + 2
+        isv = kind == "u2"
+        if kind == "u2":
+            ja, jb = 2, ny
+        elif kind.startswith("th"):
+            t = int(kind[2:])
+            ja, jb = self.jst[t], self.jet[t]
+        else:
+            ja, jb = self.js, self.je
+        shape = u.shape[1:]
+        ml = [torch.zeros(shape, dtype=torch.float64) for _ in range(n)]
+        md = [torch.ones(shape, dtype=torch.float64) for _ in range(n)]
+        mu = [torch.zeros(shape, dtype=torch.float64) for _ in range(n)]
+        vec = [torch.zeros(shape, dtype=torch.float64) for _ in range(n)]
+        for j in range(ja, jb + 1):
+            if isv:
+                lo = coef / (dyf[j - 1] * dy[j])
+                hi = coef / (dyf[j] * dy[j])
+            else:
+                lo = coef / (dy[j] * dyf[j])
+                hi = coef / (dy[j + 1] * dyf[j])
+            ex = hi * (u[j + 1] - u[j]) - lo * (u[j] - u[j - 1])
+            ml[j] = -lo * torch.ones(shape, dtype=torch.float64)
+            md[j] = (1.0 + lo + hi) * torch.ones(shape, dtype=torch.float64)
+            mu[j] = -hi * torch.ones(shape, dtype=torch.float64)
+            vec[j] = r[j] + ex
+        for j, (l, d, uu, v) in self.bc_rows(kind, time).items():
+            ml[j] = l * torch.ones(shape, dtype=torch.float64)
+            md[j] = d * torch.ones(shape, dtype=torch.float64)
+            mu[j] = uu * torch.ones(shape, dtype=torch.float64)
+            vec[j] = v * torch.ones(shape, dtype=torch.float64)
+        x = self.thomas(ml, md, mu, torch.stack(vec))
+        if isv:
+            ca, cb = 1, ny + 1
+        elif kind.startswith("th"):
+            t = int(kind[2:])
+            ca, cb = self.jst[t] - 1, self.jet[t] + 1
+        else:
+            ca, cb = self.js - 1, self.je + 1
+        return x, ca, cb
+
+    def rem_div(self, cu1, cu2, cu3):
+        """rem_div_chan: complex Poisson solve per mode, projection; returns phi."""
+        ny, dy, dyf = self.ny, self.dy, self.dyf
+        lam = self.kx ** 2 + self.kz ** 2
+        mean = torch.zeros_like(lam)
+        mean[0, 0] = 1.0
+        ml, md, mu, rhs = {}, {}, {}, {}
+        zero = torch.zeros_like(cu1[0])
+        ml[1], md[1], mu[1], rhs[1] = 0.0 * lam, 1.0 + 0.0 * lam, -(1.0 - mean), zero
+        for j in range(2, ny):
+            a = 1.0 / (dy[j] * dyf[j])
+            c = 1.0 / (dy[j + 1] * dyf[j])
+            ml[j] = a + 0.0 * lam
+            md[j] = -lam - a - c
+            mu[j] = c + 0.0 * lam
+            rhs[j] = self.ikx * cu1[j] + (cu2[j + 1] - cu2[j]) / dyf[j] + self.ikz * cu3[j]
+        ml[ny], md[ny], mu[ny], rhs[ny] = 1.0 + 0.0 * lam, -1.0 + 0.0 * lam, 0.0 * lam, zero
+        b = {1: md[1]}
+        g = {1: rhs[1]}
+        for j in range(2, ny + 1):
+            mlt = -ml[j] / b[j - 1]
+            b[j] = md[j] + mlt * mu[j - 1]
+            g[j] = rhs[j] + mlt * g[j - 1]
+        phi = {ny: g[ny] / b[ny]}
+        for j in range(ny - 1, 0, -1):
+            phi[j] = (g[j] - mu[j] * phi[j + 1]) / b[j]
+        n = ny + 2
+        phis = torch.stack([phi.get(j, zero) for j in range(n)])
+        m1 = self.rows(cu1, 1, ny)
+        cu1 = cu1 - m1 * self.ikx * phis
+        cu3 = cu3 - m1 * self.ikz * phis
+        d = torch.zeros_like(phis)
+        d[2: ny + 1] = (phis[2: ny + 1] - phis[1:ny]) / dy[2: ny + 1, None, None]
+        cu2 = cu2 - d
+        return cu1, cu2, cu3, phis
+
+    # ------------------------------------------------------------ the substep
+    def substep(self, s, rk_step, time=0.0):
+        """rk_chan_1: state dict -> new state dict (the inputs are not modified)."""
+        p, ny, nth = self.p, self.ny, self.nth
+        dy, dyf = self.dy, self.dyf
+        rs = rk_step - 1
+        temp1 = 0.5 * self.nu * self.h_bar[rs]
+        temp3 = self.zeta_bar[rs] * self.h_bar[rs]
+        temp4 = self.h_bar[rs]
+        temp5 = self.beta_bar[rs] * self.h_bar[rs]
+        nvs = p["nu_v_scale"]
+        cu1, cu2, cu3, cp = s["cu1"], s["cu2"], s["cu3"], s["cp"]
+        cth, cfth_old = s["cth"], s["cfth"]
+        m1 = self.rows(cu1, self.js, self.je)
+        m2 = self.rows(cu1, 2, ny)
+        mt = [self.rows(cu1, self.jst[n], self.jet[n]) for n in range(nth)]
+        visc = self.visc
+        # explicit spectral terms (channel.f90:39-253)
+        cs1 = torch.zeros_like(cp)
+        cs1[2: ny + 1] = (cp[2: ny + 1] - cp[1:ny]) / dy[2: ny + 1, None, None]
+        if p["f_type"] == 1:
+            force = p["px0"]
+        elif p["f_type"] == 2:
+            force = p["px0"] + p["amp_omega0"] * math.cos(p["omega0"] * time)
+        else:
+            force = 0.0
+        fmean = torch.zeros_like(cp)
+        fmean[:, 0, 0] = force
+        f1 = -self.nu * visc * cu1 + (p["ro_inv"] / p["delta"]) * cu3
+        f3 = -self.nu * visc * cu3 - (p["ro_inv"] / p["delta"]) * cu1
+        f2 = -self.nu * visc * cu2
+        notmean = torch.ones_like(visc)
+        notmean[0, 0] = 0.0
+        fts = []
+        for n in range(nth):
+            th = cth[n]
+            thi = torch.zeros_like(th)
+            thi[2: ny + 1] = (th[2: ny + 1] * dyf[1:ny, None, None] + th[1:ny] * dyf[2: ny + 1, None, None]) / (
+                2.0 * dy[2: ny + 1, None, None])
+            f2 = f2 + p["grav_y"] * notmean * thi
+            f1 = f1 + p["grav_x"] * th
+            f3 = f3 + p["grav_z"] * th
+            fts.append(-(self.nu / p["pr"][n]) * visc * th - cu1 * p["dthdx"][n] - cu3 * p["dthdz"][n])
+        # nonlinear products (channel.f90:299-459, 557-608)
+        u1, u2, u3 = self.to_phys(cu1), self.to_phys(cu2), self.to_phys(cu3)
+        ths = [self.to_phys(cth[n]) for n in range(nth)]
+        z = torch.zeros_like(u1[:1])
+        sh = lambda a: (torch.cat([a[1:], z]), torch.cat([z, a[:-1]]))  # a(j+1), a(j-1)
+        u1p, u1m = sh(u1)
+        u2p, u2m = sh(u2)
+        u3p, u3m = sh(u3)
+        c2dyf = 1.0 / (2.0 * dyf[:, None, None])
+        dyc = dy.clone()
+        dyc[0] = 1.0  # plane 0 is never consumed; keep it finite
+        interp = lambda a, am: (dyf[:, None, None] * a + torch.cat([dyf[:1], dyf[:-1]])[:, None, None] * am) / (
+            2.0 * dyc[:, None, None])
+        F = self.to_spec
+        f1 = f1 - self.ikz * F(u3 * u1) - self.ikx * F(u1 * u1) - F((u1p * u2p + u1 * u2p - u1 * u2 - u1m * u2) * c2dyf)
+        f3 = f3 - self.ikx * F(u3 * u1) - self.ikz * F(u3 * u3) - F((u3p * u2p + u3 * u2p - u3 * u2 - u3m * u2) * c2dyf)
+        f2 = f2 - self.ikx * F(interp(u1, u1m) * u2) - self.ikz * F(interp(u3, u3m) * u2) - F(
+            (0.25 * (u2 + u2p) ** 2 - 0.25 * (u2 + u2m) ** 2) / dyc[:, None, None])
+        for n in range(nth):
+            th = ths[n]
+            thp, thm = sh(th)
+            fts[n] = fts[n] - self.ikx * F(th * u1) - self.ikz * F(th * u3) - F(
+                (thp * u2p + th * u2p - th * u2 - thm * u2) * c2dyf)
+        # cr (channel.f90:39-120, 468-482) and the new RK terms
+        cr1 = cu1 - temp4 * self.ikx * cp - temp4 * fmean + temp5 * f1
+        cr3 = cu3 - temp4 * self.ikz * cp + temp5 * f3
+        cr2 = cu2 - temp4 * cs1 + temp5 * f2
+        if rk_step > 1:
+            cr1 = cr1 + temp3 * s["cf1"]
+            cr3 = cr3 + temp3 * s["cf3"]
+            cr2 = cr2 + temp3 * s["cf2"]
+        cf1 = m1 * f1 + (1 - m1) * s["cf1"]
+        cf3 = m1 * f3 + (1 - m1) * s["cf3"]
+        cf2 = m2 * f2 + (1 - m2) * s["cf2"]
+        crth, cfth = [], []
+        for n in range(nth):
+            rt = cth[n] + temp5 * fts[n]
+            if rk_step > 1:
+                rt = rt + temp3 * cfth_old[n]
+            crth.append(rt)
+            cfth.append(mt[n] * fts[n] + (1 - mt[n]) * cfth_old[n])
+        # implicit vertical solves and back to Fourier space (channel.f90:485-896)
+        out = {}
+        new_th = []
+        for n in range(nth):
+            x, ca, cb = self.cn_solve(f"th{n}", ths[n], self.to_phys(crth[n]), temp1 * nvs / p["pr"][n], time)
+            new_th.append(self.splice(cth[n], x, ca, cb))
+        x2, ca, cb = self.cn_solve("u2", u2, self.to_phys(cr2), temp1 * nvs, time)
+        n2 = self.splice(cu2, x2, ca, cb)
+        x1, ca, cb = self.cn_solve("u1", u1, self.to_phys(cr1), temp1 * nvs, time)
+        n1 = self.splice(cu1, x1, ca, cb)
+        x3, ca, cb = self.cn_solve("u3", u3, self.to_phys(cr3), temp1 * nvs, time)
+        n3 = self.splice(cu3, x3, ca, cb)
+        n1, n2, n3, phi = self.rem_div(n1, n2, n3)
+        out["cu1"], out["cu2"], out["cu3"] = n1, n2, n3
+        out["cp"] = cp + m1 * phi / temp4
+        out["cth"] = torch.stack(new_th) if nth else cth
+        out["cf1"], out["cf2"], out["cf3"] = cf1, cf2, cf3
+        out["cfth"] = torch.stack(cfth) if nth else cfth_old
+        return out
+
+    def splice(self, c_old, x_phys, ca, cb):
+        """Fourier transform of the solved planes ca..cb; other planes keep c_old."""
+        m = self.rows(c_old, ca, cb)
+        return m * self.to_spec(x_phys) + (1 - m) * c_old
+
+    def step(self, s, time=0.0):
+        for rk in (1, 2, 3):
+            s = self.substep(s, rk, time)
+        return s
+
+
+def from_ref(state, p):
+    """Reference-layout numpy state -> dict of torch active blocks."""
+    nkx, nkz = p["nx"] // 3, p["nz"] // 3
+    out = {}
+    for k in STATE:
+        a = state[k][..., : 2 * nkz + 1, : nkx + 1]
+        out[k] = torch.tensor(np.ascontiguousarray(a), dtype=torch.complex128)
+    return out
+
+
+def to_ref(s, p):
+    """Dict of torch active blocks -> reference-layout numpy arrays."""
+    nx, ny, nz = p["nx"], p["ny"], p["nz"]
+    nkx, nkz = nx // 3, nz // 3
+    out = {}
+    for k, v in s.items():
+        a = v.detach().numpy()
+        full = np.zeros(a.shape[:-3] + (ny + 2, nz + 2, nx // 2 + 1), dtype=np.complex128)
+        full[..., : 2 * nkz + 1, : nkx + 1] = a
+        out[k] = full
+    return out
