@@ -120,6 +120,56 @@ def run_form(p, gy, st, precision, steps, warmup, timing_steps):
     }
 
 
+def run_adjoint_form(p, gy, st, steps, warmup, timing_steps):
+    """The differentiation form (one full untimed run first, whatever `warmup`):
+    the gradient of the time-averaged wall shear
+    stress over `steps` steps with respect to the initial state -- forward
+    run with checkpoints, recomputation and the adjoint substeps.  A step of
+    this form is one forward step plus its share of recomputation and one
+    adjoint step."""
+    from diablo3_test_amd.adjoint import WallShearRun
+    from diablo3_test_amd.binding import ChannelSolver
+
+    npts = p["nx"] * p["ny"] * p["nz"]
+    g = ChannelSolver(p, gy)
+    g.set_state(st)
+    g.pre_first_step()
+    run = WallShearRun(g, steps, checkpoint_every=steps)
+    # one full run first: it allocates the checkpoint slots; slot 0 then holds
+    # the initial state, which stays resident for the timed run
+    run.gradient(None, fetch=False)
+    g.checkpoint(0, save=False)
+    g.synchronize()
+    t0 = time.perf_counter()
+    run.gradient(None, fetch=False)
+    g.synchronize()
+    sec = time.perf_counter() - t0
+    ms_per_step = sec / steps * 1e3
+    log(f"bench: adjoint {steps} steps in {sec:.3f} s, {ms_per_step:.2f} ms/step")
+    g.checkpoint(0, save=False)
+    g.timing_enable(True)
+    WallShearRun(g, timing_steps, checkpoint_every=timing_steps).gradient(None, fetch=False)
+    kt = g.timing_read()
+    g.timing_enable(False)
+    g.close()
+    kernels = {}
+    total_bytes = 0.0
+    for name, (ms, by, ct) in kt.items():
+        if ct:
+            total_bytes += by
+            kernels[name] = {"ms_per_step": ms / timing_steps, "launches_per_step": ct / timing_steps,
+                             "GBps": by / (ms * 1e-3) / 1e9 if ms > 0 else None}
+    step_bytes = total_bytes / timing_steps
+    return {
+        "value": npts * steps / sec / 1e6,
+        "ms_per_step": ms_per_step,
+        "kt": kt,
+        "kernels": kernels,
+        "step_hbm_frac": step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "algorithmic_GB_per_step": step_bytes / 1e9,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -128,7 +178,7 @@ def main():
     ap.add_argument("--workload", default="dns_512x257x512")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"],
                     help="the form whose throughput is `value`")
-    ap.add_argument("--forms", default="fp64,mixed",
+    ap.add_argument("--forms", default="fp64,mixed,adjoint",
                     help="forms also measured and reported under `forms` (comma list, or 'none')")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--timing-steps", type=int, default=2)
@@ -154,7 +204,13 @@ def main():
     forms = {args.precision: main_form}
     extra = [] if args.forms == "none" else [f for f in args.forms.split(",") if f and f != args.precision]
     for f in extra:
-        forms[f] = run_form(p, gy, st, f, args.steps, args.warmup, args.timing_steps)
+        if f == "adjoint":
+            if p["use_les"]:
+                log("bench: the adjoint covers the DNS substep; skipped for an LES workload")
+                continue
+            forms[f] = run_adjoint_form(p, gy, st, args.steps, args.warmup, args.timing_steps)
+        else:
+            forms[f] = run_form(p, gy, st, f, args.steps, args.warmup, args.timing_steps)
 
     kt = main_form["kt"]
     dname, (dms, dbytes, dct) = max(kt.items(), key=lambda kv: kv[1][0])

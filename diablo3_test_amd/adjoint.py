@@ -61,9 +61,10 @@ class WallShearRun:
         for rk in (1, 2, 3):
             self.s.rk_chan_1(rk, t, ts)
 
-    def loss(self, state, save_checkpoints=False):
-        """Set the initial state, run N steps, return L."""
-        self.s.set_state(state)
+    def loss(self, state=None, save_checkpoints=False):
+        """Run N steps from `state` (None: the handle's current state), return L."""
+        if state is not None:
+            self.s.set_state(state)
         total = 0.0
         for n in range(self.n):
             if save_checkpoints and n % self.k == 0:
@@ -72,8 +73,12 @@ class WallShearRun:
             total += float(self.w @ self.s.mean_profile("cu1"))
         return total / self.n
 
-    def gradient(self, state):
-        """(L, dL/dS_0) with dL/dS_0 a dict of reference-layout complex arrays."""
+    def gradient(self, state=None, fetch=True):
+        """(L, dL/dS_0), dL/dS_0 a dict of reference-layout complex arrays.
+        state None starts from the handle's current state; fetch False leaves
+        the gradient in the handle's adjoint state (and returns None for it).
+        Afterwards the forward state is a mix of recomputed fields; checkpoint
+        slot 0 holds the initial state (restore it with checkpoint(0, False))."""
         L = self.loss(state, save_checkpoints=True)
         nseg = (self.n + self.k - 1) // self.k
         base = nseg  # substep snapshots follow the segment checkpoints
@@ -87,7 +92,7 @@ class WallShearRun:
                 t = self.t0 + n * self.dt
                 ts = self.ts0 + n + 1
                 for rk in (1, 2, 3):
-                    s.checkpoint(base + 3 * (n - n0) + rk - 1, save=True)
+                    s.checkpoint(base + 3 * (n - n0) + rk - 1, save=True, light=True)
                     s.rk_chan_1(rk, t, ts)
             for n in reversed(range(n0, n1)):
                 t = self.t0 + n * self.dt
@@ -96,7 +101,7 @@ class WallShearRun:
                 for rk in (3, 2, 1):
                     s.checkpoint(base + 3 * (n - n0) + rk - 1, save=False)
                     s.adj_rk_chan_1(rk, t, ts)
-        return L, s.adj_get_state(STATE)
+        return L, (s.adj_get_state(STATE) if fetch else None)
 
 
 class WallShearLoss(torch.autograd.Function):

@@ -192,7 +192,7 @@ def load_library(path=None):
     return _LIB[path]
 
 
-NCLASS = 11
+NCLASS = 12
 
 
 def spectral_shape(p):
@@ -367,9 +367,11 @@ class ChannelSolver:
                                                out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))), "mean_profile")
         return out
 
-    def checkpoint(self, slot, save=True):
-        """Save (or restore) the forward state in device slot `slot`."""
-        self._check(self.lib.dgpu_checkpoint(self._h, int(slot), 1 if save else 0), "checkpoint")
+    def checkpoint(self, slot, save=True, light=False):
+        """Save (or restore) the forward state in device slot `slot`; a light
+        save keeps only the fields the adjoint substep reads (cu1..3, cth)."""
+        mode = (2 if light else 1) if save else 0
+        self._check(self.lib.dgpu_checkpoint(self._h, int(slot), mode), "checkpoint")
 
     def close(self):
         if self._h:

@@ -115,6 +115,7 @@ struct dgpu_handle {
   DevBuf<double> d_w;  // Ny+2 weights of dgpu_adj_add_mean
   struct Snap {
     std::vector<DevBuf<double2>> f;
+    size_t nsaved = 0;
     double w_transient = 0.0;
   };
   std::vector<Snap> snaps;
@@ -639,6 +640,13 @@ void launch_thomas(dgpu_handle* h, int field, int rs, const R* r, R* u, int ca, 
   HIPCHECK(hipGetLastError());
 }
 
+void copy_spec(dgpu_handle* h, double2* dst, const double2* src) {
+  const long long n = (long long)h->g.ny2 * h->g.sp;
+  Timed tm(h, DGPU_K_COPY, 32.0 * n);
+  hipLaunchKernelGGL(k_copy2, dim3(2048), dim3(256), 0, h->stream, dst, src, n);
+  HIPCHECK(hipGetLastError());
+}
+
 void launch_remdiv(dgpu_handle* h, bool update_cp, double temp4) {
   const Geo& g = h->g;
   RemDivArgs a;
@@ -870,13 +878,12 @@ void rk_substep_t(dgpu_handle* h, int rk_step, double time, int time_step) {
   const double2* lin_cth[DGPU_MAX_TH] = {nullptr, nullptr, nullptr, nullptr};
   for (int n = 0; n < nth; ++n) lin_cth[n] = h->cth[n].p;
   if (les) {
-    const size_t bytes = sizeof(double2) * (size_t)g.ny2 * g.sp;
     for (int f = 0; f < 3; ++f) {
-      HIPCHECK(hipMemcpyAsync(h->cu_pre[f].p, h->cu[f].p, bytes, hipMemcpyDeviceToDevice, h->stream));
+      copy_spec(h, h->cu_pre[f].p, h->cu[f].p);
       lin_cu[f] = h->cu_pre[f].p;
     }
     for (int n = 0; n < nth; ++n) {
-      HIPCHECK(hipMemcpyAsync(h->cth_pre[n].p, h->cth[n].p, bytes, hipMemcpyDeviceToDevice, h->stream));
+      copy_spec(h, h->cth_pre[n].p, h->cth[n].p);
       lin_cth[n] = h->cth_pre[n].p;
     }
     les_stage<R>(h, time);
@@ -1304,36 +1311,39 @@ void adj_substep(dgpu_handle* h, int rk_step, double time, int time_step) {
   }
 }
 
-// Checkpoint slot `slot` of the forward state (save or restore).
-void checkpoint(dgpu_handle* h, int slot, bool save) {
+// Checkpoint slot `slot` of the forward state.  mode 1 saves the whole state,
+// mode 2 only the fields the adjoint substep reads (cu1..3, cth), mode 0
+// restores what the slot holds.
+void checkpoint(dgpu_handle* h, int slot, int mode) {
   const size_t ns = (size_t)h->g.ny2 * h->g.sp;
   const int nth = h->nth;
-  std::vector<double2*> live = {h->cu[0].p, h->cu[1].p, h->cu[2].p, h->cp.p, h->cf[0].p, h->cf[1].p, h->cf[2].p};
-  for (int n = 0; n < nth; ++n) {
-    live.push_back(h->cth[n].p);
-    live.push_back(h->cfth[n].p);
-  }
+  std::vector<double2*> live = {h->cu[0].p, h->cu[1].p, h->cu[2].p};
+  for (int n = 0; n < nth; ++n) live.push_back(h->cth[n].p);
+  const size_t nlight = live.size();
+  live.push_back(h->cp.p);
+  for (int f = 0; f < 3; ++f) live.push_back(h->cf[f].p);
+  for (int n = 0; n < nth; ++n) live.push_back(h->cfth[n].p);
   if (slot < 0 || slot > 4096) throw ApiError{DGPU_EINVAL, "checkpoint slot out of range"};
+  if (mode < 0 || mode > 2) throw ApiError{DGPU_EINVAL, "checkpoint mode must be 0, 1 or 2"};
   if ((size_t)slot >= h->snaps.size()) {
-    if (!save) throw ApiError{DGPU_EINVAL, "checkpoint slot was never saved"};
+    if (mode == 0) throw ApiError{DGPU_EINVAL, "checkpoint slot was never saved"};
     h->snaps.resize(slot + 1);
   }
   auto& sn = h->snaps[slot];
-  if (sn.f.empty()) {
-    if (!save) throw ApiError{DGPU_EINVAL, "checkpoint slot was never saved"};
-    sn.f.resize(live.size());
-    for (auto& b : sn.f) b.alloc(ns);
-  }
-  for (size_t i = 0; i < live.size(); ++i) {
-    if (save)
-      HIPCHECK(hipMemcpyAsync(sn.f[i].p, live[i], sizeof(double2) * ns, hipMemcpyDeviceToDevice, h->stream));
-    else
-      HIPCHECK(hipMemcpyAsync(live[i], sn.f[i].p, sizeof(double2) * ns, hipMemcpyDeviceToDevice, h->stream));
-  }
-  if (save)
+  if (mode == 0 && sn.nsaved == 0) throw ApiError{DGPU_EINVAL, "checkpoint slot was never saved"};
+  const size_t nsave = (mode == 2) ? nlight : live.size();
+  if (mode != 0) {
+    while (sn.f.size() < nsave) {
+      sn.f.emplace_back();
+      sn.f.back().alloc(ns);
+    }
+    sn.nsaved = nsave;
+    for (size_t i = 0; i < nsave; ++i) copy_spec(h, sn.f[i].p, live[i]);
     sn.w_transient = h->w_transient;
-  else
+  } else {
+    for (size_t i = 0; i < sn.nsaved; ++i) copy_spec(h, live[i], sn.f[i].p);
     h->w_transient = sn.w_transient;
+  }
 }
 
 // -------------------------------------------------- reference layouts ----
@@ -1848,7 +1858,8 @@ int dgpu_timing_read(dgpu_handle* h, double* ms, double* bytes, int64_t* count) 
 const char* dgpu_kernel_class_name(int cls) {
   static const char* names[DGPU_NCLASS] = {"x_forward", "x_inverse", "z_forward", "z_inverse",
                                            "x_products", "spectral_rhs", "cn_solve",
-                                           "projection", "wall_bc", "les_model", "adjoint"};
+                                           "projection", "wall_bc", "les_model", "adjoint",
+                                           "state_copy"};
   return (cls >= 0 && cls < DGPU_NCLASS) ? names[cls] : "";
 }
 
@@ -1952,7 +1963,7 @@ int dgpu_mean_profile(dgpu_handle* h, int field, double* out) {
 
 int dgpu_checkpoint(dgpu_handle* h, int slot, int save) {
   if (!h) return DGPU_EINVAL;
-  return run_guarded(h, [&] { checkpoint(h, slot, save != 0); });
+  return run_guarded(h, [&] { checkpoint(h, slot, save); });
 }
 
 int dgpu_synchronize(dgpu_handle* h) {

@@ -897,6 +897,14 @@ __global__ void __launch_bounds__(256) k_remdiv(RemDivArgs a, Geo g) {
   }
 }
 
+// Device-to-device copy of n complex values (checkpoints, the LES pre-BC
+// state copy): a streaming kernel, four values per thread per iteration.
+__global__ void __launch_bounds__(256) k_copy2(double2* __restrict__ dst, const double2* __restrict__ src,
+                                               long long n) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
 // ---------------------------------------------------------- wall BCs ----
 // apply_BC_vel_lower_post / apply_BC_vel_upper_post (boundary.f90:299-498).
 struct BcPostArgs {

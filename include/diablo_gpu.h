@@ -160,10 +160,12 @@ int dgpu_adj_add_mean(dgpu_handle* h, int field, const double* w);
 /* Real part of the (0,0) mode of a forward field on planes 0..Ny+1 (the mean
    profile ume of statistics.f90:728), Ny+2 doubles. */
 int dgpu_mean_profile(dgpu_handle* h, int field, double* out);
-/* Save (save != 0) or restore the forward state (cu1..3, cp, cth, cf1..3,
-   cfth and the f_type 4 wall value) in device checkpoint slot `slot`
-   (0..4096, allocated on first save): recomputation for the adjoint. */
-int dgpu_checkpoint(dgpu_handle* h, int slot, int save);
+/* Checkpoints of the forward state in device slot `slot` (0..4096,
+   allocated on first save), for recomputation in adjoint runs.  mode 1 saves
+   the whole state (cu1..3, cp, cth, cf1..3, cfth and the f_type 4 wall
+   value), mode 2 only what dgpu_adj_rk_substep reads (cu1..3, cth), mode 0
+   restores what the slot holds. */
+int dgpu_checkpoint(dgpu_handle* h, int slot, int mode);
 
 /*
  * Per-kernel-class timing (no counterpart in the reference, whose only timer
@@ -183,7 +185,8 @@ int dgpu_checkpoint(dgpu_handle* h, int slot, int save);
 #define DGPU_K_BCPOST 8   /* wall boundary values */
 #define DGPU_K_LES 9      /* subgrid model pointwise kernels and forcing */
 #define DGPU_K_ADJOINT 10 /* adjoint-only kernels (projection, CN post, RHS, products) */
-#define DGPU_NCLASS 11
+#define DGPU_K_COPY 11    /* device copies of the state (checkpoints, LES pre-BC copy) */
+#define DGPU_NCLASS 12
 int dgpu_timing_enable(dgpu_handle* h, int enable);
 int dgpu_timing_read(dgpu_handle* h, double* ms /*[DGPU_NCLASS]*/, double* bytes /*[DGPU_NCLASS]*/,
                      int64_t* count /*[DGPU_NCLASS]*/);
