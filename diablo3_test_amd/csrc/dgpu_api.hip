@@ -4,6 +4,7 @@
 // declared in include/diablo_gpu.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -56,6 +57,16 @@ struct DevBuf {
 // registers), with at most 16 segments (1024 threads per block).  The target
 // length trades registers per thread against segments per column.
 int seg_target = 24;
+// Threads per workgroup of the x passes and of the products pass (tuning
+// knobs: DGPU_XPASS_THREADS, DGPU_XPROD_THREADS).
+int xpass_threads = 512;
+int xprod_threads = 512;
+// Plane batches of a 2-D transform: the x and z passes run batch by batch so
+// that a batch's x-spectra (the intermediate between the passes) are read
+// back while still in the last-level cache.  Budget in MB (DGPU_BATCH_MB).
+// Off by default: at 512x257x512 every batch size measured slower than one
+// launch per pass over all planes.
+int batch_mb = 0;
 int seg_len_for(int n) {
   int S = (n + seg_target - 1) / seg_target;
   if (S > 16) S = 16;
@@ -511,7 +522,7 @@ void launch_xfwd(dgpu_handle* h, const R* u, cpx<R>* T, int j0, int j1) {
   if (j1 < j0) return;
   const Geo& g = h->g;
   constexpr int KT = XTile<R>::KT;
-  const int ng = x_ngroups<R>(h, KT / 2);
+  const int ng = x_ngroups<R>(h, KT / 2, xpass_threads);
   const size_t lds = x_lds<R>(h, ng);
   dim3 grid(g.nz / KT, j1 - j0 + 1), block(ng * (g.nx / 8));
   Timed tm(h, DGPU_K_XFWD, (sizeof(R) * g.pp + 2.0 * sizeof(R) * g.tp) * (j1 - j0 + 1));
@@ -525,7 +536,7 @@ void launch_xinv(dgpu_handle* h, const cpx<R>* T, R* u, int j0, int j1) {
   if (j1 < j0) return;
   const Geo& g = h->g;
   constexpr int KT = XTile<R>::KT;
-  const int ng = x_ngroups<R>(h, KT / 2);
+  const int ng = x_ngroups<R>(h, KT / 2, xpass_threads);
   const size_t lds = x_lds<R>(h, ng);
   dim3 grid(g.nz / KT, j1 - j0 + 1), block(ng * (g.nx / 8));
   Timed tm(h, DGPU_K_XINV, (sizeof(R) * g.pp + 2.0 * sizeof(R) * g.tp) * (j1 - j0 + 1));
@@ -592,20 +603,36 @@ void launch_thomas_les(dgpu_handle* h, int field, int rs, const R* r, R* u, cons
   HIPCHECK(hipGetLastError());
 }
 
+// Planes per batch of a 2-D transform (see batch_mb).
+template <typename R>
+int batch_planes(const dgpu_handle* h) {
+  if (batch_mb <= 0) return 1 << 30;
+  const double per = 2.0 * sizeof(R) * (double)h->g.tp;
+  int b = (int)((double)batch_mb * 1048576.0 / per);
+  return b < 8 ? 8 : b;
+}
+
 // spectral -> physical for one field, all planes (fft_xz_to_physical, fft.f90:319-382)
 template <typename R>
 void to_physical(dgpu_handle* h, const double2* C, R* u, cpx<R>* T) {
-  const int n2 = h->g.ny2;
-  launch_zinv<R>(h, C, T, 0, n2 - 1);
-  launch_xinv<R>(h, T, u, 0, n2 - 1);
+  const int n2 = h->g.ny2, B = batch_planes<R>(h);
+  for (int j0 = 0; j0 < n2; j0 += B) {
+    const int j1 = std::min(n2 - 1, j0 + B - 1);
+    launch_zinv<R>(h, C, T, j0, j1);
+    launch_xinv<R>(h, T, u, j0, j1);
+  }
 }
 
 // physical -> spectral for one field on planes [j0, j1] (fft_xz_to_fourier,
 // fft.f90:247-315); with accum the transform is added to C
 template <typename R>
 void to_fourier(dgpu_handle* h, const R* u, double2* C, cpx<R>* T, int j0, int j1, bool accum = false) {
-  launch_xfwd<R>(h, u, T, j0, j1);
-  launch_zfwd<R>(h, T, C, j0, j1, accum);
+  const int B = batch_planes<R>(h);
+  for (int a = j0; a <= j1; a += B) {
+    const int b = std::min(j1, a + B - 1);
+    launch_xfwd<R>(h, u, T, a, b);
+    launch_zfwd<R>(h, T, C, a, b, accum);
+  }
 }
 
 template <typename R>
@@ -916,7 +943,7 @@ void rk_substep_t(dgpu_handle* h, int rk_step, double time, int time_step) {
     }
     for (int s = 0; s < nslot; ++s) pa.out[s] = h->tpool[s].p;
     constexpr int KT = XTile<R>::KT;
-    const int ngx = x_ngroups<R>(h, KT, 512);
+    const int ngx = x_ngroups<R>(h, KT, xprod_threads);
     const size_t lds = x_lds<R>(h, ngx);
     dim3 grid(g.nz / KT, Ny), block(ngx * (g.nx / 8));
     // reads u1, u2, u3, th once (neighbour planes from cache); writes the product x-spectra
@@ -1628,6 +1655,9 @@ void create_impl(dgpu_handle* h, const double* gy) {
 
   // CN factor tables (DGPU_SEG_TARGET overrides the target segment length)
   if (const char* e = std::getenv("DGPU_SEG_TARGET")) seg_target = std::atoi(e);
+  if (const char* e = std::getenv("DGPU_XPASS_THREADS")) xpass_threads = std::atoi(e);
+  if (const char* e = std::getenv("DGPU_XPROD_THREADS")) xprod_threads = std::atoi(e);
+  if (const char* e = std::getenv("DGPU_BATCH_MB")) batch_mb = std::atoi(e);
   const int n = g.ny2;
   h->seg_L = seg_len_for(n);
   if (h->seg_L < 0) throw ApiError{DGPU_EUNSUP, "ny too large for the segmented solver"};
