@@ -117,6 +117,21 @@ class Params(ctypes.Structure):
         return out
 
 
+class CourantParams(ctypes.Structure):
+    """Mirror of ``dgpu_courant_params`` (include/diablo_gpu.h)."""
+
+    _fields_ = [
+        ("cfl", ctypes.c_double),
+        ("max_dt", ctypes.c_double),
+        ("ly", ctypes.c_double),
+        ("time_limit", ctypes.c_double),
+        ("save_stats_time", ctypes.c_double),
+        ("save_flow_time", ctypes.c_double),
+        ("update_dt", ctypes.c_int32),
+        ("front", ctypes.c_int32),
+    ]
+
+
 class DgpuError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"{ERRORS.get(code, code)}: {msg}")
@@ -139,6 +154,13 @@ def _declare(lib, prefix, with_precision=True):
         ("destroy", [h]),
         ("set_field", [h, ctypes.c_int, P(ctypes.c_double)]),
         ("get_field", [h, ctypes.c_int, P(ctypes.c_double)]),
+        ("get_physical", [h, ctypes.c_int, P(ctypes.c_double)]),
+        ("set_physical", [h, ctypes.c_int, P(ctypes.c_double)]),
+        ("poisson_p", [h, P(ctypes.c_double)]),
+        ("set_courant", [h, P(CourantParams)]),
+        ("courant", [h, ctypes.c_double]),
+        ("get_delta_t", [h, P(ctypes.c_double)]),
+        ("set_delta_t", [h, ctypes.c_double]),
         ("apply_bc_vel_post", [h, ctypes.c_double]),
         ("rem_div", [h]),
         ("rk_substep", [h, ctypes.c_int, ctypes.c_double, ctypes.c_int]),
@@ -192,7 +214,7 @@ def load_library(path=None):
     return _LIB[path]
 
 
-NCLASS = 12
+NCLASS = 13
 
 
 def spectral_shape(p):
@@ -228,6 +250,7 @@ class ChannelSolver:
             raise DgpuError(rc, f"{prefix}create failed (bad parameters or device error)")
         self.time = 0.0
         self.time_step = 0
+        self._courant = None
 
     def _fn(self, name):
         return getattr(self.lib, self.prefix + name)
@@ -272,6 +295,32 @@ class ChannelSolver:
     def get_state(self, names=("cu1", "cu2", "cu3", "cp", "cth")):
         return {k: self.get_field(k) for k in names if not (k in PER_SCALAR and self.p.n_th == 0)}
 
+    def _phys_shape(self, name):
+        base = physical_shape(self.p)
+        return ((self.p.n_th,) + base) if name in PER_SCALAR else base
+
+    def get_physical(self, name):
+        """fft_xz_to_physical of spectral field `name` (cu1..3, cp, cth), computed
+        on the device; float64 (Ny+2, Nz+2, Nx+2) arrays ((n_th, ...) for cth)."""
+        if name not in ("cu1", "cu2", "cu3", "cp", "cth"):
+            raise ValueError(f"no physical copy of {name}")
+        a = np.zeros(self._phys_shape(name))
+        self._check(
+            self._fn("get_physical")(self._h, FIELD_IDS[name], a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+            f"get_physical({name})",
+        )
+        return a
+
+    def set_physical(self, name, arr):
+        """Spectral field `name` = fft_xz_to_fourier of physical `arr` on all planes."""
+        if name not in ("cu1", "cu2", "cu3", "cp", "cth"):
+            raise ValueError(f"no physical copy of {name}")
+        a = np.ascontiguousarray(np.asarray(arr, dtype=np.float64).reshape(self._phys_shape(name)))
+        self._check(
+            self._fn("set_physical")(self._h, FIELD_IDS[name], a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+            f"set_physical({name})",
+        )
+
     # ---- the reference's procedures
     def apply_bc_vel_mpi_post(self, time=None):
         """apply_BC_vel_mpi_post (for/boundary.f90:824-838)."""
@@ -282,14 +331,59 @@ class ChannelSolver:
         """rem_div_chan (for/channel.f90:946-1023) without the pressure update."""
         self._check(self._fn("rem_div")(self._h), "rem_div")
 
-    def pre_first_step(self):
-        """The projection part of pre_first_step (for/advance.f90:17-44).
-
-        cp is taken as supplied (a start file with P), so poisson_p_chan is not
-        called; dt is fixed, so courant is not called.
-        """
+    def pre_first_step(self, compute_pressure=False, ri=None):
+        """pre_first_step (for/advance.f90:17-44): wall values, projection,
+        poisson_p_chan when compute_pressure (start files without P; ri holds
+        Ri(1..n_th)), and courant when variable_dt is on (set_courant)."""
         self.apply_bc_vel_mpi_post()
         self.rem_div_chan()
+        if compute_pressure:
+            self.poisson_p_chan(ri)
+        if self._courant is not None:
+            self.courant()
+
+    def poisson_p_chan(self, ri=None):
+        """poisson_p_chan (for/channel.f90:1026-1234): cp from the velocity and
+        the scalars; ri = Ri(1..n_th) (parameters.f90:73)."""
+        n = self.p.n_th
+        r = np.zeros(max(n, 1))
+        if n:
+            if ri is None or len(ri) != n:
+                raise ValueError(f"ri must hold {n} Richardson numbers")
+            r[:n] = np.asarray(ri, dtype=np.float64)
+        self._check(self._fn("poisson_p")(self._h, r.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+                    "poisson_p")
+
+    def set_courant(self, cfl=None, max_dt=None, update_dt=1, ly=0.0, time_limit=1e300,
+                    save_stats_time=1e300, save_flow_time=1e300, front=False, off=False):
+        """Turn variable_dt on with courant's inputs (advance.f90:47-158), or
+        off with off=True.  max_dt defaults to the current delta_t (as
+        parameters.f90:103 sets it from DELTA_T); the output times steer the
+        step towards the next save as courant does."""
+        if off:
+            self._courant = None
+            self._check(self._fn("set_courant")(self._h, None), "set_courant")
+            return
+        c = CourantParams(cfl=float(cfl), max_dt=float(self.delta_t if max_dt is None else max_dt),
+                          ly=float(ly), time_limit=float(time_limit), save_stats_time=float(save_stats_time),
+                          save_flow_time=float(save_flow_time), update_dt=int(update_dt), front=int(bool(front)))
+        self._check(self._fn("set_courant")(self._h, ctypes.byref(c)), "set_courant")
+        self._courant = c
+
+    def courant(self, time=None):
+        """courant as pre_first_step calls it, on the current state."""
+        t = self.time if time is None else time
+        self._check(self._fn("courant")(self._h, float(t)), "courant")
+
+    @property
+    def delta_t(self):
+        d = ctypes.c_double()
+        self._check(self._fn("get_delta_t")(self._h, ctypes.byref(d)), "get_delta_t")
+        return d.value
+
+    @delta_t.setter
+    def delta_t(self, dt):
+        self._check(self._fn("set_delta_t")(self._h, float(dt)), "set_delta_t")
 
     def rk_chan_1(self, rk_step, time=None, time_step=None):
         """One RK3 substep (for/channel.f90:10-917)."""

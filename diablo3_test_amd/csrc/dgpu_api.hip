@@ -16,6 +16,7 @@
 
 #include "../../include/diablo_gpu.h"
 #include "dgpu_adjoint.h"
+#include "dgpu_init.h"
 #include "dgpu_les.h"
 
 using namespace dgpu;
@@ -100,7 +101,15 @@ struct dgpu_handle {
   DevBuf<double2> sg;
   DevBuf<double2> twx, twz;
   DevBuf<float2> twx32, twz32;  // FP32 twiddles of the mixed form
-  DevBuf<double> d_kx, d_kz, d_dy, d_dyf;
+  DevBuf<double> d_kx, d_kz, d_dy, d_dyf, d_gyf;
+  // poisson_p_chan: the spectral right-hand side (allocated on first use)
+  DevBuf<double2> pois;
+  // variable_dt (dgpu_set_courant): settings, CFL*dy(j), Front offsets,
+  // per-workgroup minima
+  bool var_dt = false;
+  dgpu_courant_params cour{};
+  int cfl_nset = 1;
+  DevBuf<double> cfl_cdy, cfl_offx, cfl_offz, cfl_part;
   // Crank-Nicolson factor tables: [field][rk] rows, field 0 u1, 1 u2, 2 u3, 3+n th_n
   DevBuf<SegRow> rows;
   // LES (allocated when use_les): velocity/scalar gradients, model outputs,
@@ -381,8 +390,33 @@ void twiddles(int n, std::vector<double2>& w) {
 
 template <typename T>
 void upload(DevBuf<T>& b, const std::vector<T>& v) {
-  b.alloc(v.size());
+  if (!b.p || b.n != v.size()) {
+    b.release();
+    b.alloc(v.size());
+  }
   HIPCHECK(hipMemcpy(b.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+}
+
+// The Crank-Nicolson tables of every field and RK substep for the current
+// h_bar: the segmented factorisation, and with use_LES the raw rows.
+void build_cn_tables(dgpu_handle* h) {
+  const int n = h->g.ny2;
+  std::vector<SegRow> rows((size_t)h->nfields * 3 * n);
+  for (int f = 0; f < h->nfields; ++f)
+    for (int rs = 0; rs < 3; ++rs) {
+      Tri t = build_tri(h, f, rs);
+      factor_rows(t, h->seg_L, rows.data() + ((size_t)f * 3 + rs) * n);
+    }
+  upload(h->rows, rows);
+  if (h->p.use_les) {
+    std::vector<TriRow> tr((size_t)h->nfields * 3 * n);
+    for (int f = 0; f < h->nfields; ++f)
+      for (int rs = 0; rs < 3; ++rs) {
+        Tri t = build_tri(h, f, rs);
+        raw_rows(h, t, f, rs, tr.data() + ((size_t)f * 3 + rs) * n);
+      }
+    upload(h->trirows, tr);
+  }
 }
 
 // ------------------------------------------------------------- timing ----
@@ -878,6 +912,187 @@ void les_stage(dgpu_handle* h, double time) {
   }
 }
 
+// ------------------------------------------------------ poisson_p_chan --
+// cp on planes 1..Ny from cu1..3 and cth (channel.f90:1026-1234).  The nine
+// derivative fields go to physical space one or two at a time (scratch r[1],
+// r[2]), their products accumulate in r[0], one forward transform gives the
+// spectral right-hand side, and k_poisson_solve adds the buoyancy term and
+// solves every mode.
+void poisson_p(dgpu_handle* h, const double* ri) {
+  const Geo& g = h->g;
+  const int j0 = 2, j1 = g.ny;
+  if (!h->pois.p) h->pois.alloc((size_t)g.ny2 * g.sp);
+  double* acc = h->r[0].p;
+  double* pa = h->r[1].p;
+  double* pb = h->r[2].p;
+  cpx<double>* T = xspec<double>(h->tpool[0]);
+  auto phys = [&](int f, int op, double* out) {
+    launch_zinv_op<double>(h, h->cu[f].p, T, j0, j1, op);
+    launch_xinv<double>(h, T, out, j0, j1);
+  };
+  const long long npts = (long long)(j1 - j0 + 1) * g.nz * g.nx;
+  auto accum = [&](int mode, const double* a, const double* b) {
+    const unsigned nb = (unsigned)std::min<long long>(4096, (npts + 255) / 256);
+    Timed tm(h, DGPU_K_INIT, (mode == 0 ? 16.0 : mode == 1 ? 24.0 : 32.0) * npts);
+    hipLaunchKernelGGL(k_poisson_accum, dim3(nb), dim3(256), 0, h->stream, acc, a, b, mode, j0, j1, g.nx, g.nz,
+                       (long long)g.pp);
+    HIPCHECK(hipGetLastError());
+  };
+  phys(0, OP_DX, pa);  // (du/dx)^2, (dv/dy)^2, (dw/dz)^2 (channel.f90:1041-1072)
+  accum(0, pa, nullptr);
+  phys(1, OP_DY_GYF, pa);
+  accum(1, pa, nullptr);
+  phys(2, OP_DZ, pa);
+  accum(1, pa, nullptr);
+  phys(0, OP_DY_C2, pa);  // 2 (du/dy)(dv/dx) (channel.f90:1075-1106)
+  phys(1, OP_DX_AVG, pb);
+  accum(2, pa, pb);
+  phys(2, OP_DY_C2, pa);  // 2 (dw/dy)(dv/dz) (channel.f90:1108-1140)
+  phys(1, OP_DZ_AVG, pb);
+  accum(2, pa, pb);
+  phys(0, OP_DZ, pa);  // 2 (du/dz)(dw/dx) (channel.f90:1142-1172)
+  phys(2, OP_DX, pb);
+  accum(2, pa, pb);
+  to_fourier<double>(h, acc, h->pois.p, T, j0, j1);
+  PoissonArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.rhs = h->pois.p;
+  a.nth = h->nth;
+  for (int n = 0; n < h->nth; ++n) {
+    a.cth[n] = h->cth[n].p;
+    a.ri[n] = ri[n];
+  }
+  a.kx = h->d_kx.p;
+  a.kz = h->d_kz.p;
+  a.dy = h->d_dy.p;
+  a.dyf = h->d_dyf.p;
+  a.gyf = h->d_gyf.p;
+  a.sb = h->sb.p;
+  a.sg = h->sg.p;
+  a.cp = h->cp.p;
+  Timed tm(h, DGPU_K_INIT, (48.0 + 16.0 * h->nth + 16.0) * g.sp * g.ny);
+  hipLaunchKernelGGL(k_poisson_solve, dim3((unsigned)((g.sp + 255) / 256)), dim3(256), 0, h->stream, a, g);
+  HIPCHECK(hipGetLastError());
+}
+
+// ------------------------------------------------------------- courant --
+// The smallest per-point limit of courant over planes [j0, j1] of the given
+// velocity buffers (physical, or spectral read as in pre_first_step).
+double cfl_min(dgpu_handle* h, const void* u0, const void* u1, const void* u2, bool spectral) {
+  const Geo& g = h->g;
+  const dgpu_params& p = h->p;
+  const bool front = h->cour.front != 0;
+  const int j0 = front ? h->jstart : 1, j1 = front ? h->jend : g.ny;
+  if (h->cfl_nset == 0 || j1 < j0) return HUGE_VAL;
+  CflArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.u[0] = u0;
+  a.u[1] = u1;
+  a.u[2] = u2;
+  a.spectral = spectral ? 1 : 0;
+  a.cdy = h->cfl_cdy.p;
+  a.cdx = h->cour.cfl * (p.lx / p.nx);  // dx = Lx/Nx (domain.f90:218)
+  a.cdz = h->cour.cfl * (p.lz / p.nz);  // dz = Lz/Nz (domain.f90:260)
+  a.offx = h->cfl_offx.p;
+  a.offz = h->cfl_offz.p;
+  a.nset = h->cfl_nset;
+  a.j0 = j0;
+  a.j1 = j1;
+  a.partial = h->cfl_part.p;
+  const long long n = (long long)(j1 - j0 + 1) * g.nz * g.nx;
+  const int nb = (int)std::min<long long>((long long)h->cfl_part.n, (n + 255) / 256);
+  {
+    Timed tm(h, DGPU_K_INIT, 24.0 * n);
+    hipLaunchKernelGGL(k_cfl_min, dim3((unsigned)nb), dim3(256), 0, h->stream, a, g);
+    HIPCHECK(hipGetLastError());
+  }
+  std::vector<double> part(nb);
+  HIPCHECK(hipMemcpyAsync(part.data(), h->cfl_part.p, sizeof(double) * nb, hipMemcpyDeviceToHost, h->stream));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  double m = HUGE_VAL;
+  for (double v : part) m = std::fmin(m, v);
+  return m;
+}
+
+void build_adj_tables(dgpu_handle* h);
+
+// delta_t, h_bar and the CN tables for a new time step.
+void set_delta_t_impl(dgpu_handle* h, double dt) {
+  if (!(dt > 0.0) || !std::isfinite(dt)) throw ApiError{DGPU_EINVAL, "delta_t must be positive and finite"};
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  h->p.delta_t = dt;
+  h->h_bar[0] = dt * (8.0 / 15.0);  // advance.f90:154-156
+  h->h_bar[1] = dt * (2.0 / 15.0);
+  h->h_bar[2] = dt * (5.0 / 15.0);
+  build_cn_tables(h);
+  if (h->adj_ready) build_adj_tables(h);
+}
+
+// The scalar part of courant (advance.f90:62-158) given the smallest per-point
+// limit `pts`: max_dt, the inertial and buoyancy periods, the caps, and the
+// scheduling towards the next output.  pi is the single-precision value of
+// fft.f90:152; the literals 0.1 and 1. - 0.025 are single precision there.
+double courant_dt(const dgpu_handle* h, double pts, double time) {
+  const dgpu_params& p = h->p;
+  const dgpu_courant_params& c = h->cour;
+  double dt = c.max_dt;
+  if (p.ro_inv != 0.0) dt = std::min(dt, 2.0 * h->pi / std::fabs(p.ro_inv / p.delta) / 20.0);
+  for (int n = 0; n < h->nth; ++n) {
+    const double nmax = std::sqrt(std::fabs(p.th_bc_ymin_c1[n]));
+    dt = std::min(dt, (double)0.1f * 2.0 * h->pi / nmax);
+  }
+  dt = std::min(dt, pts);
+  double delta_t;
+  if (dt <= 0.0)
+    delta_t = 0.00001;
+  else if (dt >= 1000.0)
+    delta_t = 1000.0;
+  else
+    delta_t = dt;
+  const double next = std::min(std::min(c.time_limit, c.save_stats_time - time),
+                               std::min(c.save_flow_time - time, c.time_limit - time));
+  const double r_max = (double)(1.0f - 0.025f);
+  // the event scheduling below replaces the capped value whatever it was
+  if (next < dt)
+    delta_t = next + 1.0e-14;
+  else if (next < (1.0 + r_max) * dt)
+    delta_t = 1.0 / (1.0 + r_max) * next;
+  else
+    delta_t = dt;
+  return delta_t;
+}
+
+void set_courant_impl(dgpu_handle* h, const dgpu_courant_params* c) {
+  if (!c) {
+    h->var_dt = false;
+    return;
+  }
+  if (!(c->cfl > 0.0) || !(c->max_dt > 0.0) || c->update_dt < 1)
+    throw ApiError{DGPU_EINVAL, "courant needs cfl > 0, max_dt > 0 and update_dt >= 1"};
+  const dgpu_params& p = h->p;
+  const int n2 = h->g.ny2;
+  h->cour = *c;
+  std::vector<double> cdy(n2, 0.0);
+  for (int j = 1; j < n2; ++j) cdy[j] = c->cfl * h->dy[j];
+  const int nset = c->front ? h->nth : 1;
+  std::vector<double> offx((size_t)std::max(nset, 1) * n2, 0.0), offz(offx.size(), 0.0);
+  if (c->front) {
+    // thermal wind of the Front flavour (advance.f90:110-122)
+    const double r = p.ro_inv / p.delta;
+    for (int n = 0; n < nset; ++n)
+      for (int j = 0; j < n2; ++j) {
+        offx[(size_t)n * n2 + j] = p.dthdz[n] * h->gyf[j] / r;
+        offz[(size_t)n * n2 + j] = (1.0 / r) * p.dthdx[n] * (h->gyf[j] - 0.5 * c->ly);
+      }
+  }
+  upload(h->cfl_cdy, cdy);
+  upload(h->cfl_offx, offx);
+  upload(h->cfl_offz, offz);
+  if (!h->cfl_part.p) h->cfl_part.alloc(1024);
+  h->cfl_nset = nset;
+  h->var_dt = true;
+}
+
 // ----------------------------------------------------------- the substep --
 // One rk_chan_1 (channel.f90:10-917).  With R = float (the mixed form) the
 // transforms, products and LES run in FP32 while the spectral state, the
@@ -1067,8 +1282,23 @@ void rk_substep_t(dgpu_handle* h, int rk_step, double time, int time_step) {
     to_fourier<R>(h, phys<R>(h->th[n]), h->cth[n].p, xspec<R>(h->tpool[3 + n]), h->jstart_th[n] - 1,
                   h->jend_th[n] + 1, incr);
 
+  // courant on the last substep of every update_dt-th step, on the velocity
+  // just solved (channel.f90:882-888); the new delta_t applies from the next
+  // substep, after this one has used temp4
+  const bool cfl_now = h->var_dt && rk_step == 3 && time_step % h->cour.update_dt == 0;
+  double cfl_pts = 0.0;
+  if (cfl_now) {
+    if (incr) {  // the mixed form holds increments in u: take the new FP64 state
+      for (int f = 0; f < 3; ++f) to_physical<double>(h, h->cu[f].p, h->r[f].p, xspec<double>(h->tpool[f]));
+      cfl_pts = cfl_min(h, h->r[0].p, h->r[1].p, h->r[2].p, false);
+    } else {
+      cfl_pts = cfl_min(h, h->u[0].p, h->u[1].p, h->u[2].p, false);
+    }
+  }
+
   // 7. projection and pressure (channel.f90:901-911)
   launch_remdiv(h, true, temp4);
+  if (cfl_now) set_delta_t_impl(h, courant_dt(h, cfl_pts, time));
 }
 
 void rk_substep(dgpu_handle* h, int rk_step, double time, int time_step) {
@@ -1091,6 +1321,26 @@ void solve_range(const dgpu_handle* h, int f, int* ca, int* cb) {
     *ca = h->jstart_th[f - 3] - 1;
     *cb = h->jend_th[f - 3] + 1;
   }
+}
+
+// Transposed CN matrices of the adjoint, all rows plain:
+// (A^T) sub(j) = c(j-1), sup(j) = a(j+1).
+void build_adj_tables(dgpu_handle* h) {
+  const int n = h->g.ny2;
+  std::vector<SegRow> rt((size_t)h->nfields * 3 * n);
+  for (int f = 0; f < h->nfields; ++f)
+    for (int rs = 0; rs < 3; ++rs) {
+      Tri t = build_tri(h, f, rs);
+      Tri tt = t;
+      for (int j = 0; j < n; ++j) {
+        tt.a[j] = (j >= 1) ? t.c[j - 1] : 0.0;
+        tt.c[j] = (j + 1 < n) ? t.a[j + 1] : 0.0;
+        tt.kind[j] = 3;
+        tt.wp[j] = tt.wm[j] = tt.vb[j] = 0.0;
+      }
+      factor_rows(tt, h->seg_L, rt.data() + ((size_t)f * 3 + rs) * n);
+    }
+  upload(h->rowsT, rt);
 }
 
 void adj_alloc(dgpu_handle* h) {
@@ -1125,23 +1375,8 @@ void adj_alloc(dgpu_handle* h) {
     zp(h->a_pA[q]);
     zp(h->a_pB[q]);
   }
-  // transposed CN matrices, all rows plain: (A^T) sub(j) = c(j-1), sup(j) = a(j+1)
-  const int n = g.ny2;
-  std::vector<SegRow> rt((size_t)h->nfields * 3 * n);
-  for (int f = 0; f < h->nfields; ++f)
-    for (int rs = 0; rs < 3; ++rs) {
-      Tri t = build_tri(h, f, rs);
-      Tri tt = t;
-      for (int j = 0; j < n; ++j) {
-        tt.a[j] = (j >= 1) ? t.c[j - 1] : 0.0;
-        tt.c[j] = (j + 1 < n) ? t.a[j + 1] : 0.0;
-        tt.kind[j] = 3;
-        tt.wp[j] = tt.wm[j] = tt.vb[j] = 0.0;
-      }
-      factor_rows(tt, h->seg_L, rt.data() + ((size_t)f * 3 + rs) * n);
-    }
-  upload(h->rowsT, rt);
-  h->d_w.alloc(n);
+  build_adj_tables(h);
+  h->d_w.alloc(g.ny2);
   h->adj_ready = true;
 }
 
@@ -1159,6 +1394,7 @@ void adj_substep(dgpu_handle* h, int rk_step, double time, int time_step) {
   const Geo& g = h->g;
   if (h->precision != DGPU_FP64 || p.use_les)
     throw ApiError{DGPU_EUNSUP, "the adjoint covers the FP64 DNS substep"};
+  if (h->var_dt) throw ApiError{DGPU_EUNSUP, "the adjoint covers a fixed time step (variable_dt is on)"};
   adj_alloc(h);
   const int rs = rk_step - 1;
   const int nth = h->nth, nf = 3 + nth, Ny = g.ny, n2 = g.ny2;
@@ -1515,6 +1751,12 @@ void free_all(dgpu_handle* h) {
   h->d_kz.release();
   h->d_dy.release();
   h->d_dyf.release();
+  h->d_gyf.release();
+  h->pois.release();
+  h->cfl_cdy.release();
+  h->cfl_offx.release();
+  h->cfl_offz.release();
+  h->cfl_part.release();
   h->rows.release();
   h->trirows.release();
   for (auto& b : h->grad) b.release();
@@ -1652,6 +1894,7 @@ void create_impl(dgpu_handle* h, const double* gy) {
   upload(h->d_kz, h->kz);
   upload(h->d_dy, h->dy);
   upload(h->d_dyf, h->dyf);
+  upload(h->d_gyf, h->gyf);
 
   // CN factor tables (DGPU_SEG_TARGET overrides the target segment length)
   if (const char* e = std::getenv("DGPU_SEG_TARGET")) seg_target = std::atoi(e);
@@ -1663,21 +1906,8 @@ void create_impl(dgpu_handle* h, const double* gy) {
   if (h->seg_L < 0) throw ApiError{DGPU_EUNSUP, "ny too large for the segmented solver"};
   h->seg_S = (n + h->seg_L - 1) / h->seg_L;
   h->nfields = 3 + h->nth;
-  std::vector<SegRow> rows((size_t)h->nfields * 3 * n);
-  for (int f = 0; f < h->nfields; ++f)
-    for (int rs = 0; rs < 3; ++rs) {
-      Tri t = build_tri(h, f, rs);
-      factor_rows(t, h->seg_L, rows.data() + ((size_t)f * 3 + rs) * n);
-    }
-  upload(h->rows, rows);
+  build_cn_tables(h);
   if (p.use_les) {
-    std::vector<TriRow> tr((size_t)h->nfields * 3 * n);
-    for (int f = 0; f < h->nfields; ++f)
-      for (int rs = 0; rs < 3; ++rs) {
-        Tri t = build_tri(h, f, rs);
-        raw_rows(h, t, f, rs, tr.data() + ((size_t)f * 3 + rs) * n);
-      }
-    upload(h->trirows, tr);
     for (int m = 0; m < 9; ++m) h->grad[m].alloc(np);
     for (int m = 0; m < 7; ++m) {
       h->tau[m].alloc(np);
@@ -1830,6 +2060,90 @@ int dgpu_get_field(dgpu_handle* h, int field, double* host) {
   });
 }
 
+// Physical-space copies of the spectral state fields (fft_xz_to_physical /
+// fft_xz_to_fourier around the flow-file I/O, phdf5.f90:1076-1082, 1190-1195,
+// 1437-1446), computed with the FP64 transforms through the handle's scratch.
+static bool physical_field_ok(dgpu_handle* h, int field) {
+  if (field == DGPU_CU1 || field == DGPU_CU2 || field == DGPU_CU3 || field == DGPU_CP) return true;
+  if (field == DGPU_CTH && h->nth > 0) return true;
+  h->err = "physical copies exist for cu1, cu2, cu3, cp and cth only";
+  return false;
+}
+
+int dgpu_get_physical(dgpu_handle* h, int field, double* host) {
+  if (!h || !host) return DGPU_EINVAL;
+  if (!physical_field_ok(h, field)) return DGPU_EINVAL;
+  return run_guarded(h, [&] {
+    const size_t per = ref_spec_doubles(h->g);
+    for (int n = 0; n < nfield_copies(h, field); ++n) {
+      void* dev;
+      bool spec;
+      field_buf(h, field, n, &dev, &spec);
+      to_physical<double>(h, (const double2*)dev, h->r[0].p, xspec<double>(h->tpool[0]));
+      std::vector<double> v((size_t)h->g.ny2 * h->g.pp);
+      HIPCHECK(hipMemcpyAsync(v.data(), h->r[0].p, sizeof(double) * v.size(), hipMemcpyDeviceToHost,
+                              h->stream));
+      HIPCHECK(hipStreamSynchronize(h->stream));
+      phys_to_ref(h, v, host + n * per);
+    }
+  });
+}
+
+int dgpu_set_physical(dgpu_handle* h, int field, const double* host) {
+  if (!h || !host) return DGPU_EINVAL;
+  if (!physical_field_ok(h, field)) return DGPU_EINVAL;
+  return run_guarded(h, [&] {
+    const size_t per = ref_spec_doubles(h->g);
+    for (int n = 0; n < nfield_copies(h, field); ++n) {
+      void* dev;
+      bool spec;
+      field_buf(h, field, n, &dev, &spec);
+      std::vector<double> v;
+      phys_from_ref(h, host + n * per, v);
+      HIPCHECK(hipMemcpyAsync(h->r[0].p, v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice,
+                              h->stream));
+      to_fourier<double>(h, h->r[0].p, (double2*)dev, xspec<double>(h->tpool[0]), 0, h->g.ny2 - 1);
+      HIPCHECK(hipStreamSynchronize(h->stream));
+    }
+  });
+}
+
+int dgpu_poisson_p(dgpu_handle* h, const double* ri) {
+  if (!h || (h->nth > 0 && !ri)) return DGPU_EINVAL;
+  return run_guarded(h, [&] {
+    poisson_p(h, ri);
+    HIPCHECK(hipStreamSynchronize(h->stream));
+  });
+}
+
+int dgpu_set_courant(dgpu_handle* h, const dgpu_courant_params* c) {
+  if (!h) return DGPU_EINVAL;
+  return run_guarded(h, [&] { set_courant_impl(h, c); });
+}
+
+int dgpu_courant(dgpu_handle* h, double time) {
+  if (!h) return DGPU_EINVAL;
+  if (!h->var_dt) {
+    h->err = "dgpu_courant needs the settings of dgpu_set_courant";
+    return DGPU_EINVAL;
+  }
+  return run_guarded(h, [&] {
+    const double pts = cfl_min(h, h->cu[0].p, h->cu[1].p, h->cu[2].p, true);
+    set_delta_t_impl(h, courant_dt(h, pts, time));
+  });
+}
+
+int dgpu_get_delta_t(dgpu_handle* h, double* delta_t) {
+  if (!h || !delta_t) return DGPU_EINVAL;
+  *delta_t = h->p.delta_t;
+  return DGPU_OK;
+}
+
+int dgpu_set_delta_t(dgpu_handle* h, double delta_t) {
+  if (!h) return DGPU_EINVAL;
+  return run_guarded(h, [&] { set_delta_t_impl(h, delta_t); });
+}
+
 int dgpu_apply_bc_vel_post(dgpu_handle* h, double time) {
   if (!h) return DGPU_EINVAL;
   return run_guarded(h, [&] { launch_bc_post(h, time); });
@@ -1889,7 +2203,7 @@ const char* dgpu_kernel_class_name(int cls) {
   static const char* names[DGPU_NCLASS] = {"x_forward", "x_inverse", "z_forward", "z_inverse",
                                            "x_products", "spectral_rhs", "cn_solve",
                                            "projection", "wall_bc", "les_model", "adjoint",
-                                           "state_copy"};
+                                           "state_copy", "init"};
   return (cls >= 0 && cls < DGPU_NCLASS) ? names[cls] : "";
 }
 

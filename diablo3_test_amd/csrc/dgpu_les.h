@@ -23,13 +23,24 @@
 
 namespace dgpu {
 
-// Derivative formed on the fly by the z inverse pass.
-enum DerivOp { OP_ID = 0, OP_DX = 1, OP_DZ = 2, OP_DY_GY = 3, OP_DY_GYF = 4 };
+// Derivative formed on the fly by the z inverse pass.  The last three are
+// the wall-normal forms of poisson_p_chan (channel.f90:1085-1141): a centred
+// difference (c(j+1) - c(j-1)) / (2 dyf(j)) and i k 0.5 (c(j) + c(j+1)).
+enum DerivOp {
+  OP_ID = 0,
+  OP_DX = 1,
+  OP_DZ = 2,
+  OP_DY_GY = 3,
+  OP_DY_GYF = 4,
+  OP_DY_C2 = 5,
+  OP_DX_AVG = 6,
+  OP_DZ_AVG = 7
+};
 
 // Unpack + zero + backward z transform of a derivative of a spectral field
 // (fft.f90:319-345 applied to the derivative arrays of les.f90).  Planes
-// where the derivative is not defined (j = 0 for OP_DY_GY, j = Ny+1 for
-// OP_DY_GYF) are written as zero.
+// where the derivative is not defined (j = 0 for OP_DY_GY and OP_DY_C2,
+// j = Ny+1 for OP_DY_GYF, OP_DY_C2 and the averages) are written as zero.
 template <typename R, bool WAVE>
 __global__ void __launch_bounds__(512) k_zinv_op(const cpx<double>* __restrict__ C, cpx<R>* __restrict__ T,
                                                  Geo g, const cpx<R>* __restrict__ tw, int j0, int op,
@@ -45,14 +56,15 @@ __global__ void __launch_bounds__(512) k_zinv_op(const cpx<double>* __restrict__
   cpx<R>* fbuf = smem + grp * fft_lds_len(g.nz);
   const int j = j0 + blockIdx.y, q = blockIdx.x * ngroups + grp;
   bool act = q < g.nq;
-  if (op == OP_DY_GY && j == 0) act = false;
-  if (op == OP_DY_GYF && j == g.ny + 1) act = false;
+  if ((op == OP_DY_GY || op == OP_DY_C2) && j == 0) act = false;
+  if (op >= OP_DY_GYF && j == g.ny + 1) act = false;
   const long long base = (long long)j * g.sp + (long long)(q < g.nq ? q : 0) * g.nkk;
   const cpx<double>* col = C + base;
   const double kxq = kx[q < g.nq ? q : 0];
   double sdy = 0.0;
   if (act && op == OP_DY_GY) sdy = 1.0 / dy[j];
   if (act && op == OP_DY_GYF) sdy = 1.0 / dyf[j];
+  if (act && op == OP_DY_C2) sdy = 1.0 / (2.0 * dyf[j]);
   const double sc = (q == 0) ? s0 : s1;
   const int off = g.nz - 2 * g.nkz - 1;
   cpx<R> v[8];
@@ -72,6 +84,9 @@ __global__ void __launch_bounds__(512) k_zinv_op(const cpx<double>* __restrict__
         case OP_DZ: z = ik(kz[kk], c); break;
         case OP_DY_GY: z = sdy * (c - col[kk - g.sp]); break;
         case OP_DY_GYF: z = sdy * (col[kk + g.sp] - c); break;
+        case OP_DY_C2: z = sdy * (col[kk + g.sp] - col[kk - g.sp]); break;
+        case OP_DX_AVG: z = ik(0.5 * kxq, c + col[kk + g.sp]); break;
+        case OP_DZ_AVG: z = ik(0.5 * kz[kk], c + col[kk + g.sp]); break;
         default: z = c; break;
       }
     }

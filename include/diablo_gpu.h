@@ -120,6 +120,19 @@ const char* dgpu_last_error(const dgpu_handle* h);
 int dgpu_set_field(dgpu_handle* h, int field, const double* host);
 int dgpu_get_field(dgpu_handle* h, int field, double* host);
 
+/*
+ * Physical-space copies of a spectral state field (cu1..3, cp, cth) in the
+ * physical reference layout (float64, Fortran order (0:Nx+1, 0:Nz+1, 0:Ny+1),
+ * n_th copies for cth).  dgpu_get_physical is `call fft_xz_to_physical(cu, u)`
+ * on a copy (the field itself is unchanged); dgpu_set_physical is
+ * `call fft_xz_to_fourier(u, cu)` on all planes 0..Ny+1, which also drops the
+ * modes outside the dealiased block.  These are the transforms around the
+ * flow-file I/O of WriteHDF5/ReadHDF5 (phdf5.f90:1076-1082, 1190-1195,
+ * 1431-1446); the file format itself is handled by the host library.
+ */
+int dgpu_get_physical(dgpu_handle* h, int field, double* host);
+int dgpu_set_physical(dgpu_handle* h, int field, const double* host);
+
 /* apply_BC_vel_mpi_post (boundary.f90:299-498, 824-838) on cu1..3. */
 int dgpu_apply_bc_vel_post(dgpu_handle* h, double time);
 /* rem_div_chan (channel.f90:946-1023): project cu1..3 onto the discrete
@@ -131,6 +144,51 @@ int dgpu_rk_substep(dgpu_handle* h, int rk_step, double time, int time_step);
 /* nsteps full steps of the driver loop (diablo.f90:98-108): time_step += 1,
    three substeps, time += delta_t.  *time and *time_step are updated. */
 int dgpu_step(dgpu_handle* h, int nsteps, double* time, int32_t* time_step);
+
+/*
+ * poisson_p_chan (channel.f90:1026-1234): cp on planes 1..Ny from the
+ * velocity (the Poisson equation for the pressure with right-hand side
+ * (du_i/dx_j)(du_j/dx_i) minus the buoyancy term Ri(n) d(th_n)/dy) and the
+ * walls' Neumann rows.  ri holds Ri(1..n_th) (parameters.f90:73; may be NULL
+ * when n_th = 0).  pre_first_step (advance.f90:17-44) calls it for start
+ * files without P.  The reference computes it in the RK carry arrays
+ * cf1..cf3, which rk_step 1 never reads; here the carry is left unchanged.
+ */
+int dgpu_poisson_p(dgpu_handle* h, const double* ri);
+
+/*
+ * Variable time step: courant (advance.f90:47-158).
+ *
+ * dgpu_set_courant turns variable_dt on (c != NULL) or off (c == NULL).
+ * While it is on, substep rk_step 3 with time_step % update_dt == 0 runs
+ * courant on the velocity just after the Crank-Nicolson solves
+ * (channel.f90:882-888): the CFL limit over the grid, the inertial and
+ * buoyancy periods, max_dt, and the scheduling towards the next output time,
+ * exactly as courant sets delta_t.  The new delta_t (and h_bar, and the CN
+ * factor tables) take effect from the next substep; dgpu_step advances time
+ * by the new delta_t after the step, as diablo.f90:108 does.
+ *
+ * dgpu_courant runs courant once on the current state, as pre_first_step
+ * does (advance.f90:36-38).  There the reference's u1..u3 share memory with
+ * cu1..cu3 (fft.f90:72-97) and hold the spectral coefficients, so courant
+ * reads those coefficients as a real array; that is reproduced.
+ */
+typedef struct dgpu_courant_params {
+  double cfl;              /* CFL */
+  double max_dt;           /* max_dt: the input DELTA_T (parameters.f90:103) */
+  double ly;               /* LY (Front flavour) */
+  double time_limit;       /* TIME_LIMIT */
+  double save_stats_time;  /* next statistics output time */
+  double save_flow_time;   /* next flow output time */
+  int32_t update_dt;       /* UPDATE_DT >= 1 */
+  int32_t front;           /* 1 when flavor == 'Front' (thermal-wind CFL) */
+} dgpu_courant_params;
+int dgpu_set_courant(dgpu_handle* h, const dgpu_courant_params* c);
+int dgpu_courant(dgpu_handle* h, double time);
+/* The time step in use, and a new one (h_bar and the CN factor tables are
+   rebuilt; delta_t > 0). */
+int dgpu_get_delta_t(dgpu_handle* h, double* delta_t);
+int dgpu_set_delta_t(dgpu_handle* h, double delta_t);
 
 /* Wait for all work queued on the handle's stream. */
 int dgpu_synchronize(dgpu_handle* h);
@@ -186,7 +244,8 @@ int dgpu_checkpoint(dgpu_handle* h, int slot, int mode);
 #define DGPU_K_LES 9      /* subgrid model pointwise kernels and forcing */
 #define DGPU_K_ADJOINT 10 /* adjoint-only kernels (projection, CN post, RHS, products) */
 #define DGPU_K_COPY 11    /* device copies of the state (checkpoints, LES pre-BC copy) */
-#define DGPU_NCLASS 12
+#define DGPU_K_INIT 12    /* Poisson right-hand side and solve, CFL reductions */
+#define DGPU_NCLASS 13
 int dgpu_timing_enable(dgpu_handle* h, int enable);
 int dgpu_timing_read(dgpu_handle* h, double* ms /*[DGPU_NCLASS]*/, double* bytes /*[DGPU_NCLASS]*/,
                      int64_t* count /*[DGPU_NCLASS]*/);

@@ -74,6 +74,12 @@ struct Solver {
   void compute_all_gradients();
   int les_chan(double time);
   int les_chan_th(double time);
+  // start-up and time-step control (oracle_init.cpp)
+  bool var_dt = false;
+  dgpu_courant_params cour{};
+  void poisson_p_chan(const double* ri);
+  double courant_points(const double* a1, const double* a2, const double* a3);
+  void courant(const double* a1, const double* a2, const double* a3, double time);
 };
 
 }  // namespace orc

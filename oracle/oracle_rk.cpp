@@ -360,6 +360,9 @@ void Solver::rk_chan_1(int rk_step, double time, int time_step) {
   solve_vel(1, U1, R1);
   solve_vel(3, U3, R3);
 
+  // variable_dt: courant on the velocity just solved (channel.f90:882-888)
+  if (var_dt && rk_step == 3 && time_step % cour.update_dt == 0) courant(U1, U2, U3, time);
+
   // ---- back to Fourier space, projection, pressure (channel.f90:891-914)
   fft_xz_to_fourier(U1);
   fft_xz_to_fourier(U2);
@@ -538,6 +541,78 @@ int orc_fft_xz_to_physical(dgpu_handle* hh, double* buf) {
   OrcHandle* h = reinterpret_cast<OrcHandle*>(hh);
   if (!h || !buf) return DGPU_EINVAL;
   h->s->fft_xz_to_physical(buf);
+  return 0;
+}
+// Physical copies of cu1..3, cp, cth around the flow-file I/O
+// (phdf5.f90:1076-1082, 1190-1195, 1431-1446): fft_xz_to_physical of a copy,
+// and fft_xz_to_fourier of the given physical field into the state.
+static bool physical_ok(int field) {
+  return field == DGPU_CU1 || field == DGPU_CU2 || field == DGPU_CU3 || field == DGPU_CP || field == DGPU_CTH;
+}
+int orc_get_physical(dgpu_handle* hh, int field, double* host) {
+  OrcHandle* h = reinterpret_cast<OrcHandle*>(hh);
+  if (!h || !host) return DGPU_EINVAL;
+  int64_t n;
+  std::vector<double>* v = field_ptr(h->s, field, &n);
+  if (!v || !physical_ok(field) || n == 0) return fail(h, DGPU_EINVAL, "no physical copy of this field");
+  std::memcpy(host, v->data(), sizeof(double) * n);
+  const int64_t NB = (int64_t)h->s->NB;
+  for (int64_t f = 0; f < n / NB; ++f) h->s->fft_xz_to_physical(host + f * NB);
+  return 0;
+}
+int orc_set_physical(dgpu_handle* hh, int field, const double* host) {
+  OrcHandle* h = reinterpret_cast<OrcHandle*>(hh);
+  if (!h || !host) return DGPU_EINVAL;
+  int64_t n;
+  std::vector<double>* v = field_ptr(h->s, field, &n);
+  if (!v || !physical_ok(field) || n == 0) return fail(h, DGPU_EINVAL, "no physical copy of this field");
+  std::memcpy(v->data(), host, sizeof(double) * n);
+  const int64_t NB = (int64_t)h->s->NB;
+  for (int64_t f = 0; f < n / NB; ++f) h->s->fft_xz_to_fourier(v->data() + f * NB);
+  return 0;
+}
+int orc_poisson_p(dgpu_handle* hh, const double* ri) {
+  OrcHandle* h = reinterpret_cast<OrcHandle*>(hh);
+  if (!h || (h->s->Nth > 0 && !ri)) return DGPU_EINVAL;
+  h->s->poisson_p_chan(ri);
+  return 0;
+}
+int orc_set_courant(dgpu_handle* hh, const dgpu_courant_params* c) {
+  OrcHandle* h = reinterpret_cast<OrcHandle*>(hh);
+  if (!h) return DGPU_EINVAL;
+  if (!c) {
+    h->s->var_dt = false;
+    return 0;
+  }
+  if (!(c->cfl > 0.0) || !(c->max_dt > 0.0) || c->update_dt < 1)
+    return fail(h, DGPU_EINVAL, "courant needs cfl > 0, max_dt > 0 and update_dt >= 1");
+  h->s->cour = *c;
+  h->s->var_dt = true;
+  return 0;
+}
+// courant as pre_first_step calls it: u1..u3 are the buffers of cu1..cu3
+int orc_courant(dgpu_handle* hh, double time) {
+  OrcHandle* h = reinterpret_cast<OrcHandle*>(hh);
+  if (!h) return DGPU_EINVAL;
+  if (!h->s->var_dt) return fail(h, DGPU_EINVAL, "orc_courant needs the settings of orc_set_courant");
+  Solver* s = h->s;
+  s->courant(s->u1.data(), s->u2.data(), s->u3.data(), time);
+  return 0;
+}
+int orc_get_delta_t(dgpu_handle* hh, double* dt) {
+  OrcHandle* h = reinterpret_cast<OrcHandle*>(hh);
+  if (!h || !dt) return DGPU_EINVAL;
+  *dt = h->s->p.delta_t;
+  return 0;
+}
+int orc_set_delta_t(dgpu_handle* hh, double dt) {
+  OrcHandle* h = reinterpret_cast<OrcHandle*>(hh);
+  if (!h || !(dt > 0.0) || !std::isfinite(dt)) return DGPU_EINVAL;
+  Solver* s = h->s;
+  s->p.delta_t = dt;
+  s->h_bar[0] = dt * (8.0 / 15.0);
+  s->h_bar[1] = dt * (2.0 / 15.0);
+  s->h_bar[2] = dt * (5.0 / 15.0);
   return 0;
 }
 // metrics: gyf, dy, dyf (Ny+2 each), kx (Nx/2), kz (2Nkz+1)

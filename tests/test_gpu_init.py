@@ -1,0 +1,127 @@
+"""poisson_p_chan and courant on the device against the oracle, and runs with
+a variable time step."""
+import numpy as np
+import pytest
+
+from diablo3_test_amd import workloads
+from helpers import gpu_solver, oracle_solver, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+COUR = dict(cfl=0.5, max_dt=1.0, ly=2.0, time_limit=1e9, save_stats_time=1e9, save_flow_time=1e9)
+
+
+def _random_state(p, gy, seed):
+    """A dealiased random state built through the oracle's transforms."""
+    o = oracle_solver(p, gy)
+    rng = np.random.default_rng(seed)
+    nx, ny, nz, nth = p["nx"], p["ny"], p["nz"], p["n_th"]
+    for k in ("cu1", "cu2", "cu3", "cth"):
+        if k == "cth" and nth == 0:
+            continue
+        lead = (nth,) if k == "cth" else ()
+        f = np.zeros(lead + (ny + 2, nz + 2, nx + 2))
+        f[..., :nz, :nx] = rng.standard_normal(lead + (ny + 2, nz, nx))
+        o.set_physical(k, f)
+    return o
+
+
+@pytest.mark.parametrize("nx,ny,nz,nth,precision", [
+    (32, 33, 32, 1, "fp64"), (64, 65, 32, 2, "fp64"), (16, 17, 64, 0, "fp64"), (128, 65, 64, 1, "mixed")])
+def test_poisson_matches_oracle(nx, ny, nz, nth, precision):
+    print(f"poisson {nx}x{ny}x{nz} nth={nth} {precision}", flush=True)
+    p = workloads.channel_params(nx, ny, nz, re=500.0, ri=1.0, n_th=nth)
+    gy = workloads.channel_grid(ny)
+    o = _random_state(p, gy, 21)
+    names = ("cu1", "cu2", "cu3", "cth") if nth else ("cu1", "cu2", "cu3")
+    g = gpu_solver(p, gy, precision)
+    g.set_state(o.get_state(names))
+    ri = [0.8, -0.3][:nth]
+    o.poisson_p_chan(ri)
+    g.poisson_p_chan(ri)
+    a = g.get_field("cp")[1:ny + 1]
+    b = o.get_field("cp")[1:ny + 1]
+    assert rel_l2(a, b) < 1e-10
+    # the rest of the state is untouched
+    for k in names:
+        assert np.array_equal(g.get_field(k), o.get_field(k)), k
+    g.close()
+
+
+def test_poisson_start_without_pressure():
+    """pre_first_step with compute_pressure on a start state, device vs oracle."""
+    print("pre_first_step with poisson 32x33x32", flush=True)
+    p, gy, ic, seed, amp = workloads.workload("small_32x33x32")
+    st = workloads.initial_state(p, gy, ic, seed, amp)
+    runs = []
+    for s in (gpu_solver(p, gy), oracle_solver(p, gy)):
+        s.set_state(st)
+        s.pre_first_step(compute_pressure=True, ri=[p["grav_y"]] * p["n_th"])
+        runs.append(s.get_state(("cu1", "cu2", "cu3", "cp")))
+    for k in runs[0]:
+        assert rel_l2(runs[0][k], runs[1][k]) < 1e-10, k
+
+
+@pytest.mark.parametrize("front", [False, True])
+def test_courant_pre_first_step_form(front):
+    """courant on the spectral buffers (as pre_first_step calls it), after the
+    state came from a forward transform on both sides."""
+    print(f"courant pre_first_step front={front}", flush=True)
+    p = workloads.channel_params(64, 33, 32, re=500.0, n_th=2)
+    p.update(ro_inv=0.4, delta=1.3, dthdx=[0.1, -0.2], dthdz=[0.3, 0.05], th_bc_ymin_c1=[0.0, 0.0])
+    gy = workloads.channel_grid(33)
+    o = _random_state(p, gy, 5)
+    g = gpu_solver(p, gy)
+    for k in ("cu1", "cu2", "cu3"):
+        g.set_physical(k, o.get_physical(k))
+        o.set_physical(k, o.get_physical(k))
+    for s in (g, o):
+        s.set_courant(front=front, **COUR)
+        s.courant(0.0)
+    assert abs(g.delta_t - o.delta_t) <= 1e-12 * o.delta_t
+    g.close()
+
+
+@pytest.mark.parametrize("precision,update_dt", [("fp64", 1), ("fp64", 3), ("mixed", 1)])
+def test_variable_dt_run(precision, update_dt):
+    """Steps with variable_dt: the time-step sequence and the state follow the
+    oracle (the mixed form to its own tolerance)."""
+    print(f"variable dt {precision} update_dt={update_dt}", flush=True)
+    p, gy, ic, seed, amp = workloads.workload("small_32x33x32")
+    st = workloads.initial_state(p, gy, ic, seed, amp)
+    runs = []
+    for s in (gpu_solver(p, gy, precision), oracle_solver(p, gy)):
+        s.set_state(st)
+        s.pre_first_step()
+        s.set_courant(update_dt=update_dt, **dict(COUR, max_dt=0.2, cfl=0.4))
+        dts = []
+        for n in range(6):
+            s.step(1)
+            dts.append((s.time, s.delta_t))
+        runs.append((s, dts))
+    (g, dg), (o, do) = runs
+    tol = 1e-10 if precision == "fp64" else 1e-5
+    for (tg, dtg), (to, dto) in zip(dg, do):
+        assert abs(dtg - dto) <= tol * dto and abs(tg - to) <= tol * max(to, 1.0)
+    assert len({d for _, d in do}) > 1  # the step did change
+    for k in ("cu1", "cu2", "cu3", "cp", "cth"):
+        assert rel_l2(g.get_field(k), o.get_field(k)) < (1e-10 if precision == "fp64" else 1e-5), k
+    g.close()
+
+
+def test_set_delta_t_rebuilds_tables():
+    """A new fixed delta_t on a running handle steps like the oracle with it."""
+    print("set_delta_t", flush=True)
+    p, gy, ic, seed, amp = workloads.workload("small_32x33x32")
+    st = workloads.initial_state(p, gy, ic, seed, amp)
+    out = []
+    for s in (gpu_solver(p, gy), oracle_solver(p, gy)):
+        s.set_state(st)
+        s.pre_first_step()
+        s.step(2)
+        s.delta_t = 0.5 * p["delta_t"]
+        s.step(2)
+        out.append(s.get_state(("cu1", "cu2", "cu3", "cp", "cth")))
+        assert s.delta_t == 0.5 * p["delta_t"]
+    for k in out[0]:
+        assert rel_l2(out[0][k], out[1][k]) < 1e-10, k
