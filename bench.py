@@ -51,6 +51,21 @@ def cpu_baseline(p, gy, st, npts):
     }
 
 
+def measured_traffic(workload, precision, cls):
+    """HBM bytes per launch of kernel class `cls` from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary for this workload, or None."""
+    import glob
+
+    for path in glob.glob(os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "hbm_traffic_*.md")):
+        for ln in open(path):
+            ln = ln.strip()
+            if ln.startswith('{"workload"'):
+                d = json.loads(ln)
+                if d["workload"] == workload and d["precision"] == precision:
+                    return d["traffic_bytes_per_launch"].get(cls)
+    return None
+
+
 def copy_probe(nbytes):
     """Device-to-device hipMemcpy of nbytes (a plain copy kernel of known
     size): calibrates the HBM byte counters and gives the copy bandwidth."""
@@ -221,7 +236,7 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS,
-        "traffic": None,
+        "traffic": measured_traffic(args.workload, args.precision, dname),
         "kernel": dname,
         "bytes_per_launch": dbytes / dct,
         "avg_launch_ms": dms / dct,
