@@ -62,6 +62,10 @@ int seg_target = 24;
 // knobs: DGPU_XPASS_THREADS, DGPU_XPROD_THREADS).
 int xpass_threads = 512;
 int xprod_threads = 512;
+// Columns per lane of the FP32-storage CN solve (1 or 2; knob DGPU_THOMAS_COLS).
+// Measured at 512x257x512: 2 columns take the mixed form's 12 solves per step
+// from 5.19 to 4.50 ms; the FP64 solves are 2% slower with 2 (5.21 -> 5.33 ms).
+int thomas_cols = 2;
 // Plane batches of a 2-D transform: the x and z passes run batch by batch so
 // that a batch's x-spectra (the intermediate between the passes) are read
 // back while still in the last-level cache.  Budget in MB (DGPU_BATCH_MB).
@@ -675,29 +679,36 @@ void launch_thomas(dgpu_handle* h, int field, int rs, const R* r, R* u, int ca, 
   const Geo& g = h->g;
   const int n = g.ny2;
   const SegRow* rows = (table ? table : h->rows.p) + ((size_t)field * 3 + rs) * n;
-  dim3 grid((unsigned)(g.pp / 64)), block(64, h->seg_S);
+  // columns per lane: thomas_cols for segments of up to 32 rows (register budget), else 1
+  // (FP32 storage gains from the wider row accesses; FP64 loses occupancy)
+  const int C = (sizeof(R) == 4 && h->seg_L <= 32 && thomas_cols == 2 && g.pp % 128 == 0) ? 2 : 1;
+  dim3 grid((unsigned)(g.pp / (64 * C))), block(64, h->seg_S);
   Timed tm(h, DGPU_K_THOMAS, 3.0 * sizeof(R) * g.pp * n);
-  switch (h->seg_L) {
-#define TCASE(L)                                                                                         \
-  case L:                                                                                                \
-    if (incr)                                                                                            \
-      hipLaunchKernelGGL((k_thomas_seg<L, R, true>), grid, block, 0, h->stream, r, u, rows, n, h->seg_S,  \
-                         g.pp, ca, cb, transient);                                                       \
-    else                                                                                                 \
-      hipLaunchKernelGGL((k_thomas_seg<L, R, false>), grid, block, 0, h->stream, r, u, rows, n, h->seg_S, \
-                         g.pp, ca, cb, transient);                                                       \
-    break;
-    TCASE(8)
-    TCASE(16)
-    TCASE(24)
-    TCASE(32)
-    TCASE(40)
-    TCASE(48)
-    TCASE(56)
-#undef TCASE
+#define TLAUNCH(L, CC)                                                                                     \
+  do {                                                                                                     \
+    if (incr)                                                                                              \
+      hipLaunchKernelGGL((k_thomas_seg<L, CC, R, true>), grid, block, 0, h->stream, r, u, rows, n,         \
+                         h->seg_S, g.pp, ca, cb, transient);                                               \
+    else                                                                                                   \
+      hipLaunchKernelGGL((k_thomas_seg<L, CC, R, false>), grid, block, 0, h->stream, r, u, rows, n,        \
+                         h->seg_S, g.pp, ca, cb, transient);                                               \
+  } while (0)
+  switch (h->seg_L * 4 + C) {
+    case 8 * 4 + 1: TLAUNCH(8, 1); break;
+    case 16 * 4 + 1: TLAUNCH(16, 1); break;
+    case 24 * 4 + 1: TLAUNCH(24, 1); break;
+    case 32 * 4 + 1: TLAUNCH(32, 1); break;
+    case 40 * 4 + 1: TLAUNCH(40, 1); break;
+    case 48 * 4 + 1: TLAUNCH(48, 1); break;
+    case 56 * 4 + 1: TLAUNCH(56, 1); break;
+    case 8 * 4 + 2: TLAUNCH(8, 2); break;
+    case 16 * 4 + 2: TLAUNCH(16, 2); break;
+    case 24 * 4 + 2: TLAUNCH(24, 2); break;
+    case 32 * 4 + 2: TLAUNCH(32, 2); break;
     default:
       throw ApiError{DGPU_EUNSUP, "unsupported Thomas segment length"};
   }
+#undef TLAUNCH
   HIPCHECK(hipGetLastError());
 }
 
@@ -1901,6 +1912,7 @@ void create_impl(dgpu_handle* h, const double* gy) {
   if (const char* e = std::getenv("DGPU_XPASS_THREADS")) xpass_threads = std::atoi(e);
   if (const char* e = std::getenv("DGPU_XPROD_THREADS")) xprod_threads = std::atoi(e);
   if (const char* e = std::getenv("DGPU_BATCH_MB")) batch_mb = std::atoi(e);
+  if (const char* e = std::getenv("DGPU_THOMAS_COLS")) thomas_cols = std::atoi(e);
   const int n = g.ny2;
   h->seg_L = seg_len_for(n);
   if (h->seg_L < 0) throw ApiError{DGPU_EUNSUP, "ny too large for the segmented solver"};

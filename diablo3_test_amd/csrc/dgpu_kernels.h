@@ -78,7 +78,7 @@ __global__ void __launch_bounds__(512) k_xfwd(const R* __restrict__ u, cpx<R>* _
     const R* rb = ra + g.nx;
     cpx<R> v[8];
 #pragma unroll
-    for (int r = 0; r < 8; ++r) v[r] = {ra[gt + G * r], rb[gt + G * r]};
+    for (int r = 0; r < 8; ++r) v[r] = {ld_in(ra + gt + G * r), ld_in(rb + gt + G * r)};
     fft_regs<R, -1, WAVE>(v, fbuf, g.nx, g.lognx, tw, gt);
     __syncthreads();  // every group's spectra are read below
     for (int idx = threadIdx.x; idx < g.nq * nr; idx += blockDim.x) {
@@ -114,7 +114,16 @@ __global__ void __launch_bounds__(512) k_xinv(const cpx<R>* __restrict__ T, R* _
     for (int idx = threadIdx.x; idx < g.nq * ngroups; idx += blockDim.x) {
       const int q = idx / ngroups, mm = idx - q * ngroups;
       const cpx<R>* src = in + (long long)q * g.nz + 2 * (m0 + mm);
-      const cpx<R> A = src[0], B = src[1];
+      // streaming loads pay off for FP32 spectra (3.46 -> 2.90 ms per step at
+      // 512x257x512) but not for FP64 ones (4.89 -> 5.40 ms)
+      cpx<R> A, B;
+      if (sizeof(R) == 4) {
+        A = ld_in(src);
+        B = ld_in(src + 1);
+      } else {
+        A = src[0];
+        B = src[1];
+      }
       cpx<R>* f = smem + mm * L;
       if (q == 0) {
         f[pidx(0)] = {A.x, B.x};
@@ -132,8 +141,8 @@ __global__ void __launch_bounds__(512) k_xinv(const cpx<R>* __restrict__ T, R* _
     R* rb = ra + g.nx;
     for (int i = gt; i < g.nx; i += G) {
       const cpx<R> z = fbuf[pidx(i)];
-      ra[i] = z.x;
-      rb[i] = z.y;
+      st_out(ra + i, z.x);
+      st_out(rb + i, z.y);
     }
   }
 }
@@ -159,7 +168,7 @@ __global__ void __launch_bounds__(512) k_zfwd(const cpx<R>* __restrict__ T, cpx<
   if (act) {
     const cpx<R>* col = T + (long long)j * g.tp + (long long)q * g.nz;
 #pragma unroll
-    for (int r = 0; r < 8; ++r) v[r] = col[gt + G * r];
+    for (int r = 0; r < 8; ++r) v[r] = ld_in(col + gt + G * r);
   } else {
 #pragma unroll
     for (int r = 0; r < 8; ++r) v[r] = {(R)0, (R)0};
@@ -174,7 +183,7 @@ __global__ void __launch_bounds__(512) k_zfwd(const cpx<R>* __restrict__ T, cpx<
       const cpx<R> z = inv * fbuf[pidx(kz_nat(g, kk))];
       cpx<double> zd = {(double)z.x, (double)z.y};
       if (accum) zd = out[kk] + zd;
-      out[kk] = zd;
+      out[kk] = zd;  // plain store: the projection and the next substep read the state
     }
   }
 }
@@ -204,9 +213,9 @@ __global__ void __launch_bounds__(512) k_zinv(const cpx<double>* __restrict__ C,
     cpx<double> z = {0.0, 0.0};
     if (act) {
       if (k <= g.nkz)
-        z = col[k];
+        z = ld_in(col + k);
       else if (k >= g.nz - g.nkz)
-        z = col[k - off];
+        z = ld_in(col + k - off);
     }
     v[r] = {(R)(sc * z.x), (R)(sc * z.y)};
   }
@@ -644,7 +653,7 @@ __global__ void __launch_bounds__(512) k_rhs(RhsArgs a, Geo g, const cpx<R>* __r
     fft_regs<R, 1, WAVE>(v, buf, g.nz, g.lognz, tw, gt);
     if (act) {
       CR* out = reinterpret_cast<CR*>(a.Tcr[c]) + to;
-      for (int k = gt; k < g.nz; k += G) st_out(out + k, buf[pidx(k)]);
+      for (int k = gt; k < g.nz; k += G) out[k] = buf[pidx(k)];
     }
   }
 }
@@ -672,95 +681,150 @@ struct SegRow {
   double ra, rb, rc;  // the matrix row itself (used for fixed rows by the increment form)
 };
 
+// C adjacent columns per lane, loaded and stored as one vector of C values:
+// a wavefront touches 64*C*sizeof(T) contiguous bytes of each row (the rows
+// of one column are a plane apart, so short row accesses leave HBM idle).
 // T is the storage type of r and u (the arithmetic is FP64).  INCR is the
 // increment form: r holds the right-hand side less the state (RHS pass with
 // incr), and the solve is for d = x - u with the same factors, A d = vec - A u:
 // assembled rows (A = I - E, vec = r + u + E u) give r + 2 E u, fixed rows
 // vec - (a u(j-1) + b u(j) + c u(j+1)).  u is overwritten with d.
-template <int L, typename T, bool INCR>
+template <int L, int C, typename T, bool INCR>
 __global__ void __launch_bounds__(1024) k_thomas_seg(const T* __restrict__ r,
                                                      T* __restrict__ u,
                                                      const SegRow* __restrict__ rows, int n, int S,
                                                      long long pp, int ca, int cb, double kscale) {
-  __shared__ double carry_f[16][64];
-  __shared__ double carry_b[16][64];
+  typedef T VN __attribute__((ext_vector_type(C)));  // C columns, one streaming access
+  __shared__ double carry_f[16][64 * C];
+  __shared__ double carry_b[16][64 * C];
   // blockDim.x == 64: a wavefront is one segment, so seg is wave-uniform and
   // the row-table reads below become scalar loads
   const int lane = threadIdx.x, seg = __builtin_amdgcn_readfirstlane(threadIdx.y);
-  const long long col = (long long)blockIdx.x * 64 + lane;
+  const long long col = ((long long)blockIdx.x * 64 + lane) * C;
   const int s0 = seg * L;
-  double v[L];
-  double w[L + 2];
+  double v[L][C];
+  double w[L + 2][C];
 #pragma unroll
   for (int t = 0; t < L + 2; ++t) {
     const int jj = s0 - 1 + t;
-    w[t] = (jj >= 0 && jj < n) ? (double)u[(long long)jj * pp + col] : 0.0;
+    if (jj >= 0 && jj < n) {
+      const VN x = __builtin_nontemporal_load(reinterpret_cast<const VN*>(u + (long long)jj * pp + col));
+#pragma unroll
+      for (int c = 0; c < C; ++c) w[t][c] = (double)x[c];
+    } else {
+#pragma unroll
+      for (int c = 0; c < C; ++c) w[t][c] = 0.0;
+    }
   }
 #pragma unroll
   for (int t = 0; t < L; ++t) {
     const int jj = s0 + t;
-    double val = 0.0;
+#pragma unroll
+    for (int c = 0; c < C; ++c) v[t][c] = 0.0;
     if (jj < n) {
       const SegRow rw = rows[jj];
-      const double ex = rw.wp * (w[t + 2] - w[t + 1]) - rw.wm * (w[t + 1] - w[t]);
-      if (rw.kind == 1)
-        val = (double)r[(long long)jj * pp + col] + (INCR ? 2.0 * ex : ex);
-      else if (rw.kind == 3)
-        val = (double)r[(long long)jj * pp + col];
-      else
-        val = (rw.kind == 2) ? rw.vb * kscale : rw.vb;
-      if (INCR && rw.kind != 1) val -= rw.ra * w[t] + rw.rb * w[t + 1] + rw.rc * w[t + 2];
+      if (rw.kind == 1 || rw.kind == 3) {
+        const VN x = __builtin_nontemporal_load(reinterpret_cast<const VN*>(r + (long long)jj * pp + col));
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+          double val = (double)x[c];
+          if (rw.kind == 1) {
+            const double ex = rw.wp * (w[t + 2][c] - w[t + 1][c]) - rw.wm * (w[t + 1][c] - w[t][c]);
+            val += INCR ? 2.0 * ex : ex;
+          }
+          v[t][c] = val;
+        }
+      } else {
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+          double val = (rw.kind == 2) ? rw.vb * kscale : rw.vb;
+          if (INCR) val -= rw.ra * w[t][c] + rw.rb * w[t + 1][c] + rw.rc * w[t + 2][c];
+          v[t][c] = val;
+        }
+      }
     }
-    v[t] = val;
   }
   // forward sweep, local part
-  double h = 0.0;
+  double h[C];
+#pragma unroll
+  for (int c = 0; c < C; ++c) h[c] = 0.0;
 #pragma unroll
   for (int t = 0; t < L; ++t) {
     const int jj = s0 + t;
     if (jj < n) {
-      h = v[t] + rows[jj].m * h;
-      v[t] = h;
+      const double m = rows[jj].m;
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        h[c] = v[t][c] + m * h[c];
+        v[t][c] = h[c];
+      }
     }
   }
   const int s1 = min(s0 + L, n) - 1;  // last row of this segment
-  {
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
     double last = 0.0;
 #pragma unroll
     for (int t = 0; t < L; ++t)
-      if (s0 + t == s1) last = v[t];
-    carry_f[seg][lane] = last;
+      if (s0 + t == s1) last = v[t][c];
+    carry_f[seg][lane * C + c] = last;
   }
   __syncthreads();
-  double gin = 0.0;
+  double gin[C];
+#pragma unroll
+  for (int c = 0; c < C; ++c) gin[c] = 0.0;
   for (int sp = 0; sp < seg; ++sp) {
     const int l1 = min(sp * L + L, n) - 1;
-    gin = carry_f[sp][lane] + rows[l1].pm * gin;
+    const double pm = rows[l1].pm;
+#pragma unroll
+    for (int c = 0; c < C; ++c) gin[c] = carry_f[sp][lane * C + c] + pm * gin[c];
   }
 #pragma unroll
   for (int t = 0; t < L; ++t) {
     const int jj = s0 + t;
-    if (jj < n) v[t] += rows[jj].pm * gin;
+    if (jj < n) {
+      const double pm = rows[jj].pm;
+#pragma unroll
+      for (int c = 0; c < C; ++c) v[t][c] += pm * gin[c];
+    }
   }
   // backward sweep, local part
-  double x = 0.0;
+  double x[C];
+#pragma unroll
+  for (int c = 0; c < C; ++c) x[c] = 0.0;
 #pragma unroll
   for (int t = L - 1; t >= 0; --t) {
     const int jj = s0 + t;
     if (jj < n) {
-      x = v[t] * rows[jj].e + rows[jj].f * x;
-      v[t] = x;
+      const double e = rows[jj].e, f = rows[jj].f;
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        x[c] = v[t][c] * e + f * x[c];
+        v[t][c] = x[c];
+      }
     }
   }
-  carry_b[seg][lane] = v[0];
+#pragma unroll
+  for (int c = 0; c < C; ++c) carry_b[seg][lane * C + c] = v[0][c];
   __syncthreads();
-  double xin = 0.0;
-  for (int sp = S - 1; sp > seg; --sp) xin = carry_b[sp][lane] + rows[sp * L].qf * xin;
+  double xin[C];
+#pragma unroll
+  for (int c = 0; c < C; ++c) xin[c] = 0.0;
+  for (int sp = S - 1; sp > seg; --sp) {
+    const double qf = rows[sp * L].qf;
+#pragma unroll
+    for (int c = 0; c < C; ++c) xin[c] = carry_b[sp][lane * C + c] + qf * xin[c];
+  }
 #pragma unroll
   for (int t = 0; t < L; ++t) {
     const int jj = s0 + t;
-    if (jj < n && jj >= ca && jj <= cb)
-      u[(long long)jj * pp + col] = (T)(v[t] + rows[jj].qf * xin);
+    if (jj < n && jj >= ca && jj <= cb) {
+      const double qf = rows[jj].qf;
+      VN o;
+#pragma unroll
+      for (int c = 0; c < C; ++c) o[c] = (T)(v[t][c] + qf * xin[c]);
+      __builtin_nontemporal_store(o, reinterpret_cast<VN*>(u + (long long)jj * pp + col));
+    }
   }
 }
 
