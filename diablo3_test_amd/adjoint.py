@@ -39,7 +39,7 @@ def wall_shear_weights(p, gy):
 
 
 class WallShearRun:
-    """N steps from a given initial state on one solver handle (FP64 DNS)."""
+    """N steps from a given initial state on one solver handle (FP64, DNS or LES)."""
 
     def __init__(self, solver, nsteps, checkpoint_every=10, time0=0.0, time_step0=0):
         if not isinstance(solver, ChannelSolver):
@@ -88,11 +88,13 @@ class WallShearRun:
         for seg in reversed(range(nseg)):
             n0, n1 = seg * self.k, min(self.n, (seg + 1) * self.k)
             s.checkpoint(seg, save=False)
+            # the LES adjoint reruns each substep and reads the whole input
+            light = not s.p.use_les
             for n in range(n0, n1):
                 t = self.t0 + n * self.dt
                 ts = self.ts0 + n + 1
                 for rk in (1, 2, 3):
-                    s.checkpoint(base + 3 * (n - n0) + rk - 1, save=True, light=True)
+                    s.checkpoint(base + 3 * (n - n0) + rk - 1, save=True, light=light)
                     s.rk_chan_1(rk, t, ts)
             for n in reversed(range(n0, n1)):
                 t = self.t0 + n * self.dt

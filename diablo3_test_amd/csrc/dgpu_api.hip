@@ -18,6 +18,7 @@
 #include "dgpu_adjoint.h"
 #include "dgpu_init.h"
 #include "dgpu_les.h"
+#include "dgpu_les_adj.h"
 
 using namespace dgpu;
 
@@ -137,6 +138,17 @@ struct dgpu_handle {
   DevBuf<double> a_pA[4 + 2 * DGPU_MAX_TH], a_pB[4 + 2 * DGPU_MAX_TH];
   DevBuf<SegRow> rowsT;
   DevBuf<double> d_w;  // Ny+2 weights of dgpu_adj_add_mean
+  // adjoint of the LES terms (allocated by the first LES adjoint call): the
+  // forward recomputation's captures (post-BC state, CN solutions), the saved
+  // forward state, the adjoint of the post-BC state, and the adjoints of
+  // nu_t, kappa_t, the gradients and the scalar gradients
+  bool les_cap = false;
+  bool les_adj_ready = false;
+  DevBuf<double2> cap_cub[3 + DGPU_MAX_TH];
+  DevBuf<double> cap_x[3 + DGPU_MAX_TH];
+  std::vector<DevBuf<double2>> fsave;
+  DevBuf<double2> a_cub[3 + DGPU_MAX_TH];
+  DevBuf<double> a_nt, a_kt[DGPU_MAX_TH], a_G[9], a_T[3][DGPU_MAX_TH];
   struct Snap {
     std::vector<DevBuf<double2>> f;
     size_t nsaved = 0;
@@ -1140,6 +1152,10 @@ void rk_substep_t(dgpu_handle* h, int rk_step, double time, int time_step) {
       lin_cth[n] = h->cth_pre[n].p;
     }
     les_stage<R>(h, time);
+    if (h->les_cap) {  // the adjoint's recomputation keeps the post-BC state
+      for (int f = 0; f < 3; ++f) copy_spec(h, h->cap_cub[f].p, h->cu[f].p);
+      for (int n = 0; n < nth; ++n) copy_spec(h, h->cap_cub[3 + n].p, h->cth[n].p);
+    }
   } else {
     // 1. state to physical space (channel.f90:279-287)
     for (int f = 0; f < 3; ++f) to_physical<R>(h, h->cu[f].p, phys<R>(h->u[f]), xspec<R>(h->tpool[f]));
@@ -1284,6 +1300,15 @@ void rk_substep_t(dgpu_handle* h, int rk_step, double time, int time_step) {
     launch_thomas<R>(h, 2, rs, phys<R>(h->r[2]), phys<R>(h->u[2]), h->jstart - 1, h->jend + 1, transient, incr);
   }
 
+  if (h->les_cap) {  // ... and the CN solutions
+    const size_t np = (size_t)h->g.ny2 * h->g.pp;
+    for (int f = 0; f < 3; ++f)
+      HIPCHECK(hipMemcpyAsync(h->cap_x[f].p, h->u[f].p, sizeof(double) * np, hipMemcpyDeviceToDevice, h->stream));
+    for (int n = 0; n < nth; ++n)
+      HIPCHECK(hipMemcpyAsync(h->cap_x[3 + n].p, h->th[n].p, sizeof(double) * np, hipMemcpyDeviceToDevice,
+                              h->stream));
+  }
+
   // 6. back to Fourier space on the solved planes (channel.f90:891-896); the
   // mixed form adds the transformed increment to the FP64 state
   to_fourier<R>(h, phys<R>(h->u[0]), h->cu[0].p, xspec<R>(h->tpool[0]), h->jstart - 1, h->jend + 1, incr);
@@ -1398,15 +1423,205 @@ const double2* fwd_spec(dgpu_handle* h, int f) { return f < 3 ? h->cu[f].p : h->
 // state is the substep's input and the adjoint state is the gradient of the
 // loss with respect to the substep's output; on exit the adjoint state is the
 // gradient with respect to the input.  The forward state is not changed.
-void adj_substep(dgpu_handle* h, int rk_step, double time, int time_step) {
-  (void)time;
-  (void)time_step;
+// ------------------------------------------------- adjoint: LES terms --
+void les_adj_alloc(dgpu_handle* h) {
+  if (h->les_adj_ready) return;
+  const Geo& g = h->g;
+  const int nth = h->nth, nf = 3 + nth;
+  const size_t ns = (size_t)g.ny2 * g.sp, npx = (size_t)g.ny2 * g.pp;
+  for (int f = 0; f < nf; ++f) {
+    h->cap_cub[f].alloc(ns);
+    h->cap_x[f].alloc(npx);
+    h->a_cub[f].alloc(ns);
+  }
+  h->fsave.resize(7 + 2 * nth);
+  for (auto& b : h->fsave) b.alloc(ns);
+  h->a_nt.alloc(npx);
+  for (int n = 0; n < nth; ++n) {
+    h->a_kt[n].alloc(npx);
+    for (int w = 0; w < 3; ++w) h->a_T[w][n].alloc(npx);
+  }
+  for (auto& b : h->a_G) b.alloc(npx);
+  h->les_adj_ready = true;
+}
+
+// The spectral state buffers in the order of fsave.
+std::vector<double2*> state_bufs(dgpu_handle* h) {
+  std::vector<double2*> v = {h->cu[0].p, h->cu[1].p, h->cu[2].p, h->cp.p, h->cf[0].p, h->cf[1].p, h->cf[2].p};
+  for (int n = 0; n < h->nth; ++n) {
+    v.push_back(h->cth[n].p);
+    v.push_back(h->cfth[n].p);
+  }
+  return v;
+}
+
+// Runs the forward substep from the current state, keeping what the adjoint
+// of the LES terms reads: the post-BC state, the CN solutions, and the
+// gradients, nu_t and kappa_t of the LES stage; then puts the input state
+// back and rebuilds the physical post-BC fields in u, th.
+void les_adj_recompute(dgpu_handle* h, int rk_step, double time, int time_step) {
+  les_adj_alloc(h);
+  std::vector<double2*> st = state_bufs(h);
+  for (size_t k = 0; k < st.size(); ++k) copy_spec(h, h->fsave[k].p, st[k]);
+  const double wt = h->w_transient;
+  h->les_cap = true;
+  try {
+    rk_substep_t<double>(h, rk_step, time, time_step);
+  } catch (...) {
+    h->les_cap = false;
+    throw;
+  }
+  h->les_cap = false;
+  for (size_t k = 0; k < st.size(); ++k) copy_spec(h, st[k], h->fsave[k].p);
+  h->w_transient = wt;
+  for (int f = 0; f < 3; ++f) to_physical<double>(h, h->cap_cub[f].p, h->u[f].p, xspec<double>(h->tpool[0]));
+  for (int n = 0; n < h->nth; ++n)
+    to_physical<double>(h, h->cap_cub[3 + n].p, h->th[n].p, xspec<double>(h->tpool[0]));
+}
+
+// Adjoint of les_stage (les.f90:20-1455 as arranged in les_stage): from the
+// adjoints of the subgrid forcings (the fbar of the RHS pass) and of nu_t,
+// kappa_t (from the CN solves) to the post-BC state adjoint a_cub and the
+// physical state adjoints a_ub (u1, u3 enter the model directly).
+void les_adj_stage(dgpu_handle* h, int rk_step) {
+  (void)rk_step;
   const dgpu_params& p = h->p;
   const Geo& g = h->g;
-  if (h->precision != DGPU_FP64 || p.use_les)
-    throw ApiError{DGPU_EUNSUP, "the adjoint covers the FP64 DNS substep"};
+  const int nth = h->nth, Ny = g.ny, n2 = g.ny2;
+  const double NxNz = (double)g.nx * g.nz;
+  const size_t npx = (size_t)n2 * g.pp;
+  cpx<double>* T0 = xspec<double>(h->tpool[0]);
+  auto grid1 = [](long long n) { return dim3((unsigned)((n + 255) / 256)); };
+  // 1. transposed stress divergence
+  {
+    AdjLesDivArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.f1 = h->a_fb[0].p;
+    a.f2 = h->a_fb[1].p;
+    a.f3 = h->a_fb[2].p;
+    a.t11 = h->ctau[0].p;
+    a.t33 = h->ctau[1].p;
+    a.t13 = h->ctau[2].p;
+    a.t12 = h->ctau[3].p;
+    a.t23 = h->ctau[4].p;
+    a.c1 = h->ctau[5].p;
+    a.c3 = h->ctau[6].p;
+    for (int n = 0; n < nth; ++n) {
+      a.fth[n] = h->a_fb[3 + n].p;
+      a.th_f1[n] = h->cthf[0][n].p;
+      a.th_f3[n] = h->cthf[1][n].p;
+      a.jst[n] = h->jstart_th[n];
+      a.jet[n] = h->jend_th[n];
+    }
+    a.kx = h->d_kx.p;
+    a.kz = h->d_kz.p;
+    a.dyf = h->d_dyf.p;
+    a.js = h->jstart;
+    a.j2 = Ny - 1;
+    a.nth = nth;
+    a.th_sign = (p.les_model_type == 5) ? -1 : 1;
+    Timed tm(h, DGPU_K_ADJOINT, 16.0 * (3 + 7 + 3 * nth) * g.sp * n2);
+    hipLaunchKernelGGL(k_adj_les_div, grid1(g.sp * n2), dim3(256), 0, h->stream, a, g);
+    HIPCHECK(hipGetLastError());
+  }
+  // 2. F^T of the stress and flux spectra (the forward transformed planes 1..Ny)
+  auto ftrans = [&](const double2* c, double* out) {
+    HIPCHECK(hipMemsetAsync(out, 0, sizeof(double) * npx, h->stream));
+    launch_zinv<double>(h, c, T0, 1, Ny, 1.0 / NxNz, 0.5 / NxNz);
+    launch_xinv<double>(h, T0, out, 1, Ny);
+  };
+  for (int m = 0; m < 7; ++m) ftrans(h->ctau[m].p, h->tau[m].p);
+  for (int n = 0; n < nth; ++n)
+    for (int w = 0; w < 2; ++w) ftrans(h->cthf[w][n].p, h->thf[w][n].p);
+  // 3. the scalar model, then the velocity model
+  LesVelArgs<double> fv;
+  std::memset(&fv, 0, sizeof(fv));
+  for (int m = 0; m < 9; ++m) fv.G[m] = h->grad[m].p;
+  fv.u1 = h->u[0].p;
+  fv.u3 = h->u[2].p;
+  fv.dy = h->d_dy.p;
+  fv.dyf = h->d_dyf.p;
+  fv.dx1 = p.lx / p.nx;
+  fv.dz1 = p.lz / p.nz;
+  fv.type = p.les_model_type;
+  fv.js = h->jstart;
+  fv.je = h->jend;
+  const long long npt = (long long)npx;
+  for (int n = 0; n < nth; ++n) {
+    AdjLesThArgs a;
+    std::memset(&a, 0, sizeof(a));
+    for (int m = 0; m < 9; ++m) a.fw.G[m] = h->grad[m].p;
+    a.fw.T1 = h->dth[0][n].p;
+    a.fw.T2 = (p.les_model_type == 5) ? h->dth[1][n].p : h->dth[0][n].p;
+    a.fw.T3 = h->dth[2][n].p;
+    a.fw.nu_t = h->nu_t.p;
+    a.fw.dy = h->d_dy.p;
+    a.fw.dyf = h->d_dyf.p;
+    a.fw.dx1 = p.lx / p.nx;
+    a.fw.dz1 = p.lz / p.nz;
+    a.fw.type = p.les_model_type;
+    a.fw.jst = h->jstart_th[n];
+    a.fw.jet = h->jend_th[n];
+    a.f1bar = h->thf[0][n].p;
+    a.f3bar = h->thf[1][n].p;
+    a.ktbar = h->a_kt[n].p;
+    for (int m = 0; m < 9; ++m) a.gbar[m] = h->a_G[m].p;
+    a.gbar_add = (n > 0) ? 1 : 0;
+    for (int w = 0; w < 3; ++w) a.tbar[w] = h->a_T[w][n].p;
+    a.ntbar = h->a_nt.p;
+    Timed tm(h, DGPU_K_ADJOINT, 8.0 * npt * 40);
+    hipLaunchKernelGGL(k_adj_les_th, grid1(npt), dim3(256), 0, h->stream, a, g);
+    HIPCHECK(hipGetLastError());
+  }
+  {
+    AdjLesVelArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.fw = fv;
+    const int bmap[8] = {0, 1, 2, 3, 4, -1, 5, 6};  // tau order t11 t33 t13 t12 t23 cr1 cr3
+    for (int k = 0; k < 8; ++k) a.bar[k] = (bmap[k] < 0) ? h->a_nt.p : h->tau[bmap[k]].p;
+    const bool th_grads = (p.les_model_type == 5 && nth > 0);
+    for (int m = 0; m < 9; ++m) {
+      a.gbar_in[m] = th_grads ? h->a_G[m].p : nullptr;
+      a.gbar[m] = h->a_G[m].p;
+    }
+    a.u1bar = h->a_ub[0].p;
+    a.u3bar = h->a_ub[2].p;
+    Timed tm(h, DGPU_K_ADJOINT, 8.0 * npt * 60);
+    hipLaunchKernelGGL(k_adj_les_vel, grid1(npt), dim3(256), 0, h->stream, a, g);
+    HIPCHECK(hipGetLastError());
+  }
+  // 4. gradients back to the post-BC spectral state: P^T, then D^T
+  double2* X = h->a_cr[0].p;
+  auto gback = [&](const double* gb, int op, double2* target) {
+    launch_xfwd<double>(h, gb, T0, 0, n2 - 1);
+    launch_zfwd<double>(h, T0, X, 0, n2 - 1, false, NxNz, 2.0 * NxNz);
+    Timed tm(h, DGPU_K_ADJOINT, 48.0 * g.sp * n2);
+    hipLaunchKernelGGL(k_adj_deriv, grid1(g.sp * n2), dim3(256), 0, h->stream, X, target, op, h->d_kx.p,
+                       h->d_kz.p, h->d_dy.p, h->d_dyf.p, g);
+    HIPCHECK(hipGetLastError());
+  };
+  const int gsrc[9] = {0, 0, 0, 1, 1, 1, 2, 2, 2};
+  const int gop[9] = {OP_DX, OP_DY_GY, OP_DZ, OP_DX, OP_DY_GYF, OP_DZ, OP_DX, OP_DY_GY, OP_DZ};
+  for (int m = 0; m < 9; ++m) gback(h->a_G[m].p, gop[m], h->a_cub[gsrc[m]].p);
+  for (int n = 0; n < nth; ++n) {
+    const int ops[3] = {OP_DX, OP_DY_GY, OP_DZ};
+    for (int w = 0; w < 3; ++w) {
+      if (w == 1 && p.les_model_type != 5) continue;
+      gback(h->a_T[w][n].p, ops[w], h->a_cub[3 + n].p);
+    }
+  }
+}
+
+void adj_substep(dgpu_handle* h, int rk_step, double time, int time_step) {
+  const dgpu_params& p = h->p;
+  const Geo& g = h->g;
+  if (h->precision != DGPU_FP64) throw ApiError{DGPU_EUNSUP, "the adjoint covers the FP64 substep"};
   if (h->var_dt) throw ApiError{DGPU_EUNSUP, "the adjoint covers a fixed time step (variable_dt is on)"};
   adj_alloc(h);
+  // the LES gate of the forward substep (channel.f90:259); with it closed
+  // nu_t and kappa_t stay zero and the substep is the DNS one
+  const bool les = p.use_les && (!p.create_new_flow || time_step > 100);
+  if (les) les_adj_recompute(h, rk_step, time, time_step);
   const int rs = rk_step - 1;
   const int nth = h->nth, nf = 3 + nth, Ny = g.ny, n2 = g.ny2;
   const double temp3 = h->zeta_bar[rs] * h->h_bar[rs];
@@ -1450,9 +1665,35 @@ void adj_substep(dgpu_handle* h, int rk_step, double time, int time_step) {
     HIPCHECK(hipMemsetAsync(adj_spec(h, f) + (size_t)ca * g.sp, 0, sizeof(double2) * g.sp * (cb - ca + 1),
                             h->stream));
   }
+  // With LES the state that the products, the LES stage and the unsolved
+  // planes see is the post-BC one: its adjoint gathers in a_cub, and the
+  // explicit spectral terms (which read the pre-BC state) add to a_cu.
+  if (les) {
+    for (int f = 0; f < nf; ++f) {
+      copy_spec(h, h->a_cub[f].p, adj_spec(h, f));
+      HIPCHECK(hipMemsetAsync(adj_spec(h, f), 0, sizeof(double2) * ns, h->stream));
+    }
+  }
   // 3. CN solves: vbar = A^-T xbar (into the physical state buffer, which
   //    step 7 refills), then rbar and ubar = E^T vbar
-  for (int f = 0; f < nf; ++f) {
+  if (les) {
+    // with nu_t, kappa_t in the rows (k_thomas_les): lambda, rbar, ubar_old
+    // and the eddy viscosity/diffusivity adjoints
+    HIPCHECK(hipMemsetAsync(h->a_nt.p, 0, sizeof(double) * npx, h->stream));
+    for (int n = 0; n < nth; ++n) HIPCHECK(hipMemsetAsync(h->a_kt[n].p, 0, sizeof(double) * npx, h->stream));
+    for (int f = 0; f < nf; ++f) {
+      const double* xb = (f < 3) ? h->r[f].p : h->rth[f - 3].p;
+      const double* uo = (f < 3) ? h->u[f].p : h->th[f - 3].p;
+      const double* nt = (f < 3) ? h->nu_t.p : h->kappa_t[f - 3].p;
+      double* ntb = (f < 3) ? h->a_nt.p : h->a_kt[f - 3].p;
+      const TriRow* rows = h->trirows.p + ((size_t)f * 3 + rs) * n2;
+      Timed tm(h, DGPU_K_ADJOINT, 96.0 * npx);
+      hipLaunchKernelGGL(k_adj_thomas_les, grid1(g.pp), dim3(256), 0, h->stream, xb, h->cap_x[f].p, uo, nt,
+                         h->a_rb[f].p, h->a_ub[f].p, ntb, h->sbl.p, h->sgl.p, rows, n2, (long long)g.pp);
+      HIPCHECK(hipGetLastError());
+    }
+  }
+  for (int f = 0; f < nf && !les; ++f) {
     double* xb = (f < 3) ? h->r[f].p : h->rth[f - 3].p;
     double* vb = (f < 3) ? h->u[f].p : h->th[f - 3].p;
     launch_thomas<double>(h, f, rs, xb, vb, 0, n2 - 1, 0.0, false, h->rowsT.p);
@@ -1529,9 +1770,13 @@ void adj_substep(dgpu_handle* h, int rk_step, double time, int time_step) {
     const double sc = (dz ? 1.0 : -1.0) / g.nz;
     launch_zinv_op<double>(h, h->a_fb[src].p, xspec<double>(h->tpool[s]), 0, n2 - 1, dz ? OP_DZ : OP_ID, sc, sc);
   }
-  // 7. the forward fields at the substep's input
-  for (int f = 0; f < 3; ++f) to_physical<double>(h, h->cu[f].p, h->u[f].p, xspec<double>(h->tpool[nslot]));
-  for (int n = 0; n < nth; ++n) to_physical<double>(h, h->cth[n].p, h->th[n].p, xspec<double>(h->tpool[nslot]));
+  // 7. the forward fields at the substep's input (with LES: after its wall
+  //    values, rebuilt by les_adj_recompute)
+  if (!les) {
+    for (int f = 0; f < 3; ++f) to_physical<double>(h, h->cu[f].p, h->u[f].p, xspec<double>(h->tpool[nslot]));
+    for (int n = 0; n < nth; ++n)
+      to_physical<double>(h, h->cth[n].p, h->th[n].p, xspec<double>(h->tpool[nslot]));
+  }
   // 8. product adjoints (x pass) and their Jacobians
   {
     AdjPairArgs a;
@@ -1578,10 +1823,43 @@ void adj_substep(dgpu_handle* h, int rk_step, double time, int time_step) {
     hipLaunchKernelGGL(k_adj_prod, grid1((long long)npx), dim3(256), 0, h->stream, a, g);
     HIPCHECK(hipGetLastError());
   }
+  if (les) les_adj_stage(h, rk_step);
   // 9. cubar += P^T ubar (the inverse transforms of the state)
   for (int f = 0; f < nf; ++f) {
     launch_xfwd<double>(h, h->a_ub[f].p, T0, 0, n2 - 1);
-    launch_zfwd<double>(h, T0, adj_spec(h, f), 0, n2 - 1, true, NxNz, 2.0 * NxNz);
+    launch_zfwd<double>(h, T0, les ? h->a_cub[f].p : adj_spec(h, f), 0, n2 - 1, true, NxNz, 2.0 * NxNz);
+  }
+  // 10. with LES, the transposed wall values (k_bc_post, k_les_bc), then the
+  //     post-BC adjoint joins the pre-BC one
+  if (les) {
+    AdjBcArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.c1 = h->a_cub[0].p;
+    a.c2 = h->a_cub[1].p;
+    a.c3 = h->a_cub[2].p;
+    a.nth = nth;
+    for (int n = 0; n < nth; ++n) {
+      a.cth[n] = h->a_cub[3 + n].p;
+      a.th_lo[n] = p.th_bc_ymin[n];
+      a.th_hi[n] = p.th_bc_ymax[n];
+    }
+    a.u_lo = p.u_bc_ymin;
+    a.v_lo = p.v_bc_ymin;
+    a.w_lo = p.w_bc_ymin;
+    a.u_hi = p.u_bc_ymax;
+    a.v_hi = p.v_bc_ymax;
+    a.w_hi = p.w_bc_ymax;
+    {
+      Timed tm(h, DGPU_K_ADJOINT, 16.0 * 12 * g.sp * (3 + nth));
+      hipLaunchKernelGGL(k_adj_les_bc, grid1(g.sp), dim3(256), 0, h->stream, a, g);
+      HIPCHECK(hipGetLastError());
+    }
+    for (int f = 0; f < nf; ++f) {
+      Timed tm(h, DGPU_K_ADJOINT, 48.0 * ns);
+      hipLaunchKernelGGL(k_axpy2, dim3(2048), dim3(256), 0, h->stream, adj_spec(h, f), h->a_cub[f].p,
+                         (long long)ns);
+      HIPCHECK(hipGetLastError());
+    }
   }
 }
 
@@ -1799,6 +2077,16 @@ void free_all(dgpu_handle* h) {
   h->a_cp.release();
   h->rowsT.release();
   h->d_w.release();
+  for (auto& b : h->cap_cub) b.release();
+  for (auto& b : h->cap_x) b.release();
+  for (auto& b : h->fsave) b.release();
+  h->fsave.clear();
+  for (auto& b : h->a_cub) b.release();
+  h->a_nt.release();
+  for (auto& b : h->a_kt) b.release();
+  for (auto& b : h->a_G) b.release();
+  for (int w = 0; w < 3; ++w)
+    for (auto& b : h->a_T[w]) b.release();
   for (auto& sn : h->snaps)
     for (auto& b : sn.f) b.release();
   h->snaps.clear();

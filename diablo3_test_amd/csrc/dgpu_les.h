@@ -20,6 +20,7 @@
 // The reference's du2dy(j) = (u2(j)-u2(j-1))/dyf(j-1) (les.f90:1713) is G22(j-1).
 #pragma once
 #include "dgpu_kernels.h"
+#include "les_point.h"
 
 namespace dgpu {
 
@@ -122,48 +123,75 @@ struct LesVelArgs {
   int js, je;           // jstart, jend
 };
 
-constexpr double kBetaSgs = 3.0;   // les.f90:60
-constexpr double kCSmag = 0.17;    // les.f90:41
-constexpr double kCAmd = 0.2887;   // les.f90:42
+using lespt::kBetaSgs;
+using lespt::kCAmd;
+using lespt::kCSmag;
 
-// GYF <-> GY interpolation weights of les.f90:150-165 and 316-327.
+// Per-plane constants of the velocity model at plane j (les.f90:146-222,
+// 371-518, 593-799): which of the GYF / GY evaluations the reference makes
+// there, and the filter factors, formed in FP64 from the FP64 metrics.
 template <typename R>
-__device__ __forceinline__ R gy_of(const R* a, long long o, long long pp, R dyfm, R dyfj, R dyj) {
-  return (a[o] * dyfm + a[o - pp] * dyfj) / ((R)2 * dyj);
+__device__ inline lespt::VelPlane<R> les_vel_plane(const LesVelArgs<R>& a, const Geo& g, int j) {
+  const int ny = g.ny;
+  lespt::VelPlane<R> P;
+  P.type = a.type;
+  const bool gyf_rows = (a.type == 5) ? (j >= a.js && j <= a.je) : (j <= ny);  // :172-182, 468-478
+  P.gyf_in = gyf_rows && j <= ny;  // the GYF model also reads GY plane j+1
+  if (a.type == 1)
+    P.gy_in = true;  // :146-168, 185-197: planes 1..Ny+1
+  else if (a.type == 4)
+    P.gy_in = (j >= 2);  // :371-464, 482-494
+  else
+    P.gy_in = (j >= 2 && j <= ny);  // :647-738 (Ny+1 never consumed)
+  P.nt_in = P.gy_in && j >= 3 && j <= ny - 1;  // ghost_les_mpi zeroes nu_t at 1, 2, Ny, Ny+1
+  P.cr_in = (j >= 2 && j <= ny);
+  P.dyj = (R)a.dy[j];
+  P.dyfj = (R)a.dyf[j];
+  P.dyfm = (R)a.dyf[j - 1];
+  P.dysu = (j <= ny) ? (R)(a.dy[j + 1] * 2.0) : (R)0;
+  const double ddxs = a.dx1 * kBetaSgs, ddzs = a.dz1 * kBetaSgs, ddy = a.dy[j], ddyf = a.dyf[j];
+  P.dxs = (R)ddxs;
+  P.dzs = (R)ddzs;
+  double dyf_f, dy_f;  // les.f90:203-222, 499-518, 775-799
+  if (a.type == 1) {
+    const double c2 = -2.0 * kCSmag * kCSmag;
+    dyf_f = c2 * pow(ddxs * ddyf * 1.0 * ddzs, 2.0 / 3.0);
+    dy_f = c2 * pow(ddxs * ddy * 2.0 * ddzs, 2.0 / 3.0);
+  } else if (a.type == 4) {
+    dyf_f = dy_f = -2.0 * kCAmd * kCAmd;
+  } else {
+    const double c2 = -2.0 * kCAmd * kCAmd * 3.0;
+    dyf_f = c2 / (1.0 / (ddxs * ddxs) + 1.0 / ((ddyf * 2.0) * (ddyf * 2.0)) + 1.0 / (ddzs * ddzs));
+    dy_f = c2 / (1.0 / (ddxs * ddxs) + 1.0 / ((ddy * 2.0) * (ddy * 2.0)) + 1.0 / (ddzs * ddzs));
+  }
+  P.dyf_fac = (R)dyf_f;
+  P.dy_fac = (R)dy_f;
+  return P;
 }
 
-// -(I3 - I4) / (I1 - I2) clipped at 0 for a strain s and rotation w (AMD
-// numerator of les.f90:597-637 with unit filter weights, or :311-361 with
-// the Rozema weights dxq, dyq, dzq).
+// The model's inputs at point o of plane j (plane j+1 only where it exists).
 template <typename R>
-__device__ __forceinline__ R amd_value(R a1, R a2, R a3, R a4, R a5, R a6, R w4, R w5, R w6, R dxq,
-                                       R dyq, R dzq, bool rozema) {
-  R v;
-  if (rozema) {
-    v = dxq * a1 * a1 * a1 + dyq * a2 * a2 * a2 + dzq * a3 * a3 * a3 + (2 * dxq + dyq) * a1 * a4 * a4 +
-        (2 * dxq + dzq) * a1 * a5 * a5 + (2 * dyq + dxq) * a2 * a4 * a4 + (2 * dyq + dzq) * a2 * a6 * a6 +
-        (2 * dzq + dxq) * a3 * a5 * a5 + (2 * dzq + dyq) * a3 * a6 * a6 +
-        2 * (dxq + dyq + dzq) * a4 * a5 * a6;
-    v = -v - dyq * a1 * w4 * w4 - dzq * a1 * w5 * w5 - dxq * a2 * w4 * w4 - dzq * a2 * w6 * w6 -
-        dxq * a3 * w5 * w5 - dyq * a3 * w6 * w6 - 2 * dzq * a4 * w5 * w6 + 2 * dyq * a5 * w4 * w6 -
-        2 * dxq * a6 * w4 * w5;
-  } else {
-    v = a1 * a1 * a1 + a2 * a2 * a2 + a3 * a3 * a3 + 3 * a1 * a4 * a4 + 3 * a1 * a5 * a5 + 3 * a2 * a4 * a4 +
-        3 * a2 * a6 * a6 + 3 * a3 * a5 * a5 + 3 * a3 * a6 * a6 + 6 * a4 * a5 * a6;
-    v = -v - a1 * w4 * w4 - a1 * w5 * w5 - a2 * w4 * w4 - a2 * w6 * w6 - a3 * w5 * w5 - a3 * w6 * w6 -
-        2 * a4 * w5 * w6 + 2 * a5 * w4 * w6 - 2 * a6 * w4 * w5;
-  }
-  if (v <= (R)0) return (R)0;
-  return v / (a1 * a1 + a2 * a2 + a3 * a3 + 2 * a4 * a4 + 2 * w4 * w4 + 2 * a5 * a5 + 2 * w5 * w5 +
-              2 * a6 * a6 + 2 * w6 * w6);
+__device__ inline void les_vel_load(const LesVelArgs<R>& a, const Geo& g, int j, long long o,
+                                    lespt::VelIn<R>& x) {
+  const long long pp = g.pp;
+  for (int m = 0; m < 9; ++m) x.c[m] = a.G[m][o];
+  const int upm[4] = {1, 3, 5, 7};   // G12 G21 G23 G32
+  const int dnm[5] = {0, 4, 8, 2, 6};  // G11 G22 G33 G13 G31
+  for (int t = 0; t < 4; ++t) x.up[t] = (j <= g.ny) ? a.G[upm[t]][o + pp] : (R)0;
+  for (int t = 0; t < 5; ++t) x.dn[t] = a.G[dnm[t]][o - pp];
+  x.u1o = a.u1[o];
+  x.u1m = a.u1[o - pp];
+  x.u3o = a.u3[o];
+  x.u3m = a.u3[o - pp];
 }
 
 // One thread per physical point of planes 1..Ny+1: the eddy viscosity, the
 // modelled stresses scaled by the filter factors of les.f90:201-222, 498-518,
 // 773-799 (applied here before the forward transform instead of after it),
-// and the explicit cross-term fluxes.  Values at points the substep never
-// consumes (GY plane 1, GYF plane Ny+1, the zeroed nu_t planes) follow the
-// reference where they are cheap and are otherwise left as the raw strain.
+// and the explicit cross-term fluxes (les.f90:890-919); the point model is
+// lespt::vel_point.  Values at points the substep never consumes (GY plane
+// 1, GYF plane Ny+1, the zeroed nu_t planes) follow the reference where they
+// are cheap and are otherwise left as the raw strain.
 template <typename R>
 __global__ void __launch_bounds__(256) k_les_vel(LesVelArgs<R> a, Geo g) {
   const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -171,143 +199,19 @@ __global__ void __launch_bounds__(256) k_les_vel(LesVelArgs<R> a, Geo g) {
   const int j = 1 + (int)(idx / pp);
   if (j > g.ny + 1) return;
   const long long o = (long long)j * pp + idx % pp;
-  const int ny = g.ny;
-  const R h5 = (R)0.5, two = (R)2, four = (R)4, one = (R)1, zero = (R)0;
-  const R* G11 = a.G[0];
-  const R* G12 = a.G[1];
-  const R* G13 = a.G[2];
-  const R* G21 = a.G[3];
-  const R* G22 = a.G[4];
-  const R* G23 = a.G[5];
-  const R* G31 = a.G[6];
-  const R* G32 = a.G[7];
-  const R* G33 = a.G[8];
-  const R dyj = (R)a.dy[j], dyfj = (R)a.dyf[j], dyfm = (R)a.dyf[j - 1];
-  const R dxs = (R)(a.dx1 * kBetaSgs), dzs = (R)(a.dz1 * kBetaSgs);
-  // strain at this plane: GYF components S11 S22 S33 S13, GY components S12 S23
-  const R S11 = G11[o], S22 = G22[o], S33 = G33[o];
-  const R S13 = h5 * (G13[o] + G31[o]);
-  const R S12 = h5 * (G12[o] + G21[o]);
-  const R S23 = h5 * (G23[o] + G32[o]);
-  // ---- GYF point (plane j)
-  R s1 = zero;
-  bool gyf_in;
-  if (a.type == 5)
-    gyf_in = (j >= a.js && j <= a.je);
-  else
-    gyf_in = (j <= ny);  // types 1, 4 scale planes 1..Ny (les.f90:172-182, 468-478)
-  const bool have_up = (j <= ny);  // GY plane j+1 exists
-  if (gyf_in && have_up) {
-    const long long u = o + pp;
-    if (a.type == 1) {
-      const R s12 = h5 * (h5 * (G12[u] + G21[u]) + S12);
-      const R s23 = h5 * (h5 * (G23[u] + G32[u]) + S23);
-      s1 = sqrt(two * S11 * S11 + four * s12 * s12 + four * S13 * S13 + two * S22 * S22 + four * s23 * s23 +
-                two * S33 * S33);
-    } else if (a.type == 4) {
-      const R dxq = dxs * dxs, dzq = dzs * dzs, dyq = (dyj * two) * (dyj * two);
-      const R a4 = h5 * (h5 * (G12[u] + G21[u]) + S12);
-      const R a6 = h5 * (h5 * (G23[u] + G32[u]) + S23);
-      const R w4 = h5 * (h5 * (G12[u] - G21[u]) + h5 * (G12[o] - G21[o]));
-      const R w5 = h5 * (G13[o] - G31[o]);
-      const R w6 = h5 * (h5 * (G23[u] - G32[u]) + h5 * (G23[o] - G32[o]));
-      s1 = amd_value(S11, S22, S33, a4, S13, a6, w4, w5, w6, dxq, dyq, dzq, true);
-    } else {
-      // invariant strain/rotation: GY components scaled with dys = 2 dy at
-      // their own plane, GYF ones with dxs, dzs (les.f90:1527-1559, 1633-1662)
-      const R dysu = (R)a.dy[j + 1] * two, dysj = dyj * two;
-      const R si4u = h5 * (G12[u] * (dysu / dxs) + G21[u] * (dxs / dysu));
-      const R si4j = h5 * (G12[o] * (dysj / dxs) + G21[o] * (dxs / dysj));
-      const R si6u = h5 * (G23[u] * (dzs / dysu) + G32[u] * (dysu / dzs));
-      const R si6j = h5 * (G23[o] * (dzs / dysj) + G32[o] * (dysj / dzs));
-      const R o4u = h5 * (G12[u] * (dysu / dxs) - G21[u] * (dxs / dysu));
-      const R o4j = h5 * (G12[o] * (dysj / dxs) - G21[o] * (dxs / dysj));
-      const R o6u = h5 * (G23[u] * (dzs / dysu) - G32[u] * (dysu / dzs));
-      const R o6j = h5 * (G23[o] * (dzs / dysj) - G32[o] * (dysj / dzs));
-      const R si5 = h5 * (G13[o] * (dzs / dxs) + G31[o] * (dxs / dzs));
-      const R o5 = h5 * (G13[o] * (dzs / dxs) - G31[o] * (dxs / dzs));
-      s1 = amd_value(S11, S22, S33, h5 * (si4u + si4j), si5, h5 * (si6u + si6j), h5 * (o4u + o4j), o5,
-                     h5 * (o6u + o6j), one, one, one, false);
-    }
-  }
-  // ---- GY point (plane j)
-  R temp = zero;
-  bool gy_in;
-  if (a.type == 1)
-    gy_in = true;                      // les.f90:146-168, 185-197: planes 1..Ny+1
-  else if (a.type == 4)
-    gy_in = (j >= 2);                  // :371-464, 482-494: planes 2..Ny+1
-  else
-    gy_in = (j >= 2 && j <= ny);       // :647-738 computes 2..Ny (Ny+1 never consumed)
-  if (gy_in) {
-    const long long m = o - pp;
-    const R i11 = gy_of(G11, o, pp, dyfm, dyfj, dyj);
-    const R i22 = gy_of(G22, o, pp, dyfm, dyfj, dyj);
-    const R i33 = gy_of(G33, o, pp, dyfm, dyfj, dyj);
-    if (a.type == 1) {
-      const R i13 = h5 * (gy_of(G13, o, pp, dyfm, dyfj, dyj) + gy_of(G31, o, pp, dyfm, dyfj, dyj));
-      temp = sqrt(two * i11 * i11 + four * S12 * S12 + four * i13 * i13 + two * i22 * i22 + four * S23 * S23 +
-                  two * i33 * i33);
-    } else if (a.type == 4) {
-      const R dxq = dxs * dxs, dzq = dzs * dzs, dyq = (dyj * two) * (dyj * two);
-      const R i13 = h5 * (gy_of(G13, o, pp, dyfm, dyfj, dyj) + gy_of(G31, o, pp, dyfm, dyfj, dyj));
-      const R w4 = h5 * (G12[o] - G21[o]);
-      const R w5 = (h5 * (G13[o] - G31[o]) * dyfm + h5 * (G13[m] - G31[m]) * dyfj) / (two * dyj);
-      const R w6 = h5 * (G23[o] - G32[o]);
-      temp = amd_value(i11, i22, i33, S12, i13, S23, w4, w5, w6, dxq, dyq, dzq, true);
-    } else {
-      const R dysj = dyj * two;
-      const R si4 = h5 * (G12[o] * (dysj / dxs) + G21[o] * (dxs / dysj));
-      const R si6 = h5 * (G23[o] * (dzs / dysj) + G32[o] * (dysj / dzs));
-      const R o4 = h5 * (G12[o] * (dysj / dxs) - G21[o] * (dxs / dysj));
-      const R o6 = h5 * (G23[o] * (dzs / dysj) - G32[o] * (dysj / dzs));
-      const R si5o = h5 * (G13[o] * (dzs / dxs) + G31[o] * (dxs / dzs));
-      const R si5m = h5 * (G13[m] * (dzs / dxs) + G31[m] * (dxs / dzs));
-      const R o5o = h5 * (G13[o] * (dzs / dxs) - G31[o] * (dxs / dzs));
-      const R o5m = h5 * (G13[m] * (dzs / dxs) - G31[m] * (dxs / dzs));
-      const R si5 = (si5o * dyfm + si5m * dyfj) / (two * dyj);
-      const R o5 = (o5o * dyfm + o5m * dyfj) / (two * dyj);
-      temp = amd_value(i11, i22, i33, si4, si5, si6, o4, o5, o6, one, one, one, false);
-    }
-  }
-  // ---- filter factors (les.f90:203-222, 499-518, 775-799)
-  // per-plane factors, formed in FP64 from the FP64 metrics
-  double dyf_f, dy_f;
-  {
-    const double ddxs = a.dx1 * kBetaSgs, ddzs = a.dz1 * kBetaSgs, ddy = a.dy[j], ddyf = a.dyf[j];
-    if (a.type == 1) {
-      const double c2 = -2.0 * kCSmag * kCSmag;
-      dyf_f = c2 * pow(ddxs * ddyf * 1.0 * ddzs, 2.0 / 3.0);
-      dy_f = c2 * pow(ddxs * ddy * 2.0 * ddzs, 2.0 / 3.0);
-    } else if (a.type == 4) {
-      dyf_f = dy_f = -2.0 * kCAmd * kCAmd;
-    } else {
-      const double c2 = -2.0 * kCAmd * kCAmd * 3.0;
-      dyf_f = c2 / (1.0 / (ddxs * ddxs) + 1.0 / ((ddyf * 2.0) * (ddyf * 2.0)) + 1.0 / (ddzs * ddzs));
-      dy_f = c2 / (1.0 / (ddxs * ddxs) + 1.0 / ((ddy * 2.0) * (ddy * 2.0)) + 1.0 / (ddzs * ddzs));
-    }
-  }
-  const R dyf_fac = (R)dyf_f, dy_fac = (R)dy_f;
-  // GYF stresses (Sij2 is handled implicitly through nu_t)
-  const R f1 = (gyf_in && have_up) ? s1 : one;
-  a.tau11[o] = dyf_fac * (f1 * S11);
-  a.tau33[o] = dyf_fac * (f1 * S33);
-  a.tau13[o] = dyf_fac * (f1 * S13);
-  // GY stresses, less the parts treated implicitly in the CN solve
-  const R du1 = (a.u1[o] - a.u1[o - pp]) / dyj, du3 = (a.u3[o] - a.u3[o - pp]) / dyj;
-  if (gy_in) {
-    a.tau12[o] = dy_fac * (temp * (S12 - h5 * du1));
-    a.tau23[o] = dy_fac * (temp * (S23 - h5 * du3));
-  } else {
-    a.tau12[o] = dy_fac * S12;
-    a.tau23[o] = dy_fac * S23;
-  }
-  // eddy viscosity at GY points, zero at planes 1, 2, Ny, Ny+1 (ghost_les_mpi)
-  R nt = zero;
-  if (j >= 3 && j <= ny - 1 && gy_in) nt = -h5 * dy_fac * temp;
-  a.nu_t[o] = (double)nt;
-  a.cr1[o] = (j >= 2 && j <= ny) ? nt * du1 : zero;
-  a.cr3[o] = (j >= 2 && j <= ny) ? nt * du3 : zero;
+  const lespt::VelPlane<R> P = les_vel_plane(a, g, j);
+  lespt::VelIn<R> x;
+  les_vel_load(a, g, j, o, x);
+  lespt::VelOut<R> y;
+  lespt::vel_point(P, x, y);
+  a.tau11[o] = y.t11;
+  a.tau33[o] = y.t33;
+  a.tau13[o] = y.t13;
+  a.tau12[o] = y.t12;
+  a.tau23[o] = y.t23;
+  a.nu_t[o] = (double)y.nt;
+  a.cr1[o] = y.cr1;
+  a.cr3[o] = y.cr3;
 }
 
 // --------------------------------------------------------- scalar model --
@@ -328,11 +232,54 @@ struct LesThArgs {
   int jst, jet;
 };
 
+// Per-plane constants of the AMD scalar model at plane j (les.f90:1238-1384).
+template <typename R>
+__device__ inline lespt::ThPlane<R> les_th_plane(const LesThArgs<R>& a, const Geo& g, int j) {
+  const int ny = g.ny;
+  lespt::ThPlane<R> P;
+  P.gyf_in = (j >= a.jst && j <= a.jet && j <= ny);
+  P.gy_in = (j >= 3 && j <= ny - 1);
+  const double ddxs = a.dx1 * kBetaSgs, ddzs = a.dz1 * kBetaSgs;
+  P.dxs = (R)ddxs;
+  P.dzs = (R)ddzs;
+  P.dyj = (R)a.dy[j];
+  P.dyfj = (R)a.dyf[j];
+  P.dyfm = (R)a.dyf[j - 1];
+  const double c2 = -2.0 * kCAmd * kCAmd * 3.0;
+  const double ddlf = a.dyf[j] * 2.0, ddl = a.dy[j] * 2.0;
+  P.dyf_fac = (R)(c2 / (1.0 / (ddxs * ddxs) + 1.0 / (ddlf * ddlf) + 1.0 / (ddzs * ddzs)));
+  P.dy_fac = (R)(c2 / (1.0 / (ddxs * ddxs) + 1.0 / (ddl * ddl) + 1.0 / (ddzs * ddzs)));
+  return P;
+}
+
+template <typename R>
+__device__ inline void les_th_load(const LesThArgs<R>& a, const Geo& g, int j, long long o, lespt::ThIn<R>& x) {
+  const long long pp = g.pp;
+  for (int m = 0; m < 9; ++m) x.c[m] = a.G[m][o];
+  x.t[0] = a.T1[o];
+  x.t[1] = a.T2[o];
+  x.t[2] = a.T3[o];
+  const bool up = j <= g.ny;
+  x.up[0] = up ? a.G[1][o + pp] : (R)0;
+  x.up[1] = up ? a.G[3][o + pp] : (R)0;
+  x.up[2] = up ? a.G[5][o + pp] : (R)0;
+  x.up[3] = up ? a.G[7][o + pp] : (R)0;
+  x.up[4] = up ? a.T2[o + pp] : (R)0;
+  x.dn[0] = a.G[0][o - pp];
+  x.dn[1] = a.G[4][o - pp];
+  x.dn[2] = a.G[8][o - pp];
+  x.dn[3] = a.G[2][o - pp];
+  x.dn[4] = a.G[6][o - pp];
+  x.dn[5] = a.T1[o - pp];
+  x.dn[6] = a.T3[o - pp];
+  x.dd = (j >= 2) ? a.G[4][o - 2 * pp] : (R)0;
+}
+
 // Types 1/4: kappa_t = nu_t and the explicit horizontal diffusion fluxes
 // kappa_t d1 th, kappa_t d3 th (les.f90:1142-1203).  Type 5: the AMD scalar
 // model at GYF (:1238-1275) and GY points (:1278-1330), kappa_t (:1373-1384)
 // and the fluxes 0.5 delta_yf s1_th d_i th (:1334-1342, 1392-1401), with the
-// filter factor applied before the forward transform.
+// filter factor applied before the forward transform (lespt::th_point).
 template <typename R>
 __global__ void __launch_bounds__(256) k_les_th(LesThArgs<R> a, Geo g) {
   const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -340,8 +287,6 @@ __global__ void __launch_bounds__(256) k_les_th(LesThArgs<R> a, Geo g) {
   const int j = 1 + (int)(idx / pp);
   if (j > g.ny + 1) return;
   const long long o = (long long)j * pp + idx % pp;
-  const int ny = g.ny;
-  const R h5 = (R)0.5, two = (R)2, zero = (R)0;
   if (a.type != 5) {
     const double kt = a.nu_t[o];
     a.kappa_t[o] = kt;
@@ -349,63 +294,14 @@ __global__ void __launch_bounds__(256) k_les_th(LesThArgs<R> a, Geo g) {
     a.f3[o] = (R)kt * a.T3[o];
     return;
   }
-  const R* G11 = a.G[0];
-  const R* G12 = a.G[1];
-  const R* G13 = a.G[2];
-  const R* G21 = a.G[3];
-  const R* G22 = a.G[4];
-  const R* G23 = a.G[5];
-  const R* G31 = a.G[6];
-  const R* G32 = a.G[7];
-  const R* G33 = a.G[8];
-  const double ddxs = a.dx1 * kBetaSgs, ddzs = a.dz1 * kBetaSgs;
-  const R dxs = (R)ddxs, dzs = (R)ddzs;
-  const R dyj = (R)a.dy[j], dyfj = (R)a.dyf[j], dyfm = (R)a.dyf[j - 1];
-  const double c2 = -2.0 * kCAmd * kCAmd * 3.0;
-  // GYF point: du2dy(j) of the reference is G22(j-1)
-  R f1 = zero, f3 = zero;
-  if (j >= a.jst && j <= a.jet && j <= ny) {
-    const long long u = o + pp, m = o - pp;
-    const R dly = dyfj * two;
-    const R dx2 = dxs * dxs, dy2 = dly * dly, dz2 = dzs * dzs;
-    const R tx = a.T1[o], tz = a.T3[o], ty = h5 * (a.T2[u] + a.T2[o]);
-    R v = -(dx2 * G11[o] * tx * tx + dy2 * G22[m] * ty * ty + dz2 * G33[o] * tz * tz +
-            (dy2 * h5 * (G12[u] + G12[o]) + dx2 * h5 * (G21[u] + G21[o])) * tx * ty +
-            (dz2 * G13[o] + dx2 * G31[o]) * tx * tz +
-            (dz2 * h5 * (G23[u] + G23[o]) + dy2 * h5 * (G32[u] + G32[o])) * ty * tz);
-    if (v <= zero)
-      v = zero;
-    else
-      v = v / ((dxs * tx) * (dxs * tx) + (dly * ty) * (dly * ty) + (dzs * tz) * (dzs * tz));
-    const double ddly = a.dyf[j] * 2.0;
-    const R dyf_fac = (R)(c2 / (1.0 / (ddxs * ddxs) + 1.0 / (ddly * ddly) + 1.0 / (ddzs * ddzs)));
-    f1 = h5 * dyf_fac * (v * tx);
-    f3 = h5 * dyf_fac * (v * tz);
-  }
-  a.f1[o] = f1;
-  a.f3[o] = f3;
-  // GY point
-  R kt = zero;
-  if (j >= 3 && j <= ny - 1) {
-    const long long m = o - pp;
-    const R dly = dyj * two;
-    const R dx2 = dxs * dxs, dy2 = dly * dly, dz2 = dzs * dzs;
-    auto gy = [&](const R* f) { return (f[o] * dyfm + f[m] * dyfj) / (two * dyj); };
-    // du2dy at GY interpolates du2dy(j) = G22(j-1) and du2dy(j-1) = G22(j-2)
-    const R d22 = (G22[m] * dyfm + G22[m - pp] * dyfj) / (two * dyj);
-    const R tx = gy(a.T1), tz = gy(a.T3), ty = a.T2[o];
-    R v = -(dx2 * gy(G11) * tx * tx + dy2 * d22 * ty * ty + dz2 * gy(G33) * tz * tz +
-            (dy2 * G12[o] + dx2 * G21[o]) * tx * ty + (dz2 * gy(G13) + dx2 * gy(G31)) * tx * tz +
-            (dz2 * G23[o] + dy2 * G32[o]) * ty * tz);
-    if (v <= zero)
-      v = zero;
-    else
-      v = v / ((dxs * tx) * (dxs * tx) + (dly * ty) * (dly * ty) + (dzs * tz) * (dzs * tz));
-    const double ddly = a.dy[j] * 2.0;
-    const R dy_fac = (R)(c2 / (1.0 / (ddxs * ddxs) + 1.0 / (ddly * ddly) + 1.0 / (ddzs * ddzs)));
-    kt = -h5 * dy_fac * v;
-  }
-  a.kappa_t[o] = (double)kt;
+  const lespt::ThPlane<R> P = les_th_plane(a, g, j);
+  lespt::ThIn<R> x;
+  les_th_load(a, g, j, o, x);
+  lespt::ThOut<R> y;
+  lespt::th_point(P, x, y);
+  a.f1[o] = y.f1;
+  a.f3[o] = y.f3;
+  a.kappa_t[o] = (double)y.kt;
 }
 
 // ---------------------------------------------- stress divergence into cf --

@@ -194,18 +194,22 @@ int dgpu_set_delta_t(dgpu_handle* h, double delta_t);
 int dgpu_synchronize(dgpu_handle* h);
 
 /*
- * Adjoint of the FP64 DNS substep (no counterpart in the reference, which has
- * no differentiation; it is the added "differentiation" form of the substep
- * for gradient-based studies).  The handle holds an adjoint state with the
- * spectral fields of the forward state (ids DGPU_CU1..DGPU_CFTH, reference
- * layout): for a real loss L of a complex field c it holds
- * dL/dRe c + i dL/dIm c.
+ * Adjoint of the FP64 substep, DNS and LES (no counterpart in the reference,
+ * which has no differentiation; it is the added "differentiation" form of
+ * the substep for gradient-based studies).  The handle holds an adjoint
+ * state with the spectral fields of the forward state (ids
+ * DGPU_CU1..DGPU_CFTH, reference layout): for a real loss L of a complex
+ * field c it holds dL/dRe c + i dL/dIm c.
  *
  * dgpu_adj_rk_substep: with the forward state equal to the INPUT of substep
  * rk_step (as rk_chan_1 would receive it) and the adjoint state equal to the
  * gradient with respect to that substep's OUTPUT, replace the adjoint state
  * by the gradient with respect to the input.  The forward state is left
- * unchanged.  Returns DGPU_EUNSUP for the mixed form and for use_les.
+ * unchanged.  Without LES (or with its gate closed) the substep is affine in
+ * cp and the RK carry, so only cu1..3 and cth of the input are read; with
+ * the LES terms the adjoint reruns the forward substep and reads the whole
+ * input state.  Returns DGPU_EUNSUP for the mixed form and while
+ * variable_dt is on.
  */
 int dgpu_adj_set_field(dgpu_handle* h, int field, const double* host);
 int dgpu_adj_get_field(dgpu_handle* h, int field, double* host);
@@ -221,8 +225,8 @@ int dgpu_mean_profile(dgpu_handle* h, int field, double* out);
 /* Checkpoints of the forward state in device slot `slot` (0..4096,
    allocated on first save), for recomputation in adjoint runs.  mode 1 saves
    the whole state (cu1..3, cp, cth, cf1..3, cfth and the f_type 4 wall
-   value), mode 2 only what dgpu_adj_rk_substep reads (cu1..3, cth), mode 0
-   restores what the slot holds. */
+   value), mode 2 only what dgpu_adj_rk_substep reads without LES (cu1..3,
+   cth), mode 0 restores what the slot holds. */
 int dgpu_checkpoint(dgpu_handle* h, int slot, int mode);
 
 /*
