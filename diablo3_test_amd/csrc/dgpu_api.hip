@@ -104,6 +104,9 @@ struct dgpu_handle {
   std::vector<DevBuf<double2>> tpool;
   DevBuf<double> sb;
   DevBuf<double2> sg;
+  // projection: 1/b_j of every mode (rows 1..Ny) and the row coefficients
+  DevBuf<double> rd_ib;
+  DevBuf<PoisRow> d_prow;
   DevBuf<double2> twx, twz;
   DevBuf<float2> twx32, twz32;  // FP32 twiddles of the mixed form
   DevBuf<double> d_kx, d_kz, d_dy, d_dyf, d_gyf;
@@ -738,19 +741,59 @@ void launch_remdiv(dgpu_handle* h, bool update_cp, double temp4) {
   a.cu2 = h->cu[1].p;
   a.cu3 = h->cu[2].p;
   a.cp = h->cp.p;
-  a.sb = h->sb.p;
-  a.sg = h->sg.p;
+  a.ib = h->rd_ib.p;
   a.kx = h->d_kx.p;
   a.kz = h->d_kz.p;
-  a.dy = h->d_dy.p;
-  a.dyf = h->d_dyf.p;
   a.inv_h = 1.0 / temp4;
   a.update_cp = update_cp ? 1 : 0;
   a.js = h->jstart;
   a.je = h->jend;
-  dim3 grid((unsigned)((g.sp + 255) / 256)), block(256);
+  // k_remdiv_seg addresses a field with 32-bit byte offsets
+  if ((double)g.ny2 * (double)g.sp * 16.0 >= 4294967295.0)
+    throw ApiError{DGPU_EUNSUP, "spectral field too large for the projection"};
   Timed tm(h, DGPU_K_REMDIV, (update_cp ? 128.0 : 96.0) * g.sp * g.ny);
-  hipLaunchKernelGGL(k_remdiv, grid, block, 0, h->stream, a, g);
+  // rows 2..Ny in segments; the block holds whole wavefronts (64 / MW
+  // segments each), the segments past Ny being empty
+  const int nr = g.ny - 1;
+  auto launch = [&](auto kern, int L, int MW) {
+    const int spw = 64 / MW;
+    const int S = ((nr + L - 1) / L + spw - 1) / spw * spw;
+    hipLaunchKernelGGL(kern, dim3((unsigned)((g.sp + MW - 1) / MW)), dim3(MW * S), 0, h->stream, a, g,
+                       h->d_prow.p, S);
+  };
+  if (nr <= 4 * 64)
+    launch(k_remdiv_seg<4, 16, 1024>, 4, 16);
+  else if (nr <= 8 * 64)
+    launch(k_remdiv_seg<8, 8, 512>, 8, 8);
+  else if (nr <= 8 * 128)
+    launch(k_remdiv_seg<8, 4, 512>, 8, 4);
+  else
+    throw ApiError{DGPU_EUNSUP, "ny too large for the projection"};
+  HIPCHECK(hipGetLastError());
+}
+
+// The projection's row coefficients and the pivots of every mode (once per
+// handle: they depend only on the grid and the wavenumbers).
+void setup_projection(dgpu_handle* h) {
+  const Geo& g = h->g;
+  const int ny = g.ny;
+  std::vector<PoisRow> rows((size_t)ny + 2, PoisRow{0.0, 0.0, 0.0, 0.0});
+  for (int j = 1; j <= ny; ++j) {
+    PoisRow& r = rows[j];
+    if (j >= 2 && j < ny) {
+      r.ml = 1.0 / (h->dy[j] * h->dyf[j]);
+      r.mu = 1.0 / (h->dy[j + 1] * h->dyf[j]);
+    } else if (j == ny) {
+      r.ml = 1.0;
+      r.mu = 0.0;
+    }
+    r.idyf = 1.0 / h->dyf[j];
+    r.idy = (j >= 2) ? 1.0 / h->dy[j] : 0.0;
+  }
+  upload(h->d_prow, rows);
+  h->rd_ib.alloc((size_t)g.ny2 * g.sp);
+  hipLaunchKernelGGL(k_remdiv_pivots, dim3((unsigned)((g.sp + 255) / 256)), dim3(256), 0, h->stream, h->rd_ib.p,
+                     h->d_prow.p, h->d_kx.p, h->d_kz.p, g);
   HIPCHECK(hipGetLastError());
 }
 
@@ -2032,6 +2075,8 @@ void free_all(dgpu_handle* h) {
   h->tpool.clear();
   h->sb.release();
   h->sg.release();
+  h->rd_ib.release();
+  h->d_prow.release();
   h->twx.release();
   h->twz.release();
   h->twx32.release();
@@ -2194,6 +2239,7 @@ void create_impl(dgpu_handle* h, const double* gy) {
   upload(h->d_dy, h->dy);
   upload(h->d_dyf, h->dyf);
   upload(h->d_gyf, h->gyf);
+  setup_projection(h);
 
   // CN factor tables (DGPU_SEG_TARGET overrides the target segment length)
   if (const char* e = std::getenv("DGPU_SEG_TARGET")) seg_target = std::atoi(e);

@@ -831,133 +831,260 @@ __global__ void __launch_bounds__(1024) k_thomas_seg(const T* __restrict__ r,
 // ----------------------------------------------------------- projection --
 // rem_div_chan (channel.f90:946-1023) with apply_BC_rem_div_mpi (:1974-2015)
 // and the complex Thomas sweeps (:1408-1457, :1499-1540); optionally the
-// pressure update cp += phi/h (:905-911).  One (q, kk) mode per lane.
+// pressure update cp += phi/h (:905-911).
+//
+// A mode's matrix depends only on lam = kx^2 + kz^2 and the grid, so its
+// pivots b_j (the forward elimination of the Thomas sweep) are computed once
+// per handle by k_remdiv_pivots and stored as 1/b_j.  The solve is then two
+// first-order linear recurrences per mode,
+//   g_j   = rhs_j + m_j g_{j-1},           m_j = -ml_j / b_{j-1}    (forward)
+//   phi_j = g_j / b_j - (mu_j / b_j) phi_{j+1}                      (backward)
+// which k_remdiv_seg splits into segments of L rows held in registers and
+// joins with a scan over the segments.
 struct RemDivArgs {
   double2* cu1;
   double2* cu2;
   double2* cu3;
   double2* cp;
-  double* sb;    // scratch b' (ny2 x sp)
-  double2* sg;   // scratch g' (ny2 x sp)
+  const double* ib;     // 1/b_j per (row, mode), rows 1..Ny (ny2 x sp)
   const double* kx;
   const double* kz;
-  const double* dy;
-  const double* dyf;
   double inv_h;  // 1/temp4
   int update_cp;
   int js, je;
 };
 
-// The sweeps run over batches of RB rows: a batch's loads are issued
-// together before its part of the recurrence, so each lane keeps several rows
-// in flight (one lane per mode gives fewer than one wavefront per SIMD).
-constexpr int RB = 8;
+// Row coefficients of the projection's tridiagonal (channel.f90:970-976,
+// :1988-2011): rows 2..Ny-1 ml = 1/(dy(j) dyf(j)), mu = 1/(dy(j+1) dyf(j)),
+// md = -lam - mu - ml; row Ny is phi(Ny) = phi(Ny-1) (ml = 1, md = -1,
+// mu = 0); row 1 has b = 1 and mu = -1 (Neumann) or 0 (Dirichlet, the (0,0)
+// mode), set per mode in the kernels.  idyf, idy: 1/dyf(j), 1/dy(j).
+struct PoisRow {
+  double ml, mu, idyf, idy;
+};
 
-__global__ void __launch_bounds__(256) k_remdiv(RemDivArgs a, Geo g) {
-  typedef cpx<double> C;
+__global__ void __launch_bounds__(256) k_remdiv_pivots(double* __restrict__ ib, const PoisRow* __restrict__ rows,
+                                                       const double* __restrict__ kx,
+                                                       const double* __restrict__ kz, Geo g) {
   const long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= g.sp) return;
   const int q = (int)(m / g.nkk), kk = (int)(m % g.nkk);
-  const double kxq = a.kx[q], kzk = a.kz[kk];
-  const double lam = kxq * kxq + kzk * kzk;
-  const bool mean = (q == 0 && kk == 0);
-  const int ny = g.ny;
+  const double lam = kx[q] * kx[q] + kz[kk] * kz[kk];
   const long long sp = g.sp;
-  C* cu1 = reinterpret_cast<C*>(a.cu1);
-  C* cu2 = reinterpret_cast<C*>(a.cu2);
-  C* cu3 = reinterpret_cast<C*>(a.cu3);
-  C* cp = reinterpret_cast<C*>(a.cp);
-  C* sg = reinterpret_cast<C*>(a.sg);
-  // row 1: Neumann phi1 = phi2, or Dirichlet phi1 = 0 for the (0,0) mode
   double bprev = 1.0;
-  double muprev = mean ? 0.0 : -1.0;
-  C gprev = {0.0, 0.0};
-  a.sb[sp + m] = bprev;
-  sg[sp + m] = gprev;
-  C cu2j = cu2[2 * sp + m];  // cu2 at row j+1 for j = 1
-  // forward sweep over rows 2..ny; row ny is phi(ny) = phi(ny-1)
-  for (int j0 = 2; j0 <= ny; j0 += RB) {
-    C d1[RB], d3[RB], d2[RB];
+  double muprev = (m == 0) ? 0.0 : -1.0;
+  ib[sp + m] = 1.0;
+  for (int j = 2; j <= g.ny; ++j) {
+    const PoisRow rw = rows[j];
+    const double md = (j < g.ny) ? -lam - rw.mu - rw.ml : -1.0;
+    const double mlt = -rw.ml / bprev;
+    const double b = md + mlt * muprev;
+    ib[(long long)j * sp + m] = 1.0 / b;
+    bprev = b;
+    muprev = rw.mu;
+  }
+}
+
+// One workgroup: MW consecutive modes x S segments of L rows.  Row 1 needs
+// no segment (g_1 = 0, and phi_1 = phi_2, or 0 for the (0,0) mode), so the
+// segments cover rows 2..Ny: segment s holds rows 2 + s L ... .  Lane
+// mm + MW sl of wavefront w is mode mm of segment s = SPW w + sl, SPW = 64/MW
+// segments per wavefront, so a wavefront reads MW-mode runs of SPW L rows
+// (the wider the runs, the better HBM serves these plane-strided reads:
+// 8-mode runs measured 1.94 ms per step at 512x257x512, 16-mode runs 1.66).
+// Every value of a segment stays in registers between the load and the
+// store.  The segments are joined by an inclusive scan over s (forward: g at
+// each segment's last row) and a reverse one (backward: phi at each
+// segment's first row): within a wavefront by shuffles, across wavefronts by
+// one exchange of the wavefront totals in LDS.
+// MAXT bounds the workgroup (and so the registers per lane): MW * S <= MAXT.
+template <int L, int MW, int MAXT>
+__global__ void __launch_bounds__(MAXT) k_remdiv_seg(RemDivArgs a, Geo g, const PoisRow* __restrict__ rows,
+                                                     int S) {
+  typedef cpx<double> C;
+  constexpr int SPW = 64 / MW;
+  constexpr int SMAX = MAXT / MW, WMAX = MAXT / 64;
+  __shared__ double tP[WMAX][MW];
+  __shared__ C tA[WMAX][MW];
+  __shared__ double tQ[WMAX][MW];
+  __shared__ C tB[WMAX][MW];
+  __shared__ C tL[SMAX][MW];
+  const int mm = threadIdx.x % MW, s = threadIdx.x / MW;
+  const int w = threadIdx.x / 64, sl = (threadIdx.x % 64) / MW;
+  const int nw = blockDim.x / 64;
+  const long long sp = g.sp;
+  const long long m = (long long)blockIdx.x * MW + mm;
+  const bool live = m < sp;
+  const long long mc = live ? m : 0;
+  const int q = (int)(mc / g.nkk), kk = (int)(mc % g.nkk);
+  const double kxq = a.kx[q], kzk = a.kz[kk];
+  const int ny = g.ny;
+  const int j0 = 2 + s * L;
+  const C Z = {0.0, 0.0};
+  // a field is below 4 GB (checked by the launcher): one 32-bit byte offset
+  // per row serves every field, from the field's base in scalar registers
+  auto ldc = [](const double2* base, unsigned ob) {
+    const double2 v = *reinterpret_cast<const double2*>(reinterpret_cast<const char*>(base) + ob);
+    return C{v.x, v.y};
+  };
+  auto stc = [](double2* base, unsigned ob, C v) {
+    *reinterpret_cast<double2*>(reinterpret_cast<char*>(base) + ob) = make_double2(v.x, v.y);
+  };
+  auto ldd = [](const double* base, unsigned ob) {
+    return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(base) + ob);
+  };
+  auto off = [&](int j) { return (unsigned)(((long long)j * sp + m) * 16); };
+  auto shup = [](C v, int d) { return C{__shfl_up(v.x, d), __shfl_up(v.y, d)}; };
+  auto shdn = [](C v, int d) { return C{__shfl_down(v.x, d), __shfl_down(v.y, d)}; };
+  // rows j0 .. j0+L-1 of cu1, cu3, cu2 (plus cu2 at j0+L) and 1/b (plus j0-1)
+  C d1[L], d3[L], d2[L + 1];
+  double ibv[L + 1];
 #pragma unroll
-    for (int t = 0; t < RB; ++t) {
-      const int j = j0 + t;
-      const long long o = (long long)j * sp + m;
-      if (j < ny) {
-        d1[t] = cu1[o];
-        d3[t] = cu3[o];
-        d2[t] = cu2[o + sp];
-      }
-    }
+  for (int t = 0; t < L; ++t) {
+    const int j = j0 + t;
+    const unsigned o = off(j);
+    const bool ok = live && j <= ny;
+    d1[t] = ok ? ldc(a.cu1, o) : Z;
+    d3[t] = ok ? ldc(a.cu3, o) : Z;
+    d2[t] = ok ? ldc(a.cu2, o) : Z;
+    ibv[t + 1] = ok ? ldd(a.ib, o >> 1) : 0.0;
+  }
+  d2[L] = (live && j0 + L <= ny) ? ldc(a.cu2, off(j0 + L)) : Z;
+  ibv[0] = (live && j0 - 1 <= ny) ? ldd(a.ib, off(j0 - 1) >> 1) : 0.0;
+
+  // forward sweep of the segment from a zero carry: g_t = gl_t + Pl_t * g(j0-1)
+  C gl[L];
+  double Pl[L];
+  {
+    C h = Z;
+    double P = 1.0;
 #pragma unroll
-    for (int t = 0; t < RB; ++t) {
+    for (int t = 0; t < L; ++t) {
       const int j = j0 + t;
-      if (j > ny) break;
-      const long long o = (long long)j * sp + m;
-      double ml, md, mu;
-      C rhs;
-      if (j < ny) {
-        ml = 1.0 / (a.dy[j] * a.dyf[j]);
-        md = -lam - 1.0 / (a.dy[j + 1] * a.dyf[j]) - 1.0 / (a.dy[j] * a.dyf[j]);
-        mu = 1.0 / (a.dy[j + 1] * a.dyf[j]);
-        rhs = ik(kxq, d1[t]) + (1.0 / a.dyf[j]) * (d2[t] - cu2j) + ik(kzk, d3[t]);
-        cu2j = d2[t];
-      } else {
-        ml = 1.0;
-        md = -1.0;
-        mu = 0.0;
-        rhs = C{0.0, 0.0};
+      if (j <= ny) {
+        const PoisRow rw = rows[j];
+        const double mj = -rw.ml * ibv[t];
+        const C rhs = (j < ny) ? ik(kxq, d1[t]) + rw.idyf * (d2[t + 1] - d2[t]) + ik(kzk, d3[t]) : Z;
+        h = rhs + mj * h;
+        P = mj * P;
       }
-      const double mlt = -ml / bprev;
-      const double b = md + mlt * muprev;
-      const C gg = rhs + mlt * gprev;
-      a.sb[o] = b;
-      sg[o] = gg;
-      bprev = b;
-      gprev = gg;
-      muprev = mu;
+      gl[t] = h;
+      Pl[t] = P;
     }
   }
-  // backward sweep + projection + pressure, rows ny..1 in batches
-  C phi_up = {0.0, 0.0};
-  for (int j0 = ny; j0 >= 1; j0 -= RB) {
-    C gg[RB], v1[RB], v3[RB], v2[RB], vp[RB];
-    double bb[RB];
+  // inclusive scan of (P, A) over the segments: in the wavefront, then across
+  {
+    double P = Pl[L - 1];
+    C A = gl[L - 1];
 #pragma unroll
-    for (int t = 0; t < RB; ++t) {
-      const int j = j0 - t;
-      const long long o = (long long)j * sp + m;
-      if (j >= 1) {
-        gg[t] = sg[o];
-        bb[t] = a.sb[o];
-        v1[t] = cu1[o];
-        v3[t] = cu3[o];
-        if (j + 1 <= ny && j + 1 >= 2) v2[t] = cu2[o + sp];
-        if (a.update_cp && j >= a.js && j <= a.je) vp[t] = cp[o];
+    for (int d = 1; d < SPW; d <<= 1) {
+      const double Pp = __shfl_up(P, d * MW);
+      const C Ap = shup(A, d * MW);
+      if (sl >= d) {
+        A = A + P * Ap;
+        P = P * Pp;
       }
     }
-#pragma unroll
-    for (int t = 0; t < RB; ++t) {
-      const int j = j0 - t;
-      if (j < 1) break;
-      const long long o = (long long)j * sp + m;
-      double mu;
-      if (j == ny)
-        mu = 0.0;
-      else if (j == 1)
-        mu = mean ? 0.0 : -1.0;
-      else
-        mu = 1.0 / (a.dy[j + 1] * a.dyf[j]);
-      C phi;
-      if (j == ny)
-        phi = (1.0 / bb[t]) * gg[t];
-      else
-        phi = (1.0 / bb[t]) * (gg[t] - mu * phi_up);
-      cu1[o] = v1[t] - ik(kxq, phi);
-      cu3[o] = v3[t] - ik(kzk, phi);
-      if (j + 1 <= ny && j + 1 >= 2) cu2[o + sp] = v2[t] - (1.0 / a.dy[j + 1]) * (phi_up - phi);
-      if (a.update_cp && j >= a.js && j <= a.je) cp[o] = vp[t] + a.inv_h * phi;
-      phi_up = phi;
+    if (sl == SPW - 1) {
+      tP[w][mm] = P;
+      tA[w][mm] = A;
     }
+    // lane sl-1's inclusive value: the composition of the segments before
+    // this one in the wavefront
+    double Pe = __shfl_up(P, MW);
+    C Ae = shup(A, MW);
+    if (sl == 0) {
+      Pe = 1.0;
+      Ae = Z;
+    }
+    __syncthreads();
+    C cin = Z;  // g at the last row of wavefront w-1 (g_1 = 0 before wavefront 0)
+    for (int w2 = 0; w2 < w; ++w2) cin = tA[w2][mm] + tP[w2][mm] * cin;
+    const C gin = Ae + Pe * cin;
+#pragma unroll
+    for (int t = 0; t < L; ++t) gl[t] = gl[t] + Pl[t] * gin;
+  }
+  // backward sweep of the segment from a zero carry: phi_t = gl_t + Pl_t * phi(j0+L)
+  {
+    C x = Z;
+    double Q = 1.0;
+#pragma unroll
+    for (int t = L - 1; t >= 0; --t) {
+      const int j = j0 + t;
+      if (j <= ny) {
+        const double e = ibv[t + 1];
+        const double f = -rows[j].mu * e;  // mu(Ny) = 0
+        x = e * gl[t] + f * x;
+        Q = f * Q;
+      }
+      gl[t] = x;
+      Pl[t] = Q;
+    }
+  }
+  // cp is read here, so that its latency overlaps the reverse scan
+  const bool cpu = a.update_cp != 0;
+  C pv[L];
+#pragma unroll
+  for (int t = 0; t < L; ++t) {
+    const int j = j0 + t;
+    pv[t] = (live && cpu && j <= ny && j >= a.js && j <= a.je) ? ldc(a.cp, off(j)) : Z;
+  }
+  // reverse inclusive scan of (Q, B): phi at each segment's first row
+  {
+    double Q = Pl[0];
+    C B = gl[0];
+#pragma unroll
+    for (int d = 1; d < SPW; d <<= 1) {
+      const double Qn = __shfl_down(Q, d * MW);
+      const C Bn = shdn(B, d * MW);
+      if (sl + d < SPW) {
+        B = B + Q * Bn;
+        Q = Q * Qn;
+      }
+    }
+    if (sl == 0) {
+      tQ[w][mm] = Q;
+      tB[w][mm] = B;
+    }
+    double Qe = __shfl_down(Q, MW);
+    C Be = shdn(B, MW);
+    if (sl == SPW - 1) {
+      Qe = 1.0;
+      Be = Z;
+    }
+    __syncthreads();
+    C ctop = Z;  // phi at the first row of wavefront w+1 (0 above the last)
+    for (int w2 = nw - 1; w2 > w; --w2) ctop = tB[w2][mm] + tQ[w2][mm] * ctop;
+    const C ptop = Be + Qe * ctop;
+#pragma unroll
+    for (int t = 0; t < L; ++t) gl[t] = gl[t] + Pl[t] * ptop;  // gl now holds phi
+  }
+  // row 1: phi_1 = (g_1 - mu_1 phi_2) / b_1 = phi_2, or 0 for the (0,0) mode
+  const C phi1 = (mc == 0) ? Z : gl[0];
+  // phi at the row below the segment, for the cu2 update of its first row
+  tL[s][mm] = gl[L - 1];
+  __syncthreads();
+  const C phim = (s > 0) ? tL[s - 1][mm] : phi1;
+  if (!live) return;
+#pragma unroll
+  for (int t = 0; t < L; ++t) {
+    const int j = j0 + t;
+    if (j > ny) break;
+    const unsigned o = off(j);
+    const C phi = gl[t];
+    stc(a.cu1, o, d1[t] - ik(kxq, phi));
+    stc(a.cu3, o, d3[t] - ik(kzk, phi));
+    stc(a.cu2, o, d2[t] - rows[j].idy * (phi - (t == 0 ? phim : gl[t - 1])));
+    if (cpu && j >= a.js && j <= a.je) stc(a.cp, o, pv[t] + a.inv_h * phi);
+  }
+  if (s == 0) {  // row 1: cu1, cu3 and cp (cu2 at row 1 is not part of the projection)
+    const unsigned o = off(1);
+    const C v1 = ldc(a.cu1, o), v3 = ldc(a.cu3, o);
+    const bool cp1 = cpu && 1 >= a.js && 1 <= a.je;
+    const C vp = cp1 ? ldc(a.cp, o) : Z;
+    stc(a.cu1, o, v1 - ik(kxq, phi1));
+    stc(a.cu3, o, v3 - ik(kzk, phi1));
+    if (cp1) stc(a.cp, o, vp + a.inv_h * phi1);
   }
 }
 

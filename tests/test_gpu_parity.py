@@ -130,6 +130,21 @@ def test_rectangular_box_and_ny():
     check_against_floor("64x41x16", errs, sensitivity(p, gy, st, 4))
 
 
+@pytest.mark.parametrize("ny", [289, 545])
+def test_tall_channel_parity(ny):
+    """Ny past 257 and past 513: the projection's wider-segment layouts
+    (8 rows per segment, 8 and 4 modes per run) against the oracle."""
+    p = workloads.channel_params(16, ny, 16, 300.0, ri=0.2, dt=1e-4)
+    gy = workloads.channel_grid(ny)
+    st = workloads.initial_state(p, gy, "white", 6, 2e-2)
+    print(f"tall channel 16x{ny}x16", flush=True)
+    g, o = pair(p, gy, st)
+    g.step(2)
+    o.step(2)
+    errs = field_errors(g.get_state(), o.get_state(), p)
+    check_against_floor(f"16x{ny}x16", errs, sensitivity(p, gy, st, 2))
+
+
 def test_cfg2_one_step_parity():
     """256x193x256, Re = 400, Ri = 0.1: one step against the oracle."""
     p, gy, ic, seed, amp = workloads.workload("cfg2_dns_256x193x256")
