@@ -61,15 +61,38 @@ class WallShearRun:
         for rk in (1, 2, 3):
             self.s.rk_chan_1(rk, t, ts)
 
+    def _nseg(self):
+        return (self.n + self.k - 1) // self.k
+
+    def _snapshot_step(self, n, n0):
+        """Step n, saving each substep's input in the snapshot slots of the
+        segment that starts at step n0 (what the adjoint substep reads: cu1..3
+        and cth, or the whole state for the LES adjoint, which reruns the
+        substep)."""
+        t = self.t0 + n * self.dt
+        ts = self.ts0 + n + 1
+        light = not self.s.p.use_les
+        base = self._nseg()  # substep snapshots follow the segment checkpoints
+        for rk in (1, 2, 3):
+            self.s.checkpoint(base + 3 * (n - n0) + rk - 1, save=True, light=light)
+            self.s.rk_chan_1(rk, t, ts)
+
     def loss(self, state=None, save_checkpoints=False):
-        """Run N steps from `state` (None: the handle's current state), return L."""
+        """Run N steps from `state` (None: the handle's current state), return L.
+        With save_checkpoints the run also keeps the segment checkpoints and the
+        substep snapshots of the last segment, which the adjoint sweep then
+        needs no recomputation for."""
         if state is not None:
             self.s.set_state(state)
+        last0 = (self._nseg() - 1) * self.k
         total = 0.0
         for n in range(self.n):
             if save_checkpoints and n % self.k == 0:
                 self.s.checkpoint(n // self.k, save=True)
-            self._step(n)
+            if save_checkpoints and n >= last0:
+                self._snapshot_step(n, last0)
+            else:
+                self._step(n)
             total += float(self.w @ self.s.mean_profile("cu1"))
         return total / self.n
 
@@ -80,22 +103,17 @@ class WallShearRun:
         Afterwards the forward state is a mix of recomputed fields; checkpoint
         slot 0 holds the initial state (restore it with checkpoint(0, False))."""
         L = self.loss(state, save_checkpoints=True)
-        nseg = (self.n + self.k - 1) // self.k
+        nseg = self._nseg()
         base = nseg  # substep snapshots follow the segment checkpoints
         s = self.s
         s.adj_zero()
         wN = self.w / self.n
         for seg in reversed(range(nseg)):
             n0, n1 = seg * self.k, min(self.n, (seg + 1) * self.k)
-            s.checkpoint(seg, save=False)
-            # the LES adjoint reruns each substep and reads the whole input
-            light = not s.p.use_les
-            for n in range(n0, n1):
-                t = self.t0 + n * self.dt
-                ts = self.ts0 + n + 1
-                for rk in (1, 2, 3):
-                    s.checkpoint(base + 3 * (n - n0) + rk - 1, save=True, light=light)
-                    s.rk_chan_1(rk, t, ts)
+            if seg < nseg - 1:  # the forward run kept the last segment's snapshots
+                s.checkpoint(seg, save=False)
+                for n in range(n0, n1):
+                    self._snapshot_step(n, n0)
             for n in reversed(range(n0, n1)):
                 t = self.t0 + n * self.dt
                 ts = self.ts0 + n + 1
