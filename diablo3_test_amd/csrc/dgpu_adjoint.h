@@ -264,29 +264,32 @@ template <bool WAVE>
 __global__ void __launch_bounds__(512) k_adj_xpairs(AdjPairArgs a, Geo g, const cpx<double>* __restrict__ tw,
                                                     int j0) {
   typedef cpx<double> C;
+  constexpr int KT = XTile<double>::KT;  // z lines per workgroup
   extern __shared__ __align__(16) unsigned char smem_raw[];
   C* smem = reinterpret_cast<C*>(smem_raw);
   const int G = g.nx >> 3;
   const int L = fft_lds_len(g.nx);
-  const int ngroups = blockDim.x / G;
+  const int ngroups = blockDim.x / G;  // a power of two dividing KT
+  const int lg_ng = 31 - __builtin_clz(ngroups);
   const int grp = threadIdx.x / G, gt = threadIdx.x % G;
   C* fbuf = smem + grp * L;
-  const int j = j0 + blockIdx.y, k = blockIdx.x;  // one z line per workgroup
+  const int j = j0 + blockIdx.y, k0 = blockIdx.x * KT;
   const double inv = 1.0 / g.nx;
-  const long long to = (long long)j * g.tp + k;
-  for (int p0 = 0; p0 < a.npairs; p0 += ngroups) {
-    const int pr = p0 + grp;
-    const bool act = pr < a.npairs;
-    __syncthreads();
-    for (int qq = g.nkx + 1 + gt; qq < g.nx - g.nkx; qq += G) fbuf[pidx(qq)] = C{0.0, 0.0};
-    if (act) {
-      for (int q = gt; q < g.nq; q += G) {
-        const long long o = to + (long long)q * g.nz;
+  for (int pr = 0; pr < a.npairs; ++pr) {
+    const int base = (pr >= 4) ? 6 + 2 * ((pr - 4) >> 1) : 0;
+    const int kind = pr < 4 ? pr : 4 + ((pr - 4) & 1);
+    for (int m0 = 0; m0 < KT; m0 += ngroups) {
+      __syncthreads();  // the previous round's output reads of the buffers are done
+      for (int qq = g.nkx + 1 + gt; qq < g.nx - g.nkx; qq += G) fbuf[pidx(qq)] = C{0.0, 0.0};
+      // consecutive threads take consecutive z lines of one kx, so a
+      // wavefront reads 16-byte values of ngroups lines as one run
+      for (int idx = threadIdx.x; idx < g.nq * ngroups; idx += blockDim.x) {
+        const int q = idx >> lg_ng, mm = idx & (ngroups - 1);
+        const long long o = (long long)j * g.tp + (long long)q * g.nz + k0 + m0 + mm;
         auto X = [&](int s) { return reinterpret_cast<const C*>(a.X[s])[o]; };
         const double kxq = a.kx[q];
         C al, be;
-        const int base = (pr >= 4) ? 6 + 2 * ((pr - 4) >> 1) : 0;
-        switch (pr < 4 ? pr : 4 + ((pr - 4) & 1)) {
+        switch (kind) {
           case 0: al = ik(-kxq, X(0)); be = X(0); break;
           case 1: al = X(1) + ik(-kxq, X(2)); be = X(2); break;
           case 2: al = X(3); be = X(5); break;
@@ -296,22 +299,22 @@ __global__ void __launch_bounds__(512) k_adj_xpairs(AdjPairArgs a, Geo g, const 
         }
         const double w = (q == 0 ? 1.0 : 0.5) * inv;
         const C A = w * al, B = w * be;
+        C* f = smem + mm * L;
         if (q == 0) {
-          fbuf[pidx(0)] = C{A.x, B.x};
+          f[pidx(0)] = C{A.x, B.x};
         } else {
-          fbuf[pidx(q)] = C{A.x - B.y, A.y + B.x};
-          fbuf[pidx(g.nx - q)] = C{A.x + B.y, B.x - A.y};
+          f[pidx(q)] = C{A.x - B.y, A.y + B.x};
+          f[pidx(g.nx - q)] = C{A.x + B.y, B.x - A.y};
         }
       }
-    }
-    __syncthreads();
-    C v[8];
+      __syncthreads();  // every group's spectrum is in place
+      C v[8];
 #pragma unroll
-    for (int r = 0; r < 8; ++r) v[r] = fbuf[pidx(gt + G * r)];
-    fft_regs<double, 1, WAVE>(v, fbuf, g.nx, g.lognx, tw, gt);
-    if (act) {
-      double* oa = a.outA[pr] + (long long)j * g.pp + (long long)k * g.nx;
-      double* ob = a.outB[pr] + (long long)j * g.pp + (long long)k * g.nx;
+      for (int r = 0; r < 8; ++r) v[r] = fbuf[pidx(gt + G * r)];
+      fft_regs<double, 1, WAVE>(v, fbuf, g.nx, g.lognx, tw, gt);
+      const long long ro = (long long)j * g.pp + (long long)(k0 + m0 + grp) * g.nx;
+      double* oa = a.outA[pr] + ro;
+      double* ob = a.outB[pr] + ro;
       for (int i = gt; i < g.nx; i += G) {
         const C z = fbuf[pidx(i)];
         oa[i] = z.x;

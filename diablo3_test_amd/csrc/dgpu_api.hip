@@ -1833,11 +1833,11 @@ void adj_substep(dgpu_handle* h, int rk_step, double time, int time_step) {
     }
     a.kx = h->d_kx.p;
     a.npairs = npairs;
-    const int G = g.nx / 8;
-    int ng = npairs;
-    while (ng > 1 && (ng * G > 512 || (size_t)ng * fft_lds_len(g.nx) * sizeof(double2) > h->lds_limit)) --ng;
-    const size_t lds = (size_t)ng * fft_lds_len(g.nx) * sizeof(double2);
-    dim3 grid(g.nz, Ny), block(ng * G);
+    // KT z lines per workgroup, one transform group per line
+    constexpr int KT = XTile<double>::KT;
+    const int ng = x_ngroups<double>(h, KT, 512);
+    const size_t lds = x_lds<double>(h, ng);
+    dim3 grid(g.nz / KT, Ny), block(ng * (g.nx / 8));
     Timed tm(h, DGPU_K_ADJOINT, (16.0 * g.tp * nslot + 16.0 * g.pp * npairs) * Ny);
     hipLaunchKernelGGL((g.nx <= 512 ? k_adj_xpairs<true> : k_adj_xpairs<false>), grid, block, lds, h->stream, a,
                        g, twx_of<double>(h), 1);
