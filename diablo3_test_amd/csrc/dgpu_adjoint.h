@@ -25,93 +25,214 @@ struct AdjRemDivArgs {
   double2* a2;
   double2* a3;
   const double2* ap;  // adjoint of cp (the map passes it through)
-  double* sb;    // scratch b'
-  double2* sg;   // scratch g'
+  const double* ib;   // 1/b_j of the forward matrix (k_remdiv_pivots)
   const double* kx;
   const double* kz;
-  const double* dy;
-  const double* dyf;
   double inv_h;
   int js, je;
 };
 
-__global__ void __launch_bounds__(256) k_adj_remdiv(AdjRemDivArgs a, Geo g) {
+// Transpose of k_remdiv_seg.  With phibar the adjoint of phi (gathered from
+// the outputs cu -= grad phi and cp += phi/h), ybar = T^-T phibar by the
+// factors of the forward matrix T = L U (pivots b_j, L's multipliers
+// ml_j / b_{j-1}): T^T = U^T L^T, so
+//   z_j = (phibar_j - mu_{j-1} z_{j-1}) / b_j                  (forward, U^T)
+//   y_j = z_j - (ml_{j+1} / b_j) y_{j+1}                        (backward, L^T)
+// and the divergence rows 2..Ny-1 take their transpose: cu1bar -= ikx y,
+// cu3bar -= ikz y, cu2bar(j+1) += y/dyf(j), cu2bar(j) -= y/dyf(j).  Row 1 is
+// no divergence row, so its y is not needed; its z_1 = phibar_1 (b_1 = 1)
+// enters segment 0 as the forward carry.  Layout, segments and scans as in
+// k_remdiv_seg.
+template <int L, int MW, int MAXT>
+__global__ void __launch_bounds__(MAXT) k_adj_remdiv_seg(AdjRemDivArgs a, Geo g, const PoisRow* __restrict__ rows,
+                                                         int S) {
   typedef cpx<double> C;
-  const long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= g.sp) return;
-  const int q = (int)(m / g.nkk), kk = (int)(m % g.nkk);
-  const double kxq = a.kx[q], kzk = a.kz[kk];
-  const double lam = kxq * kxq + kzk * kzk;
-  const bool mean = (q == 0 && kk == 0);
-  const int ny = g.ny;
+  constexpr int SPW = 64 / MW;
+  constexpr int SMAX = MAXT / MW, WMAX = MAXT / 64;
+  __shared__ double tP[WMAX][MW];
+  __shared__ C tA[WMAX][MW];
+  __shared__ double tQ[WMAX][MW];
+  __shared__ C tB[WMAX][MW];
+  __shared__ C tL[SMAX][MW];
+  __shared__ C z1s[MW];
+  const int mm = threadIdx.x % MW, s = threadIdx.x / MW;
+  const int w = threadIdx.x / 64, sl = (threadIdx.x % 64) / MW;
+  const int nw = blockDim.x / 64;
   const long long sp = g.sp;
-  C* a1 = reinterpret_cast<C*>(a.a1);
-  C* a2 = reinterpret_cast<C*>(a.a2);
-  C* a3 = reinterpret_cast<C*>(a.a3);
-  const C* ap = reinterpret_cast<const C*>(a.ap);
-  C* sg = reinterpret_cast<C*>(a.sg);
-  // rows of T (forward rem_div): row 1 (1, mu1), rows 2..ny-1 (ml, md, mu), row ny (1, -1)
-  auto ml_of = [&](int j) -> double { return (j == ny) ? 1.0 : ((j == 1) ? 0.0 : 1.0 / (a.dy[j] * a.dyf[j])); };
-  auto md_of = [&](int j) -> double {
-    if (j == 1) return 1.0;
-    if (j == ny) return -1.0;
-    return -lam - 1.0 / (a.dy[j + 1] * a.dyf[j]) - 1.0 / (a.dy[j] * a.dyf[j]);
+  const long long m = (long long)blockIdx.x * MW + mm;
+  const bool live = m < sp;
+  const long long mc = live ? m : 0;
+  const int q = (int)(mc / g.nkk), kk = (int)(mc % g.nkk);
+  const double kxq = a.kx[q], kzk = a.kz[kk];
+  const double mu1 = (mc == 0) ? 0.0 : -1.0;
+  const int ny = g.ny;
+  const int j0 = 2 + s * L;
+  const C Z = {0.0, 0.0};
+  auto ldc = [](const double2* base, unsigned ob) {
+    const double2 v = *reinterpret_cast<const double2*>(reinterpret_cast<const char*>(base) + ob);
+    return C{v.x, v.y};
   };
-  auto mu_of = [&](int j) -> double {
-    if (j == 1) return mean ? 0.0 : -1.0;
-    if (j == ny) return 0.0;
-    return 1.0 / (a.dy[j + 1] * a.dyf[j]);
+  auto stc = [](double2* base, unsigned ob, C v) {
+    *reinterpret_cast<double2*>(reinterpret_cast<char*>(base) + ob) = make_double2(v.x, v.y);
   };
-  auto phibar = [&](int j) -> C {
-    const long long o = (long long)j * sp + m;
-    C v = ik(kxq, a1[o]) + ik(kzk, a3[o]);
-    if (j >= 2) v = v - (1.0 / a.dy[j]) * a2[o];
-    if (j + 1 <= ny) v = v + (1.0 / a.dy[j + 1]) * a2[o + sp];
-    if (j >= a.js && j <= a.je) v = v + a.inv_h * ap[o];
-    return v;
+  auto ldd = [](const double* base, unsigned ob) {
+    return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(base) + ob);
   };
-  // forward elimination of T^T y = phibar: sub(j) = mu(j-1), sup(j) = ml(j+1)
-  double bprev = md_of(1);
-  C gprev = phibar(1);
-  a.sb[sp + m] = bprev;
-  sg[sp + m] = gprev;
-  for (int j = 2; j <= ny; ++j) {
-    const long long o = (long long)j * sp + m;
-    const double mlt = -mu_of(j - 1) / bprev;
-    const double b = md_of(j) + mlt * ml_of(j);
-    const C gg = phibar(j) + mlt * gprev;
-    a.sb[o] = b;
-    sg[o] = gg;
-    bprev = b;
-    gprev = gg;
+  auto off = [&](int j) { return (unsigned)(((long long)j * sp + m) * 16); };
+  auto shup = [](C v, int d) { return C{__shfl_up(v.x, d), __shfl_up(v.y, d)}; };
+  auto shdn = [](C v, int d) { return C{__shfl_down(v.x, d), __shfl_down(v.y, d)}; };
+  auto inp = [&](int j) { return j >= a.js && j <= a.je; };
+  // adjoints at rows j0 .. j0+L-1 (plus cu2bar at j0+L) and 1/b
+  C d1[L], d3[L], d2[L + 1], pb[L];
+  double ibv[L];
+#pragma unroll
+  for (int t = 0; t < L; ++t) {
+    const int j = j0 + t;
+    const unsigned o = off(j);
+    const bool ok = live && j <= ny;
+    d1[t] = ok ? ldc(a.a1, o) : Z;
+    d3[t] = ok ? ldc(a.a3, o) : Z;
+    d2[t] = ok ? ldc(a.a2, o) : Z;
+    pb[t] = (ok && inp(j)) ? ldc(a.ap, o) : Z;
+    ibv[t] = ok ? ldd(a.ib, o >> 1) : 0.0;
   }
-  // back substitution; rows 2..ny-1 carry the divergence: rhs_j = ikx cu1 +
-  // (cu2(j+1) - cu2(j))/dyf(j) + ikz cu3, so cu1bar -= ikx y, cu3bar -= ikz y,
-  // cu2bar(j+1) += y/dyf(j), cu2bar(j) -= y/dyf(j)
-  C yup = {0.0, 0.0};
-  for (int j = ny; j >= 1; --j) {
-    const long long o = (long long)j * sp + m;
-    C y;
-    if (j == ny)
-      y = (1.0 / a.sb[o]) * sg[o];
-    else
-      y = (1.0 / a.sb[o]) * (sg[o] - ml_of(j + 1) * yup);
-    const bool div_row = (j >= 2 && j <= ny - 1);
-    const bool div_up = (j + 1 >= 2 && j + 1 <= ny - 1);
-    if (div_row) {
-      a1[o] = a1[o] - ik(kxq, y);
-      a3[o] = a3[o] - ik(kzk, y);
-    }
-    // cu2bar at row j+1: the rhs rows j and j+1 touch it
-    if (j + 1 <= ny + 1) {
-      C d = {0.0, 0.0};
-      if (div_row) d = d + (1.0 / a.dyf[j]) * y;
-      if (div_up) d = d - (1.0 / a.dyf[j + 1]) * yup;
-      if (div_row || div_up) a2[o + sp] = a2[o + sp] + d;
-    }
-    yup = y;
+  d2[L] = (live && j0 + L <= ny) ? ldc(a.a2, off(j0 + L)) : Z;
+  // phibar of each row
+#pragma unroll
+  for (int t = 0; t < L; ++t) {
+    const int j = j0 + t;
+    if (j > ny) break;
+    C v = ik(kxq, d1[t]) + ik(kzk, d3[t]) - rows[j].idy * d2[t];
+    if (j + 1 <= ny) v = v + rows[j + 1].idy * d2[t + 1];
+    if (inp(j)) v = v + a.inv_h * pb[t];
+    pb[t] = v;
   }
-  // cu2bar at row 1 is not touched (rhs row 1 is a BC row)
+  if (s == 0) {  // z_1 = phibar_1 (row 1: no cu2bar(1) term)
+    C z1 = Z;
+    if (live) {
+      const unsigned o = off(1);
+      z1 = ik(kxq, ldc(a.a1, o)) + ik(kzk, ldc(a.a3, o));
+      if (ny >= 2) z1 = z1 + rows[2].idy * d2[0];
+      if (inp(1)) z1 = z1 + a.inv_h * ldc(a.ap, o);
+    }
+    z1s[mm] = z1;
+  }
+  // forward sweep (U^T) of the segment from a zero carry: z_t = gl_t + Pl_t * z(j0-1)
+  C gl[L];
+  double Pl[L];
+  {
+    C h = Z;
+    double P = 1.0;
+#pragma unroll
+    for (int t = 0; t < L; ++t) {
+      const int j = j0 + t;
+      if (j <= ny) {
+        const double mup = (j == 2) ? mu1 : rows[j - 1].mu;
+        const double f = -mup * ibv[t];
+        h = ibv[t] * pb[t] + f * h;
+        P = f * P;
+      }
+      gl[t] = h;
+      Pl[t] = P;
+    }
+  }
+  {
+    double P = Pl[L - 1];
+    C A = gl[L - 1];
+#pragma unroll
+    for (int d = 1; d < SPW; d <<= 1) {
+      const double Pp = __shfl_up(P, d * MW);
+      const C Ap = shup(A, d * MW);
+      if (sl >= d) {
+        A = A + P * Ap;
+        P = P * Pp;
+      }
+    }
+    if (sl == SPW - 1) {
+      tP[w][mm] = P;
+      tA[w][mm] = A;
+    }
+    double Pe = __shfl_up(P, MW);
+    C Ae = shup(A, MW);
+    if (sl == 0) {
+      Pe = 1.0;
+      Ae = Z;
+    }
+    __syncthreads();
+    C cin = z1s[mm];  // z at row 1 enters wavefront 0
+    for (int w2 = 0; w2 < w; ++w2) cin = tA[w2][mm] + tP[w2][mm] * cin;
+    const C zin = Ae + Pe * cin;
+#pragma unroll
+    for (int t = 0; t < L; ++t) gl[t] = gl[t] + Pl[t] * zin;
+  }
+  // backward sweep (L^T) of the segment from a zero carry: y_t = gl_t + Pl_t * y(j0+L)
+  {
+    C x = Z;
+    double Q = 1.0;
+#pragma unroll
+    for (int t = L - 1; t >= 0; --t) {
+      const int j = j0 + t;
+      if (j <= ny) {
+        const double f = (j < ny) ? -rows[j + 1].ml * ibv[t] : 0.0;
+        x = gl[t] + f * x;
+        Q = f * Q;
+      }
+      gl[t] = x;
+      Pl[t] = Q;
+    }
+  }
+  {
+    double Q = Pl[0];
+    C B = gl[0];
+#pragma unroll
+    for (int d = 1; d < SPW; d <<= 1) {
+      const double Qn = __shfl_down(Q, d * MW);
+      const C Bn = shdn(B, d * MW);
+      if (sl + d < SPW) {
+        B = B + Q * Bn;
+        Q = Q * Qn;
+      }
+    }
+    if (sl == 0) {
+      tQ[w][mm] = Q;
+      tB[w][mm] = B;
+    }
+    double Qe = __shfl_down(Q, MW);
+    C Be = shdn(B, MW);
+    if (sl == SPW - 1) {
+      Qe = 1.0;
+      Be = Z;
+    }
+    __syncthreads();
+    C ctop = Z;
+    for (int w2 = nw - 1; w2 > w; --w2) ctop = tB[w2][mm] + tQ[w2][mm] * ctop;
+    const C ytop = Be + Qe * ctop;
+#pragma unroll
+    for (int t = 0; t < L; ++t) gl[t] = gl[t] + Pl[t] * ytop;  // gl now holds y
+  }
+  // y at the row below the segment (row 1 for segment 0, not a divergence row)
+  tL[s][mm] = gl[L - 1];
+  __syncthreads();
+  const C ym = (s > 0) ? tL[s - 1][mm] : Z;
+  if (!live) return;
+#pragma unroll
+  for (int t = 0; t < L; ++t) {
+    const int j = j0 + t;
+    if (j > ny) break;
+    const unsigned o = off(j);
+    const C y = gl[t];
+    const bool dr = j <= ny - 1;                 // j >= 2 holds for every segment row
+    const bool dm = j - 1 >= 2 && j - 1 <= ny - 1;
+    const C yprev = (t == 0) ? ym : gl[t - 1];
+    C d = d2[t];
+    if (dm) d = d + rows[j - 1].idyf * yprev;
+    if (dr) d = d - rows[j].idyf * y;
+    stc(a.a2, o, d);
+    if (dr) {
+      stc(a.a1, o, d1[t] - ik(kxq, y));
+      stc(a.a3, o, d3[t] - ik(kzk, y));
+    }
+  }
 }
 
 // ------------------------------------------------------ CN adjoint ----

@@ -1684,17 +1684,27 @@ void adj_substep(dgpu_handle* h, int rk_step, double time, int time_step) {
     a.a2 = h->a_cu[1].p;
     a.a3 = h->a_cu[2].p;
     a.ap = h->a_cp.p;
-    a.sb = h->sb.p;
-    a.sg = h->sg.p;
+    a.ib = h->rd_ib.p;
     a.kx = h->d_kx.p;
     a.kz = h->d_kz.p;
-    a.dy = h->d_dy.p;
-    a.dyf = h->d_dyf.p;
     a.inv_h = 1.0 / temp4;
     a.js = h->jstart;
     a.je = h->jend;
     Timed tm(h, DGPU_K_ADJOINT, 128.0 * g.sp * Ny);
-    hipLaunchKernelGGL(k_adj_remdiv, grid1(g.sp), dim3(256), 0, h->stream, a, g);
+    // segments and workgroup shape as in launch_remdiv
+    const int nr = g.ny - 1;
+    auto launch = [&](auto kern, int L, int MW) {
+      const int spw = 64 / MW;
+      const int S = ((nr + L - 1) / L + spw - 1) / spw * spw;
+      hipLaunchKernelGGL(kern, dim3((unsigned)((g.sp + MW - 1) / MW)), dim3(MW * S), 0, h->stream, a, g,
+                         h->d_prow.p, S);
+    };
+    if (nr <= 4 * 64)
+      launch(k_adj_remdiv_seg<4, 16, 1024>, 4, 16);
+    else if (nr <= 8 * 64)
+      launch(k_adj_remdiv_seg<8, 8, 512>, 8, 8);
+    else
+      launch(k_adj_remdiv_seg<8, 4, 512>, 8, 4);  // launch_remdiv has rejected ny beyond this
     HIPCHECK(hipGetLastError());
   }
   // 2. the forward transform of the solved planes: xbar = F^T cubar there, and

@@ -85,6 +85,27 @@ def test_adjoint_substep_matches_torch(name, rk):
         assert e < 1e-11, (k, e)
 
 
+@pytest.mark.parametrize("ny", [289, 545])
+def test_adjoint_substep_tall_channel(ny):
+    """Ny past 257 and past 513: the transposed projection's 8-row segment
+    layouts against torch autograd."""
+    p = workloads.channel_params(16, ny, 16, 200.0, ri=0.1, dt=2e-4)
+    gy = workloads.channel_grid(p["ny"])
+    s = seeded(p, gy)
+    lam = random_adjoint(p, 77)
+    print(f"adjoint substep 16x{ny}x16 rk=2", flush=True)
+    g = gpu_solver(p, gy)
+    g.set_state(s)
+    g.adj_set_state(lam)
+    g.adj_rk_chan_1(2, 0.3, 5)
+    got = g.adj_get_state(STATE)
+    want = torch_vjp(p, gy, s, lam, 2, 0.3)
+    errs = {k: rel_l2(active(got[k], p), active(want[k], p)) for k in STATE}
+    report(f"adj 16x{ny}x16", errs)
+    for k, e in errs.items():
+        assert e < 1e-11, (k, e)
+
+
 def test_wall_shear_gradient_matches_torch():
     """L over 3 steps with checkpoints every 2: HIP checkpointed adjoint vs torch autograd."""
     p = workloads.channel_params(16, 17, 16, 200.0, ri=0.1, dt=2e-3)
