@@ -62,7 +62,7 @@ __global__ void __launch_bounds__(MAXT) k_adj_remdiv_seg(AdjRemDivArgs a, Geo g,
   const long long m = (long long)blockIdx.x * MW + mm;
   const bool live = m < sp;
   const long long mc = live ? m : 0;
-  const int q = (int)(mc / g.nkk), kk = (int)(mc % g.nkk);
+  const int q = (int)((unsigned)mc / (unsigned)g.nkk), kk = (int)((unsigned)mc % (unsigned)g.nkk);
   const double kxq = a.kx[q], kzk = a.kz[kk];
   const double mu1 = (mc == 0) ? 0.0 : -1.0;
   const int ny = g.ny;
@@ -247,7 +247,7 @@ __global__ void __launch_bounds__(256) k_adj_cnpost(const double* __restrict__ v
   const long long pp = g.pp;
   const int n = g.ny2;
   if (idx >= pp * n) return;
-  const int j = (int)(idx / pp);
+  const int j = plane_of(idx, pp);
   const SegRow rj = rows[j];
   const double vj = vbar[idx];
   double ub = 0.0;
@@ -292,7 +292,7 @@ __global__ void __launch_bounds__(256) k_adj_rhs_a(AdjRhsArgs a, Geo g) {
   typedef cpx<double> C;
   const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= g.sp * g.ny2) return;
-  const int j = (int)(idx / g.sp);
+  const int j = plane_of(idx, g.sp);
   const int nc = 3 + a.nth;
   for (int c = 0; c < nc; ++c) {
     C* cf = reinterpret_cast<C*>(a.acf[c]);
@@ -314,9 +314,9 @@ __global__ void __launch_bounds__(256) k_adj_rhs_b(AdjRhsArgs a, Geo g) {
   typedef cpx<double> C;
   const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= g.sp * g.ny2) return;
-  const int j = (int)(idx / g.sp);
-  const long long mm = idx % g.sp;
-  const int q = (int)(mm / g.nkk), kk = (int)(mm % g.nkk);
+  const int j = plane_of(idx, g.sp);
+  const long long mm = (long long)((unsigned)idx % (unsigned)g.sp);
+  const int q = (int)((unsigned)mm / (unsigned)g.nkk), kk = (int)((unsigned)mm % (unsigned)g.nkk);
   const double kxq = a.kx[q], kzk = a.kz[kk];
   const double lap = kxq * kxq + kzk * kzk;
   const double visc = (a.beta == 1.0) ? lap : pow(lap, a.beta);
@@ -469,7 +469,7 @@ __global__ void __launch_bounds__(256) k_adj_prod(AdjProdArgs a, Geo g) {
   const long long pp = g.pp;
   const int n = g.ny2;
   if (idx >= pp * n) return;
-  const int j = (int)(idx / pp);
+  const int j = plane_of(idx, pp);
   const long long o = idx, up = idx + pp, dn = idx - pp;
   const bool hu = j + 1 < n, hd = j >= 1;
   auto V = [&](const double* f, long long i, bool ok) { return ok ? f[i] : 0.0; };

@@ -2197,6 +2197,9 @@ void create_impl(dgpu_handle* h, const double* gy) {
   }
   HIPCHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
 
+  // kernels index fields with 32-bit arithmetic (plane_of)
+  if ((double)g.ny2 * (double)g.pp >= 4294967296.0)
+    throw ApiError{DGPU_EUNSUP, "grid too large: a field must have fewer than 2^32 points"};
   const size_t ns = (size_t)g.ny2 * g.sp, np = (size_t)g.ny2 * g.pp, nt = (size_t)g.ny2 * g.tp;
   for (int f = 0; f < 3; ++f) {
     h->cu[f].alloc(ns);

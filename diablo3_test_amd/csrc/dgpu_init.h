@@ -56,7 +56,7 @@ __global__ void __launch_bounds__(256) k_poisson_solve(PoissonArgs a, Geo g) {
   typedef cpx<double> C;
   const long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= g.sp) return;
-  const int q = (int)(m / g.nkk), kk = (int)(m % g.nkk);
+  const int q = (int)((unsigned)m / (unsigned)g.nkk), kk = (int)((unsigned)m % (unsigned)g.nkk);
   const double kxq = a.kx[q], kzk = a.kz[kk];
   const double lam = kxq * kxq + kzk * kzk;
   const bool mean = (q == 0 && kk == 0);

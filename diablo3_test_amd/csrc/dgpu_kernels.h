@@ -34,6 +34,14 @@ struct Geo {
   long long tp;       // x-spectra plane: nq*Nz
 };
 
+// Plane of flat index idx in arrays of `plane` entries per plane.  Every
+// field has fewer than 2^32 entries (checked when a handle is created), so a
+// 32-bit division serves: a 64-bit one by a run-time divisor costs a
+// per-thread subroutine of dozens of instructions.
+__device__ __forceinline__ int plane_of(long long idx, long long plane) {
+  return (int)((unsigned)idx / (unsigned)plane);
+}
+
 // natural z index of packed mode kk (inverse of the pack at fft.f90:308-311)
 __device__ __forceinline__ int kz_nat(const Geo& g, int kk) {
   return kk <= g.nkz ? kk : g.nz - 2 * g.nkz - 1 + kk;
@@ -868,7 +876,7 @@ __global__ void __launch_bounds__(256) k_remdiv_pivots(double* __restrict__ ib, 
                                                        const double* __restrict__ kz, Geo g) {
   const long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= g.sp) return;
-  const int q = (int)(m / g.nkk), kk = (int)(m % g.nkk);
+  const int q = (int)((unsigned)m / (unsigned)g.nkk), kk = (int)((unsigned)m % (unsigned)g.nkk);
   const double lam = kx[q] * kx[q] + kz[kk] * kz[kk];
   const long long sp = g.sp;
   double bprev = 1.0;
@@ -916,7 +924,7 @@ __global__ void __launch_bounds__(MAXT) k_remdiv_seg(RemDivArgs a, Geo g, const 
   const long long m = (long long)blockIdx.x * MW + mm;
   const bool live = m < sp;
   const long long mc = live ? m : 0;
-  const int q = (int)(mc / g.nkk), kk = (int)(mc % g.nkk);
+  const int q = (int)((unsigned)mc / (unsigned)g.nkk), kk = (int)((unsigned)mc % (unsigned)g.nkk);
   const double kxq = a.kx[q], kzk = a.kz[kk];
   const int ny = g.ny;
   const int j0 = 2 + s * L;

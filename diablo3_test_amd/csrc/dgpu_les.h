@@ -196,9 +196,9 @@ template <typename R>
 __global__ void __launch_bounds__(256) k_les_vel(LesVelArgs<R> a, Geo g) {
   const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long pp = g.pp;
-  const int j = 1 + (int)(idx / pp);
+  const int j = 1 + plane_of(idx, pp);
   if (j > g.ny + 1) return;
-  const long long o = (long long)j * pp + idx % pp;
+  const long long o = (long long)j * pp + (long long)((unsigned)idx % (unsigned)pp);
   const lespt::VelPlane<R> P = les_vel_plane(a, g, j);
   lespt::VelIn<R> x;
   les_vel_load(a, g, j, o, x);
@@ -284,9 +284,9 @@ template <typename R>
 __global__ void __launch_bounds__(256) k_les_th(LesThArgs<R> a, Geo g) {
   const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long pp = g.pp;
-  const int j = 1 + (int)(idx / pp);
+  const int j = 1 + plane_of(idx, pp);
   if (j > g.ny + 1) return;
-  const long long o = (long long)j * pp + idx % pp;
+  const long long o = (long long)j * pp + (long long)((unsigned)idx % (unsigned)pp);
   if (a.type != 5) {
     const double kt = a.nu_t[o];
     a.kappa_t[o] = kt;
@@ -334,10 +334,10 @@ __global__ void __launch_bounds__(256) k_les_div(LesDivArgs a, Geo g) {
   typedef cpx<double> C;
   const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long sp = g.sp;
-  const int j = (int)(idx / sp);
+  const int j = plane_of(idx, sp);
   if (j >= g.ny2) return;
   const long long m = idx % sp;
-  const int q = (int)(m / g.nkk), kk = (int)(m % g.nkk);
+  const int q = (int)((unsigned)m / (unsigned)g.nkk), kk = (int)((unsigned)m % (unsigned)g.nkk);
   const double kxq = a.kx[q], kzk = a.kz[kk];
   const long long o = idx;
   auto L = [&](const double2* p, long long i) { return reinterpret_cast<const C*>(p)[i]; };

@@ -37,10 +37,10 @@ __global__ void __launch_bounds__(256) k_adj_les_div(AdjLesDivArgs a, Geo g) {
   typedef cpx<double> C;
   const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long sp = g.sp;
-  const int j = (int)(idx / sp);
+  const int j = plane_of(idx, sp);
   if (j >= g.ny2) return;
   const long long m = idx % sp;
-  const int q = (int)(m / g.nkk), kk = (int)(m % g.nkk);
+  const int q = (int)((unsigned)m / (unsigned)g.nkk), kk = (int)((unsigned)m % (unsigned)g.nkk);
   const double kxq = a.kx[q], kzk = a.kz[kk];
   const long long o = idx;
   auto L = [&](const double2* p, long long i) { return reinterpret_cast<const C*>(p)[i]; };
@@ -93,7 +93,7 @@ struct AdjLesVelArgs {
 __global__ void __launch_bounds__(256) k_adj_les_vel(AdjLesVelArgs a, Geo g) {
   const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long pp = g.pp;
-  const int j = (int)(idx / pp);
+  const int j = plane_of(idx, pp);
   if (j > g.ny + 1) return;
   const long long o = idx;
   double gb[9];
@@ -152,7 +152,7 @@ struct AdjLesThArgs {
 __global__ void __launch_bounds__(256) k_adj_les_th(AdjLesThArgs a, Geo g) {
   const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long pp = g.pp;
-  const int j = (int)(idx / pp);
+  const int j = plane_of(idx, pp);
   if (j > g.ny + 1) return;
   const long long o = idx;
   if (a.fw.type != 5) {
@@ -212,10 +212,10 @@ __global__ void __launch_bounds__(256) k_adj_deriv(const double2* __restrict__ X
   typedef cpx<double> C;
   const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long sp = g.sp;
-  const int j = (int)(idx / sp);
+  const int j = plane_of(idx, sp);
   if (j >= g.ny2) return;
   const long long m = idx % sp;
-  const int q = (int)(m / g.nkk), kk = (int)(m % g.nkk);
+  const int q = (int)((unsigned)m / (unsigned)g.nkk), kk = (int)((unsigned)m % (unsigned)g.nkk);
   const C* x = reinterpret_cast<const C*>(X);
   C v = {0, 0};
   switch (op) {
