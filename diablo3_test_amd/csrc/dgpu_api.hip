@@ -63,6 +63,8 @@ int seg_target = 24;
 // knobs: DGPU_XPASS_THREADS, DGPU_XPROD_THREADS).
 int xpass_threads = 512;
 int xprod_threads = 512;
+// columns per lane of the LES CN solve (DGPU_LES_COLS)
+int les_cols = 2;
 // Columns per lane of the FP32-storage CN solve (1 or 2; knob DGPU_THOMAS_COLS).
 // Measured at 512x257x512: 2 columns take the mixed form's 12 solves per step
 // from 5.19 to 4.50 ms; the FP64 solves are 2% slower with 2 (5.21 -> 5.33 ms).
@@ -648,11 +650,19 @@ void launch_thomas_les(dgpu_handle* h, int field, int rs, const R* r, R* u, cons
   const Geo& g = h->g;
   const int n = g.ny2;
   const TriRow* rows = h->trirows.p + ((size_t)field * 3 + rs) * n;
-  dim3 grid((unsigned)((g.pp + 255) / 256)), block(256);
-  // reads r, u, nu_t; writes b' and g' and reads them back; writes u
+  // reads r, u, nu_t; writes b' and g' and reads them back; writes u.  Two
+  // columns per lane (one 2-wide access per row and array) when the plane
+  // allows it.
   Timed tm(h, DGPU_K_THOMAS, (3.0 * sizeof(R) + 8.0 + 32.0) * g.pp * n);
-  hipLaunchKernelGGL(k_thomas_les<R>, grid, block, 0, h->stream, r, u, nt, h->sbl.p, h->sgl.p, rows, n, g.pp,
-                     ca, cb, transient, incr ? 1 : 0);
+  if (les_cols == 2 && g.pp % 2 == 0) {
+    dim3 grid((unsigned)((g.pp / 2 + 255) / 256)), block(256);
+    hipLaunchKernelGGL((k_thomas_les<R, 2>), grid, block, 0, h->stream, r, u, nt, h->sbl.p, h->sgl.p, rows, n,
+                       g.pp, ca, cb, transient, incr ? 1 : 0);
+  } else {
+    dim3 grid((unsigned)((g.pp + 255) / 256)), block(256);
+    hipLaunchKernelGGL((k_thomas_les<R, 1>), grid, block, 0, h->stream, r, u, nt, h->sbl.p, h->sgl.p, rows, n,
+                       g.pp, ca, cb, transient, incr ? 1 : 0);
+  }
   HIPCHECK(hipGetLastError());
 }
 
@@ -2261,6 +2271,7 @@ void create_impl(dgpu_handle* h, const double* gy) {
   if (const char* e = std::getenv("DGPU_XPROD_THREADS")) xprod_threads = std::atoi(e);
   if (const char* e = std::getenv("DGPU_BATCH_MB")) batch_mb = std::atoi(e);
   if (const char* e = std::getenv("DGPU_THOMAS_COLS")) thomas_cols = std::atoi(e);
+  if (const char* e = std::getenv("DGPU_LES_COLS")) les_cols = std::atoi(e);
   const int n = g.ny2;
   h->seg_L = seg_len_for(n);
   if (h->seg_L < 0) throw ApiError{DGPU_EUNSUP, "ny too large for the segmented solver"};

@@ -457,25 +457,32 @@ struct TriRow {
 };
 
 // T is the storage type of r and u; the sweeps run in FP64 with b' and g' in
-// FP64 scratch.  incr has the meaning of k_thomas_seg's flag.
-template <typename T>
+// FP64 scratch.  incr has the meaning of k_thomas_seg's flag.  Each lane
+// solves C adjacent columns (one C-wide access per row and array), so a
+// wavefront's row accesses are 64 C values long.
+template <typename T, int C>
 __global__ void __launch_bounds__(256) k_thomas_les(const T* __restrict__ r, T* __restrict__ u,
                                                     const double* __restrict__ nt, double* __restrict__ sb,
                                                     double* __restrict__ sg, const TriRow* __restrict__ rows,
                                                     int n, long long pp, int ca, int cb, double kscale,
                                                     int incr) {
-  const long long col = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  typedef T VT __attribute__((ext_vector_type(C)));
+  typedef double VD __attribute__((ext_vector_type(C)));
+  const long long col = ((long long)blockIdx.x * blockDim.x + threadIdx.x) * C;
   if (col >= pp) return;
-  auto U = [&](int j) { return (double)u[(long long)j * pp + col]; };
-  auto N = [&](int j) { return nt[(long long)j * pp + col]; };
-  // row coefficients of this lane
-  auto row = [&](int j, double& a, double& b, double& c, double& vec, double um, double uc, double up) {
+  auto at = [&](int j) { return (long long)j * pp + col; };
+  auto U = [&](int j) { return __builtin_convertvector(*reinterpret_cast<const VT*>(u + at(j)), VD); };
+  auto N = [&](int j) { return *reinterpret_cast<const VD*>(nt + at(j)); };
+  auto R = [&](int j) { return __builtin_convertvector(*reinterpret_cast<const VT*>(r + at(j)), VD); };
+  const VD zero = (VD)0.0;
+  // row coefficients of this lane's columns
+  auto row = [&](int j, VD& a, VD& b, VD& c, VD& vec, VD um, VD uc, VD up) {
     const TriRow& rw = rows[j];
-    a = rw.a;
-    b = rw.b;
-    c = rw.c;
+    a = (VD)rw.a;
+    b = (VD)rw.b;
+    c = (VD)rw.c;
     if (rw.kind == 1) {
-      double nl, nu;  // eddy viscosity on the lower / upper face of the row
+      VD nl, nu;  // eddy viscosity on the lower / upper face of the row
       if (rw.vtype) {
         nl = 0.5 * (N(j) + N(j - 1));
         nu = 0.5 * (N(j) + N(j + 1));
@@ -486,47 +493,46 @@ __global__ void __launch_bounds__(256) k_thomas_les(const T* __restrict__ r, T* 
       a -= rw.lm * nl;
       b += rw.lp * nu + rw.lm * nl;
       c -= rw.lp * nu;
-      vec = (double)r[(long long)j * pp + col] + (incr ? uc : 0.0) + rw.wp * (up - uc) - rw.wm * (uc - um) +
+      vec = R(j) + (incr ? uc : zero) + rw.wp * (up - uc) - rw.wm * (uc - um) +
             (rw.lp * nu * (up - uc) - rw.lm * nl * (uc - um));
     } else {
-      vec = (rw.kind == 2) ? rw.vb * kscale : rw.vb;
+      vec = (VD)((rw.kind == 2) ? rw.vb * kscale : rw.vb);
     }
   };
-  double um = 0.0, uc = U(0), up = (n > 1) ? U(1) : 0.0;
-  double a, b, c, vec;
+  VD um = zero, uc = U(0), up = (n > 1) ? U(1) : zero;
+  VD a, b, c, vec;
   row(0, a, b, c, vec, um, uc, up);
-  double bp = b, gp = vec, cprev = c;
-  sb[col] = bp;
-  sg[col] = gp;
+  VD bp = b, gp = vec, cprev = c;
+  *reinterpret_cast<VD*>(sb + at(0)) = bp;
+  *reinterpret_cast<VD*>(sg + at(0)) = gp;
   for (int j = 1; j < n; ++j) {
     um = uc;
     uc = up;
-    up = (j + 1 < n) ? U(j + 1) : 0.0;
+    up = (j + 1 < n) ? U(j + 1) : zero;
     row(j, a, b, c, vec, um, uc, up);
-    const double mlt = -a / bp;
+    const VD mlt = -a / bp;
     bp = b + mlt * cprev;
     gp = vec + mlt * gp;
     cprev = c;
-    const long long o = (long long)j * pp + col;
-    sb[o] = bp;
-    sg[o] = gp;
+    *reinterpret_cast<VD*>(sb + at(j)) = bp;
+    *reinterpret_cast<VD*>(sg + at(j)) = gp;
   }
   // backward: x(n-1) = g'/b', x(j) = (g'(j) - c(j) x(j+1)) / b'(j)
-  double x = gp / bp;
-  if (n - 1 >= ca && n - 1 <= cb) {
-    const long long o = (long long)(n - 1) * pp + col;
-    u[o] = (T)(x - (incr ? (double)u[o] : 0.0));
-  }
+  VD x = gp / bp;
+  auto put = [&](int j, VD v) {
+    VT* p = reinterpret_cast<VT*>(u + at(j));
+    *p = __builtin_convertvector(v - (incr ? __builtin_convertvector(*p, VD) : zero), VT);
+  };
+  if (n - 1 >= ca && n - 1 <= cb) put(n - 1, x);
   for (int j = n - 2; j >= ca; --j) {
     const TriRow& rw = rows[j];
-    double c = rw.c;
+    VD cc = (VD)rw.c;
     if (rw.kind == 1) {
-      const double nu = rw.vtype ? 0.5 * (N(j) + N(j + 1)) : N(j + 1);
-      c -= rw.lp * nu;
+      const VD nu = rw.vtype ? 0.5 * (N(j) + N(j + 1)) : N(j + 1);
+      cc -= rw.lp * nu;
     }
-    const long long o = (long long)j * pp + col;
-    x = (sg[o] - c * x) / sb[o];
-    if (j <= cb) u[o] = (T)(x - (incr ? (double)u[o] : 0.0));
+    x = (*reinterpret_cast<const VD*>(sg + at(j)) - cc * x) / *reinterpret_cast<const VD*>(sb + at(j));
+    if (j <= cb) put(j, x);
   }
 }
 
