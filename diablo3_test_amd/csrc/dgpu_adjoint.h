@@ -236,27 +236,8 @@ __global__ void __launch_bounds__(MAXT) k_adj_remdiv_seg(AdjRemDivArgs a, Geo g,
 }
 
 // ------------------------------------------------------ CN adjoint ----
-// After the transposed solve vbar = A^-T xbar (k_thomas_seg with the
-// transposed factor table), the explicit half vec = r + E u of the assembled
-// rows gives rbar = vbar on those rows and ubar = E^T vbar, with
-// E u = wp (u(j+1) - u(j)) - wm (u(j) - u(j-1)).
-__global__ void __launch_bounds__(256) k_adj_cnpost(const double* __restrict__ vbar, double* __restrict__ rbar,
-                                                    double* __restrict__ ubar, const SegRow* __restrict__ rows,
-                                                    Geo g) {
-  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long long pp = g.pp;
-  const int n = g.ny2;
-  if (idx >= pp * n) return;
-  const int j = plane_of(idx, pp);
-  const SegRow rj = rows[j];
-  const double vj = vbar[idx];
-  double ub = 0.0;
-  if (rj.kind == 1) ub -= (rj.wp + rj.wm) * vj;
-  if (j >= 1 && rows[j - 1].kind == 1) ub += rows[j - 1].wp * vbar[idx - pp];
-  if (j + 1 < n && rows[j + 1].kind == 1) ub += rows[j + 1].wm * vbar[idx + pp];
-  rbar[idx] = (rj.kind == 1) ? vj : 0.0;
-  ubar[idx] = ub;
-}
+// The transposed solve vbar = A^-T xbar and its epilogue (rbar = vbar on the
+// assembled rows, ubar = E^T vbar) are k_thomas_seg with ADJ set.
 
 // ------------------------------------------------- spectral RHS adjoint ----
 // Part a (per spectral entry): fbar = cfbar_new + temp5 crbar on the rows

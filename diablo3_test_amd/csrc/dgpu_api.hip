@@ -738,6 +738,39 @@ void launch_thomas(dgpu_handle* h, int field, int rs, const R* r, R* u, int ca, 
   HIPCHECK(hipGetLastError());
 }
 
+// The adjoint's transposed CN solve vbar = A^-T xbar (factor table rowsT) with
+// its epilogue: rbar = vbar on the assembled rows, ubar = E^T vbar.
+void launch_thomas_adj(dgpu_handle* h, int field, int rs, const double* xb, double* rbar, double* ubar) {
+  const Geo& g = h->g;
+  const int n = g.ny2;
+  const size_t off = ((size_t)field * 3 + rs) * n;
+  const SegRow* rows = h->rowsT.p + off;
+  const SegRow* prow = h->rows.p + off;
+  const int C = (h->seg_L <= 32 && thomas_cols == 2 && g.pp % 128 == 0) ? 2 : 1;
+  dim3 grid((unsigned)(g.pp / (64 * C))), block(64, h->seg_S);
+  Timed tm(h, DGPU_K_THOMAS, 24.0 * g.pp * n);
+#define TLAUNCH(L, CC)                                                                                     \
+  hipLaunchKernelGGL((k_thomas_seg<L, CC, double, false, true>), grid, block, 0, h->stream, xb,             \
+                     (double*)nullptr, rows, n, h->seg_S, g.pp, 0, n - 1, 0.0, prow, rbar, ubar)
+  switch (h->seg_L * 4 + C) {
+    case 8 * 4 + 1: TLAUNCH(8, 1); break;
+    case 16 * 4 + 1: TLAUNCH(16, 1); break;
+    case 24 * 4 + 1: TLAUNCH(24, 1); break;
+    case 32 * 4 + 1: TLAUNCH(32, 1); break;
+    case 40 * 4 + 1: TLAUNCH(40, 1); break;
+    case 48 * 4 + 1: TLAUNCH(48, 1); break;
+    case 56 * 4 + 1: TLAUNCH(56, 1); break;
+    case 8 * 4 + 2: TLAUNCH(8, 2); break;
+    case 16 * 4 + 2: TLAUNCH(16, 2); break;
+    case 24 * 4 + 2: TLAUNCH(24, 2); break;
+    case 32 * 4 + 2: TLAUNCH(32, 2); break;
+    default:
+      throw ApiError{DGPU_EUNSUP, "unsupported Thomas segment length"};
+  }
+#undef TLAUNCH
+  HIPCHECK(hipGetLastError());
+}
+
 void copy_spec(dgpu_handle* h, double2* dst, const double2* src) {
   const long long n = (long long)h->g.ny2 * h->g.sp;
   Timed tm(h, DGPU_K_COPY, 32.0 * n);
@@ -1738,8 +1771,7 @@ void adj_substep(dgpu_handle* h, int rk_step, double time, int time_step) {
       HIPCHECK(hipMemsetAsync(adj_spec(h, f), 0, sizeof(double2) * ns, h->stream));
     }
   }
-  // 3. CN solves: vbar = A^-T xbar (into the physical state buffer, which
-  //    step 7 refills), then rbar and ubar = E^T vbar
+  // 3. CN solves: vbar = A^-T xbar, and from it rbar and ubar = E^T vbar
   if (les) {
     // with nu_t, kappa_t in the rows (k_thomas_les): lambda, rbar, ubar_old
     // and the eddy viscosity/diffusivity adjoints
@@ -1758,14 +1790,8 @@ void adj_substep(dgpu_handle* h, int rk_step, double time, int time_step) {
     }
   }
   for (int f = 0; f < nf && !les; ++f) {
-    double* xb = (f < 3) ? h->r[f].p : h->rth[f - 3].p;
-    double* vb = (f < 3) ? h->u[f].p : h->th[f - 3].p;
-    launch_thomas<double>(h, f, rs, xb, vb, 0, n2 - 1, 0.0, false, h->rowsT.p);
-    const SegRow* rows = h->rows.p + ((size_t)f * 3 + rs) * n2;
-    Timed tm(h, DGPU_K_ADJOINT, 32.0 * npx);
-    hipLaunchKernelGGL(k_adj_cnpost, grid1((long long)npx), dim3(256), 0, h->stream, vb, h->a_rb[f].p,
-                       h->a_ub[f].p, rows, g);
-    HIPCHECK(hipGetLastError());
+    const double* xb = (f < 3) ? h->r[f].p : h->rth[f - 3].p;
+    launch_thomas_adj(h, f, rs, xb, h->a_rb[f].p, h->a_ub[f].p);
   }
   // 4. crbar = P^T rbar on planes 1..Ny (the inverse transform of cr)
   for (int f = 0; f < nf; ++f) {

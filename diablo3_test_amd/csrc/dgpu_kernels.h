@@ -697,11 +697,20 @@ struct SegRow {
 // incr), and the solve is for d = x - u with the same factors, A d = vec - A u:
 // assembled rows (A = I - E, vec = r + u + E u) give r + 2 E u, fixed rows
 // vec - (a u(j-1) + b u(j) + c u(j+1)).  u is overwritten with d.
-template <int L, int C, typename T, bool INCR>
+//
+// ADJ (the adjoint's transposed solve vbar = A^-T xbar, rows of kind 3): the
+// solution is not stored; instead rbar = vbar on the forward matrix's
+// assembled rows (0 elsewhere) and ubar = E^T vbar, with E u = wp (u(j+1) -
+// u(j)) - wm (u(j) - u(j-1)) from the forward row table prow, go to rbar and
+// ubar for every row.
+template <int L, int C, typename T, bool INCR, bool ADJ = false>
 __global__ void __launch_bounds__(1024) k_thomas_seg(const T* __restrict__ r,
                                                      T* __restrict__ u,
                                                      const SegRow* __restrict__ rows, int n, int S,
-                                                     long long pp, int ca, int cb, double kscale) {
+                                                     long long pp, int ca, int cb, double kscale,
+                                                     const SegRow* __restrict__ prow = nullptr,
+                                                     double* __restrict__ rbar = nullptr,
+                                                     double* __restrict__ ubar = nullptr) {
   typedef T VN __attribute__((ext_vector_type(C)));  // C columns, one streaming access
   __shared__ double carry_f[16][64 * C];
   __shared__ double carry_b[16][64 * C];
@@ -712,10 +721,13 @@ __global__ void __launch_bounds__(1024) k_thomas_seg(const T* __restrict__ r,
   const int s0 = seg * L;
   double v[L][C];
   double w[L + 2][C];
+  // u (the state, for the explicit term and the increment form) is not read
+  // by the adjoint's transposed solve, whose rows are all plain (kind 3) and
+  // which passes no u
 #pragma unroll
   for (int t = 0; t < L + 2; ++t) {
     const int jj = s0 - 1 + t;
-    if (jj >= 0 && jj < n) {
+    if (!ADJ && jj >= 0 && jj < n) {
       const VN x = __builtin_nontemporal_load(reinterpret_cast<const VN*>(u + (long long)jj * pp + col));
 #pragma unroll
       for (int c = 0; c < C; ++c) w[t][c] = (double)x[c];
@@ -823,15 +835,67 @@ __global__ void __launch_bounds__(1024) k_thomas_seg(const T* __restrict__ r,
 #pragma unroll
     for (int c = 0; c < C; ++c) xin[c] = carry_b[sp][lane * C + c] + qf * xin[c];
   }
+  if constexpr (ADJ) {
+    // the solution of every row of the segment, then its neighbours' edge rows
 #pragma unroll
-  for (int t = 0; t < L; ++t) {
-    const int jj = s0 + t;
-    if (jj < n && jj >= ca && jj <= cb) {
-      const double qf = rows[jj].qf;
-      VN o;
+    for (int t = 0; t < L; ++t) {
+      const int jj = s0 + t;
+      const double qf = rows[jj < n ? jj : n - 1].qf;
 #pragma unroll
-      for (int c = 0; c < C; ++c) o[c] = (T)(v[t][c] + qf * xin[c]);
-      __builtin_nontemporal_store(o, reinterpret_cast<VN*>(u + (long long)jj * pp + col));
+      for (int c = 0; c < C; ++c) v[t][c] += qf * xin[c];
+    }
+    __syncthreads();  // every lane is done with the carries
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      double last = 0.0;
+#pragma unroll
+      for (int t = 0; t < L; ++t)
+        if (s0 + t == s1) last = v[t][c];
+      carry_f[seg][lane * C + c] = last;     // row s1 of this segment
+      carry_b[seg][lane * C + c] = v[0][c];  // row s0
+    }
+    __syncthreads();
+    double vm[C], vp[C];  // rows s0 - 1 and s1 + 1
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      vm[c] = seg > 0 ? carry_f[seg - 1][lane * C + c] : 0.0;
+      vp[c] = seg + 1 < S ? carry_b[seg + 1][lane * C + c] : 0.0;
+    }
+#pragma unroll
+    for (int t = 0; t < L; ++t) {
+      const int jj = s0 + t;
+      if (jj < n) {
+        const SegRow rj = prow[jj];
+        const bool km = jj >= 1 && prow[jj - 1].kind == 1, kp = jj + 1 < n && prow[jj + 1].kind == 1;
+        const double wpm = km ? prow[jj - 1].wp : 0.0, wmp = kp ? prow[jj + 1].wm : 0.0;
+        typedef double VD __attribute__((ext_vector_type(C)));
+        VD rb, ub;
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+          const double x = v[t][c];
+          const double xm = (t == 0) ? vm[c] : v[t > 0 ? t - 1 : 0][c];
+          const double xp = (jj == s1) ? vp[c] : v[t + 1 < L ? t + 1 : t][c];
+          double b = wpm * xm + wmp * xp;
+          if (rj.kind == 1) b -= (rj.wp + rj.wm) * x;
+          ub[c] = b;
+          rb[c] = (rj.kind == 1) ? x : 0.0;
+        }
+        const long long o = (long long)jj * pp + col;
+        __builtin_nontemporal_store(rb, reinterpret_cast<VD*>(rbar + o));
+        __builtin_nontemporal_store(ub, reinterpret_cast<VD*>(ubar + o));
+      }
+    }
+  } else {
+#pragma unroll
+    for (int t = 0; t < L; ++t) {
+      const int jj = s0 + t;
+      if (jj < n && jj >= ca && jj <= cb) {
+        const double qf = rows[jj].qf;
+        VN o;
+#pragma unroll
+        for (int c = 0; c < C; ++c) o[c] = (T)(v[t][c] + qf * xin[c]);
+        __builtin_nontemporal_store(o, reinterpret_cast<VN*>(u + (long long)jj * pp + col));
+      }
     }
   }
 }
