@@ -931,7 +931,7 @@ void les_stage(dgpu_handle* h, double time) {
   // the state itself (les.f90:586-588, 1207-1209, 1229-1231)
   for (int f = 0; f < 3; ++f) to_physical<R>(h, h->cu[f].p, phys<R>(h->u[f]), xspec<R>(h->tpool[f]));
   for (int n = 0; n < nth; ++n) to_physical<R>(h, h->cth[n].p, phys<R>(h->th[n]), xspec<R>(h->tpool[3 + n]));
-  // velocity model
+  // velocity and scalar models in one pass over the points
   {
     LesVelArgs<R> a;
     for (int m = 0; m < 9; ++m) a.G[m] = phys<R>(h->grad[m]);
@@ -952,40 +952,39 @@ void les_stage(dgpu_handle* h, double time) {
     a.type = p.les_model_type;
     a.js = h->jstart;
     a.je = h->jend;
+    LesThAll<R> th;
+    std::memset(&th, 0, sizeof(th));
+    th.nth = nth;
+    for (int n = 0; n < nth; ++n) {
+      LesThArgs<R>& t = th.s[n];
+      for (int m = 0; m < 9; ++m) t.G[m] = phys<R>(h->grad[m]);
+      t.T1 = phys<R>(h->dth[0][n]);
+      t.T2 = (p.les_model_type == 5) ? phys<R>(h->dth[1][n]) : phys<R>(h->dth[0][n]);
+      t.T3 = phys<R>(h->dth[2][n]);
+      t.nu_t = h->nu_t.p;
+      t.f1 = phys<R>(h->thf[0][n]);
+      t.f3 = phys<R>(h->thf[1][n]);
+      t.kappa_t = h->kappa_t[n].p;
+      t.dy = h->d_dy.p;
+      t.dyf = h->d_dyf.p;
+      t.dx1 = p.lx / p.nx;
+      t.dz1 = p.lz / p.nz;
+      t.type = p.les_model_type;
+      t.jst = h->jstart_th[n];
+      t.jet = h->jend_th[n];
+    }
     const long long npt = g.pp * (Ny + 1);
     dim3 grid((unsigned)((npt + 255) / 256)), block(256);
-    Timed tm(h, DGPU_K_LES, ((9.0 + 2.0 + 7.0) * sizeof(R) + 8.0) * npt);
-    hipLaunchKernelGGL(k_les_vel<R>, grid, block, 0, h->stream, a, g);
+    Timed tm(h, DGPU_K_LES,
+             ((9.0 + 2.0 + 7.0) * sizeof(R) + 8.0 + nth * ((3.0 + 2.0) * sizeof(R) + 8.0)) * npt);
+    hipLaunchKernelGGL(k_les_model<R>, grid, block, 0, h->stream, a, th, g);
     HIPCHECK(hipGetLastError());
   }
   for (int m = 0; m < 7; ++m)
     to_fourier<R>(h, phys<R>(h->tau[m]), h->ctau[m].p, xspec<R>(h->tpool[0]), 1, Ny);
-  // scalar model
-  for (int n = 0; n < nth; ++n) {
-    LesThArgs<R> a;
-    for (int m = 0; m < 9; ++m) a.G[m] = phys<R>(h->grad[m]);
-    a.T1 = phys<R>(h->dth[0][n]);
-    a.T2 = (p.les_model_type == 5) ? phys<R>(h->dth[1][n]) : phys<R>(h->dth[0][n]);
-    a.T3 = phys<R>(h->dth[2][n]);
-    a.nu_t = h->nu_t.p;
-    a.f1 = phys<R>(h->thf[0][n]);
-    a.f3 = phys<R>(h->thf[1][n]);
-    a.kappa_t = h->kappa_t[n].p;
-    a.dy = h->d_dy.p;
-    a.dyf = h->d_dyf.p;
-    a.dx1 = p.lx / p.nx;
-    a.dz1 = p.lz / p.nz;
-    a.type = p.les_model_type;
-    a.jst = h->jstart_th[n];
-    a.jet = h->jend_th[n];
-    const long long npt = g.pp * (Ny + 1);
-    dim3 grid((unsigned)((npt + 255) / 256)), block(256);
-    Timed tm(h, DGPU_K_LES, ((9.0 + 3.0 + 2.0) * sizeof(R) + 16.0) * npt);
-    hipLaunchKernelGGL(k_les_th<R>, grid, block, 0, h->stream, a, g);
-    HIPCHECK(hipGetLastError());
+  for (int n = 0; n < nth; ++n)
     for (int w = 0; w < 2; ++w)
       to_fourier<R>(h, phys<R>(h->thf[w][n]), h->cthf[w][n].p, xspec<R>(h->tpool[0]), 1, Ny);
-  }
   // divergence into the subgrid forcing
   {
     LesDivArgs a;

@@ -185,35 +185,6 @@ __device__ inline void les_vel_load(const LesVelArgs<R>& a, const Geo& g, int j,
   x.u3m = a.u3[o - pp];
 }
 
-// One thread per physical point of planes 1..Ny+1: the eddy viscosity, the
-// modelled stresses scaled by the filter factors of les.f90:201-222, 498-518,
-// 773-799 (applied here before the forward transform instead of after it),
-// and the explicit cross-term fluxes (les.f90:890-919); the point model is
-// lespt::vel_point.  Values at points the substep never consumes (GY plane
-// 1, GYF plane Ny+1, the zeroed nu_t planes) follow the reference where they
-// are cheap and are otherwise left as the raw strain.
-template <typename R>
-__global__ void __launch_bounds__(256) k_les_vel(LesVelArgs<R> a, Geo g) {
-  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long long pp = g.pp;
-  const int j = 1 + plane_of(idx, pp);
-  if (j > g.ny + 1) return;
-  const long long o = (long long)j * pp + (long long)((unsigned)idx % (unsigned)pp);
-  const lespt::VelPlane<R> P = les_vel_plane(a, g, j);
-  lespt::VelIn<R> x;
-  les_vel_load(a, g, j, o, x);
-  lespt::VelOut<R> y;
-  lespt::vel_point(P, x, y);
-  a.tau11[o] = y.t11;
-  a.tau33[o] = y.t33;
-  a.tau13[o] = y.t13;
-  a.tau12[o] = y.t12;
-  a.tau23[o] = y.t23;
-  a.nu_t[o] = (double)y.nt;
-  a.cr1[o] = y.cr1;
-  a.cr3[o] = y.cr3;
-}
-
 // --------------------------------------------------------- scalar model --
 template <typename R>
 struct LesThArgs {
@@ -275,33 +246,77 @@ __device__ inline void les_th_load(const LesThArgs<R>& a, const Geo& g, int j, l
   x.dd = (j >= 2) ? a.G[4][o - 2 * pp] : (R)0;
 }
 
-// Types 1/4: kappa_t = nu_t and the explicit horizontal diffusion fluxes
-// kappa_t d1 th, kappa_t d3 th (les.f90:1142-1203).  Type 5: the AMD scalar
-// model at GYF (:1238-1275) and GY points (:1278-1330), kappa_t (:1373-1384)
-// and the fluxes 0.5 delta_yf s1_th d_i th (:1334-1342, 1392-1401), with the
-// filter factor applied before the forward transform (lespt::th_point).
+// One thread per physical point of planes 1..Ny+1, both subgrid models.
+// Velocity: the eddy viscosity, the modelled stresses scaled by the filter
+// factors of les.f90:201-222, 498-518, 773-799 (applied here before the
+// forward transform instead of after it), and the explicit cross-term fluxes
+// (les.f90:890-919); the point model is lespt::vel_point.  Values at points
+// the substep never consumes (GY plane 1, GYF plane Ny+1, the zeroed nu_t
+// planes) follow the reference where they are cheap and are otherwise left
+// as the raw strain.  Scalars, each: types 1/4 kappa_t = nu_t and the
+// explicit horizontal diffusion fluxes kappa_t d1 th, kappa_t d3 th
+// (les.f90:1142-1203); type 5 the AMD scalar model at GYF (:1238-1275) and
+// GY points (:1278-1330), kappa_t (:1373-1384) and the fluxes
+// 0.5 delta_yf s1_th d_i th (:1334-1342, 1392-1401), with the filter factor
+// applied before the forward transform (lespt::th_point).  The scalar model
+// reads the same velocity gradients (planes and components) as the velocity
+// model: they are taken from registers instead of being read again.
 template <typename R>
-__global__ void __launch_bounds__(256) k_les_th(LesThArgs<R> a, Geo g) {
+struct LesThAll {
+  LesThArgs<R> s[MAXTH];
+  int nth;
+};
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_les_model(LesVelArgs<R> a, LesThAll<R> th, Geo g) {
   const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long pp = g.pp;
   const int j = 1 + plane_of(idx, pp);
   if (j > g.ny + 1) return;
   const long long o = (long long)j * pp + (long long)((unsigned)idx % (unsigned)pp);
-  if (a.type != 5) {
-    const double kt = a.nu_t[o];
-    a.kappa_t[o] = kt;
-    a.f1[o] = (R)kt * a.T1[o];
-    a.f3[o] = (R)kt * a.T3[o];
-    return;
+  const lespt::VelPlane<R> P = les_vel_plane(a, g, j);
+  lespt::VelIn<R> x;
+  les_vel_load(a, g, j, o, x);
+  lespt::VelOut<R> y;
+  lespt::vel_point(P, x, y);
+  a.tau11[o] = y.t11;
+  a.tau33[o] = y.t33;
+  a.tau13[o] = y.t13;
+  a.tau12[o] = y.t12;
+  a.tau23[o] = y.t23;
+  a.nu_t[o] = (double)y.nt;
+  a.cr1[o] = y.cr1;
+  a.cr3[o] = y.cr3;
+  for (int n = 0; n < th.nth; ++n) {
+    const LesThArgs<R>& t = th.s[n];
+    if (t.type != 5) {
+      const double kt = (double)y.nt;
+      t.kappa_t[o] = kt;
+      t.f1[o] = (R)kt * t.T1[o];
+      t.f3[o] = (R)kt * t.T3[o];
+      continue;
+    }
+    const lespt::ThPlane<R> Pt = les_th_plane(t, g, j);
+    lespt::ThIn<R> xt;
+#pragma unroll
+    for (int m = 0; m < 9; ++m) xt.c[m] = x.c[m];
+    xt.t[0] = t.T1[o];
+    xt.t[1] = t.T2[o];
+    xt.t[2] = t.T3[o];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) xt.up[m] = x.up[m];  // G12 G21 G23 G32 at j+1
+    xt.up[4] = (j <= g.ny) ? t.T2[o + pp] : (R)0;
+#pragma unroll
+    for (int m = 0; m < 5; ++m) xt.dn[m] = x.dn[m];  // G11 G22 G33 G13 G31 at j-1
+    xt.dn[5] = t.T1[o - pp];
+    xt.dn[6] = t.T3[o - pp];
+    xt.dd = (j >= 2) ? t.G[4][o - 2 * pp] : (R)0;
+    lespt::ThOut<R> yt;
+    lespt::th_point(Pt, xt, yt);
+    t.f1[o] = yt.f1;
+    t.f3[o] = yt.f3;
+    t.kappa_t[o] = (double)yt.kt;
   }
-  const lespt::ThPlane<R> P = les_th_plane(a, g, j);
-  lespt::ThIn<R> x;
-  les_th_load(a, g, j, o, x);
-  lespt::ThOut<R> y;
-  lespt::th_point(P, x, y);
-  a.f1[o] = y.f1;
-  a.f3[o] = y.f3;
-  a.kappa_t[o] = (double)y.kt;
 }
 
 // ---------------------------------------------- stress divergence into cf --

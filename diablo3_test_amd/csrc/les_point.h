@@ -129,7 +129,7 @@ struct VelOut {
   R t11, t33, t13, t12, t23, nt, cr1, cr3;
 };
 
-// Per-plane constants and masks of k_les_vel.
+// Per-plane constants and masks of the velocity model (k_les_model).
 template <typename R>
 struct VelPlane {
   int type;
@@ -232,7 +232,7 @@ LP_HD R vel_gy_scalar(const VelPlane<R>& P, const VelIn<R>& x, Amd<R>* zout = nu
   return amd_value(z, dxq, dyq, dzq, P.type == 4);
 }
 
-// All outputs of k_les_vel at one point (les.f90:201-222, 498-518, 773-799,
+// All outputs of the velocity model (k_les_model) at one point (les.f90:201-222, 498-518, 773-799,
 // 890-919): filter-scaled stresses, nu_t, and the cross-term fluxes.
 template <typename R>
 LP_HD void vel_point(const VelPlane<R>& P, const VelIn<R>& x, VelOut<R>& y) {
