@@ -2292,6 +2292,7 @@ void create_impl(dgpu_handle* h, const double* gy) {
 
   // CN factor tables (DGPU_SEG_TARGET overrides the target segment length)
   if (const char* e = std::getenv("DGPU_SEG_TARGET")) seg_target = std::atoi(e);
+  if (seg_target < 1) throw ApiError{DGPU_EINVAL, "DGPU_SEG_TARGET must be a positive row count"};
   if (const char* e = std::getenv("DGPU_XPASS_THREADS")) xpass_threads = std::atoi(e);
   if (const char* e = std::getenv("DGPU_XPROD_THREADS")) xprod_threads = std::atoi(e);
   if (const char* e = std::getenv("DGPU_BATCH_MB")) batch_mb = std::atoi(e);
