@@ -916,14 +916,16 @@ void les_stage(dgpu_handle* h, double time) {
     int op;
   } gsrc[9] = {{cu1, OP_DX}, {cu1, OP_DY_GY}, {cu1, OP_DZ},  {cu2, OP_DX}, {cu2, OP_DY_GYF},
                {cu2, OP_DZ}, {cu3, OP_DX},    {cu3, OP_DY_GY}, {cu3, OP_DZ}};
+  // the y differences (G12, G22, G32, the scalars' T2) are formed from the
+  // physical state inside k_les_model
   for (int m = 0; m < 9; ++m) {
+    if (gsrc[m].op == OP_DY_GY || gsrc[m].op == OP_DY_GYF) continue;
     launch_zinv_op<R>(h, gsrc[m].c, T, 0, n2 - 1, gsrc[m].op);
     launch_xinv<R>(h, T, phys<R>(h->grad[m]), 0, n2 - 1);
   }
   for (int n = 0; n < nth; ++n) {
     const int ops[3] = {OP_DX, OP_DY_GY, OP_DZ};
-    for (int w = 0; w < 3; ++w) {
-      if (w == 1 && p.les_model_type != 5) continue;
+    for (int w = 0; w < 3; w += 2) {
       launch_zinv_op<R>(h, h->cth[n].p, T, 0, n2 - 1, ops[w]);
       launch_xinv<R>(h, T, phys<R>(h->dth[w][n]), 0, n2 - 1);
     }
@@ -973,11 +975,24 @@ void les_stage(dgpu_handle* h, double time) {
       t.jst = h->jstart_th[n];
       t.jet = h->jend_th[n];
     }
+    LesDyArgs<R> dy;
+    std::memset(&dy, 0, sizeof(dy));
+    dy.u2 = phys<R>(h->u[1]);
+    for (int n = 0; n < nth; ++n) {
+      dy.th[n] = phys<R>(h->th[n]);
+      dy.t2[n] = phys<R>(h->dth[1][n]);
+    }
+    dy.g12 = phys<R>(h->grad[1]);
+    dy.g22 = phys<R>(h->grad[4]);
+    dy.g32 = phys<R>(h->grad[7]);
+    dy.store = h->les_cap ? 1 : 0;  // the adjoint's gathers read them
     const long long npt = g.pp * (Ny + 1);
     dim3 grid((unsigned)((npt + 255) / 256)), block(256);
+    // 6 gradients, u1, u2, u3 in; 5 stresses, 2 fluxes, nu_t out; per scalar
+    // th and 2 gradients in, 2 fluxes and kappa_t out
     Timed tm(h, DGPU_K_LES,
-             ((9.0 + 2.0 + 7.0) * sizeof(R) + 8.0 + nth * ((3.0 + 2.0) * sizeof(R) + 8.0)) * npt);
-    hipLaunchKernelGGL(k_les_model<R>, grid, block, 0, h->stream, a, th, g);
+             ((6.0 + 3.0 + 7.0) * sizeof(R) + 8.0 + nth * ((3.0 + 2.0) * sizeof(R) + 8.0)) * npt);
+    hipLaunchKernelGGL(k_les_model<R>, grid, block, 0, h->stream, a, th, dy, g);
     HIPCHECK(hipGetLastError());
   }
   for (int m = 0; m < 7; ++m)

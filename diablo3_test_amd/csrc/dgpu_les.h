@@ -267,16 +267,61 @@ struct LesThAll {
   int nth;
 };
 
+// The four y-difference gradients -- G12 = (u1(j) - u1(j-1))/dy(j),
+// G32 likewise, G22 = (u2(j+1) - u2(j))/dyf(j) (0 at Ny+1) and per scalar
+// T2 = (th(j) - th(j-1))/dy(j) -- commute with the x-z transforms, so they are
+// formed here from the physical state instead of by four inverse transforms
+// of the differenced spectra (the forms of k_zinv_op's OP_DY_GY / OP_DY_GYF).
+// With store set (the adjoint's recomputation) they are also written out
+// (planes 0..Ny+1), where the adjoint's gathers read them.
 template <typename R>
-__global__ void __launch_bounds__(256) k_les_model(LesVelArgs<R> a, LesThAll<R> th, Geo g) {
+struct LesDyArgs {
+  const R* u2;
+  const R* th[MAXTH];
+  R* g12;
+  R* g22;
+  R* g32;
+  R* t2[MAXTH];
+  int store;
+};
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_les_model(LesVelArgs<R> a, LesThAll<R> th, LesDyArgs<R> dy, Geo g) {
   const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long pp = g.pp;
+  const int ny = g.ny;
   const int j = 1 + plane_of(idx, pp);
-  if (j > g.ny + 1) return;
+  if (j > ny + 1) return;
   const long long o = (long long)j * pp + (long long)((unsigned)idx % (unsigned)pp);
+  const bool up = j <= ny;  // plane j+1 exists
+  const R idy = (R)(1.0 / a.dy[j]), idyp = up ? (R)(1.0 / a.dy[j + 1]) : (R)0;
+  // y differences at planes j (and j+1, j-1, j-2 where the models read them)
+  const R u1m = a.u1[o - pp], u1c = a.u1[o], u1p = up ? a.u1[o + pp] : (R)0;
+  const R u3m = a.u3[o - pp], u3c = a.u3[o], u3p = up ? a.u3[o + pp] : (R)0;
+  const R u2m = dy.u2[o - pp], u2c = dy.u2[o], u2p = up ? dy.u2[o + pp] : (R)0;
+  const R g12 = (u1c - u1m) * idy, g12p = up ? (u1p - u1c) * idyp : (R)0;
+  const R g32 = (u3c - u3m) * idy, g32p = up ? (u3p - u3c) * idyp : (R)0;
+  const R g22 = up ? (u2p - u2c) * (R)(1.0 / a.dyf[j]) : (R)0;
+  const R g22m = (u2c - u2m) * (R)(1.0 / a.dyf[j - 1]);
   const lespt::VelPlane<R> P = les_vel_plane(a, g, j);
   lespt::VelIn<R> x;
-  les_vel_load(a, g, j, o, x);
+  for (int m = 0; m < 9; ++m) x.c[m] = (m == 1 || m == 4 || m == 7) ? (R)0 : a.G[m][o];
+  x.c[1] = g12;
+  x.c[4] = g22;
+  x.c[7] = g32;
+  x.up[0] = g12p;
+  x.up[1] = up ? a.G[3][o + pp] : (R)0;
+  x.up[2] = up ? a.G[5][o + pp] : (R)0;
+  x.up[3] = g32p;
+  x.dn[0] = a.G[0][o - pp];
+  x.dn[1] = g22m;
+  x.dn[2] = a.G[8][o - pp];
+  x.dn[3] = a.G[2][o - pp];
+  x.dn[4] = a.G[6][o - pp];
+  x.u1o = u1c;
+  x.u1m = u1m;
+  x.u3o = u3c;
+  x.u3m = u3m;
   lespt::VelOut<R> y;
   lespt::vel_point(P, x, y);
   a.tau11[o] = y.t11;
@@ -287,8 +332,24 @@ __global__ void __launch_bounds__(256) k_les_model(LesVelArgs<R> a, LesThAll<R> 
   a.nu_t[o] = (double)y.nt;
   a.cr1[o] = y.cr1;
   a.cr3[o] = y.cr3;
+  if (dy.store) {
+    dy.g12[o] = g12;
+    dy.g22[o] = g22;
+    dy.g32[o] = g32;
+    if (j == 1) {  // plane 0: the differences the transform route gives there
+      dy.g12[o - pp] = (R)0;
+      dy.g32[o - pp] = (R)0;
+      dy.g22[o - pp] = g22m;
+    }
+  }
   for (int n = 0; n < th.nth; ++n) {
     const LesThArgs<R>& t = th.s[n];
+    const R thm = dy.th[n][o - pp], thc = dy.th[n][o], thp = up ? dy.th[n][o + pp] : (R)0;
+    const R t2 = (thc - thm) * idy, t2p = up ? (thp - thc) * idyp : (R)0;
+    if (dy.store) {
+      dy.t2[n][o] = t2;
+      if (j == 1) dy.t2[n][o - pp] = (R)0;
+    }
     if (t.type != 5) {
       const double kt = (double)y.nt;
       t.kappa_t[o] = kt;
@@ -301,16 +362,16 @@ __global__ void __launch_bounds__(256) k_les_model(LesVelArgs<R> a, LesThAll<R> 
 #pragma unroll
     for (int m = 0; m < 9; ++m) xt.c[m] = x.c[m];
     xt.t[0] = t.T1[o];
-    xt.t[1] = t.T2[o];
+    xt.t[1] = t2;
     xt.t[2] = t.T3[o];
 #pragma unroll
     for (int m = 0; m < 4; ++m) xt.up[m] = x.up[m];  // G12 G21 G23 G32 at j+1
-    xt.up[4] = (j <= g.ny) ? t.T2[o + pp] : (R)0;
+    xt.up[4] = t2p;
 #pragma unroll
     for (int m = 0; m < 5; ++m) xt.dn[m] = x.dn[m];  // G11 G22 G33 G13 G31 at j-1
     xt.dn[5] = t.T1[o - pp];
     xt.dn[6] = t.T3[o - pp];
-    xt.dd = (j >= 2) ? t.G[4][o - 2 * pp] : (R)0;
+    xt.dd = (j >= 2) ? (dy.u2[o - pp] - dy.u2[o - 2 * pp]) * (R)(1.0 / a.dyf[j - 2]) : (R)0;
     lespt::ThOut<R> yt;
     lespt::th_point(Pt, xt, yt);
     t.f1[o] = yt.f1;
