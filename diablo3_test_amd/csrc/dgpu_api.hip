@@ -587,7 +587,7 @@ void launch_xfwd(dgpu_handle* h, const R* u, cpx<R>* T, int j0, int j1) {
 }
 
 template <typename R>
-void launch_xinv(dgpu_handle* h, const cpx<R>* T, R* u, int j0, int j1) {
+void launch_xinv(dgpu_handle* h, const cpx<R>* T, R* u, int j0, int j1, const double* dkx = nullptr) {
   if (j1 < j0) return;
   const Geo& g = h->g;
   constexpr int KT = XTile<R>::KT;
@@ -596,7 +596,7 @@ void launch_xinv(dgpu_handle* h, const cpx<R>* T, R* u, int j0, int j1) {
   dim3 grid(g.nz / KT, j1 - j0 + 1), block(ng * (g.nx / 8));
   Timed tm(h, DGPU_K_XINV, (sizeof(R) * g.pp + 2.0 * sizeof(R) * g.tp) * (j1 - j0 + 1));
   hipLaunchKernelGGL((g.nx <= 512 ? k_xinv<R, true> : k_xinv<R, false>), grid, block, lds, h->stream, T, u, g,
-                     twx_of<R>(h), j0);
+                     twx_of<R>(h), j0, dkx);
   HIPCHECK(hipGetLastError());
 }
 
@@ -916,23 +916,34 @@ void les_stage(dgpu_handle* h, double time) {
     int op;
   } gsrc[9] = {{cu1, OP_DX}, {cu1, OP_DY_GY}, {cu1, OP_DZ},  {cu2, OP_DX}, {cu2, OP_DY_GYF},
                {cu2, OP_DZ}, {cu3, OP_DX},    {cu3, OP_DY_GY}, {cu3, OP_DZ}};
-  // the y differences (G12, G22, G32, the scalars' T2) are formed from the
-  // physical state inside k_les_model
+  // the state itself (les.f90:586-588, 1207-1209, 1229-1231); its x-spectra
+  // stay in tpool[f] (all planes, without plane batches) for the x derivatives
+  for (int f = 0; f < 3; ++f) to_physical<R>(h, h->cu[f].p, phys<R>(h->u[f]), xspec<R>(h->tpool[f]));
+  for (int n = 0; n < nth; ++n) to_physical<R>(h, h->cth[n].p, phys<R>(h->th[n]), xspec<R>(h->tpool[3 + n]));
+  // x derivatives: i kx times the state's x-spectra, then the x inverse; z
+  // derivatives: i kz inside the z inverse; the y differences (G12, G22,
+  // G32, the scalars' T2) are formed from the physical state in k_les_model
+  const bool share = batch_mb <= 0;
+  T = xspec<R>(h->tpool[3 + nth]);  // scratch past the state's x-spectra
   for (int m = 0; m < 9; ++m) {
     if (gsrc[m].op == OP_DY_GY || gsrc[m].op == OP_DY_GYF) continue;
+    if (gsrc[m].op == OP_DX && share) {
+      launch_xinv<R>(h, xspec<R>(h->tpool[m / 3]), phys<R>(h->grad[m]), 0, n2 - 1, h->d_kx.p);
+      continue;
+    }
     launch_zinv_op<R>(h, gsrc[m].c, T, 0, n2 - 1, gsrc[m].op);
     launch_xinv<R>(h, T, phys<R>(h->grad[m]), 0, n2 - 1);
   }
   for (int n = 0; n < nth; ++n) {
-    const int ops[3] = {OP_DX, OP_DY_GY, OP_DZ};
-    for (int w = 0; w < 3; w += 2) {
-      launch_zinv_op<R>(h, h->cth[n].p, T, 0, n2 - 1, ops[w]);
-      launch_xinv<R>(h, T, phys<R>(h->dth[w][n]), 0, n2 - 1);
+    if (share) {
+      launch_xinv<R>(h, xspec<R>(h->tpool[3 + n]), phys<R>(h->dth[0][n]), 0, n2 - 1, h->d_kx.p);
+    } else {
+      launch_zinv_op<R>(h, h->cth[n].p, T, 0, n2 - 1, OP_DX);
+      launch_xinv<R>(h, T, phys<R>(h->dth[0][n]), 0, n2 - 1);
     }
+    launch_zinv_op<R>(h, h->cth[n].p, T, 0, n2 - 1, OP_DZ);
+    launch_xinv<R>(h, T, phys<R>(h->dth[2][n]), 0, n2 - 1);
   }
-  // the state itself (les.f90:586-588, 1207-1209, 1229-1231)
-  for (int f = 0; f < 3; ++f) to_physical<R>(h, h->cu[f].p, phys<R>(h->u[f]), xspec<R>(h->tpool[f]));
-  for (int n = 0; n < nth; ++n) to_physical<R>(h, h->cth[n].p, phys<R>(h->th[n]), xspec<R>(h->tpool[3 + n]));
   // velocity and scalar models in one pass over the points
   {
     LesVelArgs<R> a;

@@ -101,10 +101,12 @@ __global__ void __launch_bounds__(512) k_xfwd(const R* __restrict__ u, cpx<R>* _
 
 // Inverse x transform: dealias (fft.f90:372-378) and c2r (:380), unnormalised.
 // FFTW's c2r reads only the real part of the kx = 0 input.  Rows 2m and 2m+1
-// are synthesised together: Z = A + iB with A, B extended Hermitian.
+// are synthesised together: Z = A + iB with A, B extended Hermitian.  With dkx
+// the x-spectra are multiplied by i kx first (the x derivative).
 template <typename R, bool WAVE>
 __global__ void __launch_bounds__(512) k_xinv(const cpx<R>* __restrict__ T, R* __restrict__ u,
-                                               Geo g, const cpx<R>* __restrict__ tw, int j0) {
+                                               Geo g, const cpx<R>* __restrict__ tw, int j0,
+                                               const double* __restrict__ dkx = nullptr) {
   constexpr int KT = XTile<R>::KT;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<R>* smem = reinterpret_cast<cpx<R>*>(smem_raw);
@@ -131,6 +133,11 @@ __global__ void __launch_bounds__(512) k_xinv(const cpx<R>* __restrict__ T, R* _
       } else {
         A = src[0];
         B = src[1];
+      }
+      if (dkx) {
+        const R k = (R)dkx[q];
+        A = ik(k, A);
+        B = ik(k, B);
       }
       cpx<R>* f = smem + mm * L;
       if (q == 0) {
