@@ -370,7 +370,7 @@ __global__ void __launch_bounds__(512) k_adj_xpairs(AdjPairArgs a, Geo g, const 
   extern __shared__ __align__(16) unsigned char smem_raw[];
   C* smem = reinterpret_cast<C*>(smem_raw);
   const int G = g.nx >> 3;
-  const int L = fft_lds_len(g.nx);
+  const int L = x_group_stride(g.nx);
   const int ngroups = blockDim.x / G;  // a power of two dividing KT
   const int lg_ng = 31 - __builtin_clz(ngroups);
   const int grp = threadIdx.x / G, gt = threadIdx.x % G;

@@ -542,7 +542,7 @@ int x_ngroups(const dgpu_handle* h, int jobs, int max_threads = 512) {
   const Geo& g = h->g;
   const int G = g.nx / 8;
   int ng = jobs;
-  auto lds = [&](int n) { return (size_t)n * fft_lds_len(g.nx) * sizeof(cpx<R>); };
+  auto lds = [&](int n) { return (size_t)n * x_group_stride(g.nx) * sizeof(cpx<R>); };
   while (ng > 1 && (ng * G > max_threads || lds(ng) > h->lds_limit)) ng /= 2;
   if (ng * G > max_threads || lds(ng) > h->lds_limit)
     throw ApiError{DGPU_EUNSUP, "Nx too large for the x-transform pass"};
@@ -551,7 +551,7 @@ int x_ngroups(const dgpu_handle* h, int jobs, int max_threads = 512) {
 
 template <typename R>
 size_t x_lds(const dgpu_handle* h, int ng) {
-  return (size_t)ng * fft_lds_len(h->g.nx) * sizeof(cpx<R>);
+  return (size_t)ng * x_group_stride(h->g.nx) * sizeof(cpx<R>);
 }
 
 template <typename R>

@@ -121,6 +121,13 @@ __device__ __forceinline__ void dft8(cpx<R>* v) {
 __device__ __forceinline__ int pidx(int i) { return i + (i >> 3); }
 __host__ __device__ constexpr int fft_lds_len(int n) { return n + (n >> 3); }
 
+// Distance between the transform buffers of the groups of an x pass.  The x
+// passes also access the buffers across groups (consecutive lanes, the same
+// element of consecutive groups' buffers); a stride of fft_lds_len(N), a
+// multiple of the bank count, would put all those lanes in the same banks, so
+// the buffers are spaced one element further apart.
+__host__ __device__ constexpr int x_group_stride(int n) { return fft_lds_len(n) + 1; }
+
 // Synchronisation between the passes of one transform.  When the G = N/8
 // lanes of a transform lie inside one wavefront (G <= 64, groups aligned to
 // G), LDS operations of the wavefront execute in program order, so a

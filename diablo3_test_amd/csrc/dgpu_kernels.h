@@ -71,7 +71,7 @@ __global__ void __launch_bounds__(512) k_xfwd(const R* __restrict__ u, cpx<R>* _
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<R>* smem = reinterpret_cast<cpx<R>*>(smem_raw);
   const int G = g.nx >> 3;
-  const int L = fft_lds_len(g.nx);
+  const int L = x_group_stride(g.nx);
   const int ngroups = blockDim.x / G;
   const int grp = threadIdx.x / G, gt = threadIdx.x % G;
   cpx<R>* fbuf = smem + grp * L;
@@ -111,7 +111,7 @@ __global__ void __launch_bounds__(512) k_xinv(const cpx<R>* __restrict__ T, R* _
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<R>* smem = reinterpret_cast<cpx<R>*>(smem_raw);
   const int G = g.nx >> 3;
-  const int L = fft_lds_len(g.nx);
+  const int L = x_group_stride(g.nx);
   const int ngroups = blockDim.x / G;
   const int grp = threadIdx.x / G, gt = threadIdx.x % G;
   cpx<R>* fbuf = smem + grp * L;
@@ -326,7 +326,7 @@ __global__ void __launch_bounds__(512) k_xprod_all(ProdAllArgs a, Geo g,
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<R>* smem = reinterpret_cast<cpx<R>*>(smem_raw);
   const int G = g.nx >> 3;  // 8 values of a row per thread
-  const int L = fft_lds_len(g.nx);
+  const int L = x_group_stride(g.nx);
   const int ngroups = blockDim.x / G;
   const int grp = threadIdx.x / G, gt = threadIdx.x % G;
   cpx<R>* fbuf = smem + grp * L;
