@@ -69,6 +69,9 @@ int les_cols = 2;
 // Measured at 512x257x512: 2 columns take the mixed form's 12 solves per step
 // from 5.19 to 4.50 ms; the FP64 solves are 2% slower with 2 (5.21 -> 5.33 ms).
 int thomas_cols = 2;
+// CN solves of the non-LES substep in spectral space (knob DGPU_SPECTRAL_CN,
+// 0 = the physical-space solves between the transforms, as the reference).
+int spectral_cn = 1;
 // Plane batches of a 2-D transform: the x and z passes run batch by batch so
 // that a batch's x-spectra (the intermediate between the passes) are read
 // back while still in the last-level cache.  Budget in MB (DGPU_BATCH_MB).
@@ -698,26 +701,31 @@ void to_fourier(dgpu_handle* h, const R* u, double2* C, cpx<R>* T, int j0, int j
   }
 }
 
+// The CN solve of one field on planes [ca, cb]: physical columns (r, u with
+// plane pitch pp), or with spec the dealiased spectra (r = cr, u = the
+// spectral state, FP64, 2 sp real columns a plane).
 template <typename R>
 void launch_thomas(dgpu_handle* h, int field, int rs, const R* r, R* u, int ca, int cb, double transient,
-                   bool incr, const SegRow* table = nullptr) {
+                   bool incr, const SegRow* table = nullptr, bool spec = false) {
   const Geo& g = h->g;
   const int n = g.ny2;
   const SegRow* rows = (table ? table : h->rows.p) + ((size_t)field * 3 + rs) * n;
+  const long long pitch = spec ? 2 * g.sp : g.pp;
+  const long long ncol = spec ? 2 * g.sp : 0, nbc = spec ? 1 : 0;
   // columns per lane: thomas_cols for segments of up to 32 rows (register budget), else 1.
   // The wider row runs (64 C values) pay for the lower occupancy: at 512x257x512 the FP64
   // solve takes 4.34 ms per step with two columns and 4.70 with one.
-  const int C = (h->seg_L <= 32 && thomas_cols == 2 && g.pp % 128 == 0) ? 2 : 1;
-  dim3 grid((unsigned)(g.pp / (64 * C))), block(64, h->seg_S);
-  Timed tm(h, DGPU_K_THOMAS, 3.0 * sizeof(R) * g.pp * n);
+  const int C = (h->seg_L <= 32 && thomas_cols == 2 && (spec || g.pp % 128 == 0)) ? 2 : 1;
+  dim3 grid((unsigned)((pitch + 64 * C - 1) / (64 * C))), block(64, h->seg_S);
+  Timed tm(h, DGPU_K_THOMAS, 3.0 * sizeof(R) * pitch * n);
 #define TLAUNCH(L, CC)                                                                                     \
   do {                                                                                                     \
     if (incr)                                                                                              \
       hipLaunchKernelGGL((k_thomas_seg<L, CC, R, true>), grid, block, 0, h->stream, r, u, rows, n,         \
-                         h->seg_S, g.pp, ca, cb, transient);                                               \
+                         h->seg_S, pitch, ca, cb, transient, nullptr, nullptr, nullptr, ncol, nbc);       \
     else                                                                                                   \
       hipLaunchKernelGGL((k_thomas_seg<L, CC, R, false>), grid, block, 0, h->stream, r, u, rows, n,        \
-                         h->seg_S, g.pp, ca, cb, transient);                                               \
+                         h->seg_S, pitch, ca, cb, transient, nullptr, nullptr, nullptr, ncol, nbc);       \
   } while (0)
   switch (h->seg_L * 4 + C) {
     case 8 * 4 + 1: TLAUNCH(8, 1); break;
@@ -1249,6 +1257,11 @@ void rk_substep_t(dgpu_handle* h, int rk_step, double time, int time_step) {
   const int ncr = 3 + nth;
   // LES gate (channel.f90:259)
   const bool les = p.use_les && (!p.create_new_flow || time_step > 100);
+  // Without LES every CN matrix row depends on j alone and the wall values are
+  // uniform in x and z, so the vertical solves commute with the horizontal
+  // transforms: they run on the dealiased spectra (FP64 in both precision
+  // forms), and cr never goes to physical space nor the solution back.
+  const bool spec = spectral_cn && !p.use_les;
   // The explicit spectral terms (channel.f90:39-253) read the state before
   // les_chan applies its wall values, so the LES path keeps that copy.
   const double2* lin_cu[3] = {h->cu[0].p, h->cu[1].p, h->cu[2].p};
@@ -1317,7 +1330,9 @@ void rk_substep_t(dgpu_handle* h, int rk_step, double time, int time_step) {
     a.cu2 = lin_cu[1];
     a.cu3 = lin_cu[2];
     a.cp = h->cp.p;
-    a.incr = !incr ? 0 : (les ? 1 : 2);
+    a.incr = (!incr || spec) ? 0 : (les ? 1 : 2);
+    if (spec)  // cr in the (otherwise idle) cr x-spectra buffers, at plane pitch sp
+      for (int c = 0; c < ncr; ++c) a.crs[c] = h->tpool[nslot + c].p;
     a.cub1 = h->cu[0].p;
     a.cub2 = h->cu[1].p;
     a.cub3 = h->cu[2].p;
@@ -1373,8 +1388,9 @@ void rk_substep_t(dgpu_handle* h, int rk_step, double time, int time_step) {
     dim3 grid(g.nq, Ny), block(ng * G);
     // x-spectra of the products in, cr x-spectra out; cu1..3, cp, cth, cf1..3,
     // cfth read and cf1..3, cfth written (neighbour planes come from cache)
+    const double crbytes = spec ? 16.0 * g.sp * ncr : 2.0 * sizeof(R) * g.tp * ncr;
     Timed tm(h, DGPU_K_RHS,
-             (2.0 * sizeof(R) * g.tp * (nslot + ncr) + 16.0 * g.sp * (4 + 2 * nth + 3 + nth + 3 + nth)) * Ny);
+             (2.0 * sizeof(R) * g.tp * nslot + crbytes + 16.0 * g.sp * (4 + 2 * nth + 3 + nth + 3 + nth)) * Ny);
     hipLaunchKernelGGL((g.nz <= 512 ? k_rhs<R, true> : k_rhs<R, false>), grid, block, lds, h->stream, a, g,
                        twz_of<R>(h), 1);
     HIPCHECK(hipGetLastError());
@@ -1382,10 +1398,13 @@ void rk_substep_t(dgpu_handle* h, int rk_step, double time, int time_step) {
 
   // 4. cr to physical space (second half of fft_xz_to_physical, :485-487, :621)
   // slot order of cr: 0 cr1, 1 cr3, 2 cr2, 3+n crth_n
-  launch_xinv<R>(h, xspec<R>(h->tpool[nslot + 0]), phys<R>(h->r[0]), 1, Ny);
-  launch_xinv<R>(h, xspec<R>(h->tpool[nslot + 1]), phys<R>(h->r[2]), 1, Ny);
-  launch_xinv<R>(h, xspec<R>(h->tpool[nslot + 2]), phys<R>(h->r[1]), 1, Ny);
-  for (int n = 0; n < nth; ++n) launch_xinv<R>(h, xspec<R>(h->tpool[nslot + 3 + n]), phys<R>(h->rth[n]), 1, Ny);
+  if (!spec) {
+    launch_xinv<R>(h, xspec<R>(h->tpool[nslot + 0]), phys<R>(h->r[0]), 1, Ny);
+    launch_xinv<R>(h, xspec<R>(h->tpool[nslot + 1]), phys<R>(h->r[2]), 1, Ny);
+    launch_xinv<R>(h, xspec<R>(h->tpool[nslot + 2]), phys<R>(h->r[1]), 1, Ny);
+    for (int n = 0; n < nth; ++n)
+      launch_xinv<R>(h, xspec<R>(h->tpool[nslot + 3 + n]), phys<R>(h->rth[n]), 1, Ny);
+  }
 
   // 5. implicit vertical solves (channel.f90:656-877)
   double transient = h->w_transient;
@@ -1403,6 +1422,17 @@ void rk_substep_t(dgpu_handle* h, int rk_step, double time, int time_step) {
                          transient, incr);
     launch_thomas_les<R>(h, 2, rs, phys<R>(h->r[2]), phys<R>(h->u[2]), h->nu_t.p, h->jstart - 1, h->jend + 1,
                          transient, incr);
+  } else if (spec) {
+    auto crs = [&](int c) { return reinterpret_cast<const double*>(h->tpool[nslot + c].p); };
+    auto spc = [&](DevBuf<double2>& b) { return reinterpret_cast<double*>(b.p); };
+    for (int n = 0; n < nth; ++n)
+      launch_thomas<double>(h, 3 + n, rs, crs(3 + n), spc(h->cth[n]), h->jstart_th[n] - 1, h->jend_th[n] + 1,
+                            transient, false, nullptr, true);
+    launch_thomas<double>(h, 1, rs, crs(2), spc(h->cu[1]), 1, Ny + 1, transient, false, nullptr, true);
+    launch_thomas<double>(h, 0, rs, crs(0), spc(h->cu[0]), h->jstart - 1, h->jend + 1, transient, false, nullptr,
+                          true);
+    launch_thomas<double>(h, 2, rs, crs(1), spc(h->cu[2]), h->jstart - 1, h->jend + 1, transient, false, nullptr,
+                          true);
   } else {
     for (int n = 0; n < nth; ++n)
       launch_thomas<R>(h, 3 + n, rs, phys<R>(h->rth[n]), phys<R>(h->th[n]), h->jstart_th[n] - 1,
@@ -1423,12 +1453,14 @@ void rk_substep_t(dgpu_handle* h, int rk_step, double time, int time_step) {
 
   // 6. back to Fourier space on the solved planes (channel.f90:891-896); the
   // mixed form adds the transformed increment to the FP64 state
-  to_fourier<R>(h, phys<R>(h->u[0]), h->cu[0].p, xspec<R>(h->tpool[0]), h->jstart - 1, h->jend + 1, incr);
-  to_fourier<R>(h, phys<R>(h->u[1]), h->cu[1].p, xspec<R>(h->tpool[1]), 1, Ny + 1, incr);
-  to_fourier<R>(h, phys<R>(h->u[2]), h->cu[2].p, xspec<R>(h->tpool[2]), h->jstart - 1, h->jend + 1, incr);
-  for (int n = 0; n < nth; ++n)
-    to_fourier<R>(h, phys<R>(h->th[n]), h->cth[n].p, xspec<R>(h->tpool[3 + n]), h->jstart_th[n] - 1,
-                  h->jend_th[n] + 1, incr);
+  if (!spec) {
+    to_fourier<R>(h, phys<R>(h->u[0]), h->cu[0].p, xspec<R>(h->tpool[0]), h->jstart - 1, h->jend + 1, incr);
+    to_fourier<R>(h, phys<R>(h->u[1]), h->cu[1].p, xspec<R>(h->tpool[1]), 1, Ny + 1, incr);
+    to_fourier<R>(h, phys<R>(h->u[2]), h->cu[2].p, xspec<R>(h->tpool[2]), h->jstart - 1, h->jend + 1, incr);
+    for (int n = 0; n < nth; ++n)
+      to_fourier<R>(h, phys<R>(h->th[n]), h->cth[n].p, xspec<R>(h->tpool[3 + n]), h->jstart_th[n] - 1,
+                    h->jend_th[n] + 1, incr);
+  }
 
   // courant on the last substep of every update_dt-th step, on the velocity
   // just solved (channel.f90:882-888); the new delta_t applies from the next
@@ -1436,7 +1468,7 @@ void rk_substep_t(dgpu_handle* h, int rk_step, double time, int time_step) {
   const bool cfl_now = h->var_dt && rk_step == 3 && time_step % h->cour.update_dt == 0;
   double cfl_pts = 0.0;
   if (cfl_now) {
-    if (incr) {  // the mixed form holds increments in u: take the new FP64 state
+    if (incr || spec) {  // u holds increments or the substep's input: take the new FP64 state
       for (int f = 0; f < 3; ++f) to_physical<double>(h, h->cu[f].p, h->r[f].p, xspec<double>(h->tpool[f]));
       cfl_pts = cfl_min(h, h->r[0].p, h->r[1].p, h->r[2].p, false);
     } else {
@@ -2323,6 +2355,7 @@ void create_impl(dgpu_handle* h, const double* gy) {
   if (const char* e = std::getenv("DGPU_XPROD_THREADS")) xprod_threads = std::atoi(e);
   if (const char* e = std::getenv("DGPU_BATCH_MB")) batch_mb = std::atoi(e);
   if (const char* e = std::getenv("DGPU_THOMAS_COLS")) thomas_cols = std::atoi(e);
+  if (const char* e = std::getenv("DGPU_SPECTRAL_CN")) spectral_cn = std::atoi(e) != 0;
   if (const char* e = std::getenv("DGPU_LES_COLS")) les_cols = std::atoi(e);
   const int n = g.ny2;
   h->seg_L = seg_len_for(n);

@@ -470,6 +470,9 @@ __global__ void __launch_bounds__(512) k_xprod_all(ProdAllArgs a, Geo g,
 struct RhsArgs {
   const void* X[MAXSLOT];         // x-spectra of the products (transform precision)
   void* Tcr[3 + MAXTH];           // out: x-spectra of cr1, cr3, cr2, crth_n (slot order)
+  // out, instead of Tcr when set: cr itself (FP64 spectra, same slot order)
+  // for the CN solve in spectral space; no backward z transform is run
+  double2* crs[3 + MAXTH];
   const double2* cu1;             // state the explicit terms read
   const double2* cu2;
   const double2* cu3;
@@ -516,6 +519,7 @@ __global__ void __launch_bounds__(512) k_rhs(RhsArgs a, Geo g, const cpx<R>* __r
   const int j = j0 + blockIdx.y, q = blockIdx.x;
   const int L = fft_lds_len(g.nz);
   const int nslot = 6 + 2 * a.nth;
+  const bool spec = a.crs[0] != nullptr;
   const bool in1 = j >= a.js && j <= a.je;
   const bool in2 = j >= 2 && j <= g.ny;
   bool inT[MAXTH];
@@ -563,7 +567,12 @@ __global__ void __launch_bounds__(512) k_rhs(RhsArgs a, Geo g, const cpx<R>* __r
       const CR z = smem[(long long)s * L + pn];
       return invnz * C{(double)z.x, (double)z.y};
     };
-    auto put = [&](int s, C v) { smem[(long long)s * L + pn] = CR{(R)v.x, (R)v.y}; };
+    auto put = [&](int s, C v) {
+      if (spec)
+        reinterpret_cast<C*>(a.crs[s])[o] = v;
+      else
+        smem[(long long)s * L + pn] = CR{(R)v.x, (R)v.y};
+    };
     auto ld = [&](const double2* p) -> C { return reinterpret_cast<const C*>(p)[o]; };
     C cu1 = {0, 0}, cu3 = {0, 0};
     if (in1 || a.nth > 0) {
@@ -646,6 +655,7 @@ __global__ void __launch_bounds__(512) k_rhs(RhsArgs a, Geo g, const cpx<R>* __r
       put(3 + n, rt);
     }
   }
+  if (spec) return;  // cr stays spectral
   // zero the dealiased gap of the cr slots (fft.f90:336-340)
   const int ncr = 3 + a.nth;
   for (int c = 0; c < ncr; ++c)
@@ -705,6 +715,12 @@ struct SegRow {
 // assembled rows (A = I - E, vec = r + u + E u) give r + 2 E u, fixed rows
 // vec - (a u(j-1) + b u(j) + c u(j+1)).  u is overwritten with d.
 //
+// A column is any sequence of values a plane (pp) apart.  The physical solve
+// has one per (x, z) point; the spectral solve (ncol = 2 sp, nbc = 1) one per
+// real or imaginary part of a dealiased (kx, kz) mode, where only the mean
+// mode's real part carries the wall values.  ncol = 0 means every launched
+// column exists, nbc = 0 that every column takes the wall values.
+//
 // ADJ (the adjoint's transposed solve vbar = A^-T xbar, rows of kind 3): the
 // solution is not stored; instead rbar = vbar on the forward matrix's
 // assembled rows (0 elsewhere) and ubar = E^T vbar, with E u = wp (u(j+1) -
@@ -717,7 +733,8 @@ __global__ void __launch_bounds__(1024) k_thomas_seg(const T* __restrict__ r,
                                                      long long pp, int ca, int cb, double kscale,
                                                      const SegRow* __restrict__ prow = nullptr,
                                                      double* __restrict__ rbar = nullptr,
-                                                     double* __restrict__ ubar = nullptr) {
+                                                     double* __restrict__ ubar = nullptr,
+                                                     long long ncol = 0, long long nbc = 0) {
   typedef T VN __attribute__((ext_vector_type(C)));  // C columns, one streaming access
   __shared__ double carry_f[16][64 * C];
   __shared__ double carry_b[16][64 * C];
@@ -725,6 +742,10 @@ __global__ void __launch_bounds__(1024) k_thomas_seg(const T* __restrict__ r,
   // the row-table reads below become scalar loads
   const int lane = threadIdx.x, seg = __builtin_amdgcn_readfirstlane(threadIdx.y);
   const long long col = ((long long)blockIdx.x * 64 + lane) * C;
+  // lanes past the last column (the spectral solve's column count is not a
+  // multiple of 64 C) take part in the carries but touch no memory; C divides
+  // ncol, so a lane's columns are all in range or all out
+  const bool inb = ncol == 0 || col < ncol;
   const int s0 = seg * L;
   double v[L][C];
   double w[L + 2][C];
@@ -734,7 +755,7 @@ __global__ void __launch_bounds__(1024) k_thomas_seg(const T* __restrict__ r,
 #pragma unroll
   for (int t = 0; t < L + 2; ++t) {
     const int jj = s0 - 1 + t;
-    if (!ADJ && jj >= 0 && jj < n) {
+    if (!ADJ && inb && jj >= 0 && jj < n) {
       const VN x = __builtin_nontemporal_load(reinterpret_cast<const VN*>(u + (long long)jj * pp + col));
 #pragma unroll
       for (int c = 0; c < C; ++c) w[t][c] = (double)x[c];
@@ -750,7 +771,7 @@ __global__ void __launch_bounds__(1024) k_thomas_seg(const T* __restrict__ r,
     for (int c = 0; c < C; ++c) v[t][c] = 0.0;
     if (jj < n) {
       const SegRow rw = rows[jj];
-      if (rw.kind == 1 || rw.kind == 3) {
+      if ((rw.kind == 1 || rw.kind == 3) && inb) {
         const VN x = __builtin_nontemporal_load(reinterpret_cast<const VN*>(r + (long long)jj * pp + col));
 #pragma unroll
         for (int c = 0; c < C; ++c) {
@@ -761,10 +782,13 @@ __global__ void __launch_bounds__(1024) k_thomas_seg(const T* __restrict__ r,
           }
           v[t][c] = val;
         }
-      } else {
+      } else if (rw.kind == 0 || rw.kind == 2) {
 #pragma unroll
         for (int c = 0; c < C; ++c) {
-          double val = (rw.kind == 2) ? rw.vb * kscale : rw.vb;
+          // the wall values are uniform in x and z: every column of the
+          // physical solve, only the mean mode's real part (column 0) of the
+          // spectral one (nbc = 1)
+          double val = (nbc == 0 || col + c < nbc) ? ((rw.kind == 2) ? rw.vb * kscale : rw.vb) : 0.0;
           if (INCR) val -= rw.ra * w[t][c] + rw.rb * w[t + 1][c] + rw.rc * w[t + 2][c];
           v[t][c] = val;
         }
@@ -888,15 +912,17 @@ __global__ void __launch_bounds__(1024) k_thomas_seg(const T* __restrict__ r,
           rb[c] = (rj.kind == 1) ? x : 0.0;
         }
         const long long o = (long long)jj * pp + col;
-        __builtin_nontemporal_store(rb, reinterpret_cast<VD*>(rbar + o));
-        __builtin_nontemporal_store(ub, reinterpret_cast<VD*>(ubar + o));
+        if (inb) {
+          __builtin_nontemporal_store(rb, reinterpret_cast<VD*>(rbar + o));
+          __builtin_nontemporal_store(ub, reinterpret_cast<VD*>(ubar + o));
+        }
       }
     }
   } else {
 #pragma unroll
     for (int t = 0; t < L; ++t) {
       const int jj = s0 + t;
-      if (jj < n && jj >= ca && jj <= cb) {
+      if (inb && jj < n && jj >= ca && jj <= cb) {
         const double qf = rows[jj].qf;
         VN o;
 #pragma unroll

@@ -119,6 +119,28 @@ def test_variant_three_steps(name):
     check_against_floor(name, errs, sensitivity(p, gy, st, 3))
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_physical_space_cn_solves(precision, monkeypatch):
+    """DGPU_SPECTRAL_CN=0: the CN solves in physical space between the
+    transforms (the reference's arrangement) against the oracle, as the
+    default spectral solves are checked by every other test."""
+    monkeypatch.setenv("DGPU_SPECTRAL_CN", "0")
+    p, gy, ic, seed, amp = workloads.workload("small_32x33x32")
+    p.update(VARIANTS["neumann_upper"])
+    st = workloads.initial_state(p, gy, ic, seed, amp)
+    print(f"physical-space CN solves, {precision}", flush=True)
+    g = prepared(gpu_solver(p, gy, precision=precision), st)
+    o = prepared(oracle_solver(p, gy), st)
+    g.step(3)
+    o.step(3)
+    errs = field_errors(g.get_state(), o.get_state(), p)
+    if precision == "fp64":
+        check_against_floor("physical CN", errs, sensitivity(p, gy, st, 3))
+    else:
+        report("physical CN mixed", errs)
+        assert max(errs[k] for k in ("cu1", "cu2", "cu3", "cth")) < 1e-6
+
+
 def test_rectangular_box_and_ny():
     p = workloads.channel_params(64, 41, 16, 300.0, ri=0.2, dt=2e-3)
     gy = workloads.channel_grid(41)
