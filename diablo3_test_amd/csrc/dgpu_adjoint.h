@@ -427,7 +427,7 @@ __global__ void __launch_bounds__(512) k_adj_xpairs(AdjPairArgs a, Geo g, const 
 }
 
 // ------------------------------------------ products adjoint, gather ----
-// ubar(j) += sum of dP/du(j) Pbar over the products P at rows j-1, j, j+1
+// ubar(j) (+)= sum of dP/du(j) Pbar over the products P at rows j-1, j, j+1
 // (channel.f90:299-459, 557-608); Pbar is zero on rows the forward skipped.
 struct AdjProdArgs {
   const double* u1;
@@ -443,6 +443,7 @@ struct AdjProdArgs {
   const double* dy;
   const double* dyf;
   int nth;
+  int store;  // 1: ubar = the gathered sum (nothing else has written it); 0: ubar +=
 };
 
 __global__ void __launch_bounds__(256) k_adj_prod(AdjProdArgs a, Geo g) {
@@ -502,11 +503,17 @@ __global__ void __launch_bounds__(256) k_adj_prod(AdjProdArgs a, Geo g) {
     b2 += -cj * (tc + tm) * B(p4, o, true) + cm * (tc + tm) * B(p4, dn, hd);
     b3 += tc * A(p5, o, true);
     bt += u3c * A(p5, o, true);
-    a.ubth[s][o] += bt;
+    a.ubth[s][o] = a.store ? bt : a.ubth[s][o] + bt;
   }
-  a.ub1[o] += b1;
-  a.ub2[o] += b2;
-  a.ub3[o] += b3;
+  if (a.store) {
+    a.ub1[o] = b1;
+    a.ub2[o] = b2;
+    a.ub3[o] = b3;
+  } else {
+    a.ub1[o] += b1;
+    a.ub2[o] += b2;
+    a.ub3[o] += b3;
+  }
 }
 
 // The (0,0)-mode entries of a spectral field: a[j] (real part) over planes.

@@ -747,19 +747,30 @@ void launch_thomas(dgpu_handle* h, int field, int rs, const R* r, R* u, int ca, 
 }
 
 // The adjoint's transposed CN solve vbar = A^-T xbar (factor table rowsT) with
-// its epilogue: rbar = vbar on the assembled rows, ubar = E^T vbar.
-void launch_thomas_adj(dgpu_handle* h, int field, int rs, const double* xb, double* rbar, double* ubar) {
+// its epilogue: rbar = vbar on the assembled rows, ubar = E^T vbar.  Physical
+// columns (xb, rbar, ubar with plane pitch pp, all rows), or with spec the
+// spectral adjoints: xb == ubar is the state adjoint, solved on its planes
+// [ca, cb] and updated in place, rbar the adjoint of cr (2 sp real columns a
+// plane).
+void launch_thomas_adj(dgpu_handle* h, int field, int rs, const double* xb, double* rbar, double* ubar,
+                       bool spec = false, int ca = 0, int cb = -1) {
   const Geo& g = h->g;
   const int n = g.ny2;
   const size_t off = ((size_t)field * 3 + rs) * n;
   const SegRow* rows = h->rowsT.p + off;
   const SegRow* prow = h->rows.p + off;
-  const int C = (h->seg_L <= 32 && thomas_cols == 2 && g.pp % 128 == 0) ? 2 : 1;
-  dim3 grid((unsigned)(g.pp / (64 * C))), block(64, h->seg_S);
-  Timed tm(h, DGPU_K_THOMAS, 24.0 * g.pp * n);
+  const long long pitch = spec ? 2 * g.sp : g.pp;
+  const long long ncol = spec ? 2 * g.sp : 0;
+  if (!spec) {
+    ca = 0;
+    cb = n - 1;
+  }
+  const int C = (h->seg_L <= 32 && thomas_cols == 2 && (spec || g.pp % 128 == 0)) ? 2 : 1;
+  dim3 grid((unsigned)((pitch + 64 * C - 1) / (64 * C))), block(64, h->seg_S);
+  Timed tm(h, DGPU_K_THOMAS, 24.0 * pitch * n);
 #define TLAUNCH(L, CC)                                                                                     \
   hipLaunchKernelGGL((k_thomas_seg<L, CC, double, false, true>), grid, block, 0, h->stream, xb,             \
-                     (double*)nullptr, rows, n, h->seg_S, g.pp, 0, n - 1, 0.0, prow, rbar, ubar)
+                     (double*)nullptr, rows, n, h->seg_S, pitch, ca, cb, 0.0, prow, rbar, ubar, ncol, 0LL)
   switch (h->seg_L * 4 + C) {
     case 8 * 4 + 1: TLAUNCH(8, 1); break;
     case 16 * 4 + 1: TLAUNCH(16, 1); break;
@@ -776,6 +787,13 @@ void launch_thomas_adj(dgpu_handle* h, int field, int rs, const double* xb, doub
       throw ApiError{DGPU_EUNSUP, "unsupported Thomas segment length"};
   }
 #undef TLAUNCH
+  HIPCHECK(hipGetLastError());
+}
+
+void launch_herm0(dgpu_handle* h, double2* c, int ca, int cb) {
+  if (cb < ca) return;
+  hipLaunchKernelGGL(k_herm0, dim3((unsigned)((h->g.nkz + 64) / 64), (unsigned)(cb - ca + 1)), dim3(64), 0,
+                     h->stream, c, h->g, ca, cb);
   HIPCHECK(hipGetLastError());
 }
 
@@ -1425,14 +1443,19 @@ void rk_substep_t(dgpu_handle* h, int rk_step, double time, int time_step) {
   } else if (spec) {
     auto crs = [&](int c) { return reinterpret_cast<const double*>(h->tpool[nslot + c].p); };
     auto spc = [&](DevBuf<double2>& b) { return reinterpret_cast<double*>(b.p); };
-    for (int n = 0; n < nth; ++n)
+    for (int n = 0; n < nth; ++n) {
       launch_thomas<double>(h, 3 + n, rs, crs(3 + n), spc(h->cth[n]), h->jstart_th[n] - 1, h->jend_th[n] + 1,
                             transient, false, nullptr, true);
+      launch_herm0(h, h->cth[n].p, h->jstart_th[n] - 1, h->jend_th[n] + 1);
+    }
     launch_thomas<double>(h, 1, rs, crs(2), spc(h->cu[1]), 1, Ny + 1, transient, false, nullptr, true);
+    launch_herm0(h, h->cu[1].p, 1, Ny + 1);
     launch_thomas<double>(h, 0, rs, crs(0), spc(h->cu[0]), h->jstart - 1, h->jend + 1, transient, false, nullptr,
                           true);
+    launch_herm0(h, h->cu[0].p, h->jstart - 1, h->jend + 1);
     launch_thomas<double>(h, 2, rs, crs(1), spc(h->cu[2]), h->jstart - 1, h->jend + 1, transient, false, nullptr,
                           true);
+    launch_herm0(h, h->cu[2].p, h->jstart - 1, h->jend + 1);
   } else {
     for (int n = 0; n < nth; ++n)
       launch_thomas<R>(h, 3 + n, rs, phys<R>(h->rth[n]), phys<R>(h->th[n]), h->jstart_th[n] - 1,
@@ -1807,9 +1830,12 @@ void adj_substep(dgpu_handle* h, int rk_step, double time, int time_step) {
       launch(k_adj_remdiv_seg<8, 4, 512>, 8, 4);  // launch_remdiv has rejected ny beyond this
     HIPCHECK(hipGetLastError());
   }
+  // Without LES the forward solved on the spectra (spectral_cn): so does the
+  // transposed solve, in place on the state adjoint, writing crbar directly
+  const bool spec = spectral_cn && !p.use_les;
   // 2. the forward transform of the solved planes: xbar = F^T cubar there, and
   //    those planes of cubar are consumed (the other planes pass through)
-  for (int f = 0; f < nf; ++f) {
+  for (int f = 0; f < nf && !spec; ++f) {
     int ca, cb;
     solve_range(h, f, &ca, &cb);
     double* xb = (f < 3) ? h->r[f].p : h->rth[f - 3].p;
@@ -1847,11 +1873,19 @@ void adj_substep(dgpu_handle* h, int rk_step, double time, int time_step) {
     }
   }
   for (int f = 0; f < nf && !les; ++f) {
-    const double* xb = (f < 3) ? h->r[f].p : h->rth[f - 3].p;
-    launch_thomas_adj(h, f, rs, xb, h->a_rb[f].p, h->a_ub[f].p);
+    if (spec) {
+      int ca, cb;
+      solve_range(h, f, &ca, &cb);
+      launch_herm0(h, adj_spec(h, f), ca, cb);
+      double* lam = reinterpret_cast<double*>(adj_spec(h, f));
+      launch_thomas_adj(h, f, rs, lam, reinterpret_cast<double*>(h->a_cr[f].p), lam, true, ca, cb);
+    } else {
+      const double* xb = (f < 3) ? h->r[f].p : h->rth[f - 3].p;
+      launch_thomas_adj(h, f, rs, xb, h->a_rb[f].p, h->a_ub[f].p);
+    }
   }
   // 4. crbar = P^T rbar on planes 1..Ny (the inverse transform of cr)
-  for (int f = 0; f < nf; ++f) {
+  for (int f = 0; f < nf && !spec; ++f) {
     launch_xfwd<double>(h, h->a_rb[f].p, T0, 1, Ny);
     launch_zfwd<double>(h, T0, h->a_cr[f].p, 1, Ny, false, NxNz, 2.0 * NxNz);
   }
@@ -1966,7 +2000,8 @@ void adj_substep(dgpu_handle* h, int rk_step, double time, int time_step) {
     a.dy = h->d_dy.p;
     a.dyf = h->d_dyf.p;
     a.nth = nth;
-    Timed tm(h, DGPU_K_ADJOINT, 8.0 * npx * (3 + nth + 2 * (4 + 2 * nth) + 2 * (3 + nth)));
+    a.store = spec ? 1 : 0;  // the spectral transposed solve wrote no ubar
+    Timed tm(h, DGPU_K_ADJOINT, 8.0 * npx * (3 + nth + 2 * (4 + 2 * nth) + (spec ? 1 : 2) * (3 + nth)));
     hipLaunchKernelGGL(k_adj_prod, grid1((long long)npx), dim3(256), 0, h->stream, a, g);
     HIPCHECK(hipGetLastError());
   }

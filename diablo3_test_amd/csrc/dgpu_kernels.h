@@ -726,14 +726,21 @@ struct SegRow {
 // assembled rows (0 elsewhere) and ubar = E^T vbar, with E u = wp (u(j+1) -
 // u(j)) - wm (u(j) - u(j-1)) from the forward row table prow, go to rbar and
 // ubar for every row.
+//
+// The spectral transposed solve runs in place (r == ubar, the spectral state
+// adjoint): input rows outside [ca, cb] (the planes the forward did not
+// solve) are zero for the solve and pass through to ubar, which therefore
+// holds the adjoint state's old value there plus E^T vbar.  Each row of a
+// column is read, then written, by one lane, so r and ubar carry no
+// __restrict__.
 template <int L, int C, typename T, bool INCR, bool ADJ = false>
-__global__ void __launch_bounds__(1024) k_thomas_seg(const T* __restrict__ r,
+__global__ void __launch_bounds__(1024) k_thomas_seg(const T* r,
                                                      T* __restrict__ u,
                                                      const SegRow* __restrict__ rows, int n, int S,
                                                      long long pp, int ca, int cb, double kscale,
                                                      const SegRow* __restrict__ prow = nullptr,
                                                      double* __restrict__ rbar = nullptr,
-                                                     double* __restrict__ ubar = nullptr,
+                                                     double* ubar = nullptr,
                                                      long long ncol = 0, long long nbc = 0) {
   typedef T VN __attribute__((ext_vector_type(C)));  // C columns, one streaming access
   __shared__ double carry_f[16][64 * C];
@@ -779,6 +786,10 @@ __global__ void __launch_bounds__(1024) k_thomas_seg(const T* __restrict__ r,
           if (rw.kind == 1) {
             const double ex = rw.wp * (w[t + 2][c] - w[t + 1][c]) - rw.wm * (w[t + 1][c] - w[t][c]);
             val += INCR ? 2.0 * ex : ex;
+          }
+          if (ADJ && (jj < ca || jj > cb)) {  // passes through (w is free in ADJ)
+            w[t + 1][c] = val;
+            val = 0.0;
           }
           v[t][c] = val;
         }
@@ -908,7 +919,7 @@ __global__ void __launch_bounds__(1024) k_thomas_seg(const T* __restrict__ r,
           const double xp = (jj == s1) ? vp[c] : v[t + 1 < L ? t + 1 : t][c];
           double b = wpm * xm + wmp * xp;
           if (rj.kind == 1) b -= (rj.wp + rj.wm) * x;
-          ub[c] = b;
+          ub[c] = b + w[t + 1][c];
           rb[c] = (rj.kind == 1) ? x : 0.0;
         }
         const long long o = (long long)jj * pp + col;
@@ -930,6 +941,27 @@ __global__ void __launch_bounds__(1024) k_thomas_seg(const T* __restrict__ r,
         __builtin_nontemporal_store(o, reinterpret_cast<VN*>(u + (long long)jj * pp + col));
       }
     }
+  }
+}
+
+// The kx = 0 column of a spectral field on planes [ca, cb], projected as the
+// physical round trip F P projects it (FFTW's c2r reads the real part of the
+// kx = 0 line): c(kz) -> (c(kz) + conj c(-kz)) / 2, Im c(0) -> 0.  The spectral
+// CN solve applies it to its solution (the reference's solution is the
+// transform of a real field), the spectral transposed solve to its input; for
+// a state that already has the symmetry it changes nothing.
+__global__ void k_herm0(double2* __restrict__ c, Geo g, int ca, int cb) {
+  const int kk = blockIdx.x * blockDim.x + threadIdx.x;  // 0..nkz
+  const int j = ca + blockIdx.y;
+  if (kk > g.nkz || j > cb) return;
+  double2* row = c + (long long)j * g.sp;  // q = 0
+  if (kk == 0) {
+    row[0].y = 0.0;
+  } else {
+    const double2 a = row[kk], b = row[g.nkk - kk];
+    const double2 s = {0.5 * (a.x + b.x), 0.5 * (a.y - b.y)};
+    row[kk] = s;
+    row[g.nkk - kk] = double2{s.x, -s.y};
   }
 }
 
