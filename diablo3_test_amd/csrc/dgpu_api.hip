@@ -55,10 +55,13 @@ struct DevBuf {
   }
 };
 
-// Rows per segment of the CN solver: a multiple of 8 (the rows live in
-// registers), with at most 16 segments (1024 threads per block).  The target
-// length trades registers per thread against segments per column.
-int seg_target = 24;
+// Rows per segment of the CN solver: one of the instantiated lengths (the rows
+// live in registers), with at most 16 segments (1024 threads per block).  The
+// target length trades registers per thread against segments per column.  At
+// 512x257x512 20 rows (13 segments) take the spectral solves from 2.47 to 2.34
+// ms per step against 24 (11 segments): the 126-VGPR kernel runs 4 waves per
+// SIMD, so a CU holds one block, and 13 waves fill more of its 16 slots.
+int seg_target = 20;
 // Threads per workgroup of the x passes and of the products pass (tuning
 // knobs: DGPU_XPASS_THREADS, DGPU_XPROD_THREADS).
 int xpass_threads = 512;
@@ -79,12 +82,13 @@ int spectral_cn = 1;
 // launch per pass over all planes.
 int batch_mb = 0;
 int seg_len_for(int n) {
+  static const int lens[] = {8, 16, 20, 24, 28, 32, 40, 48, 56};  // the launch switches' cases
   int S = (n + seg_target - 1) / seg_target;
   if (S > 16) S = 16;
-  int L = (n + S - 1) / S;
-  L = (L + 7) / 8 * 8;
-  if (L > 56 || (n + L - 1) / L > 16) return -1;
-  return L;
+  const int L0 = (n + S - 1) / S;
+  for (int L : lens)
+    if (L >= L0) return (n + L - 1) / L > 16 ? -1 : L;
+  return -1;
 }
 
 }  // namespace
@@ -730,14 +734,18 @@ void launch_thomas(dgpu_handle* h, int field, int rs, const R* r, R* u, int ca, 
   switch (h->seg_L * 4 + C) {
     case 8 * 4 + 1: TLAUNCH(8, 1); break;
     case 16 * 4 + 1: TLAUNCH(16, 1); break;
+    case 20 * 4 + 1: TLAUNCH(20, 1); break;
     case 24 * 4 + 1: TLAUNCH(24, 1); break;
+    case 28 * 4 + 1: TLAUNCH(28, 1); break;
     case 32 * 4 + 1: TLAUNCH(32, 1); break;
     case 40 * 4 + 1: TLAUNCH(40, 1); break;
     case 48 * 4 + 1: TLAUNCH(48, 1); break;
     case 56 * 4 + 1: TLAUNCH(56, 1); break;
     case 8 * 4 + 2: TLAUNCH(8, 2); break;
     case 16 * 4 + 2: TLAUNCH(16, 2); break;
+    case 20 * 4 + 2: TLAUNCH(20, 2); break;
     case 24 * 4 + 2: TLAUNCH(24, 2); break;
+    case 28 * 4 + 2: TLAUNCH(28, 2); break;
     case 32 * 4 + 2: TLAUNCH(32, 2); break;
     default:
       throw ApiError{DGPU_EUNSUP, "unsupported Thomas segment length"};
@@ -774,14 +782,18 @@ void launch_thomas_adj(dgpu_handle* h, int field, int rs, const double* xb, doub
   switch (h->seg_L * 4 + C) {
     case 8 * 4 + 1: TLAUNCH(8, 1); break;
     case 16 * 4 + 1: TLAUNCH(16, 1); break;
+    case 20 * 4 + 1: TLAUNCH(20, 1); break;
     case 24 * 4 + 1: TLAUNCH(24, 1); break;
+    case 28 * 4 + 1: TLAUNCH(28, 1); break;
     case 32 * 4 + 1: TLAUNCH(32, 1); break;
     case 40 * 4 + 1: TLAUNCH(40, 1); break;
     case 48 * 4 + 1: TLAUNCH(48, 1); break;
     case 56 * 4 + 1: TLAUNCH(56, 1); break;
     case 8 * 4 + 2: TLAUNCH(8, 2); break;
     case 16 * 4 + 2: TLAUNCH(16, 2); break;
+    case 20 * 4 + 2: TLAUNCH(20, 2); break;
     case 24 * 4 + 2: TLAUNCH(24, 2); break;
+    case 28 * 4 + 2: TLAUNCH(28, 2); break;
     case 32 * 4 + 2: TLAUNCH(32, 2); break;
     default:
       throw ApiError{DGPU_EUNSUP, "unsupported Thomas segment length"};
