@@ -715,7 +715,7 @@ void launch_thomas(dgpu_handle* h, int field, int rs, const R* r, R* u, int ca, 
   const int n = g.ny2;
   const SegRow* rows = (table ? table : h->rows.p) + ((size_t)field * 3 + rs) * n;
   const long long pitch = spec ? 2 * g.sp : g.pp;
-  const long long ncol = spec ? 2 * g.sp : 0, nbc = spec ? 1 : 0;
+  const long long ncol = pitch, nbc = spec ? 1 : 0;  // lanes past ncol (a partial last tile) touch no memory
   // columns per lane: thomas_cols for segments of up to 32 rows (register budget), else 1.
   // The wider row runs (64 C values) pay for the lower occupancy: at 512x257x512 the FP64
   // solve takes 4.34 ms per step with two columns and 4.70 with one.
@@ -768,7 +768,7 @@ void launch_thomas_adj(dgpu_handle* h, int field, int rs, const double* xb, doub
   const SegRow* rows = h->rowsT.p + off;
   const SegRow* prow = h->rows.p + off;
   const long long pitch = spec ? 2 * g.sp : g.pp;
-  const long long ncol = spec ? 2 * g.sp : 0;
+  const long long ncol = pitch;
   if (!spec) {
     ca = 0;
     cb = n - 1;

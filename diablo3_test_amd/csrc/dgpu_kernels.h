@@ -719,7 +719,8 @@ struct SegRow {
 // has one per (x, z) point; the spectral solve (ncol = 2 sp, nbc = 1) one per
 // real or imaginary part of a dealiased (kx, kz) mode, where only the mean
 // mode's real part carries the wall values.  ncol = 0 means every launched
-// column exists, nbc = 0 that every column takes the wall values.
+// column exists (the launchers always pass the column count), nbc = 0 that
+// every column takes the wall values.
 //
 // ADJ (the adjoint's transposed solve vbar = A^-T xbar, rows of kind 3): the
 // solution is not stored; instead rbar = vbar on the forward matrix's
