@@ -710,26 +710,29 @@ void to_fourier(dgpu_handle* h, const R* u, double2* C, cpx<R>* T, int j0, int j
 // spectral state, FP64, 2 sp real columns a plane).
 template <typename R>
 void launch_thomas(dgpu_handle* h, int field, int rs, const R* r, R* u, int ca, int cb, double transient,
-                   bool incr, const SegRow* table = nullptr, bool spec = false) {
+                   bool incr, const SegRow* table = nullptr, bool spec = false,
+                   const CnFields<R>* batch = nullptr) {
   const Geo& g = h->g;
   const int n = g.ny2;
   const SegRow* rows = (table ? table : h->rows.p) + ((size_t)field * 3 + rs) * n;
+  const CnFields<R> mf = batch ? *batch : CnFields<R>{};
+  const int nf = batch ? batch->nf : 1;
   const long long pitch = spec ? 2 * g.sp : g.pp;
   const long long ncol = pitch, nbc = spec ? 1 : 0;  // lanes past ncol (a partial last tile) touch no memory
   // columns per lane: thomas_cols for segments of up to 32 rows (register budget), else 1.
   // The wider row runs (64 C values) pay for the lower occupancy: at 512x257x512 the FP64
   // solve takes 4.34 ms per step with two columns and 4.70 with one.
   const int C = (h->seg_L <= 32 && thomas_cols == 2 && (spec || g.pp % 128 == 0)) ? 2 : 1;
-  dim3 grid((unsigned)((pitch + 64 * C - 1) / (64 * C))), block(64, h->seg_S);
-  Timed tm(h, DGPU_K_THOMAS, 3.0 * sizeof(R) * pitch * n);
+  dim3 grid((unsigned)((pitch + 64 * C - 1) / (64 * C)), (unsigned)nf), block(64, h->seg_S);
+  Timed tm(h, DGPU_K_THOMAS, 3.0 * sizeof(R) * pitch * n * nf);
 #define TLAUNCH(L, CC)                                                                                     \
   do {                                                                                                     \
     if (incr)                                                                                              \
       hipLaunchKernelGGL((k_thomas_seg<L, CC, R, true>), grid, block, 0, h->stream, r, u, rows, n,         \
-                         h->seg_S, pitch, ca, cb, transient, nullptr, nullptr, nullptr, ncol, nbc);       \
+                         h->seg_S, pitch, ca, cb, transient, nullptr, nullptr, nullptr, ncol, nbc, mf);   \
     else                                                                                                   \
       hipLaunchKernelGGL((k_thomas_seg<L, CC, R, false>), grid, block, 0, h->stream, r, u, rows, n,        \
-                         h->seg_S, pitch, ca, cb, transient, nullptr, nullptr, nullptr, ncol, nbc);       \
+                         h->seg_S, pitch, ca, cb, transient, nullptr, nullptr, nullptr, ncol, nbc, mf);   \
   } while (0)
   switch (h->seg_L * 4 + C) {
     case 8 * 4 + 1: TLAUNCH(8, 1); break;
@@ -1453,21 +1456,22 @@ void rk_substep_t(dgpu_handle* h, int rk_step, double time, int time_step) {
     launch_thomas_les<R>(h, 2, rs, phys<R>(h->r[2]), phys<R>(h->u[2]), h->nu_t.p, h->jstart - 1, h->jend + 1,
                          transient, incr);
   } else if (spec) {
-    auto crs = [&](int c) { return reinterpret_cast<const double*>(h->tpool[nslot + c].p); };
-    auto spc = [&](DevBuf<double2>& b) { return reinterpret_cast<double*>(b.p); };
-    for (int n = 0; n < nth; ++n) {
-      launch_thomas<double>(h, 3 + n, rs, crs(3 + n), spc(h->cth[n]), h->jstart_th[n] - 1, h->jend_th[n] + 1,
-                            transient, false, nullptr, true);
-      launch_herm0(h, h->cth[n].p, h->jstart_th[n] - 1, h->jend_th[n] + 1);
-    }
-    launch_thomas<double>(h, 1, rs, crs(2), spc(h->cu[1]), 1, Ny + 1, transient, false, nullptr, true);
-    launch_herm0(h, h->cu[1].p, 1, Ny + 1);
-    launch_thomas<double>(h, 0, rs, crs(0), spc(h->cu[0]), h->jstart - 1, h->jend + 1, transient, false, nullptr,
-                          true);
-    launch_herm0(h, h->cu[0].p, h->jstart - 1, h->jend + 1);
-    launch_thomas<double>(h, 2, rs, crs(1), spc(h->cu[2]), h->jstart - 1, h->jend + 1, transient, false, nullptr,
-                          true);
-    launch_herm0(h, h->cu[2].p, h->jstart - 1, h->jend + 1);
+    // every field's solve in one launch (cr slot order: 0 cr1, 1 cr3, 2 cr2, 3+n crth_n)
+    CnFields<double> mf{};
+    auto add = [&](int field, int slot, DevBuf<double2>& state, int ca, int cb) {
+      const int k = mf.nf++;
+      mf.r[k] = reinterpret_cast<const double*>(h->tpool[nslot + slot].p);
+      mf.u[k] = reinterpret_cast<double*>(state.p);
+      mf.rows[k] = h->rows.p + ((size_t)field * 3 + rs) * g.ny2;
+      mf.ca[k] = ca;
+      mf.cb[k] = cb;
+    };
+    add(0, 0, h->cu[0], h->jstart - 1, h->jend + 1);
+    add(1, 2, h->cu[1], 1, Ny + 1);
+    add(2, 1, h->cu[2], h->jstart - 1, h->jend + 1);
+    for (int n = 0; n < nth; ++n) add(3 + n, 3 + n, h->cth[n], h->jstart_th[n] - 1, h->jend_th[n] + 1);
+    launch_thomas<double>(h, 0, rs, mf.r[0], mf.u[0], mf.ca[0], mf.cb[0], transient, false, nullptr, true, &mf);
+    for (int k = 0; k < mf.nf; ++k) launch_herm0(h, reinterpret_cast<double2*>(mf.u[k]), mf.ca[k], mf.cb[k]);
   } else {
     for (int n = 0; n < nth; ++n)
       launch_thomas<R>(h, 3 + n, rs, phys<R>(h->rth[n]), phys<R>(h->th[n]), h->jstart_th[n] - 1,

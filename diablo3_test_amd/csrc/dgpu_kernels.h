@@ -734,6 +734,20 @@ struct SegRow {
 // holds the adjoint state's old value there plus E^T vbar.  Each row of a
 // column is read, then written, by one lane, so r and ubar carry no
 // __restrict__.
+//
+// CnFields batches the solves of several fields into one launch (blockIdx.y
+// = field; nf = 0: the single field of the plain arguments): one launch
+// keeps the CUs busy through the last blocks of each field.
+constexpr int CN_MAXF = 3 + MAXTH;
+template <typename T>
+struct CnFields {
+  const T* r[CN_MAXF];
+  T* u[CN_MAXF];
+  const SegRow* rows[CN_MAXF];
+  int ca[CN_MAXF], cb[CN_MAXF];
+  int nf;
+};
+
 template <int L, int C, typename T, bool INCR, bool ADJ = false>
 __global__ void __launch_bounds__(1024) k_thomas_seg(const T* r,
                                                      T* __restrict__ u,
@@ -742,8 +756,17 @@ __global__ void __launch_bounds__(1024) k_thomas_seg(const T* r,
                                                      const SegRow* __restrict__ prow = nullptr,
                                                      double* __restrict__ rbar = nullptr,
                                                      double* ubar = nullptr,
-                                                     long long ncol = 0, long long nbc = 0) {
+                                                     long long ncol = 0, long long nbc = 0,
+                                                     CnFields<T> mf = CnFields<T>{}) {
   typedef T VN __attribute__((ext_vector_type(C)));  // C columns, one streaming access
+  if (mf.nf > 0) {
+    const int f = blockIdx.y;
+    r = mf.r[f];
+    u = mf.u[f];
+    rows = mf.rows[f];
+    ca = mf.ca[f];
+    cb = mf.cb[f];
+  }
   __shared__ double carry_f[16][64 * C];
   __shared__ double carry_b[16][64 * C];
   // blockDim.x == 64: a wavefront is one segment, so seg is wave-uniform and
