@@ -146,6 +146,24 @@ def test_variant_three_steps(name):
     check_against_floor(name, errs, sensitivity(p, gy, st, 3))
 
 
+def test_three_scalars():
+    """Three scalars with their own Pr, wall conditions and background
+    gradients: the product slots, the RHS and the batched CN launch (six
+    fields) against the oracle over 3 steps."""
+    p = workloads.channel_params(32, 33, 32, 300.0, ri=0.2, dt=2e-3, n_th=3)
+    p.update(pr=[1.0, 0.7, 2.0], th_bc_ymin=[0, 1, 0], th_bc_ymax=[1, 0, 0],
+             th_bc_ymin_c1=[-1.0, 0.2, 0.0], th_bc_ymax_c1=[0.1, 1.0, 0.5],
+             dthdx=[0.0, 0.1, -0.05], dthdz=[0.02, 0.0, 0.1], grav_x=0.1, grav_z=-0.05)
+    gy = workloads.channel_grid(33)
+    st = workloads.initial_state(p, gy, "white", 9, 2e-2)
+    print("three scalars 32x33x32, 3 steps", flush=True)
+    g, o = pair(p, gy, st)
+    g.step(3)
+    o.step(3)
+    errs = field_errors(g.get_state(), o.get_state(), p)
+    check_against_floor("three scalars", errs, sensitivity(p, gy, st, 3))
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_physical_space_cn_solves(precision, monkeypatch):
     """DGPU_SPECTRAL_CN=0: the CN solves in physical space between the
