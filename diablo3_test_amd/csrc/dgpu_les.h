@@ -533,9 +533,12 @@ struct TriRow {
 };
 
 // T is the storage type of r and u; the sweeps run in FP64 with b' and g' in
-// FP64 scratch.  incr has the meaning of k_thomas_seg's flag.  Each lane
-// solves C adjacent columns (one C-wide access per row and array), so a
-// wavefront's row accesses are 64 C values long.
+// FP64 scratch.  incr has the meaning of k_thomas_seg's flag: the solve is for
+// d = x - u, with the right-hand side vec - A u formed in the forward sweep
+// from the u(j-1), u(j), u(j+1) it holds anyway (reading u back in the
+// backward sweep would put a load behind every store of the serial sweep).
+// Each lane solves C adjacent columns (one C-wide access per row and array),
+// so a wavefront's row accesses are 64 C values long.
 template <typename T, int C>
 __global__ void __launch_bounds__(256) k_thomas_les(const T* __restrict__ r, T* __restrict__ u,
                                                     const double* __restrict__ nt, double* __restrict__ sb,
@@ -574,6 +577,7 @@ __global__ void __launch_bounds__(256) k_thomas_les(const T* __restrict__ r, T* 
     } else {
       vec = (VD)((rw.kind == 2) ? rw.vb * kscale : rw.vb);
     }
+    if (incr) vec -= a * um + b * uc + c * up;
   };
   VD um = zero, uc = U(0), up = (n > 1) ? U(1) : zero;
   VD a, b, c, vec;
@@ -595,10 +599,7 @@ __global__ void __launch_bounds__(256) k_thomas_les(const T* __restrict__ r, T* 
   }
   // backward: x(n-1) = g'/b', x(j) = (g'(j) - c(j) x(j+1)) / b'(j)
   VD x = gp / bp;
-  auto put = [&](int j, VD v) {
-    VT* p = reinterpret_cast<VT*>(u + at(j));
-    *p = __builtin_convertvector(v - (incr ? __builtin_convertvector(*p, VD) : zero), VT);
-  };
+  auto put = [&](int j, VD v) { *reinterpret_cast<VT*>(u + at(j)) = __builtin_convertvector(v, VT); };
   if (n - 1 >= ca && n - 1 <= cb) put(n - 1, x);
   for (int j = n - 2; j >= ca; --j) {
     const TriRow& rw = rows[j];
