@@ -594,16 +594,18 @@ void launch_xfwd(dgpu_handle* h, const R* u, cpx<R>* T, int j0, int j1) {
 }
 
 template <typename R>
-void launch_xinv(dgpu_handle* h, const cpx<R>* T, R* u, int j0, int j1, const double* dkx = nullptr) {
+void launch_xinv(dgpu_handle* h, const cpx<R>* T, R* u, int j0, int j1, const double* dkx = nullptr,
+                 const FieldBatch* fb = nullptr) {
   if (j1 < j0) return;
   const Geo& g = h->g;
   constexpr int KT = XTile<R>::KT;
   const int ng = x_ngroups<R>(h, KT / 2, xpass_threads);
   const size_t lds = x_lds<R>(h, ng);
-  dim3 grid(g.nz / KT, j1 - j0 + 1), block(ng * (g.nx / 8));
-  Timed tm(h, DGPU_K_XINV, (sizeof(R) * g.pp + 2.0 * sizeof(R) * g.tp) * (j1 - j0 + 1));
+  const int nf = fb ? fb->nf : 1;
+  dim3 grid(g.nz / KT, j1 - j0 + 1, nf), block(ng * (g.nx / 8));
+  Timed tm(h, DGPU_K_XINV, (sizeof(R) * g.pp + 2.0 * sizeof(R) * g.tp) * (j1 - j0 + 1) * nf);
   hipLaunchKernelGGL((g.nx <= 512 ? k_xinv<R, true> : k_xinv<R, false>), grid, block, lds, h->stream, T, u, g,
-                     twx_of<R>(h), j0, dkx);
+                     twx_of<R>(h), j0, dkx, fb ? *fb : FieldBatch{});
   HIPCHECK(hipGetLastError());
 }
 
@@ -623,15 +625,17 @@ void launch_zfwd(dgpu_handle* h, const cpx<R>* T, double2* C, int j0, int j1, bo
 
 template <typename R>
 void launch_zinv(dgpu_handle* h, const double2* C, cpx<R>* T, int j0, int j1, double s0 = 1.0,
-                 double s1 = 1.0) {
+                 double s1 = 1.0, const FieldBatch* fb = nullptr) {
   if (j1 < j0) return;
   const Geo& g = h->g;
   const int ng = z_ngroups<R>(h);
   const size_t lds = z_lds<R>(h, ng);
-  dim3 grid((g.nq + ng - 1) / ng, j1 - j0 + 1), block(ng * (g.nz / 8));
-  Timed tm(h, DGPU_K_ZINV, (2.0 * sizeof(R) * g.tp + 16.0 * g.sp) * (j1 - j0 + 1));
+  const int nf = fb ? fb->nf : 1;
+  dim3 grid((g.nq + ng - 1) / ng, j1 - j0 + 1, nf), block(ng * (g.nz / 8));
+  Timed tm(h, DGPU_K_ZINV, (2.0 * sizeof(R) * g.tp + 16.0 * g.sp) * (j1 - j0 + 1) * nf);
   hipLaunchKernelGGL((g.nz <= 512 ? k_zinv<R, true> : k_zinv<R, false>), grid, block, lds, h->stream,
-                     reinterpret_cast<const cpx<double>*>(C), T, g, twz_of<R>(h), j0, s0, s1);
+                     reinterpret_cast<const cpx<double>*>(C), T, g, twz_of<R>(h), j0, s0, s1,
+                     fb ? *fb : FieldBatch{});
   HIPCHECK(hipGetLastError());
 }
 
@@ -691,6 +695,27 @@ void to_physical(dgpu_handle* h, const double2* C, R* u, cpx<R>* T) {
     launch_zinv<R>(h, C, T, j0, j1);
     launch_xinv<R>(h, T, u, j0, j1);
   }
+}
+
+// spectral -> physical for several fields (C[k] -> u[k] through the x-spectra
+// T[k]): every field's z pass in one launch, then every field's x pass.
+template <typename R>
+void to_physical_fields(dgpu_handle* h, int nf, const double2* const* C, R* const* u, cpx<R>* const* T) {
+  if (batch_planes<R>(h) < h->g.ny2 || nf > MAXBF) {  // plane batches: field by field
+    for (int k = 0; k < nf; ++k) to_physical<R>(h, C[k], u[k], T[k]);
+    return;
+  }
+  FieldBatch zb{}, xb{};
+  for (int k = 0; k < nf; ++k) {
+    zb.in[k] = C[k];
+    zb.out[k] = T[k];
+    xb.in[k] = T[k];
+    xb.out[k] = u[k];
+  }
+  zb.nf = xb.nf = nf;
+  const int n2 = h->g.ny2;
+  launch_zinv<R>(h, C[0], T[0], 0, n2 - 1, 1.0, 1.0, &zb);
+  launch_xinv<R>(h, T[0], u[0], 0, n2 - 1, nullptr, &xb);
 }
 
 // physical -> spectral for one field on planes [j0, j1] (fft_xz_to_fourier,
@@ -1316,9 +1341,15 @@ void rk_substep_t(dgpu_handle* h, int rk_step, double time, int time_step) {
     }
   } else {
     // 1. state to physical space (channel.f90:279-287)
-    for (int f = 0; f < 3; ++f) to_physical<R>(h, h->cu[f].p, phys<R>(h->u[f]), xspec<R>(h->tpool[f]));
-    for (int n = 0; n < nth; ++n)
-      to_physical<R>(h, h->cth[n].p, phys<R>(h->th[n]), xspec<R>(h->tpool[3 + n]));
+    const double2* C[3 + DGPU_MAX_TH];
+    R* U[3 + DGPU_MAX_TH];
+    cpx<R>* T[3 + DGPU_MAX_TH];
+    for (int f = 0; f < 3 + nth; ++f) {
+      C[f] = f < 3 ? h->cu[f].p : h->cth[f - 3].p;
+      U[f] = f < 3 ? phys<R>(h->u[f]) : phys<R>(h->th[f - 3]);
+      T[f] = xspec<R>(h->tpool[f]);
+    }
+    to_physical_fields<R>(h, 3 + nth, C, U, T);
   }
 
   // 2. products, x-transformed (channel.f90:299-459, 557-608)

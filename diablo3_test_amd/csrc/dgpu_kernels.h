@@ -47,6 +47,16 @@ __device__ __forceinline__ int kz_nat(const Geo& g, int kk) {
   return kk <= g.nkz ? kk : g.nz - 2 * g.nkz - 1 + kk;
 }
 
+// Several fields through one transform pass in one launch: blockIdx.z picks
+// the field's input and output (nf = 0: the single field of the plain
+// arguments).  One launch keeps the CUs busy through each field's last blocks.
+constexpr int MAXBF = 8;
+struct FieldBatch {
+  const void* in[MAXBF];
+  void* out[MAXBF];
+  int nf;
+};
+
 // ---------------------------------------------------------------- x pass --
 // Real-pair post-processing: z = FFT(a + i b) -> A = FFT(a), B = FFT(b) at q.
 template <typename R>
@@ -106,7 +116,12 @@ __global__ void __launch_bounds__(512) k_xfwd(const R* __restrict__ u, cpx<R>* _
 template <typename R, bool WAVE>
 __global__ void __launch_bounds__(512) k_xinv(const cpx<R>* __restrict__ T, R* __restrict__ u,
                                                Geo g, const cpx<R>* __restrict__ tw, int j0,
-                                               const double* __restrict__ dkx = nullptr) {
+                                               const double* __restrict__ dkx = nullptr,
+                                               FieldBatch fb = FieldBatch{}) {
+  if (fb.nf > 0) {
+    T = reinterpret_cast<const cpx<R>*>(fb.in[blockIdx.z]);
+    u = reinterpret_cast<R*>(fb.out[blockIdx.z]);
+  }
   constexpr int KT = XTile<R>::KT;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<R>* smem = reinterpret_cast<cpx<R>*>(smem_raw);
@@ -209,7 +224,11 @@ __global__ void __launch_bounds__(512) k_zfwd(const cpx<R>* __restrict__ T, cpx<
 template <typename R, bool WAVE>
 __global__ void __launch_bounds__(512) k_zinv(const cpx<double>* __restrict__ C, cpx<R>* __restrict__ T,
                                                Geo g, const cpx<R>* __restrict__ tw, int j0, double s0,
-                                               double s1) {
+                                               double s1, FieldBatch fb = FieldBatch{}) {
+  if (fb.nf > 0) {
+    C = reinterpret_cast<const cpx<double>*>(fb.in[blockIdx.z]);
+    T = reinterpret_cast<cpx<R>*>(fb.out[blockIdx.z]);
+  }
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<R>* smem = reinterpret_cast<cpx<R>*>(smem_raw);
   const int G = g.nz >> 3;
