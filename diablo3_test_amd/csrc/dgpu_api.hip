@@ -984,8 +984,17 @@ void les_stage(dgpu_handle* h, double time) {
                {cu2, OP_DZ}, {cu3, OP_DX},    {cu3, OP_DY_GY}, {cu3, OP_DZ}};
   // the state itself (les.f90:586-588, 1207-1209, 1229-1231); its x-spectra
   // stay in tpool[f] (all planes, without plane batches) for the x derivatives
-  for (int f = 0; f < 3; ++f) to_physical<R>(h, h->cu[f].p, phys<R>(h->u[f]), xspec<R>(h->tpool[f]));
-  for (int n = 0; n < nth; ++n) to_physical<R>(h, h->cth[n].p, phys<R>(h->th[n]), xspec<R>(h->tpool[3 + n]));
+  {
+    const double2* C[3 + DGPU_MAX_TH];
+    R* U[3 + DGPU_MAX_TH];
+    cpx<R>* T[3 + DGPU_MAX_TH];
+    for (int f = 0; f < 3 + nth; ++f) {
+      C[f] = f < 3 ? h->cu[f].p : h->cth[f - 3].p;
+      U[f] = f < 3 ? phys<R>(h->u[f]) : phys<R>(h->th[f - 3]);
+      T[f] = xspec<R>(h->tpool[f]);
+    }
+    to_physical_fields<R>(h, 3 + nth, C, U, T);
+  }
   // x derivatives: i kx times the state's x-spectra, then the x inverse; z
   // derivatives: i kz inside the z inverse; the y differences (G12, G22,
   // G32, the scalars' T2) are formed from the physical state in k_les_model
@@ -2000,10 +2009,16 @@ void adj_substep(dgpu_handle* h, int rk_step, double time, int time_step) {
   }
   // 7. the forward fields at the substep's input (with LES: after its wall
   //    values, rebuilt by les_adj_recompute)
-  if (!les) {
-    for (int f = 0; f < 3; ++f) to_physical<double>(h, h->cu[f].p, h->u[f].p, xspec<double>(h->tpool[nslot]));
-    for (int n = 0; n < nth; ++n)
-      to_physical<double>(h, h->cth[n].p, h->th[n].p, xspec<double>(h->tpool[nslot]));
+  if (!les) {  // each field through its own idle cr x-spectra buffer, all in one launch per pass
+    const double2* C[3 + DGPU_MAX_TH];
+    double* U[3 + DGPU_MAX_TH];
+    cpx<double>* T[3 + DGPU_MAX_TH];
+    for (int f = 0; f < nf; ++f) {
+      C[f] = f < 3 ? h->cu[f].p : h->cth[f - 3].p;
+      U[f] = f < 3 ? h->u[f].p : h->th[f - 3].p;
+      T[f] = xspec<double>(h->tpool[nslot + f]);
+    }
+    to_physical_fields<double>(h, nf, C, U, T);
   }
   // 8. product adjoints (x pass) and their Jacobians
   {
