@@ -657,22 +657,24 @@ void launch_zinv_op(dgpu_handle* h, const double2* C, cpx<R>* T, int j0, int j1,
 // The CN solve when the rows depend on nu_t / kappa_t per column.
 template <typename R>
 void launch_thomas_les(dgpu_handle* h, int field, int rs, const R* r, R* u, const double* nt, int ca,
-                       int cb, double transient, bool incr) {
+                       int cb, double transient, bool incr, const LesCnFields<R>* batch = nullptr) {
   const Geo& g = h->g;
   const int n = g.ny2;
   const TriRow* rows = h->trirows.p + ((size_t)field * 3 + rs) * n;
+  const LesCnFields<R> mf = batch ? *batch : LesCnFields<R>{};
+  const int nf = batch ? batch->nf : 1;
   // reads r, u, nu_t; writes b' and g' and reads them back; writes u.  Two
   // columns per lane (one 2-wide access per row and array) when the plane
   // allows it.
-  Timed tm(h, DGPU_K_THOMAS, (3.0 * sizeof(R) + 8.0 + 32.0) * g.pp * n);
+  Timed tm(h, DGPU_K_THOMAS, (3.0 * sizeof(R) + 8.0 + 32.0) * g.pp * n * nf);
   if (les_cols == 2 && g.pp % 2 == 0) {
-    dim3 grid((unsigned)((g.pp / 2 + 255) / 256)), block(256);
+    dim3 grid((unsigned)((g.pp / 2 + 255) / 256), (unsigned)nf), block(256);
     hipLaunchKernelGGL((k_thomas_les<R, 2>), grid, block, 0, h->stream, r, u, nt, h->sbl.p, h->sgl.p, rows, n,
-                       g.pp, ca, cb, transient, incr ? 1 : 0);
+                       g.pp, ca, cb, transient, incr ? 1 : 0, mf);
   } else {
-    dim3 grid((unsigned)((g.pp + 255) / 256)), block(256);
+    dim3 grid((unsigned)((g.pp + 255) / 256), (unsigned)nf), block(256);
     hipLaunchKernelGGL((k_thomas_les<R, 1>), grid, block, 0, h->stream, r, u, nt, h->sbl.p, h->sgl.p, rows, n,
-                       g.pp, ca, cb, transient, incr ? 1 : 0);
+                       g.pp, ca, cb, transient, incr ? 1 : 0, mf);
   }
   HIPCHECK(hipGetLastError());
 }
@@ -1486,15 +1488,24 @@ void rk_substep_t(dgpu_handle* h, int rk_step, double time, int time_step) {
     h->w_transient = transient;
   }
   if (p.use_les) {
-    // with use_LES the eddy viscosity enters every CN row (channel.f90:519-547, 636-689)
+    // with use_LES the eddy viscosity enters every CN row (channel.f90:519-547, 636-689);
+    // every field's solve in one launch
+    LesCnFields<R> mf{};
+    auto add = [&](int field, DevBuf<double>& rb, DevBuf<double>& ub, const double* nt, int ca, int cb) {
+      const int k = mf.nf++;
+      mf.r[k] = phys<R>(rb);
+      mf.u[k] = phys<R>(ub);
+      mf.nt[k] = nt;
+      mf.rows[k] = h->trirows.p + ((size_t)field * 3 + rs) * g.ny2;
+      mf.ca[k] = ca;
+      mf.cb[k] = cb;
+    };
     for (int n = 0; n < nth; ++n)
-      launch_thomas_les<R>(h, 3 + n, rs, phys<R>(h->rth[n]), phys<R>(h->th[n]), h->kappa_t[n].p,
-                           h->jstart_th[n] - 1, h->jend_th[n] + 1, transient, incr);
-    launch_thomas_les<R>(h, 1, rs, phys<R>(h->r[1]), phys<R>(h->u[1]), h->nu_t.p, 1, Ny + 1, transient, incr);
-    launch_thomas_les<R>(h, 0, rs, phys<R>(h->r[0]), phys<R>(h->u[0]), h->nu_t.p, h->jstart - 1, h->jend + 1,
-                         transient, incr);
-    launch_thomas_les<R>(h, 2, rs, phys<R>(h->r[2]), phys<R>(h->u[2]), h->nu_t.p, h->jstart - 1, h->jend + 1,
-                         transient, incr);
+      add(3 + n, h->rth[n], h->th[n], h->kappa_t[n].p, h->jstart_th[n] - 1, h->jend_th[n] + 1);
+    add(1, h->r[1], h->u[1], h->nu_t.p, 1, Ny + 1);
+    add(0, h->r[0], h->u[0], h->nu_t.p, h->jstart - 1, h->jend + 1);
+    add(2, h->r[2], h->u[2], h->nu_t.p, h->jstart - 1, h->jend + 1);
+    launch_thomas_les<R>(h, 0, rs, mf.r[0], mf.u[0], mf.nt[0], mf.ca[0], mf.cb[0], transient, incr, &mf);
   } else if (spec) {
     // every field's solve in one launch (cr slot order: 0 cr1, 1 cr3, 2 cr2, 3+n crth_n)
     CnFields<double> mf{};
@@ -2479,8 +2490,10 @@ void create_impl(dgpu_handle* h, const double* gy) {
       h->lesfth[nn].alloc(ns);
       h->cth_pre[nn].alloc(ns);
     }
-    h->sbl.alloc(np);
-    h->sgl.alloc(np);
+    // b', g' of the LES CN solves, one region per field (the fields of a
+    // substep are solved in one launch)
+    h->sbl.alloc(np * (3 + h->nth));
+    h->sgl.alloc(np * (3 + h->nth));
   }
   HIPCHECK(hipDeviceSynchronize());
 }

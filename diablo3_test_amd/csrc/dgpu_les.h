@@ -539,12 +539,39 @@ struct TriRow {
 // backward sweep would put a load behind every store of the serial sweep).
 // Each lane solves C adjacent columns (one C-wide access per row and array),
 // so a wavefront's row accesses are 64 C values long.
+//
+// LesCnFields batches the solves of several fields into one launch
+// (blockIdx.y = field, each with its own b', g' scratch of n pp values; nf = 0:
+// the single field of the plain arguments).  A solve is one sequential sweep
+// per lane, so one field fills only a few waves per CU; the batch gives the
+// CUs the other fields' sweeps to hide their latency behind.
+template <typename T>
+struct LesCnFields {
+  const T* r[3 + MAXTH];
+  T* u[3 + MAXTH];
+  const double* nt[3 + MAXTH];
+  const TriRow* rows[3 + MAXTH];
+  int ca[3 + MAXTH], cb[3 + MAXTH];
+  int nf;
+};
+
 template <typename T, int C>
 __global__ void __launch_bounds__(256) k_thomas_les(const T* __restrict__ r, T* __restrict__ u,
                                                     const double* __restrict__ nt, double* __restrict__ sb,
                                                     double* __restrict__ sg, const TriRow* __restrict__ rows,
                                                     int n, long long pp, int ca, int cb, double kscale,
-                                                    int incr) {
+                                                    int incr, LesCnFields<T> mf = LesCnFields<T>{}) {
+  if (mf.nf > 0) {
+    const int f = blockIdx.y;
+    r = mf.r[f];
+    u = mf.u[f];
+    nt = mf.nt[f];
+    rows = mf.rows[f];
+    ca = mf.ca[f];
+    cb = mf.cb[f];
+    sb += (long long)f * n * pp;
+    sg += (long long)f * n * pp;
+  }
   typedef T VT __attribute__((ext_vector_type(C)));
   typedef double VD __attribute__((ext_vector_type(C)));
   const long long col = ((long long)blockIdx.x * blockDim.x + threadIdx.x) * C;
