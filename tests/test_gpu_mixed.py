@@ -3,7 +3,7 @@ solves and projection) against the FP64 oracle and the FP64 GPU path.
 
 Stated errors per field after N steps: velocity, scalar, pressure, the
 discrete divergence, and the wall shear stress of statistics.f90:143-146.
-The north star's bar for the wall shear is 1e-5 relative; the test bars
+The required bar for the wall shear is 1e-5 relative; the test bars
 below for the other fields are the measured errors with a margin, so a
 regression shows."""
 import numpy as np

@@ -1,6 +1,6 @@
 """HIP path vs the CPU oracle on the same seeded inputs, through the C-ABI.
 
-The bar (BASELINE north star): relative L2 error per field <= 1e-10 after 10
+The bar: relative L2 error per field <= 1e-10 after 10
 steps of the 64x65x64 case.  Smaller cases cover the BC and forcing
 variants the reference supports; full-size cases check size-independent
 properties (divergence-free projection, the RK3 single-mode closed form).
