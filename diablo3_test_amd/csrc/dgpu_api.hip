@@ -580,16 +580,17 @@ size_t z_lds(const dgpu_handle* h, int ng) {
 }
 
 template <typename R>
-void launch_xfwd(dgpu_handle* h, const R* u, cpx<R>* T, int j0, int j1) {
+void launch_xfwd(dgpu_handle* h, const R* u, cpx<R>* T, int j0, int j1, const FieldBatch* fb = nullptr) {
   if (j1 < j0) return;
   const Geo& g = h->g;
   constexpr int KT = XTile<R>::KT;
   const int ng = x_ngroups<R>(h, KT / 2, xpass_threads);
   const size_t lds = x_lds<R>(h, ng);
-  dim3 grid(g.nz / KT, j1 - j0 + 1), block(ng * (g.nx / 8));
-  Timed tm(h, DGPU_K_XFWD, (sizeof(R) * g.pp + 2.0 * sizeof(R) * g.tp) * (j1 - j0 + 1));
+  const int nf = fb ? fb->nf : 1;
+  dim3 grid(g.nz / KT, j1 - j0 + 1, nf), block(ng * (g.nx / 8));
+  Timed tm(h, DGPU_K_XFWD, (sizeof(R) * g.pp + 2.0 * sizeof(R) * g.tp) * (j1 - j0 + 1) * nf);
   hipLaunchKernelGGL((g.nx <= 512 ? k_xfwd<R, true> : k_xfwd<R, false>), grid, block, lds, h->stream, u, T, g,
-                     twx_of<R>(h), j0);
+                     twx_of<R>(h), j0, fb ? *fb : FieldBatch{});
   HIPCHECK(hipGetLastError());
 }
 
@@ -611,15 +612,17 @@ void launch_xinv(dgpu_handle* h, const cpx<R>* T, R* u, int j0, int j1, const do
 
 template <typename R>
 void launch_zfwd(dgpu_handle* h, const cpx<R>* T, double2* C, int j0, int j1, bool accum, double s0 = 1.0,
-                 double s1 = 1.0) {
+                 double s1 = 1.0, const FieldBatch* fb = nullptr) {
   if (j1 < j0) return;
   const Geo& g = h->g;
   const int ng = z_ngroups<R>(h);
   const size_t lds = z_lds<R>(h, ng);
-  dim3 grid((g.nq + ng - 1) / ng, j1 - j0 + 1), block(ng * (g.nz / 8));
-  Timed tm(h, DGPU_K_ZFWD, (2.0 * sizeof(R) * g.tp + (accum ? 32.0 : 16.0) * g.sp) * (j1 - j0 + 1));
+  const int nf = fb ? fb->nf : 1;
+  dim3 grid((g.nq + ng - 1) / ng, j1 - j0 + 1, nf), block(ng * (g.nz / 8));
+  Timed tm(h, DGPU_K_ZFWD, (2.0 * sizeof(R) * g.tp + (accum ? 32.0 : 16.0) * g.sp) * (j1 - j0 + 1) * nf);
   hipLaunchKernelGGL((g.nz <= 512 ? k_zfwd<R, true> : k_zfwd<R, false>), grid, block, lds, h->stream, T,
-                     reinterpret_cast<cpx<double>*>(C), g, twz_of<R>(h), j0, accum ? 1 : 0, (R)s0, (R)s1);
+                     reinterpret_cast<cpx<double>*>(C), g, twz_of<R>(h), j0, accum ? 1 : 0, (R)s0, (R)s1,
+                     fb ? *fb : FieldBatch{});
   HIPCHECK(hipGetLastError());
 }
 
@@ -729,6 +732,30 @@ void to_fourier(dgpu_handle* h, const R* u, double2* C, cpx<R>* T, int j0, int j
     const int b = std::min(j1, a + B - 1);
     launch_xfwd<R>(h, u, T, a, b);
     launch_zfwd<R>(h, T, C, a, b, accum);
+  }
+}
+
+// physical -> spectral for several fields on planes [j0, j1] (u[k] -> C[k]
+// through the x-spectra T[k]): every field's x pass in one launch, then every
+// field's z pass.
+template <typename R>
+void to_fourier_fields(dgpu_handle* h, int nf, R* const* u, double2* const* C, cpx<R>* const* T, int j0, int j1) {
+  for (int k0 = 0; k0 < nf; k0 += MAXBF) {
+    const int nb = std::min(MAXBF, nf - k0);
+    if (batch_planes<R>(h) < h->g.ny2) {  // plane batches: field by field
+      for (int k = k0; k < k0 + nb; ++k) to_fourier<R>(h, u[k], C[k], T[k], j0, j1);
+      continue;
+    }
+    FieldBatch xb{}, zb{};
+    for (int k = 0; k < nb; ++k) {
+      xb.in[k] = u[k0 + k];
+      xb.out[k] = T[k0 + k];
+      zb.in[k] = T[k0 + k];
+      zb.out[k] = C[k0 + k];
+    }
+    xb.nf = zb.nf = nb;
+    launch_xfwd<R>(h, u[k0], T[k0], j0, j1, &xb);
+    launch_zfwd<R>(h, T[k0], C[k0], j0, j1, false, 1.0, 1.0, &zb);
   }
 }
 
@@ -1083,11 +1110,24 @@ void les_stage(dgpu_handle* h, double time) {
     hipLaunchKernelGGL(k_les_model<R>, grid, block, 0, h->stream, a, th, dy, g);
     HIPCHECK(hipGetLastError());
   }
-  for (int m = 0; m < 7; ++m)
-    to_fourier<R>(h, phys<R>(h->tau[m]), h->ctau[m].p, xspec<R>(h->tpool[0]), 1, Ny);
-  for (int n = 0; n < nth; ++n)
-    for (int w = 0; w < 2; ++w)
-      to_fourier<R>(h, phys<R>(h->thf[w][n]), h->cthf[w][n].p, xspec<R>(h->tpool[0]), 1, Ny);
+  {  // the stresses and fluxes to Fourier space, batched over fields (the
+     // x-spectra buffers are idle until the products pass)
+    R* U[7 + 2 * DGPU_MAX_TH];
+    double2* C[7 + 2 * DGPU_MAX_TH];
+    cpx<R>* T[7 + 2 * DGPU_MAX_TH];
+    int nf = 0;
+    for (int m = 0; m < 7; ++m, ++nf) {
+      U[nf] = phys<R>(h->tau[m]);
+      C[nf] = h->ctau[m].p;
+    }
+    for (int n = 0; n < nth; ++n)
+      for (int w = 0; w < 2; ++w, ++nf) {
+        U[nf] = phys<R>(h->thf[w][n]);
+        C[nf] = h->cthf[w][n].p;
+      }
+    for (int k = 0; k < nf; ++k) T[k] = xspec<R>(h->tpool[k % (int)h->tpool.size()]);
+    to_fourier_fields<R>(h, nf, U, C, T, 1, Ny);
+  }
   // divergence into the subgrid forcing
   {
     LesDivArgs a;

@@ -76,7 +76,12 @@ __device__ __forceinline__ void split_pair(const cpx<R>* f, int n, int q, R scal
 // LDS ngroups*fft_lds_len(Nx) cpx.
 template <typename R, bool WAVE>
 __global__ void __launch_bounds__(512) k_xfwd(const R* __restrict__ u, cpx<R>* __restrict__ T,
-                                               Geo g, const cpx<R>* __restrict__ tw, int j0) {
+                                               Geo g, const cpx<R>* __restrict__ tw, int j0,
+                                               FieldBatch fb = FieldBatch{}) {
+  if (fb.nf > 0) {
+    u = reinterpret_cast<const R*>(fb.in[blockIdx.z]);
+    T = reinterpret_cast<cpx<R>*>(fb.out[blockIdx.z]);
+  }
   constexpr int KT = XTile<R>::KT;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<R>* smem = reinterpret_cast<cpx<R>*>(smem_raw);
@@ -185,7 +190,11 @@ __global__ void __launch_bounds__(512) k_xinv(const cpx<R>* __restrict__ T, R* _
 template <typename R, bool WAVE>
 __global__ void __launch_bounds__(512) k_zfwd(const cpx<R>* __restrict__ T, cpx<double>* __restrict__ C,
                                                Geo g, const cpx<R>* __restrict__ tw, int j0, int accum,
-                                               R s0, R s1) {
+                                               R s0, R s1, FieldBatch fb = FieldBatch{}) {
+  if (fb.nf > 0) {
+    T = reinterpret_cast<const cpx<R>*>(fb.in[blockIdx.z]);
+    C = reinterpret_cast<cpx<double>*>(fb.out[blockIdx.z]);
+  }
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<R>* smem = reinterpret_cast<cpx<R>*>(smem_raw);
   const int G = g.nz >> 3;
