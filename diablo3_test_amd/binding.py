@@ -94,6 +94,8 @@ class Params(ctypes.Structure):
         ("dthdz", ctypes.c_double * MAX_TH),
         ("use_les", ctypes.c_int32),
         ("create_new_flow", ctypes.c_int32),
+        ("flavor_front", ctypes.c_int32),
+        ("ly", ctypes.c_double),
     ]
 
     @classmethod
@@ -206,6 +208,7 @@ def load_library(path=None):
             ("adj_add_mean", [h, ctypes.c_int, d]),
             ("mean_profile", [h, ctypes.c_int, d]),
             ("checkpoint", [h, ctypes.c_int, ctypes.c_int]),
+            ("check_state", [h]),
         ]:
             fn = getattr(lib, "dgpu_" + name)
             fn.argtypes = args
@@ -417,6 +420,17 @@ class ChannelSolver:
 
     def synchronize(self):
         self._check(self._fn("synchronize")(self._h), "synchronize")
+
+    def check_state(self):
+        """True when the state (cu1..3, cp, cth, the RK carry and, with LES,
+        nu_t/kappa_t) is finite; False when dgpu_check_state reports
+        DGPU_ESTATE (``self.state_error`` then names the fields)."""
+        rc = self.lib.dgpu_check_state(self._h)
+        if rc == -5:
+            self.state_error = self.lib.dgpu_last_error(self._h).decode()
+            return False
+        self._check(rc, "check_state")
+        return True
 
     # ---- adjoint of the FP64 DNS substep (HIP library only)
     def adj_set_field(self, name, arr):

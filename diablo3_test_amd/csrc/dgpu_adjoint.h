@@ -258,6 +258,10 @@ struct AdjRhsArgs {
   double grav_x, grav_y, grav_z;
   double pr[MAXTH], dthdx[MAXTH], dthdz[MAXTH];
   double temp3, temp4, temp5;
+  // Front thermal-wind coefficients, as in RhsArgs (null unless Front)
+  const double* fr_yf;
+  const double* fr_y;
+  double fr_sx, fr_sz, fr_dr;
   int rk_step, nth;
   int js, je;
   int jst[MAXTH], jet[MAXTH];
@@ -319,12 +323,33 @@ __global__ void __launch_bounds__(256) k_adj_rhs_b(AdjRhsArgs a, Geo g) {
   bp = bp - (a.temp4 / a.dy[j > 0 ? j : 1]) * r2;
   if (j + 1 <= ny && inr(1, j + 1)) bp = bp + (a.temp4 / a.dy[j + 1]) * L(a.acr[1], idx + sp);
   const C f2up = (j + 1 < g.ny2) ? L(a.fbar[1], idx + sp) : zero;
+  // The Front thermal wind (forcing.f90:50-100).  A term alpha i k x of the
+  // forward adds -alpha i k xbar to the adjoint of x; the fbar are zero
+  // outside their fields' row ranges.
+  if (a.fr_yf) {
+    const double cy = a.fr_yf[j];
+    b1 = b1 + (a.fr_sx * cy) * ik(kzk, f1) - (a.fr_sz * cy) * ik(kxq, f1);
+    b3 = b3 + (a.fr_sx * cy) * ik(kzk, f3) - (a.fr_sz * cy) * ik(kxq, f3);
+    // f1(j) += sz dr (u2(j) + u2(j+1))/2, f3(j) -= sx dr (u2(j) + u2(j+1))/2
+    C f1s = f1, f3s = f3;
+    if (j > 0) {
+      f1s = f1s + L(a.fbar[0], idx - sp);
+      f3s = f3s + L(a.fbar[2], idx - sp);
+    }
+    b2 = b2 + (0.5 * a.fr_sz * a.fr_dr) * f1s - (0.5 * a.fr_sx * a.fr_dr) * f3s;
+    const double cyy = a.fr_y[j];
+    b2 = b2 + (a.fr_sx * cyy) * ik(kzk, f2) - (a.fr_sz * cyy) * ik(kxq, f2);
+  }
   for (int n = 0; n < a.nth; ++n) {
     const C ft = L(a.fbar[3 + n], idx);
     const C rt = inr(3 + n, j) ? L(a.acr[3 + n], idx) : zero;
     b1 = b1 - a.dthdx[n] * ft;
     b3 = b3 - a.dthdz[n] * ft;
     C bt = rt + (-(a.nu / a.pr[n]) * visc) * ft + a.grav_x * f1 + a.grav_z * f3;
+    if (a.fr_yf) {
+      const double cy = a.fr_yf[j];
+      bt = bt + (a.dthdx[n] * cy) * ik(kzk, ft) - (a.dthdz[n] * cy) * ik(kxq, ft);
+    }
     if (!mean) {
       // f2(j) += grav_y (dyf(j-1) th(j) + dyf(j) th(j-1)) / (2 dy(j)) on rows 2..Ny
       if (j >= 2 && j <= ny) bt = bt + (a.grav_y * a.dyf[j - 1] / (2.0 * a.dy[j])) * f2;

@@ -55,33 +55,57 @@ struct DevBuf {
   }
 };
 
-// Rows per segment of the CN solver: one of the instantiated lengths (the rows
-// live in registers), with at most 16 segments (1024 threads per block).  The
-// target length trades registers per thread against segments per column.  At
-// 512x257x512 20 rows (13 segments) take the spectral solves from 2.47 to 2.34
-// ms per step against 24 (11 segments): the 126-VGPR kernel runs 4 waves per
-// SIMD, so a CU holds one block, and 13 waves fill more of its 16 slots.
-int seg_target = 20;
-// Threads per workgroup of the x passes and of the products pass (tuning
-// knobs: DGPU_XPASS_THREADS, DGPU_XPROD_THREADS).
-int xpass_threads = 512;
-int xprod_threads = 512;
-// columns per lane of the LES CN solve (DGPU_LES_COLS)
-int les_cols = 2;
-// Columns per lane of the FP32-storage CN solve (1 or 2; knob DGPU_THOMAS_COLS).
-// Measured at 512x257x512: 2 columns take the mixed form's 12 solves per step
-// from 5.19 to 4.50 ms; the FP64 solves are 2% slower with 2 (5.21 -> 5.33 ms).
-int thomas_cols = 2;
-// CN solves of the non-LES substep in spectral space (knob DGPU_SPECTRAL_CN,
-// 0 = the physical-space solves between the transforms, as the reference).
-int spectral_cn = 1;
-// Plane batches of a 2-D transform: the x and z passes run batch by batch so
-// that a batch's x-spectra (the intermediate between the passes) are read
-// back while still in the last-level cache.  Budget in MB (DGPU_BATCH_MB).
-// Off by default: at 512x257x512 every batch size measured slower than one
-// launch per pass over all planes.
-int batch_mb = 0;
-int seg_len_for(int n) {
+// Tuning knobs.  Each handle reads them once, at dgpu_create, from the
+// environment (DGPU_SEG_TARGET, DGPU_XPASS_THREADS, DGPU_XPROD_THREADS,
+// DGPU_LES_COLS, DGPU_THOMAS_COLS, DGPU_SPECTRAL_CN, DGPU_BATCH_MB), with the
+// defaults below when a variable is unset, and keeps its own copy: creating a
+// handle never changes how another one runs.
+struct Knobs {
+  // Rows per segment of the CN solver: one of the instantiated lengths (the
+  // rows live in registers), with at most 16 segments (1024 threads per
+  // block).  The target length trades registers per thread against segments
+  // per column.  At 512x257x512 20 rows (13 segments) take the spectral solves
+  // from 2.47 to 2.34 ms per step against 24 (11 segments): the 126-VGPR
+  // kernel runs 4 waves per SIMD, so a CU holds one block, and 13 waves fill
+  // more of its 16 slots.
+  int seg_target = 20;
+  // Threads per workgroup of the x passes and of the products pass.
+  int xpass_threads = 512;
+  int xprod_threads = 512;
+  // Columns per lane of the LES CN solve.
+  int les_cols = 2;
+  // Columns per lane of the FP32-storage CN solve (1 or 2).  Measured at
+  // 512x257x512: 2 columns take the mixed form's 12 solves per step from 5.19
+  // to 4.50 ms; the FP64 solves are 2% slower with 2 (5.21 -> 5.33 ms).
+  int thomas_cols = 2;
+  // CN solves of the non-LES substep in spectral space (0 = the
+  // physical-space solves between the transforms, as the reference).
+  int spectral_cn = 1;
+  // Plane batches of a 2-D transform: the x and z passes run batch by batch
+  // so that a batch's x-spectra (the intermediate between the passes) are
+  // read back while still in the last-level cache.  Budget in MB.  Off by
+  // default: at 512x257x512 every batch size measured slower than one launch
+  // per pass over all planes.
+  int batch_mb = 0;
+};
+
+Knobs knobs_from_env() {
+  Knobs k;
+  auto rd = [](const char* name, int& v) {
+    if (const char* e = std::getenv(name)) v = std::atoi(e);
+  };
+  rd("DGPU_SEG_TARGET", k.seg_target);
+  rd("DGPU_XPASS_THREADS", k.xpass_threads);
+  rd("DGPU_XPROD_THREADS", k.xprod_threads);
+  rd("DGPU_LES_COLS", k.les_cols);
+  rd("DGPU_THOMAS_COLS", k.thomas_cols);
+  rd("DGPU_SPECTRAL_CN", k.spectral_cn);
+  rd("DGPU_BATCH_MB", k.batch_mb);
+  k.spectral_cn = k.spectral_cn != 0;
+  return k;
+}
+
+int seg_len_for(int n, int seg_target) {
   static const int lens[] = {8, 16, 20, 24, 28, 32, 40, 48, 56};  // the launch switches' cases
   int S = (n + seg_target - 1) / seg_target;
   if (S > 16) S = 16;
@@ -105,6 +129,7 @@ struct dgpu_handle {
   double w_transient;
   hipStream_t stream = nullptr;
   std::string err;
+  Knobs kn;  // tuning knobs, read at dgpu_create
 
   // device state (layouts described in dgpu_kernels.h)
   DevBuf<double2> cu[3], cp, cth[DGPU_MAX_TH], cf[3], cfth[DGPU_MAX_TH];
@@ -119,6 +144,11 @@ struct dgpu_handle {
   DevBuf<double2> twx, twz;
   DevBuf<float2> twx32, twz32;  // FP32 twiddles of the mixed form
   DevBuf<double> d_kx, d_kz, d_dy, d_dyf, d_gyf;
+  // Front thermal wind: (gyf(j) - Ly/2) delta/Ro_inv on planes 0..Ny+1, then
+  // the same at gy(j) (allocated only for flavor == 'Front' with Ro_inv != 0)
+  DevBuf<double> d_front;
+  // per-block flags of the non-finite scan (dgpu_check_state)
+  DevBuf<int> d_flags;
   // poisson_p_chan: the spectral right-hand side (allocated on first use)
   DevBuf<double2> pois;
   // variable_dt (dgpu_set_courant): settings, CFL*dy(j), Front offsets,
@@ -584,7 +614,7 @@ void launch_xfwd(dgpu_handle* h, const R* u, cpx<R>* T, int j0, int j1, const Fi
   if (j1 < j0) return;
   const Geo& g = h->g;
   constexpr int KT = XTile<R>::KT;
-  const int ng = x_ngroups<R>(h, KT / 2, xpass_threads);
+  const int ng = x_ngroups<R>(h, KT / 2, h->kn.xpass_threads);
   const size_t lds = x_lds<R>(h, ng);
   const int nf = fb ? fb->nf : 1;
   dim3 grid(g.nz / KT, j1 - j0 + 1, nf), block(ng * (g.nx / 8));
@@ -600,7 +630,7 @@ void launch_xinv(dgpu_handle* h, const cpx<R>* T, R* u, int j0, int j1, const do
   if (j1 < j0) return;
   const Geo& g = h->g;
   constexpr int KT = XTile<R>::KT;
-  const int ng = x_ngroups<R>(h, KT / 2, xpass_threads);
+  const int ng = x_ngroups<R>(h, KT / 2, h->kn.xpass_threads);
   const size_t lds = x_lds<R>(h, ng);
   const int nf = fb ? fb->nf : 1;
   dim3 grid(g.nz / KT, j1 - j0 + 1, nf), block(ng * (g.nx / 8));
@@ -670,7 +700,7 @@ void launch_thomas_les(dgpu_handle* h, int field, int rs, const R* r, R* u, cons
   // columns per lane (one 2-wide access per row and array) when the plane
   // allows it.
   Timed tm(h, DGPU_K_THOMAS, (3.0 * sizeof(R) + 8.0 + 32.0) * g.pp * n * nf);
-  if (les_cols == 2 && g.pp % 2 == 0) {
+  if (h->kn.les_cols == 2 && g.pp % 2 == 0) {
     dim3 grid((unsigned)((g.pp / 2 + 255) / 256), (unsigned)nf), block(256);
     hipLaunchKernelGGL((k_thomas_les<R, 2>), grid, block, 0, h->stream, r, u, nt, h->sbl.p, h->sgl.p, rows, n,
                        g.pp, ca, cb, transient, incr ? 1 : 0, mf);
@@ -682,12 +712,12 @@ void launch_thomas_les(dgpu_handle* h, int field, int rs, const R* r, R* u, cons
   HIPCHECK(hipGetLastError());
 }
 
-// Planes per batch of a 2-D transform (see batch_mb).
+// Planes per batch of a 2-D transform (see Knobs::batch_mb).
 template <typename R>
 int batch_planes(const dgpu_handle* h) {
-  if (batch_mb <= 0) return 1 << 30;
+  if (h->kn.batch_mb <= 0) return 1 << 30;
   const double per = 2.0 * sizeof(R) * (double)h->g.tp;
-  int b = (int)((double)batch_mb * 1048576.0 / per);
+  int b = (int)((double)h->kn.batch_mb * 1048576.0 / per);
   return b < 8 ? 8 : b;
 }
 
@@ -773,10 +803,10 @@ void launch_thomas(dgpu_handle* h, int field, int rs, const R* r, R* u, int ca, 
   const int nf = batch ? batch->nf : 1;
   const long long pitch = spec ? 2 * g.sp : g.pp;
   const long long ncol = pitch, nbc = spec ? 1 : 0;  // lanes past ncol (a partial last tile) touch no memory
-  // columns per lane: thomas_cols for segments of up to 32 rows (register budget), else 1.
+  // columns per lane: Knobs::thomas_cols for segments of up to 32 rows (register budget), else 1.
   // The wider row runs (64 C values) pay for the lower occupancy: at 512x257x512 the FP64
   // solve takes 4.34 ms per step with two columns and 4.70 with one.
-  const int C = (h->seg_L <= 32 && thomas_cols == 2 && (spec || g.pp % 128 == 0)) ? 2 : 1;
+  const int C = (h->seg_L <= 32 && h->kn.thomas_cols == 2 && (spec || g.pp % 128 == 0)) ? 2 : 1;
   dim3 grid((unsigned)((pitch + 64 * C - 1) / (64 * C)), (unsigned)nf), block(64, h->seg_S);
   Timed tm(h, DGPU_K_THOMAS, 3.0 * sizeof(R) * pitch * n * nf);
 #define TLAUNCH(L, CC)                                                                                     \
@@ -830,7 +860,7 @@ void launch_thomas_adj(dgpu_handle* h, int field, int rs, const double* xb, doub
     ca = 0;
     cb = n - 1;
   }
-  const int C = (h->seg_L <= 32 && thomas_cols == 2 && (spec || g.pp % 128 == 0)) ? 2 : 1;
+  const int C = (h->seg_L <= 32 && h->kn.thomas_cols == 2 && (spec || g.pp % 128 == 0)) ? 2 : 1;
   dim3 grid((unsigned)((pitch + 64 * C - 1) / (64 * C))), block(64, h->seg_S);
   Timed tm(h, DGPU_K_THOMAS, 24.0 * pitch * n);
 #define TLAUNCH(L, CC)                                                                                     \
@@ -1027,7 +1057,7 @@ void les_stage(dgpu_handle* h, double time) {
   // x derivatives: i kx times the state's x-spectra, then the x inverse; z
   // derivatives: i kz inside the z inverse; the y differences (G12, G22,
   // G32, the scalars' T2) are formed from the physical state in k_les_model
-  const bool share = batch_mb <= 0;
+  const bool share = h->kn.batch_mb <= 0;
   T = xspec<R>(h->tpool[3 + nth]);  // scratch past the state's x-spectra
   for (int m = 0; m < 9; ++m) {
     if (gsrc[m].op == OP_DY_GY || gsrc[m].op == OP_DY_GYF) continue;
@@ -1351,6 +1381,90 @@ void set_courant_impl(dgpu_handle* h, const dgpu_courant_params* c) {
 // explicit spectral terms, the CN solves and the projection stay FP64; the
 // RHS pass and the solves then work on increments (RHS less state, new state
 // less old), so the FP64 state is never rounded to FP32 as a whole.
+// Arrays of doubles scanned for NaN and Inf, one per blockIdx.y.
+struct ScanList {
+  const double* a[3 * (3 + DGPU_MAX_TH) + 1];
+  long long n[3 * (3 + DGPU_MAX_TH) + 1];
+};
+constexpr int kScanBlocks = 128;  // blocks per array
+
+// Each block writes one flag: 1 when its grid-stride share of the array
+// holds a value that is not finite.  Every flag is written, so none needs
+// clearing first.
+__global__ void __launch_bounds__(256) k_nonfinite(ScanList L, int* __restrict__ flags) {
+  const double* a = L.a[blockIdx.y];
+  const long long n = L.n[blockIdx.y];
+  int bad = 0;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+    bad |= !isfinite(a[i]);
+  bad = __syncthreads_or(bad);
+  if (threadIdx.x == 0) flags[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = bad;
+}
+
+// The finiteness check of dgpu_check_state: the spectral state, the RK
+// carry and, with LES, the eddy viscosity and diffusivities.
+int check_state_impl(dgpu_handle* h) {
+  const Geo& g = h->g;
+  ScanList L{};
+  const char* names[3 * (3 + DGPU_MAX_TH) + 1];
+  int nf = 0;
+  const long long ns = 2LL * g.sp * g.ny2;  // doubles of a spectral field
+  auto add = [&](const void* p, long long n, const char* name) {
+    if (!p) return;
+    L.a[nf] = static_cast<const double*>(p);
+    L.n[nf] = n;
+    names[nf++] = name;
+  };
+  static const char* thn[] = {"cth", "cfth", "kappa_t"};
+  add(h->cu[0].p, ns, "cu1");
+  add(h->cu[1].p, ns, "cu2");
+  add(h->cu[2].p, ns, "cu3");
+  add(h->cp.p, ns, "cp");
+  add(h->cf[0].p, ns, "cf1");
+  add(h->cf[1].p, ns, "cf2");
+  add(h->cf[2].p, ns, "cf3");
+  for (int n = 0; n < h->nth; ++n) {
+    add(h->cth[n].p, ns, thn[0]);
+    add(h->cfth[n].p, ns, thn[1]);
+  }
+  if (h->p.use_les) {  // physical FP64 arrays in both forms
+    const long long np = (long long)g.pp * g.ny2;
+    add(h->nu_t.p, np, "nu_t");
+    for (int n = 0; n < h->nth; ++n) add(h->kappa_t[n].p, np, thn[2]);
+  }
+  if (!h->d_flags.p) h->d_flags.alloc((size_t)kScanBlocks * (3 * (3 + DGPU_MAX_TH) + 1));
+  hipLaunchKernelGGL(k_nonfinite, dim3(kScanBlocks, nf), dim3(256), 0, h->stream, L, h->d_flags.p);
+  HIPCHECK(hipGetLastError());
+  std::vector<int> fl((size_t)kScanBlocks * nf);
+  HIPCHECK(hipMemcpyAsync(fl.data(), h->d_flags.p, fl.size() * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  std::string bad;
+  for (int f = 0; f < nf; ++f)
+    for (int b = 0; b < kScanBlocks; ++b)
+      if (fl[(size_t)f * kScanBlocks + b]) {
+        bad += bad.empty() ? names[f] : std::string(", ") + names[f];
+        break;
+      }
+  if (!bad.empty()) throw ApiError{DGPU_ESTATE, "non-finite values in " + bad};
+  return DGPU_OK;
+}
+
+// The Front thermal-wind coefficients of RhsArgs / AdjRhsArgs (null pointers
+// when the flavour has none, forcing.f90:50).
+template <typename A>
+void set_front(const dgpu_handle* h, A& a) {
+  const dgpu_params& p = h->p;
+  if (!h->d_front.p) return;
+  a.fr_yf = h->d_front.p;
+  a.fr_y = h->d_front.p + h->g.ny2;
+  a.fr_sx = a.fr_sz = 0.0;
+  for (int n = 0; n < h->nth; ++n) {
+    a.fr_sx += p.dthdx[n];
+    a.fr_sz += p.dthdz[n];
+  }
+  a.fr_dr = p.delta / p.ro_inv;
+}
+
 template <typename R>
 void rk_substep_t(dgpu_handle* h, int rk_step, double time, int time_step) {
   const dgpu_params& p = h->p;
@@ -1370,7 +1484,7 @@ void rk_substep_t(dgpu_handle* h, int rk_step, double time, int time_step) {
   // uniform in x and z, so the vertical solves commute with the horizontal
   // transforms: they run on the dealiased spectra (FP64 in both precision
   // forms), and cr never goes to physical space nor the solution back.
-  const bool spec = spectral_cn && !p.use_les;
+  const bool spec = h->kn.spectral_cn && !p.use_les;
   // The explicit spectral terms (channel.f90:39-253) read the state before
   // les_chan applies its wall values, so the LES path keeps that copy.
   const double2* lin_cu[3] = {h->cu[0].p, h->cu[1].p, h->cu[2].p};
@@ -1425,7 +1539,7 @@ void rk_substep_t(dgpu_handle* h, int rk_step, double time, int time_step) {
     }
     for (int s = 0; s < nslot; ++s) pa.out[s] = h->tpool[s].p;
     constexpr int KT = XTile<R>::KT;
-    const int ngx = x_ngroups<R>(h, KT, xprod_threads);
+    const int ngx = x_ngroups<R>(h, KT, h->kn.xprod_threads);
     const size_t lds = x_lds<R>(h, ngx);
     dim3 grid(g.nz / KT, Ny), block(ngx * (g.nx / 8));
     // reads u1, u2, u3, th once (neighbour planes from cache); writes the product x-spectra
@@ -1489,6 +1603,7 @@ void rk_substep_t(dgpu_handle* h, int rk_step, double time, int time_step) {
       a.force = p.px0 + p.amp_omega0 * std::cos(p.omega0 * time);  // :113-118
     else
       a.force = 0.0;
+    set_front(h, a);
     a.rk_step = rk_step;
     a.nth = nth;
     a.js = h->jstart;
@@ -1939,7 +2054,7 @@ void adj_substep(dgpu_handle* h, int rk_step, double time, int time_step) {
   }
   // Without LES the forward solved on the spectra (spectral_cn): so does the
   // transposed solve, in place on the state adjoint, writing crbar directly
-  const bool spec = spectral_cn && !p.use_les;
+  const bool spec = h->kn.spectral_cn && !p.use_les;
   // 2. the forward transform of the solved planes: xbar = F^T cubar there, and
   //    those planes of cubar are consumed (the other planes pass through)
   for (int f = 0; f < nf && !spec; ++f) {
@@ -2034,6 +2149,7 @@ void adj_substep(dgpu_handle* h, int rk_step, double time, int time_step) {
     a.temp3 = temp3;
     a.temp4 = temp4;
     a.temp5 = temp5;
+    set_front(h, a);
     a.rk_step = rk_step;
     a.nth = nth;
     a.js = h->jstart;
@@ -2338,6 +2454,8 @@ void free_all(dgpu_handle* h) {
   h->d_dy.release();
   h->d_dyf.release();
   h->d_gyf.release();
+  h->d_front.release();
+  h->d_flags.release();
   h->pois.release();
   h->cfl_cdy.release();
   h->cfl_offx.release();
@@ -2494,19 +2612,20 @@ void create_impl(dgpu_handle* h, const double* gy) {
   upload(h->d_dy, h->dy);
   upload(h->d_dyf, h->dyf);
   upload(h->d_gyf, h->gyf);
+  if (p.flavor_front && p.ro_inv != 0.0 && h->nth > 0) {
+    const int n2 = g.ny2;
+    std::vector<double> fr(2 * (size_t)n2, 0.0);
+    const double dr = p.delta / p.ro_inv;
+    for (int j = 0; j < n2; ++j) fr[j] = (h->gyf[j] - 0.5 * p.ly) * dr;
+    for (int j = 1; j < n2; ++j) fr[n2 + j] = (h->gy[j] - 0.5 * p.ly) * dr;
+    upload(h->d_front, fr);
+  }
   setup_projection(h);
 
   // CN factor tables (DGPU_SEG_TARGET overrides the target segment length)
-  if (const char* e = std::getenv("DGPU_SEG_TARGET")) seg_target = std::atoi(e);
-  if (seg_target < 1) throw ApiError{DGPU_EINVAL, "DGPU_SEG_TARGET must be a positive row count"};
-  if (const char* e = std::getenv("DGPU_XPASS_THREADS")) xpass_threads = std::atoi(e);
-  if (const char* e = std::getenv("DGPU_XPROD_THREADS")) xprod_threads = std::atoi(e);
-  if (const char* e = std::getenv("DGPU_BATCH_MB")) batch_mb = std::atoi(e);
-  if (const char* e = std::getenv("DGPU_THOMAS_COLS")) thomas_cols = std::atoi(e);
-  if (const char* e = std::getenv("DGPU_SPECTRAL_CN")) spectral_cn = std::atoi(e) != 0;
-  if (const char* e = std::getenv("DGPU_LES_COLS")) les_cols = std::atoi(e);
+  if (h->kn.seg_target < 1) throw ApiError{DGPU_EINVAL, "DGPU_SEG_TARGET must be a positive row count"};
   const int n = g.ny2;
-  h->seg_L = seg_len_for(n);
+  h->seg_L = seg_len_for(n, h->kn.seg_target);
   if (h->seg_L < 0) throw ApiError{DGPU_EUNSUP, "ny too large for the segmented solver"};
   h->seg_S = (n + h->seg_L - 1) / h->seg_L;
   h->nfields = 3 + h->nth;
@@ -2576,6 +2695,7 @@ int dgpu_create(const dgpu_params* p, const double* gy, int precision, dgpu_hand
   if (!h) return DGPU_ENOMEM;
   h->p = *p;
   h->precision = precision;
+  h->kn = knobs_from_env();
   int rc = run_guarded(h, [&] { create_impl(h, gy); });
   if (rc != DGPU_OK) {
     free_all(h);
@@ -2914,6 +3034,11 @@ int dgpu_mean_profile(dgpu_handle* h, int field, double* out) {
 int dgpu_checkpoint(dgpu_handle* h, int slot, int save) {
   if (!h) return DGPU_EINVAL;
   return run_guarded(h, [&] { checkpoint(h, slot, save); });
+}
+
+int dgpu_check_state(dgpu_handle* h) {
+  if (!h) return DGPU_EINVAL;
+  return run_guarded(h, [&] { check_state_impl(h); });
 }
 
 int dgpu_synchronize(dgpu_handle* h) {

@@ -529,6 +529,13 @@ struct RhsArgs {
   const double2* lesf3;
   const double2* lesfth[MAXTH];
   double temp3, temp4, temp5, force;  // force = px0 (+ oscillation), 0 if none
+  // thermal-wind advection of the Front flavour (forcing.f90:50-100), null
+  // otherwise: fr_yf[j] = (gyf(j) - Ly/2) delta/Ro_inv, fr_y[j] the same at
+  // gy(j); fr_sx, fr_sz the sums of dthdx, dthdz over the scalars (the
+  // velocity terms of every scalar add up), fr_dr = delta/Ro_inv
+  const double* fr_yf;
+  const double* fr_y;
+  double fr_sx, fr_sz, fr_dr;
   int rk_step, nth;
   int js, je;
   int jst[MAXTH], jet[MAXTH];
@@ -616,6 +623,12 @@ __global__ void __launch_bounds__(512) k_rhs(RhsArgs a, Geo g, const cpx<R>* __r
         f1 = f1 + a.grav_x * th;
         f3 = f3 + a.grav_z * th;
       }
+      if (a.fr_yf) {  // forcing.f90:55-72
+        const double cy = a.fr_yf[j];
+        const C u2m = 0.5 * (ld(a.cu2) + reinterpret_cast<const C*>(a.cu2)[o + g.sp]);
+        f1 = f1 - (a.fr_sx * cy) * ik(kzk, cu1) + (a.fr_sz * cy) * ik(kxq, cu1) + (a.fr_sz * a.fr_dr) * u2m;
+        f3 = f3 - (a.fr_sx * cy) * ik(kzk, cu3) + (a.fr_sz * cy) * ik(kxq, cu3) - (a.fr_sx * a.fr_dr) * u2m;
+      }
       if (a.lesf1) {
         f1 = f1 + reinterpret_cast<const C*>(a.lesf1)[o];
         f3 = f3 + reinterpret_cast<const C*>(a.lesf3)[o];
@@ -656,6 +669,10 @@ __global__ void __launch_bounds__(512) k_rhs(RhsArgs a, Geo g, const cpx<R>* __r
           f2 = f2 + (a.grav_y / (2.0 * cdy)) * (cdyfm * th + cdyf * thm);
         }
       }
+      if (a.fr_y) {  // forcing.f90:74-84
+        const double cy = a.fr_y[j];
+        f2 = f2 - (a.fr_sx * cy) * ik(kzk, cu2) + (a.fr_sz * cy) * ik(kxq, cu2);
+      }
       if (a.lesf2) f2 = f2 + reinterpret_cast<const C*>(a.lesf2)[o];
       f2 = f2 - (Z(4) + ik(kzk, Z(5)));
       C r2 = cu2;
@@ -672,6 +689,10 @@ __global__ void __launch_bounds__(512) k_rhs(RhsArgs a, Geo g, const cpx<R>* __r
       C th = reinterpret_cast<const C*>(a.cth[n])[o];
       C ft = (-(a.nu / a.pr[n]) * visc) * th;
       ft = ft - a.dthdx[n] * cu1 - a.dthdz[n] * cu3;
+      if (a.fr_yf) {  // forcing.f90:87-96
+        const double cy = a.fr_yf[j];
+        ft = ft - (a.dthdx[n] * cy) * ik(kzk, th) + (a.dthdz[n] * cy) * ik(kxq, th);
+      }
       if (a.lesfth[n]) ft = ft + reinterpret_cast<const C*>(a.lesfth[n])[o];
       ft = ft - (Z(6 + 2 * n) + ik(kzk, Z(7 + 2 * n)));
       C rt = th;

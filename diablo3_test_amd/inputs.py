@@ -222,6 +222,8 @@ def channel_params(c, nx, ny, nz, n_th):
         th_bc_ymin_c1=list(c["th_bc_ymin_c1"]), th_bc_ymax_c1=list(c["th_bc_ymax_c1"]),
         dthdx=list(c["dthdx"]), dthdz=list(c["dthdz"]),
         use_les=int(c["use_les"]), create_new_flow=int(c["create_new_flow"]),
+        # flavor == 'Front' is a blank-padded Fortran comparison (forcing.f90:50)
+        flavor_front=int(c["flavor"].rstrip() == "Front"), ly=c["ly"],
     )
 
 

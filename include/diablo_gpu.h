@@ -101,6 +101,12 @@ typedef struct dgpu_params {
   double dthdx[DGPU_MAX_TH], dthdz[DGPU_MAX_TH]; /* background scalar gradients */
   int32_t use_les;
   int32_t create_new_flow;    /* LES gate: LES only if !create_new_flow || time_step>100 */
+  /* flavor == 'Front' (parameters.f90:25): with ro_inv != 0 the substep adds
+     the advection by the thermal wind of the background gradients dthdx,
+     dthdz, a wind that varies linearly about y = ly/2
+     (user_rhs_chan_fourier, forcing.f90:50-100). */
+  int32_t flavor_front;
+  double ly;                  /* LY, the height of the domain (parameters.f90:31) */
 } dgpu_params;
 
 typedef struct dgpu_handle dgpu_handle;
@@ -192,6 +198,16 @@ int dgpu_set_delta_t(dgpu_handle* h, double delta_t);
 
 /* Wait for all work queued on the handle's stream. */
 int dgpu_synchronize(dgpu_handle* h);
+
+/*
+ * Finiteness check of the state (no counterpart in the reference, whose run
+ * goes on with a non-finite field until its statistics or a flow save show
+ * it).  Scans cu1..3, cp, cth, the RK carry cf1..3, cfth and, with LES,
+ * nu_t and kappa_t for NaN and Inf on the device; returns DGPU_OK, or
+ * DGPU_ESTATE with the offending fields named in dgpu_last_error.  Waits for
+ * the handle's stream.
+ */
+int dgpu_check_state(dgpu_handle* h);
 
 /*
  * Adjoint of the FP64 substep, DNS and LES (no counterpart in the reference,

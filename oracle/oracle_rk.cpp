@@ -119,7 +119,40 @@ void Solver::rk_chan_1(int rk_step, double time, int time_step) {
           cfth[ci(i, k, j)] = cfth[ci(i, k, j)] - cu1[ci(i, k, j)] * p.dthdx[n] - cu3[ci(i, k, j)] * p.dthdz[n];
         }
   }
-  // user_rhs_chan_fourier (forcing.f90:37-113) is a no-op unless flavor=='Front'
+  // user_rhs_chan_fourier (forcing.f90:37-113): the thermal-wind advection of
+  // the Front flavour (:50-100); a no-op otherwise
+  if (p.flavor_front && p.ro_inv != 0.0) {
+    const double Ly = p.ly, dl = p.delta, ro = p.ro_inv;
+    for (int n = 0; n < Nth; ++n) {
+      const double dx = p.dthdx[n], dz = p.dthdz[n];
+      cplx* cth = CTH(th, n);
+      cplx* cfth = CTH(fth, n);
+#pragma omp parallel for schedule(static)
+      for (int j = js; j <= je; ++j)  // forcing.f90:55-72
+        for (int k = 0; k <= twoNkz; ++k)
+          for (int i = 0; i < Nxp; ++i) {
+            const cplx u2m = 0.5 * (cu2[ci(i, k, j)] + cu2[ci(i, k, j + 1)]);
+            cf1[ci(i, k, j)] = cf1[ci(i, k, j)] - (dx * (gyf[j] - 0.5 * Ly) * dl / ro) * ik(kz[k], cu1[ci(i, k, j)]) -
+                               (-1.0 * dz * (gyf[j] - 0.5 * Ly) * dl / ro) * ik(kx[i], cu1[ci(i, k, j)]) -
+                               (-1.0 * dz * dl / ro) * u2m;
+            cf3[ci(i, k, j)] = cf3[ci(i, k, j)] - (dx * (gyf[j] - 0.5 * Ly) * dl / ro) * ik(kz[k], cu3[ci(i, k, j)]) -
+                               (-1.0 * dz * (gyf[j] - 0.5 * Ly) * dl / ro) * ik(kx[i], cu3[ci(i, k, j)]) -
+                               (dx * dl / ro) * u2m;
+          }
+#pragma omp parallel for schedule(static)
+      for (int j = 2; j <= Nyp; ++j)  // :74-84
+        for (int k = 0; k <= twoNkz; ++k)
+          for (int i = 0; i < Nxp; ++i)
+            cf2[ci(i, k, j)] = cf2[ci(i, k, j)] - (dx * (gy[j] - 0.5 * Ly) * dl / ro) * ik(kz[k], cu2[ci(i, k, j)]) -
+                               (-1.0 * dz * (gy[j] - 0.5 * Ly) * dl / ro) * ik(kx[i], cu2[ci(i, k, j)]);
+#pragma omp parallel for schedule(static)
+      for (int j = jstart_th[n]; j <= jend_th[n]; ++j)  // :87-96
+        for (int k = 0; k <= twoNkz; ++k)
+          for (int i = 0; i < Nxp; ++i)
+            cfth[ci(i, k, j)] = cfth[ci(i, k, j)] - (dl / ro) * dx * (gyf[j] - 0.5 * Ly) * ik(kz[k], cth[ci(i, k, j)]) -
+                                (dl / ro) * (-1.0) * dz * (gyf[j] - 0.5 * Ly) * ik(kx[i], cth[ci(i, k, j)]);
+    }
+  }
 
   // ---- LES gate (channel.f90:259) or plain inverse FFTs (:279-287)
   if (p.use_les && (!p.create_new_flow || time_step > 100)) {

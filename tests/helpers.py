@@ -10,7 +10,9 @@ from diablo3_test_amd import workloads
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
-ORACLE_LIB = os.path.join(ORACLE_DIR, "liboracle.so")
+# DGPU_ORACLE_LIB points the tests at another build of the oracle (the
+# planted-mistake builds of tests/oracle_mutations.py)
+ORACLE_LIB = os.environ.get("DGPU_ORACLE_LIB") or os.path.join(ORACLE_DIR, "liboracle.so")
 
 _oracle = None
 

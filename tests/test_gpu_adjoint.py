@@ -57,6 +57,7 @@ VARIANTS = {
                     th_bc_ymax_c1=[0.2], v_bc_ymax=1),
     "coriolis_gravity": dict(ro_inv=0.5, grav_x=0.2, grav_z=-0.1, grav_y=0.4, dthdx=[0.1], dthdz=[0.05],
                              pr=[0.7]),
+    "front": dict(flavor_front=1, ly=1.5, ro_inv=0.5, delta=1.5, dthdx=[0.2], dthdz=[-0.1], pr=[0.7]),
 }
 
 

@@ -6,7 +6,6 @@ variants the reference supports; full-size cases check size-independent
 properties (divergence-free projection, the RK3 single-mode closed form).
 """
 import math
-import os
 
 import numpy as np
 import pytest
@@ -129,6 +128,11 @@ VARIANTS = {
     "coriolis_gravity": dict(ro_inv=0.7, delta=2.0, grav_x=0.3, grav_z=-0.2, grav_y=0.5,
                              dthdx=[0.1], dthdz=[-0.05], pr=[0.7]),
     "hyperviscosity_anisotropic": dict(beta=2.0, nu_v_scale=0.5),
+    "front_thermal_wind": dict(flavor_front=1, ly=1.5, ro_inv=0.6, delta=1.5, dthdx=[0.3], dthdz=[-0.2],
+                               pr=[0.9]),
+    "front_two_scalars": dict(flavor_front=1, ly=0.5, ro_inv=0.8, delta=1.0, n_th=2, pr=[0.9, 1.3],
+                              th_bc_ymin=[0, 1], th_bc_ymax=[0, 0], th_bc_ymin_c1=[-1.0, 0.1],
+                              th_bc_ymax_c1=[1.0, 0.4], dthdx=[0.3, -0.1], dthdz=[-0.2, 0.25]),
     "no_scalar": dict(n_th=0, pr=[], th_bc_ymin=[], th_bc_ymax=[], th_bc_ymin_c1=[],
                       th_bc_ymax_c1=[], dthdx=[], dthdz=[]),
 }
@@ -186,6 +190,31 @@ def test_physical_space_cn_solves(precision, monkeypatch):
         assert max(errs[k] for k in ("cu1", "cu2", "cu3", "cth")) < 1e-6
 
 
+def test_knobs_are_per_handle(monkeypatch):
+    """Tuning knobs are read once per handle at creation: a handle made with
+    DGPU_SPECTRAL_CN=0 solves in physical space (x forward transforms of the
+    solutions), and a handle made afterwards with the variable unset takes
+    the default spectral path (no x forward pass at all), while the first
+    handle keeps its own setting."""
+    p, gy, ic, seed, amp = workloads.workload("small_32x33x32")
+    st = workloads.initial_state(p, gy, ic, seed, amp)
+    monkeypatch.setenv("DGPU_SPECTRAL_CN", "0")
+    phys = prepared(gpu_solver(p, gy), st)
+    monkeypatch.delenv("DGPU_SPECTRAL_CN")
+    spec = prepared(gpu_solver(p, gy), st)
+
+    def xfwd_launches(s):
+        s.timing_enable(True)
+        s.step(1)
+        n = s.timing_read()["x_forward"][2]
+        s.timing_enable(False)
+        return n
+
+    assert xfwd_launches(spec) == 0
+    assert xfwd_launches(phys) > 0
+    assert xfwd_launches(spec) == 0
+
+
 def test_rectangular_box_and_ny():
     p = workloads.channel_params(64, 41, 16, 300.0, ri=0.2, dt=2e-3)
     gy = workloads.channel_grid(41)
@@ -224,8 +253,7 @@ def test_cfg2_one_step_parity():
     assert max(errs.values()) <= 1e-10
 
 
-@pytest.mark.skipif(not os.environ.get("DGPU_LONG_TESTS"),
-                    reason="the oracle takes about 2 minutes for 100 steps (set DGPU_LONG_TESTS=1)")
+@pytest.mark.timeout(900)
 def test_cfg2_hundred_steps_parity():
     """256x193x256, Re = 400, Ri = 0.1: 100 steps against the oracle, every
     field within 1e-10 relative L2 (measured on MI355X: velocities and scalar

@@ -31,6 +31,10 @@ class TorchChannel:
         self.nkx, self.nkz = self.nx // 3, self.nz // 3
         self.nq, self.nkk = self.nkx + 1, 2 * self.nkz + 1
         gyf, dy, dyf = metrics(gy)
+        gyfull = np.zeros(self.ny + 2)
+        gyfull[1:] = gy
+        self.gyf = torch.tensor(gyf, dtype=torch.float64)
+        self.gy = torch.tensor(gyfull, dtype=torch.float64)
         self.dy = torch.tensor(dy, dtype=torch.float64)
         self.dyf = torch.tensor(dyf, dtype=torch.float64)
         kx, kz = workloads.wavenumbers(p)
@@ -256,6 +260,18 @@ class TorchChannel:
             f1 = f1 + p["grav_x"] * th
             f3 = f3 + p["grav_z"] * th
             fts.append(-(self.nu / p["pr"][n]) * visc * th - cu1 * p["dthdx"][n] - cu3 * p["dthdz"][n])
+        # thermal wind of the Front flavour (forcing.f90:50-100)
+        if p.get("flavor_front", 0) and p["ro_inv"] != 0.0:
+            dr = p["delta"] / p["ro_inv"]
+            cyf = ((self.gyf - 0.5 * p["ly"]) * dr)[:, None, None]
+            cy = ((self.gy - 0.5 * p["ly"]) * dr)[:, None, None]
+            u2a = 0.5 * (cu2 + torch.cat([cu2[1:], torch.zeros_like(cu2[:1])]))
+            for n in range(nth):
+                dx, dz = p["dthdx"][n], p["dthdz"][n]
+                f1 = f1 - dx * cyf * self.ikz * cu1 + dz * cyf * self.ikx * cu1 + dz * dr * u2a
+                f3 = f3 - dx * cyf * self.ikz * cu3 + dz * cyf * self.ikx * cu3 - dx * dr * u2a
+                f2 = f2 - dx * cy * self.ikz * cu2 + dz * cy * self.ikx * cu2
+                fts[n] = fts[n] - dx * cyf * self.ikz * cth[n] + dz * cyf * self.ikx * cth[n]
         # nonlinear products (channel.f90:299-459, 557-608)
         u1, u2, u3 = self.to_phys(cu1), self.to_phys(cu2), self.to_phys(cu3)
         ths = [self.to_phys(cth[n]) for n in range(nth)]
