@@ -8,7 +8,14 @@ Beside the GPU number the line carries
 * roofline: the dominant kernel class's algorithmic HBM bytes over its
   kernel time, timed live with HIP events on the library's stream;
 * cpu_baseline: the CPU oracle (C++ FP64 restatement, OpenMP) timed on the
-  host cores on a bounded sample of the same workload.
+  host cores on a bounded sample of the same workload (the median of three
+  single steps);
+* forms: the other forms of the metric -- mixed precision, the adjoint
+  (checkpoints every 10 steps, and with one checkpoint), the AMD-Verstappen
+  LES in FP64 and mixed -- each with its per-kernel-class split, its
+  algorithmic bytes per step, the same without the two-pass transforms'
+  x-spectra intermediate, and the reference algorithm's one-pass
+  compulsory traffic (model_GB_per_step) over its step time.
 """
 import argparse
 import json
@@ -27,9 +34,33 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(p, gy, st, npts):
-    """One RK3 step (three rk_chan_1 substeps) of the same state on the CPU oracle."""
+def host_cpu():
+    """lscpu's model name and physical core count of this host."""
+    import subprocess
+
+    model, cores_per_socket, sockets = None, None, None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for ln in out.splitlines():
+            k, _, v = ln.partition(":")
+            v = v.strip()
+            if k.strip() == "Model name":
+                model = v
+            elif k.strip() == "Core(s) per socket":
+                cores_per_socket = int(v)
+            elif k.strip() == "Socket(s)":
+                sockets = int(v)
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    phys = cores_per_socket * sockets if cores_per_socket and sockets else None
+    return model, phys
+
+
+def cpu_baseline(p, gy, st, npts, nsteps=3):
+    """RK3 steps (three rk_chan_1 substeps each) of the same state on the CPU
+    oracle, timed one by one; the median step gives the rate."""
     import ctypes
+    import statistics
 
     from diablo3_test_amd.binding import ChannelSolver, _declare
 
@@ -37,17 +68,25 @@ def cpu_baseline(p, gy, st, npts):
     o = ChannelSolver(p, gy, lib=lib, prefix="orc_")
     o.set_state(st)
     o.pre_first_step()
-    t0 = time.perf_counter()
-    o.step(1)
-    dt = time.perf_counter() - t0
+    times = []
+    for i in range(nsteps):
+        t0 = time.perf_counter()
+        o.step(1)
+        times.append(time.perf_counter() - t0)
+        log(f"bench: CPU oracle step {i + 1}/{nsteps}: {times[-1]:.2f} s")
     o.close()
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    dt = statistics.median(times)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    model, phys = host_cpu()
     return {
         "value": npts / dt / 1e6,
         "unit": "Mpts·RK3-steps/s",
-        "cores": cores,
+        "cores": threads,
         "kind": "port",
-        "sample": f"1 RK3 step of the same state, {dt:.1f} s wall, OpenMP threads={cores}",
+        "sample": (f"median of {nsteps} single RK3 steps of the same state "
+                   f"({', '.join(f'{t:.2f}' for t in times)} s), 1 process x {threads} OpenMP threads"),
+        "cpu_model": model,
+        "physical_cores": phys,
     }
 
 
@@ -64,6 +103,29 @@ def measured_traffic(workload, precision, cls):
                 if d["workload"] == workload and d["precision"] == precision:
                     return d["traffic_bytes_per_launch"].get(cls)
     return None
+
+
+def model_bytes_per_step(p, form):
+    """The one-pass compulsory-traffic model of a step of the reference's
+    algorithm (23 two-dimensional transforms per DNS substep, 59 with the
+    AMD-Verstappen LES, each reading its input once and writing its output
+    once; real CN solves of 4 fields; projection; RHS assembly fused into the
+    transforms; the products' extra physical reads; LES pointwise work), in
+    bytes: P = Nx Nz (Ny+2) physical points, S = (Nx/2)(2 Nz/3 + 1)(Ny+2)
+    spectral modes, r bytes per real.  None for the adjoint form."""
+    if form.startswith("adjoint"):
+        return None
+    r = 4.0 if form.endswith("mixed") else 8.0
+    les = form.startswith("les")
+    P = p["nx"] * p["nz"] * (p["ny"] + 2)
+    S = (p["nx"] // 2) * (2 * (p["nz"] // 3) + 1) * (p["ny"] + 2)
+    nfft = 59 if les else 23
+    sub = nfft * (r * P + 2 * r * S)
+    sub += 4 * ((r + 16) * P + (8 * P if les else 0))
+    sub += 128 * S + 17 * 2 * r * S + 8 * r * P
+    if les:
+        sub += 30 * r * P + 5 * 2 * r * S
+    return 3 * sub
 
 
 def copy_probe(nbytes):
@@ -111,10 +173,18 @@ def run_form(p, gy, st, precision, steps, warmup, timing_steps):
     g.timing_enable(True)
     g.step(timing_steps)
     kt = g.timing_read()
+    tb = g.timing_read_intermediate()
     g.timing_enable(False)
     g.close()
+    return summarize(npts, steps, sec, kt, tb, timing_steps)
+
+
+def summarize(npts, steps, sec, kt, tb, timing_steps):
+    """Throughput and the per-kernel-class split of a timed run."""
+    ms_per_step = sec / steps * 1e3
     total_ms = sum(v[0] for v in kt.values())
     total_bytes = sum(v[1] for v in kt.values())
+    total_tbytes = sum(tb.values())
     kernels = {}
     for name, (ms, by, ct) in kt.items():
         if ct:
@@ -122,6 +192,8 @@ def run_form(p, gy, st, precision, steps, warmup, timing_steps):
                 "ms_per_step": ms / timing_steps,
                 "launches_per_step": ct / timing_steps,
                 "GBps": by / (ms * 1e-3) / 1e9 if ms > 0 else None,
+                # without the x-spectra intermediate of the two-pass transforms
+                "GBps_one_pass": (by - tb[name]) / (ms * 1e-3) / 1e9 if ms > 0 else None,
                 "share": ms / total_ms if total_ms > 0 else None,
             }
     step_bytes = total_bytes / timing_steps
@@ -129,19 +201,23 @@ def run_form(p, gy, st, precision, steps, warmup, timing_steps):
         "value": npts * steps / sec / 1e6,
         "ms_per_step": ms_per_step,
         "kt": kt,
+        "tb": tb,
         "kernels": kernels,
         "step_hbm_frac": step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "algorithmic_GB_per_step": step_bytes / 1e9,
+        "one_pass_GB_per_step": (total_bytes - total_tbytes) / timing_steps / 1e9,
     }
 
 
-def run_adjoint_form(p, gy, st, steps, warmup, timing_steps):
-    """The differentiation form (one full untimed run first, whatever `warmup`):
-    the gradient of the time-averaged wall shear
-    stress over `steps` steps with respect to the initial state -- forward
-    run with checkpoints, recomputation and the adjoint substeps.  A step of
-    this form is one forward step plus its share of recomputation and one
-    adjoint step."""
+def run_adjoint_form(p, gy, st, steps, warmup, timing_steps, checkpoint_every):
+    """The differentiation form (one full untimed run first, whatever
+    `warmup`): the gradient of the time-averaged wall shear stress over
+    `steps` steps with respect to the initial state -- the forward run saving
+    a checkpoint every `checkpoint_every` steps, then per segment (last
+    first) the recomputation from its checkpoint and the adjoint substeps.
+    The last segment is never recomputed: the forward run keeps its substep
+    inputs.  A step of this form is one forward step, its share of
+    recomputation and one adjoint step."""
     from diablo3_test_amd.adjoint import WallShearRun
     from diablo3_test_amd.binding import ChannelSolver
 
@@ -149,7 +225,7 @@ def run_adjoint_form(p, gy, st, steps, warmup, timing_steps):
     g = ChannelSolver(p, gy)
     g.set_state(st)
     g.pre_first_step()
-    run = WallShearRun(g, steps, checkpoint_every=steps)
+    run = WallShearRun(g, steps, checkpoint_every=checkpoint_every)
     # one full run first: it allocates the checkpoint slots; slot 0 then holds
     # the initial state, which stays resident for the timed run
     run.gradient(None, fetch=False)
@@ -159,30 +235,19 @@ def run_adjoint_form(p, gy, st, steps, warmup, timing_steps):
     run.gradient(None, fetch=False)
     g.synchronize()
     sec = time.perf_counter() - t0
-    ms_per_step = sec / steps * 1e3
-    log(f"bench: adjoint {steps} steps in {sec:.3f} s, {ms_per_step:.2f} ms/step")
+    log(f"bench: adjoint (checkpoints every {checkpoint_every}) {steps} steps in {sec:.3f} s, "
+        f"{sec / steps * 1e3:.2f} ms/step")
     g.checkpoint(0, save=False)
     g.timing_enable(True)
-    WallShearRun(g, timing_steps, checkpoint_every=timing_steps).gradient(None, fetch=False)
+    WallShearRun(g, timing_steps, checkpoint_every=min(checkpoint_every, timing_steps)).gradient(None, fetch=False)
     kt = g.timing_read()
+    tb = g.timing_read_intermediate()
     g.timing_enable(False)
     g.close()
-    kernels = {}
-    total_bytes = 0.0
-    for name, (ms, by, ct) in kt.items():
-        if ct:
-            total_bytes += by
-            kernels[name] = {"ms_per_step": ms / timing_steps, "launches_per_step": ct / timing_steps,
-                             "GBps": by / (ms * 1e-3) / 1e9 if ms > 0 else None}
-    step_bytes = total_bytes / timing_steps
-    return {
-        "value": npts * steps / sec / 1e6,
-        "ms_per_step": ms_per_step,
-        "kt": kt,
-        "kernels": kernels,
-        "step_hbm_frac": step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
-        "algorithmic_GB_per_step": step_bytes / 1e9,
-    }
+    out = summarize(npts, steps, sec, kt, tb, timing_steps)
+    out["checkpoint_every"] = checkpoint_every
+    out["steps"] = steps
+    return out
 
 
 def main():
@@ -193,8 +258,12 @@ def main():
     ap.add_argument("--workload", default="dns_512x257x512")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"],
                     help="the form whose throughput is `value`")
-    ap.add_argument("--forms", default="fp64,mixed,adjoint",
-                    help="forms also measured and reported under `forms` (comma list, or 'none')")
+    ap.add_argument("--forms", default="fp64,mixed,adjoint,adjoint_one_segment,les,les_mixed",
+                    help="forms also measured and reported under `forms` (comma list, or 'none'): "
+                         "fp64, mixed, adjoint (50 steps, checkpoints every 10), adjoint_one_segment "
+                         "(--steps steps, one checkpoint: no recomputation), les and les_mixed (AMD-Verstappen LES "
+                         "on the LES workload of the same grid)")
+    ap.add_argument("--les-workload", default="cfg3_les_512x257x512")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--timing-steps", type=int, default=2)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -218,18 +287,34 @@ def main():
     main_form = run_form(p, gy, st, args.precision, args.steps, args.warmup, args.timing_steps)
     forms = {args.precision: main_form}
     extra = [] if args.forms == "none" else [f for f in args.forms.split(",") if f and f != args.precision]
+    les_case = None
     for f in extra:
-        if f == "adjoint":
+        if f.startswith("adjoint"):
             if p["use_les"]:
-                log("bench: the adjoint covers the DNS substep; skipped for an LES workload")
+                log("bench: the adjoint form is timed on the DNS workload; skipped for an LES workload")
                 continue
-            forms[f] = run_adjoint_form(p, gy, st, args.steps, args.warmup, args.timing_steps)
-        else:
+            if f == "adjoint":  # 50 steps with a checkpoint every 10 (the recomputing cadence)
+                forms[f] = run_adjoint_form(p, gy, st, max(args.steps, 50), args.warmup, args.timing_steps, 10)
+            else:
+                forms[f] = run_adjoint_form(p, gy, st, args.steps, args.warmup, args.timing_steps, args.steps)
+        elif f in ("les", "les_mixed"):
+            if les_case is None:
+                lp, lgy, lic, lseed, lamp = workloads.workload(args.les_workload)
+                log(f"bench: LES forms on {args.les_workload}")
+                les_case = (lp, lgy, workloads.initial_state(lp, lgy, lic, lseed, lamp))
+            lp, lgy, lst = les_case
+            forms[f] = run_form(lp, lgy, lst, "mixed" if f == "les_mixed" else "fp64", args.steps, args.warmup,
+                                args.timing_steps)
+            forms[f]["workload"] = args.les_workload
+        elif f in ("fp64", "mixed"):
             forms[f] = run_form(p, gy, st, f, args.steps, args.warmup, args.timing_steps)
+        else:
+            raise SystemExit(f"unknown form {f}")
 
     kt = main_form["kt"]
     dname, (dms, dbytes, dct) = max(kt.items(), key=lambda kv: kv[1][0])
     achieved = dbytes / (dms * 1e-3) / 1e9
+    one_pass = (dbytes - main_form["tb"][dname]) / (dms * 1e-3) / 1e9
     roofline = {
         "bound": "hbm",
         "achieved": achieved,
@@ -240,6 +325,10 @@ def main():
         "kernel": dname,
         "bytes_per_launch": dbytes / dct,
         "avg_launch_ms": dms / dct,
+        # the same kernel's bytes without its x-spectra intermediate (what a
+        # one-pass 2-D transform would move) over the same time
+        "achieved_one_pass": one_pass,
+        "frac_one_pass": one_pass / HBM_PEAK_GBS,
     }
     base = forms.get("fp64", main_form)
     line = {
@@ -267,22 +356,35 @@ def main():
         "step_hbm_frac": main_form["step_hbm_frac"],
         "algorithmic_GB_per_step": main_form["algorithmic_GB_per_step"],
         "kernels": main_form["kernels"],
-        "forms": {
-            name: {
-                "value": f["value"],
-                "ms_per_step": f["ms_per_step"],
-                "vs_fp64": f["value"] / base["value"],
-                "step_hbm_frac": f["step_hbm_frac"],
-                "algorithmic_GB_per_step": f["algorithmic_GB_per_step"],
-                "kernels_ms_per_step": {k: v["ms_per_step"] for k, v in f["kernels"].items()},
-            }
-            for name, f in forms.items()
-        },
+        "forms": {},
     }
+    for name, f in forms.items():
+        fp = les_case[0] if f.get("workload", args.workload) != args.workload else p
+        mb = model_bytes_per_step(fp, name)
+        line["forms"][name] = {
+            "value": f["value"],
+            "ms_per_step": f["ms_per_step"],
+            "vs_fp64": f["value"] / base["value"],
+            "workload": f.get("workload", args.workload),
+            "step_hbm_frac": f["step_hbm_frac"],
+            "algorithmic_GB_per_step": f["algorithmic_GB_per_step"],
+            "one_pass_GB_per_step": f["one_pass_GB_per_step"],
+            # the reference algorithm's one-pass compulsory traffic over this
+            # form's step time
+            "model_GB_per_step": mb / 1e9 if mb else None,
+            "model_hbm_frac": mb / (f["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS if mb else None,
+            "kernels": {k: {kk: v[kk] for kk in ("ms_per_step", "GBps", "GBps_one_pass")}
+                        for k, v in f["kernels"].items()},
+        }
+        if "checkpoint_every" in f:
+            line["forms"][name]["checkpoint_every"] = f["checkpoint_every"]
+            line["forms"][name]["steps"] = f["steps"]
+    if "les" in forms and "les_mixed" in forms:
+        line["forms"]["les_mixed"]["vs_les"] = forms["les_mixed"]["value"] / forms["les"]["value"]
     if probe:
         line["copy_probe"] = probe
     if not args.no_cpu_baseline:
-        log("bench: timing the CPU oracle on one step of the same state")
+        log("bench: timing the CPU oracle on three steps of the same state")
         line["cpu_baseline"] = cpu_baseline(p, gy, st, npts)
     else:
         line["cpu_baseline"] = None

@@ -196,6 +196,8 @@ def load_library(path=None):
         lib.dgpu_timing_read.argtypes = [ctypes.c_void_p, P(ctypes.c_double), P(ctypes.c_double),
                                          P(ctypes.c_int64)]
         lib.dgpu_timing_read.restype = ctypes.c_int
+        lib.dgpu_timing_read_intermediate.argtypes = [ctypes.c_void_p, P(ctypes.c_double)]
+        lib.dgpu_timing_read_intermediate.restype = ctypes.c_int
         lib.dgpu_kernel_class_name.argtypes = [ctypes.c_int]
         lib.dgpu_kernel_class_name.restype = ctypes.c_char_p
         h = ctypes.c_void_p
@@ -418,6 +420,13 @@ class ChannelSolver:
             out[name] = (ms[c], by[c], ct[c])
         return out
 
+    def timing_read_intermediate(self):
+        """{kernel class: x-spectra bytes} since timing_enable: the part of the
+        algorithmic bytes a one-pass 2-D transform would not move."""
+        tb = (ctypes.c_double * NCLASS)()
+        self._check(self.lib.dgpu_timing_read_intermediate(self._h, tb), "timing_read_intermediate")
+        return {self.lib.dgpu_kernel_class_name(c).decode(): tb[c] for c in range(NCLASS)}
+
     def synchronize(self):
         self._check(self._fn("synchronize")(self._h), "synchronize")
 
@@ -425,7 +434,10 @@ class ChannelSolver:
         """True when the state (cu1..3, cp, cth, the RK carry and, with LES,
         nu_t/kappa_t) is finite; False when dgpu_check_state reports
         DGPU_ESTATE (``self.state_error`` then names the fields)."""
-        rc = self.lib.dgpu_check_state(self._h)
+        return self._check_state(self.lib.dgpu_check_state)
+
+    def _check_state(self, fn):
+        rc = fn(self._h)
         if rc == -5:
             self.state_error = self.lib.dgpu_last_error(self._h).decode()
             return False

@@ -204,13 +204,14 @@ struct dgpu_handle {
   bool timing = false;
   struct Mark {
     int cls;
-    double bytes;
+    double bytes, tbytes;
     hipEvent_t a, b;
   };
   std::vector<Mark> marks;
   std::vector<hipEvent_t> event_pool;
   double t_ms[DGPU_NCLASS] = {0};
   double t_bytes[DGPU_NCLASS] = {0};
+  double t_tbytes[DGPU_NCLASS] = {0};
   long long t_count[DGPU_NCLASS] = {0};
 };
 
@@ -491,12 +492,16 @@ hipEvent_t take_event(dgpu_handle* h) {
 
 // Brackets one kernel launch with events when timing is on.  `bytes` is the
 // launch's algorithmic HBM traffic (compulsory reads + writes).
+// tbytes: the part of `bytes` that is x-spectra traffic (the intermediate
+// of a two-pass 2-D transform, written by one pass and read by the next),
+// which a one-pass transform would not move.
 struct Timed {
   dgpu_handle* h;
   int cls;
-  double bytes;
+  double bytes, tbytes;
   hipEvent_t a = nullptr;
-  Timed(dgpu_handle* h_, int cls_, double bytes_) : h(h_), cls(cls_), bytes(bytes_) {
+  Timed(dgpu_handle* h_, int cls_, double bytes_, double tbytes_ = 0.0)
+      : h(h_), cls(cls_), bytes(bytes_), tbytes(tbytes_) {
     if (h->timing) {
       a = take_event(h);
       HIPCHECK(hipEventRecord(a, h->stream));
@@ -514,7 +519,7 @@ struct Timed {
         b = nullptr;
       }
       if (b && hipEventRecord(b, h->stream) == hipSuccess) {
-        h->marks.push_back({cls, bytes, a, b});
+        h->marks.push_back({cls, bytes, tbytes, a, b});
       } else {
         h->event_pool.push_back(a);
         if (b) h->event_pool.push_back(b);
@@ -531,6 +536,7 @@ void collect_timing(dgpu_handle* h) {
     HIPCHECK(hipEventElapsedTime(&ms, m.a, m.b));
     h->t_ms[m.cls] += ms;
     h->t_bytes[m.cls] += m.bytes;
+    h->t_tbytes[m.cls] += m.tbytes;
     h->t_count[m.cls] += 1;
     h->event_pool.push_back(m.a);
     h->event_pool.push_back(m.b);
@@ -618,7 +624,8 @@ void launch_xfwd(dgpu_handle* h, const R* u, cpx<R>* T, int j0, int j1, const Fi
   const size_t lds = x_lds<R>(h, ng);
   const int nf = fb ? fb->nf : 1;
   dim3 grid(g.nz / KT, j1 - j0 + 1, nf), block(ng * (g.nx / 8));
-  Timed tm(h, DGPU_K_XFWD, (sizeof(R) * g.pp + 2.0 * sizeof(R) * g.tp) * (j1 - j0 + 1) * nf);
+  Timed tm(h, DGPU_K_XFWD, (sizeof(R) * g.pp + 2.0 * sizeof(R) * g.tp) * (j1 - j0 + 1) * nf,
+           2.0 * sizeof(R) * g.tp * (j1 - j0 + 1) * nf);
   hipLaunchKernelGGL((g.nx <= 512 ? k_xfwd<R, true> : k_xfwd<R, false>), grid, block, lds, h->stream, u, T, g,
                      twx_of<R>(h), j0, fb ? *fb : FieldBatch{});
   HIPCHECK(hipGetLastError());
@@ -634,7 +641,8 @@ void launch_xinv(dgpu_handle* h, const cpx<R>* T, R* u, int j0, int j1, const do
   const size_t lds = x_lds<R>(h, ng);
   const int nf = fb ? fb->nf : 1;
   dim3 grid(g.nz / KT, j1 - j0 + 1, nf), block(ng * (g.nx / 8));
-  Timed tm(h, DGPU_K_XINV, (sizeof(R) * g.pp + 2.0 * sizeof(R) * g.tp) * (j1 - j0 + 1) * nf);
+  Timed tm(h, DGPU_K_XINV, (sizeof(R) * g.pp + 2.0 * sizeof(R) * g.tp) * (j1 - j0 + 1) * nf,
+           2.0 * sizeof(R) * g.tp * (j1 - j0 + 1) * nf);
   hipLaunchKernelGGL((g.nx <= 512 ? k_xinv<R, true> : k_xinv<R, false>), grid, block, lds, h->stream, T, u, g,
                      twx_of<R>(h), j0, dkx, fb ? *fb : FieldBatch{});
   HIPCHECK(hipGetLastError());
@@ -649,7 +657,8 @@ void launch_zfwd(dgpu_handle* h, const cpx<R>* T, double2* C, int j0, int j1, bo
   const size_t lds = z_lds<R>(h, ng);
   const int nf = fb ? fb->nf : 1;
   dim3 grid((g.nq + ng - 1) / ng, j1 - j0 + 1, nf), block(ng * (g.nz / 8));
-  Timed tm(h, DGPU_K_ZFWD, (2.0 * sizeof(R) * g.tp + (accum ? 32.0 : 16.0) * g.sp) * (j1 - j0 + 1) * nf);
+  Timed tm(h, DGPU_K_ZFWD, (2.0 * sizeof(R) * g.tp + (accum ? 32.0 : 16.0) * g.sp) * (j1 - j0 + 1) * nf,
+           2.0 * sizeof(R) * g.tp * (j1 - j0 + 1) * nf);
   hipLaunchKernelGGL((g.nz <= 512 ? k_zfwd<R, true> : k_zfwd<R, false>), grid, block, lds, h->stream, T,
                      reinterpret_cast<cpx<double>*>(C), g, twz_of<R>(h), j0, accum ? 1 : 0, (R)s0, (R)s1,
                      fb ? *fb : FieldBatch{});
@@ -665,7 +674,8 @@ void launch_zinv(dgpu_handle* h, const double2* C, cpx<R>* T, int j0, int j1, do
   const size_t lds = z_lds<R>(h, ng);
   const int nf = fb ? fb->nf : 1;
   dim3 grid((g.nq + ng - 1) / ng, j1 - j0 + 1, nf), block(ng * (g.nz / 8));
-  Timed tm(h, DGPU_K_ZINV, (2.0 * sizeof(R) * g.tp + 16.0 * g.sp) * (j1 - j0 + 1) * nf);
+  Timed tm(h, DGPU_K_ZINV, (2.0 * sizeof(R) * g.tp + 16.0 * g.sp) * (j1 - j0 + 1) * nf,
+           2.0 * sizeof(R) * g.tp * (j1 - j0 + 1) * nf);
   hipLaunchKernelGGL((g.nz <= 512 ? k_zinv<R, true> : k_zinv<R, false>), grid, block, lds, h->stream,
                      reinterpret_cast<const cpx<double>*>(C), T, g, twz_of<R>(h), j0, s0, s1,
                      fb ? *fb : FieldBatch{});
@@ -680,7 +690,8 @@ void launch_zinv_op(dgpu_handle* h, const double2* C, cpx<R>* T, int j0, int j1,
   const int ng = z_ngroups<R>(h);
   const size_t lds = z_lds<R>(h, ng);
   dim3 grid((g.nq + ng - 1) / ng, j1 - j0 + 1), block(ng * (g.nz / 8));
-  Timed tm(h, DGPU_K_ZINV, (2.0 * sizeof(R) * g.tp + 16.0 * g.sp) * (j1 - j0 + 1));
+  Timed tm(h, DGPU_K_ZINV, (2.0 * sizeof(R) * g.tp + 16.0 * g.sp) * (j1 - j0 + 1),
+           2.0 * sizeof(R) * g.tp * (j1 - j0 + 1));
   hipLaunchKernelGGL((g.nz <= 512 ? k_zinv_op<R, true> : k_zinv_op<R, false>), grid, block, lds, h->stream,
                      reinterpret_cast<const cpx<double>*>(C), T, g, twz_of<R>(h), j0, op, h->d_kx.p,
                      h->d_kz.p, h->d_dy.p, h->d_dyf.p, s0, s1);
@@ -1375,18 +1386,14 @@ void set_courant_impl(dgpu_handle* h, const dgpu_courant_params* c) {
   h->var_dt = true;
 }
 
-// ----------------------------------------------------------- the substep --
-// One rk_chan_1 (channel.f90:10-917).  With R = float (the mixed form) the
-// transforms, products and LES run in FP32 while the spectral state, the
-// explicit spectral terms, the CN solves and the projection stay FP64; the
-// RHS pass and the solves then work on increments (RHS less state, new state
-// less old), so the FP64 state is never rounded to FP32 as a whole.
+// ------------------------------------------------- state finiteness --
 // Arrays of doubles scanned for NaN and Inf, one per blockIdx.y.
 struct ScanList {
   const double* a[3 * (3 + DGPU_MAX_TH) + 1];
   long long n[3 * (3 + DGPU_MAX_TH) + 1];
 };
 constexpr int kScanBlocks = 128;  // blocks per array
+constexpr int kScanFields = 3 * (3 + DGPU_MAX_TH) + 1;  // arrays per scan, at most
 
 // Each block writes one flag: 1 when its grid-stride share of the array
 // holds a value that is not finite.  Every flag is written, so none needs
@@ -1406,7 +1413,7 @@ __global__ void __launch_bounds__(256) k_nonfinite(ScanList L, int* __restrict__
 int check_state_impl(dgpu_handle* h) {
   const Geo& g = h->g;
   ScanList L{};
-  const char* names[3 * (3 + DGPU_MAX_TH) + 1];
+  const char* names[kScanFields];
   int nf = 0;
   const long long ns = 2LL * g.sp * g.ny2;  // doubles of a spectral field
   auto add = [&](const void* p, long long n, const char* name) {
@@ -1432,11 +1439,12 @@ int check_state_impl(dgpu_handle* h) {
     add(h->nu_t.p, np, "nu_t");
     for (int n = 0; n < h->nth; ++n) add(h->kappa_t[n].p, np, thn[2]);
   }
-  if (!h->d_flags.p) h->d_flags.alloc((size_t)kScanBlocks * (3 * (3 + DGPU_MAX_TH) + 1));
-  hipLaunchKernelGGL(k_nonfinite, dim3(kScanBlocks, nf), dim3(256), 0, h->stream, L, h->d_flags.p);
-  HIPCHECK(hipGetLastError());
+  if (!h->d_flags.p) h->d_flags.alloc((size_t)kScanBlocks * kScanFields);
+  int* flags = h->d_flags.p;
   std::vector<int> fl((size_t)kScanBlocks * nf);
-  HIPCHECK(hipMemcpyAsync(fl.data(), h->d_flags.p, fl.size() * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  hipLaunchKernelGGL(k_nonfinite, dim3(kScanBlocks, nf), dim3(256), 0, h->stream, L, flags);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipMemcpyAsync(fl.data(), flags, fl.size() * sizeof(int), hipMemcpyDeviceToHost, h->stream));
   HIPCHECK(hipStreamSynchronize(h->stream));
   std::string bad;
   for (int f = 0; f < nf; ++f)
@@ -1465,6 +1473,12 @@ void set_front(const dgpu_handle* h, A& a) {
   a.fr_dr = p.delta / p.ro_inv;
 }
 
+// ----------------------------------------------------------- the substep --
+// One rk_chan_1 (channel.f90:10-917).  With R = float (the mixed form) the
+// transforms, products and LES run in FP32 while the spectral state, the
+// explicit spectral terms, the CN solves and the projection stay FP64; the
+// RHS pass and the solves then work on increments (RHS less state, new state
+// less old), so the FP64 state is never rounded to FP32 as a whole.
 template <typename R>
 void rk_substep_t(dgpu_handle* h, int rk_step, double time, int time_step) {
   const dgpu_params& p = h->p;
@@ -1543,7 +1557,8 @@ void rk_substep_t(dgpu_handle* h, int rk_step, double time, int time_step) {
     const size_t lds = x_lds<R>(h, ngx);
     dim3 grid(g.nz / KT, Ny), block(ngx * (g.nx / 8));
     // reads u1, u2, u3, th once (neighbour planes from cache); writes the product x-spectra
-    Timed tm(h, DGPU_K_XPROD, (sizeof(R) * g.pp * (3 + nth) + 2.0 * sizeof(R) * g.tp * nslot) * Ny);
+    Timed tm(h, DGPU_K_XPROD, (sizeof(R) * g.pp * (3 + nth) + 2.0 * sizeof(R) * g.tp * nslot) * Ny,
+             2.0 * sizeof(R) * g.tp * nslot * Ny);
     hipLaunchKernelGGL((g.nx <= 512 ? k_xprod_all<R, true> : k_xprod_all<R, false>), grid, block, lds, h->stream,
                        pa, g, twx_of<R>(h), 1);
     HIPCHECK(hipGetLastError());
@@ -1620,7 +1635,8 @@ void rk_substep_t(dgpu_handle* h, int rk_step, double time, int time_step) {
     // cfth read and cf1..3, cfth written (neighbour planes come from cache)
     const double crbytes = spec ? 16.0 * g.sp * ncr : 2.0 * sizeof(R) * g.tp * ncr;
     Timed tm(h, DGPU_K_RHS,
-             (2.0 * sizeof(R) * g.tp * nslot + crbytes + 16.0 * g.sp * (4 + 2 * nth + 3 + nth + 3 + nth)) * Ny);
+             (2.0 * sizeof(R) * g.tp * nslot + crbytes + 16.0 * g.sp * (4 + 2 * nth + 3 + nth + 3 + nth)) * Ny,
+             (2.0 * sizeof(R) * g.tp * nslot + (spec ? 0.0 : crbytes)) * Ny);
     hipLaunchKernelGGL((g.nz <= 512 ? k_rhs<R, true> : k_rhs<R, false>), grid, block, lds, h->stream, a, g,
                        twz_of<R>(h), 1);
     HIPCHECK(hipGetLastError());
@@ -2204,7 +2220,7 @@ void adj_substep(dgpu_handle* h, int rk_step, double time, int time_step) {
     const int ng = x_ngroups<double>(h, KT, 512);
     const size_t lds = x_lds<double>(h, ng);
     dim3 grid(g.nz / KT, Ny), block(ng * (g.nx / 8));
-    Timed tm(h, DGPU_K_ADJOINT, (16.0 * g.tp * nslot + 16.0 * g.pp * npairs) * Ny);
+    Timed tm(h, DGPU_K_ADJOINT, (16.0 * g.tp * nslot + 16.0 * g.pp * npairs) * Ny, 16.0 * g.tp * nslot * Ny);
     hipLaunchKernelGGL((g.nx <= 512 ? k_adj_xpairs<true> : k_adj_xpairs<false>), grid, block, lds, h->stream, a,
                        g, twx_of<double>(h), 1);
     HIPCHECK(hipGetLastError());
@@ -2908,6 +2924,7 @@ int dgpu_timing_enable(dgpu_handle* h, int enable) {
     for (int c = 0; c < DGPU_NCLASS; ++c) {
       h->t_ms[c] = 0.0;
       h->t_bytes[c] = 0.0;
+      h->t_tbytes[c] = 0.0;
       h->t_count[c] = 0;
     }
   });
@@ -2922,6 +2939,14 @@ int dgpu_timing_read(dgpu_handle* h, double* ms, double* bytes, int64_t* count) 
       bytes[c] = h->t_bytes[c];
       count[c] = h->t_count[c];
     }
+  });
+}
+
+int dgpu_timing_read_intermediate(dgpu_handle* h, double* tbytes) {
+  if (!h || !tbytes) return DGPU_EINVAL;
+  return run_guarded(h, [&] {
+    collect_timing(h);
+    for (int c = 0; c < DGPU_NCLASS; ++c) tbytes[c] = h->t_tbytes[c];
   });
 }
 

@@ -209,6 +209,7 @@ int dgpu_synchronize(dgpu_handle* h);
  */
 int dgpu_check_state(dgpu_handle* h);
 
+
 /*
  * Adjoint of the FP64 substep, DNS and LES (no counterpart in the reference,
  * which has no differentiation; it is the added "differentiation" form of
@@ -270,6 +271,11 @@ int dgpu_timing_enable(dgpu_handle* h, int enable);
 int dgpu_timing_read(dgpu_handle* h, double* ms /*[DGPU_NCLASS]*/, double* bytes /*[DGPU_NCLASS]*/,
                      int64_t* count /*[DGPU_NCLASS]*/);
 const char* dgpu_kernel_class_name(int cls);
+/* The part of the bytes dgpu_timing_read reports that is x-spectra traffic:
+   the intermediate of the two-pass 2-D transforms, written by an x pass and
+   read by a z pass (or the reverse).  A one-pass transform moves none of it,
+   so bytes - tbytes is the class's traffic under that model. */
+int dgpu_timing_read_intermediate(dgpu_handle* h, double* tbytes /*[DGPU_NCLASS]*/);
 
 /* Sizes of the reference-layout buffers (in doubles) for a field id. */
 int64_t dgpu_field_doubles(const dgpu_handle* h, int field);
