@@ -211,6 +211,7 @@ def load_library(path=None):
             ("mean_profile", [h, ctypes.c_int, d]),
             ("checkpoint", [h, ctypes.c_int, ctypes.c_int]),
             ("check_state", [h]),
+            ("check_state_noalloc", [h]),
         ]:
             fn = getattr(lib, "dgpu_" + name)
             fn.argtypes = args
@@ -435,6 +436,10 @@ class ChannelSolver:
         nu_t/kappa_t) is finite; False when dgpu_check_state reports
         DGPU_ESTATE (``self.state_error`` then names the fields)."""
         return self._check_state(self.lib.dgpu_check_state)
+
+    def check_state_noalloc(self):
+        """check_state through dgpu_check_state_noalloc (no device allocation)."""
+        return self._check_state(self.lib.dgpu_check_state_noalloc)
 
     def _check_state(self, fn):
         rc = fn(self._h)

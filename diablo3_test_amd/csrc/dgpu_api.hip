@@ -10,6 +10,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -149,6 +150,8 @@ struct dgpu_handle {
   DevBuf<double> d_front;
   // per-block flags of the non-finite scan (dgpu_check_state)
   DevBuf<int> d_flags;
+  // address of the static scratch array on this handle's device
+  void* scratch = nullptr;
   // poisson_p_chan: the spectral right-hand side (allocated on first use)
   DevBuf<double2> pois;
   // variable_dt (dgpu_set_courant): settings, CFL*dy(j), Front offsets,
@@ -1395,6 +1398,35 @@ struct ScanList {
 constexpr int kScanBlocks = 128;  // blocks per array
 constexpr int kScanFields = 3 * (3 + DGPU_MAX_TH) + 1;  // arrays per scan, at most
 
+// Scratch of dgpu_check_state_noalloc: the scan's per-block flags in static
+// device memory, so that the call allocates nothing.  Each GPU the library
+// runs on holds one copy (12.5 KiB) until the process ends; every call on
+// that GPU shares it, so calls that overlap on one GPU race.  Host code
+// reaches it only through the address hipGetSymbolAddress returns for the
+// current device (scratch_address); the name itself is a host stand-in.
+constexpr int kScratchInts = kScanBlocks * kScanFields;
+static __device__ __align__(256) int scratch[kScratchInts];
+
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mu;
+void* scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device, looked up once
+// per device (at dgpu_create).
+void* scratch_address() {
+  int dev = 0;
+  HIPCHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) throw ApiError{DGPU_EUNSUP, "device index beyond the scratch table"};
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    HIPCHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) throw ApiError{DGPU_EHIP, "hipGetSymbolAddress returned no address for the scratch array"};
+    scratch_ptr[dev] = ptr;
+  }
+  return scratch_ptr[dev];
+}
+
 // Each block writes one flag: 1 when its grid-stride share of the array
 // holds a value that is not finite.  Every flag is written, so none needs
 // clearing first.
@@ -1409,8 +1441,11 @@ __global__ void __launch_bounds__(256) k_nonfinite(ScanList L, int* __restrict__
 }
 
 // The finiteness check of dgpu_check_state: the spectral state, the RK
-// carry and, with LES, the eddy viscosity and diffusivities.
-int check_state_impl(dgpu_handle* h) {
+// carry and, with LES, the eddy viscosity and diffusivities.  flags holds
+// kScanBlocks ints per scanned array on the device (the handle's buffer,
+// allocated on first use, or the static scratch with noalloc), host the
+// same on the host (a stack array with noalloc).
+int check_state_impl(dgpu_handle* h, bool noalloc) {
   const Geo& g = h->g;
   ScanList L{};
   const char* names[kScanFields];
@@ -1439,12 +1474,26 @@ int check_state_impl(dgpu_handle* h) {
     add(h->nu_t.p, np, "nu_t");
     for (int n = 0; n < h->nth; ++n) add(h->kappa_t[n].p, np, thn[2]);
   }
-  if (!h->d_flags.p) h->d_flags.alloc((size_t)kScanBlocks * kScanFields);
-  int* flags = h->d_flags.p;
-  std::vector<int> fl((size_t)kScanBlocks * nf);
+  if (nf > kScanFields) throw ApiError{DGPU_EINVAL, "more arrays than the scan holds"};
+  int* flags;
+  int fl_stack[kScratchInts];
+  std::vector<int> fl_vec;
+  int* fl = fl_stack;
+  if (noalloc) {
+    // the static scratch of this handle's GPU, looked up at dgpu_create
+    if (!h->scratch) throw ApiError{DGPU_EHIP, "no scratch address for this handle's device"};
+    if ((size_t)kScanBlocks * nf > (size_t)kScratchInts)
+      throw ApiError{DGPU_EINVAL, "scan larger than the scratch array (kScanBlocks x arrays)"};
+    flags = static_cast<int*>(h->scratch);
+  } else {
+    if (!h->d_flags.p) h->d_flags.alloc((size_t)kScanBlocks * kScanFields);
+    flags = h->d_flags.p;
+    fl_vec.resize((size_t)kScanBlocks * nf);
+    fl = fl_vec.data();
+  }
   hipLaunchKernelGGL(k_nonfinite, dim3(kScanBlocks, nf), dim3(256), 0, h->stream, L, flags);
   HIPCHECK(hipGetLastError());
-  HIPCHECK(hipMemcpyAsync(fl.data(), flags, fl.size() * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  HIPCHECK(hipMemcpyAsync(fl, flags, (size_t)kScanBlocks * nf * sizeof(int), hipMemcpyDeviceToHost, h->stream));
   HIPCHECK(hipStreamSynchronize(h->stream));
   std::string bad;
   for (int f = 0; f < nf; ++f)
@@ -2571,6 +2620,7 @@ void create_impl(dgpu_handle* h, const double* gy) {
       HIPCHECK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds_limit));
   }
   HIPCHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  h->scratch = scratch_address();
 
   // kernels index fields with 32-bit arithmetic (plane_of)
   if ((double)g.ny2 * (double)g.pp >= 4294967296.0)
@@ -3063,7 +3113,12 @@ int dgpu_checkpoint(dgpu_handle* h, int slot, int save) {
 
 int dgpu_check_state(dgpu_handle* h) {
   if (!h) return DGPU_EINVAL;
-  return run_guarded(h, [&] { check_state_impl(h); });
+  return run_guarded(h, [&] { check_state_impl(h, false); });
+}
+
+int dgpu_check_state_noalloc(dgpu_handle* h) {
+  if (!h) return DGPU_EINVAL;
+  return run_guarded(h, [&] { check_state_impl(h, true); });
 }
 
 int dgpu_synchronize(dgpu_handle* h) {

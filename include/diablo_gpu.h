@@ -209,6 +209,18 @@ int dgpu_synchronize(dgpu_handle* h);
  */
 int dgpu_check_state(dgpu_handle* h);
 
+/*
+ * dgpu_check_state without any device allocation (no counterpart in the
+ * reference: it is for callers that cannot allocate device memory, such as
+ * a tight serving loop or a harness that forbids hipMalloc).  Same scan and
+ * results as dgpu_check_state, whose per-block flags it keeps in a static
+ * device array of 12.5 KiB (128 blocks x 25 arrays of ints) instead of a
+ * buffer the handle allocates on first use; scans of more arrays are
+ * rejected with DGPU_EINVAL.  Each GPU holds one such array until the
+ * process ends, shared by every call on that GPU: calls that overlap on one
+ * GPU race.  Calls no hipMalloc or hipFree.
+ */
+int dgpu_check_state_noalloc(dgpu_handle* h);
 
 /*
  * Adjoint of the FP64 substep, DNS and LES (no counterpart in the reference,

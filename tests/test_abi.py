@@ -46,7 +46,7 @@ def test_oracle_exports_same_api():
     # GPU-only: the timing hooks, and the adjoint with its checkpoint and
     # mean-profile helpers (the adjoint's checker is the torch reimplementation)
     gpu_only = ("dgpu_timing", "dgpu_kernel_class_name", "dgpu_adj_", "dgpu_mean_profile", "dgpu_checkpoint",
-                "dgpu_check_state")
+                "dgpu_check_state")  # also matches dgpu_check_state_noalloc
     for n in declared_functions():
         if n.startswith(gpu_only):
             continue
