@@ -124,22 +124,45 @@ class WallShearRun:
         return L, (s.adj_get_state(STATE) if fetch else None)
 
 
+def _load(solver, fields, rest):
+    """Put (cu1, cu2, cu3, cth) and the held fields `rest` into the handle:
+    device tensors through their data pointers on torch's current stream,
+    host tensors and arrays through host copies."""
+    for k, v in list(rest.items()) + list(fields.items()):
+        if isinstance(v, torch.Tensor) and v.is_cuda:
+            solver.set_field_device(k, v.detach().contiguous())
+        elif isinstance(v, torch.Tensor):
+            solver.set_field(k, v.detach().numpy())
+        else:
+            solver.set_field(k, v)
+
+
 class WallShearLoss(torch.autograd.Function):
     """L(cu1, cu2, cu3, cth) for a WallShearRun; the other state fields (cp,
     the RK carry) are held fixed at `rest`.  Tensors are complex128 in the
-    reference layout; backward runs the checkpointed HIP adjoint."""
+    reference layout; backward runs the checkpointed HIP adjoint.  Device
+    tensors go in and the gradients come out through the handle's device
+    entry points on torch's current stream, with no host copy of a field;
+    host tensors take the host copies."""
 
     @staticmethod
     def forward(ctx, cu1, cu2, cu3, cth, run, rest):
-        state = dict(rest)
-        for k, v in (("cu1", cu1), ("cu2", cu2), ("cu3", cu3), ("cth", cth)):
-            state[k] = v.detach().cpu().numpy()
-        ctx.run = run
-        ctx.state = state
-        return torch.tensor(run.loss(state), dtype=torch.float64)
+        fields = {"cu1": cu1.detach(), "cu2": cu2.detach(), "cu3": cu3.detach(), "cth": cth.detach()}
+        ctx.run, ctx.rest, ctx.fields = run, rest, fields
+        ctx.device = cu1.device
+        _load(run.s, fields, rest)
+        return torch.tensor(run.loss(None), dtype=torch.float64, device=cu1.device)
 
     @staticmethod
     def backward(ctx, g):
-        _, grad = ctx.run.gradient(ctx.state)
-        out = [g * torch.from_numpy(grad[k]) for k in ("cu1", "cu2", "cu3", "cth")]
+        run = ctx.run
+        _load(run.s, ctx.fields, ctx.rest)
+        run.gradient(None, fetch=False)
+        out = []
+        for k in ("cu1", "cu2", "cu3", "cth"):
+            if ctx.device.type == "cuda":
+                grad = run.s.adj_get_field_device(k, device=ctx.device)
+            else:
+                grad = torch.from_numpy(run.s.adj_get_field(k))
+            out.append(g * grad)
         return out[0], out[1], out[2], out[3], None, None

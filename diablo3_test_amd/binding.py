@@ -211,6 +211,10 @@ def load_library(path=None):
             ("mean_profile", [h, ctypes.c_int, d]),
             ("checkpoint", [h, ctypes.c_int, ctypes.c_int]),
             ("check_state", [h]),
+            ("set_field_device", [h, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
+            ("get_field_device", [h, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
+            ("adj_set_field_device", [h, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
+            ("adj_get_field_device", [h, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
             ("check_state_noalloc", [h]),
         ]:
             fn = getattr(lib, "dgpu_" + name)
@@ -293,6 +297,56 @@ class ChannelSolver:
             f"get_field({name})",
         )
         return a
+
+    # ---- the same exchange with torch tensors in device memory
+    def _device_tensor(self, name, t):
+        import torch
+
+        dt = torch.float64 if name in PHYSICAL else torch.complex128
+        if not (isinstance(t, torch.Tensor) and t.is_cuda):
+            raise TypeError(f"{name}: a device tensor is required")
+        if t.dtype != dt or tuple(t.shape) != self._shape(name) or not t.is_contiguous():
+            raise ValueError(f"{name}: expected a contiguous {dt} tensor of shape {self._shape(name)}, "
+                             f"got {t.dtype} {tuple(t.shape)}")
+        return t
+
+    @staticmethod
+    def _stream(t, stream):
+        import torch
+
+        s = stream if stream is not None else torch.cuda.current_stream(t.device)
+        return ctypes.c_void_p(s.cuda_stream)
+
+    def _new_device(self, name, device):
+        import torch
+
+        dt = torch.float64 if name in PHYSICAL else torch.complex128
+        return torch.empty(self._shape(name), dtype=dt, device=device)
+
+    def set_field_device(self, name, t, stream=None):
+        """set_field from a device tensor (its data_ptr), ordered on `stream`
+        (default: torch's current stream on the tensor's device)."""
+        t = self._device_tensor(name, t)
+        self._check(self.lib.dgpu_set_field_device(self._h, FIELD_IDS[name], ctypes.c_void_p(t.data_ptr()),
+                                                   self._stream(t, stream)), f"set_field_device({name})")
+
+    def get_field_device(self, name, out=None, stream=None, device="cuda"):
+        """get_field into a device tensor (new, or `out`), ordered on `stream`."""
+        t = self._device_tensor(name, out if out is not None else self._new_device(name, device))
+        self._check(self.lib.dgpu_get_field_device(self._h, FIELD_IDS[name], ctypes.c_void_p(t.data_ptr()),
+                                                   self._stream(t, stream)), f"get_field_device({name})")
+        return t
+
+    def adj_set_field_device(self, name, t, stream=None):
+        t = self._device_tensor(name, t)
+        self._check(self.lib.dgpu_adj_set_field_device(self._h, FIELD_IDS[name], ctypes.c_void_p(t.data_ptr()),
+                                                       self._stream(t, stream)), f"adj_set_field_device({name})")
+
+    def adj_get_field_device(self, name, out=None, stream=None, device="cuda"):
+        t = self._device_tensor(name, out if out is not None else self._new_device(name, device))
+        self._check(self.lib.dgpu_adj_get_field_device(self._h, FIELD_IDS[name], ctypes.c_void_p(t.data_ptr()),
+                                                       self._stream(t, stream)), f"adj_get_field_device({name})")
+        return t
 
     def set_state(self, state):
         for k, v in state.items():

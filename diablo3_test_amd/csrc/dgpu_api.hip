@@ -152,6 +152,8 @@ struct dgpu_handle {
   DevBuf<int> d_flags;
   // address of the static scratch array on this handle's device
   void* scratch = nullptr;
+  // ordering events of the copies on a caller's stream (dgpu_*_field_device)
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
   // poisson_p_chan: the spectral right-hand side (allocated on first use)
   DevBuf<double2> pois;
   // variable_dt (dgpu_set_courant): settings, CFL*dy(j), Front offsets,
@@ -2423,6 +2425,112 @@ void phys_to_ref(const dgpu_handle* h, const std::vector<double>& in, double* re
         ref[i + NX2 * (k + NZ2 * j)] = in[(size_t)j * g.pp + (size_t)k * g.nx + i];
 }
 
+// ---------------------------------------- fields in caller device memory --
+// The reference layouts on the device (dgpu_*_field_device): the same
+// mappings as spec_from_ref / spec_to_ref / phys_from_ref / phys_to_ref, one
+// thread per element.  The reference-layout side is written in full (zeros
+// outside the dealiased block or the physical points), so the caller's
+// buffer is fully defined.
+__global__ void __launch_bounds__(256) k_spec_from_ref(const double2* __restrict__ ref, double2* __restrict__ dst,
+                                                       Geo g) {
+  const long long n = (long long)g.ny2 * g.sp;
+  const long long nxc = g.nx / 2 + 1, nz2 = g.nz + 2;
+  for (long long t = (long long)blockIdx.x * 256 + threadIdx.x; t < n; t += (long long)gridDim.x * 256) {
+    const int j = (int)(t / g.sp);
+    const int m = (int)(t - (long long)j * g.sp);
+    const int q = m / g.nkk, kk = m - q * g.nkk;
+    dst[t] = ref[q + nxc * (kk + nz2 * j)];
+  }
+}
+
+__global__ void __launch_bounds__(256) k_spec_to_ref(const double2* __restrict__ src, double2* __restrict__ ref,
+                                                     Geo g) {
+  const long long nxc = g.nx / 2 + 1, nz2 = g.nz + 2;
+  const long long n = nxc * nz2 * g.ny2;
+  for (long long t = (long long)blockIdx.x * 256 + threadIdx.x; t < n; t += (long long)gridDim.x * 256) {
+    const int j = (int)(t / (nxc * nz2));
+    const long long r = t - (long long)j * nxc * nz2;
+    const int kk = (int)(r / nxc), q = (int)(r - (long long)kk * nxc);
+    double2 v = make_double2(0.0, 0.0);
+    if (q < g.nq && kk < g.nkk) v = src[(long long)j * g.sp + (long long)q * g.nkk + kk];
+    ref[t] = v;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_phys_from_ref(const double* __restrict__ ref, double* __restrict__ dst,
+                                                       Geo g) {
+  const long long n = (long long)g.ny2 * g.pp;
+  const long long nx2 = g.nx + 2, nz2 = g.nz + 2;
+  for (long long t = (long long)blockIdx.x * 256 + threadIdx.x; t < n; t += (long long)gridDim.x * 256) {
+    const int j = (int)(t / g.pp);
+    const int m = (int)(t - (long long)j * g.pp);
+    const int k = m / g.nx, i = m - k * g.nx;
+    dst[t] = ref[i + nx2 * (k + nz2 * j)];
+  }
+}
+
+__global__ void __launch_bounds__(256) k_phys_to_ref(const double* __restrict__ src, double* __restrict__ ref, Geo g) {
+  const long long nx2 = g.nx + 2, nz2 = g.nz + 2;
+  const long long n = nx2 * nz2 * g.ny2;
+  for (long long t = (long long)blockIdx.x * 256 + threadIdx.x; t < n; t += (long long)gridDim.x * 256) {
+    const int j = (int)(t / (nx2 * nz2));
+    const long long r = t - (long long)j * nx2 * nz2;
+    const int k = (int)(r / nx2), i = (int)(r - (long long)k * nx2);
+    ref[t] = (i < g.nx && k < g.nz) ? src[(long long)j * g.pp + (long long)k * g.nx + i] : 0.0;
+  }
+}
+
+bool field_buf(dgpu_handle* h, int f, int n, void** dev, bool* spec);
+bool adj_field_buf(dgpu_handle* h, int f, int n, void** dev);
+int nfield_copies(const dgpu_handle* h, int f);
+void adj_alloc(dgpu_handle* h);
+size_t ref_spec_doubles(const Geo& g);
+
+// Copy field `field` between the handle (forward or adjoint state) and the
+// caller's device buffer `dev` in the reference layout, on the caller's
+// stream `cs`.  The copy waits for the work already queued on the handle's
+// stream, and the handle's later work waits for the copy; the host does not
+// wait.
+void field_device_copy(dgpu_handle* h, int field, void* dev, hipStream_t cs, bool to_handle, bool adjoint) {
+  const Geo& g = h->g;
+  if (adjoint) {
+    if (field < 0 || field > DGPU_CFTH) throw ApiError{DGPU_EINVAL, "bad adjoint field id"};
+    adj_alloc(h);
+  } else if (field < 0 || field >= DGPU_NFIELDS) {
+    throw ApiError{DGPU_EINVAL, "bad field id"};
+  }
+  if (!h->ev_in) {
+    HIPCHECK(hipEventCreateWithFlags(&h->ev_in, hipEventDisableTiming));
+    HIPCHECK(hipEventCreateWithFlags(&h->ev_out, hipEventDisableTiming));
+  }
+  HIPCHECK(hipEventRecord(h->ev_in, h->stream));
+  HIPCHECK(hipStreamWaitEvent(cs, h->ev_in, 0));
+  const size_t per = ref_spec_doubles(g);
+  const dim3 grid(2048), block(256);
+  for (int n = 0; n < nfield_copies(h, field); ++n) {
+    void* buf = nullptr;
+    bool spec = true;
+    if (adjoint)
+      adj_field_buf(h, field, n, &buf);
+    else
+      field_buf(h, field, n, &buf, &spec);
+    double* ref = static_cast<double*>(dev) + n * per;
+    if (spec && to_handle)
+      hipLaunchKernelGGL(k_spec_from_ref, grid, block, 0, cs, reinterpret_cast<const double2*>(ref),
+                         static_cast<double2*>(buf), g);
+    else if (spec)
+      hipLaunchKernelGGL(k_spec_to_ref, grid, block, 0, cs, static_cast<const double2*>(buf),
+                         reinterpret_cast<double2*>(ref), g);
+    else if (to_handle)
+      hipLaunchKernelGGL(k_phys_from_ref, grid, block, 0, cs, ref, static_cast<double*>(buf), g);
+    else
+      hipLaunchKernelGGL(k_phys_to_ref, grid, block, 0, cs, static_cast<const double*>(buf), ref, g);
+    HIPCHECK(hipGetLastError());
+  }
+  HIPCHECK(hipEventRecord(h->ev_out, cs));
+  HIPCHECK(hipStreamWaitEvent(h->stream, h->ev_out, 0));
+}
+
 // (buffer, is_spectral) for field id f, scalar n
 bool adj_field_buf(dgpu_handle* h, int f, int n, void** dev) {
   switch (f) {
@@ -2781,6 +2889,8 @@ int dgpu_destroy(dgpu_handle* h) {
     (void)hipEventDestroy(m.b);
   }
   for (auto e : h->event_pool) (void)hipEventDestroy(e);
+  if (h->ev_in) (void)hipEventDestroy(h->ev_in);
+  if (h->ev_out) (void)hipEventDestroy(h->ev_out);
   free_all(h);
   (void)hipStreamDestroy(h->stream);
   delete h;
@@ -3046,6 +3156,30 @@ int dgpu_adj_get_field(dgpu_handle* h, int field, double* host) {
       spec_to_ref(h, v, host + n * per);
     }
   });
+}
+
+int dgpu_set_field_device(dgpu_handle* h, int field, const double* dev, void* stream) {
+  if (!h || !dev) return DGPU_EINVAL;
+  return run_guarded(h, [&] {
+    field_device_copy(h, field, const_cast<double*>(dev), static_cast<hipStream_t>(stream), true, false);
+  });
+}
+
+int dgpu_get_field_device(dgpu_handle* h, int field, double* dev, void* stream) {
+  if (!h || !dev) return DGPU_EINVAL;
+  return run_guarded(h, [&] { field_device_copy(h, field, dev, static_cast<hipStream_t>(stream), false, false); });
+}
+
+int dgpu_adj_set_field_device(dgpu_handle* h, int field, const double* dev, void* stream) {
+  if (!h || !dev) return DGPU_EINVAL;
+  return run_guarded(h, [&] {
+    field_device_copy(h, field, const_cast<double*>(dev), static_cast<hipStream_t>(stream), true, true);
+  });
+}
+
+int dgpu_adj_get_field_device(dgpu_handle* h, int field, double* dev, void* stream) {
+  if (!h || !dev) return DGPU_EINVAL;
+  return run_guarded(h, [&] { field_device_copy(h, field, dev, static_cast<hipStream_t>(stream), false, true); });
 }
 
 int dgpu_adj_zero(dgpu_handle* h) {

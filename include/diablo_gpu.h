@@ -139,6 +139,19 @@ int dgpu_get_field(dgpu_handle* h, int field, double* host);
 int dgpu_get_physical(dgpu_handle* h, int field, double* host);
 int dgpu_set_physical(dgpu_handle* h, int field, const double* host);
 
+/*
+ * The same exchange with a buffer in device memory (for the torch side,
+ * which passes tensor.data_ptr() and its current stream): dev holds the
+ * field in the reference layout above (dgpu_field_doubles(h, field)
+ * doubles), and the layout conversion runs on `stream` (a hipStream_t; NULL
+ * is the null stream).  The copy is ordered after the work already queued on
+ * the handle's stream, and the handle's later work after the copy, through
+ * events; the host does not wait.  dgpu_get_field_device writes the whole
+ * reference-layout buffer (zeros outside the dealiased block).
+ */
+int dgpu_set_field_device(dgpu_handle* h, int field, const double* dev, void* stream);
+int dgpu_get_field_device(dgpu_handle* h, int field, double* dev, void* stream);
+
 /* apply_BC_vel_mpi_post (boundary.f90:299-498, 824-838) on cu1..3. */
 int dgpu_apply_bc_vel_post(dgpu_handle* h, double time);
 /* rem_div_chan (channel.f90:946-1023): project cu1..3 onto the discrete
@@ -243,6 +256,9 @@ int dgpu_check_state_noalloc(dgpu_handle* h);
 int dgpu_adj_set_field(dgpu_handle* h, int field, const double* host);
 int dgpu_adj_get_field(dgpu_handle* h, int field, double* host);
 int dgpu_adj_zero(dgpu_handle* h);
+/* The adjoint state through device buffers, as dgpu_set_field_device. */
+int dgpu_adj_set_field_device(dgpu_handle* h, int field, const double* dev, void* stream);
+int dgpu_adj_get_field_device(dgpu_handle* h, int field, double* dev, void* stream);
 int dgpu_adj_rk_substep(dgpu_handle* h, int rk_step, double time, int time_step);
 /* Adds w[j] (j = 0..Ny+1) to the real part of the (0,0) mode of the adjoint
    of field (cu1, cu2, cu3 or cp): the gradient of a loss of the mean profile,

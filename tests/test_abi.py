@@ -43,10 +43,11 @@ def test_oracle_exports_same_api():
     oracle_lib()
     out = subprocess.run(["nm", "-D", "--defined-only", ORACLE_LIB], capture_output=True, text=True).stdout
     exported = set(re.findall(r"\bT (orc_\w+)", out))
-    # GPU-only: the timing hooks, and the adjoint with its checkpoint and
-    # mean-profile helpers (the adjoint's checker is the torch reimplementation)
+    # GPU-only: the timing hooks, the adjoint with its checkpoint and
+    # mean-profile helpers (the adjoint's checker is the torch
+    # reimplementation), the state check and the device-memory exchange
     gpu_only = ("dgpu_timing", "dgpu_kernel_class_name", "dgpu_adj_", "dgpu_mean_profile", "dgpu_checkpoint",
-                "dgpu_check_state")  # also matches dgpu_check_state_noalloc
+                "dgpu_check_state", "dgpu_set_field_device", "dgpu_get_field_device")  # also matches dgpu_check_state_noalloc
     for n in declared_functions():
         if n.startswith(gpu_only):
             continue
