@@ -400,6 +400,7 @@ __global__ void __launch_bounds__(512) k_adj_xpairs(AdjPairArgs a, Geo g, const 
   const int lg_ng = 31 - __builtin_clz(ngroups);
   const int grp = threadIdx.x / G, gt = threadIdx.x % G;
   C* fbuf = smem + grp * L;
+  const C* twl = lds_twiddles(smem + ngroups * L, tw, g.nx);
   const int j = j0 + blockIdx.y, k0 = blockIdx.x * KT;
   const double inv = 1.0 / g.nx;
   for (int pr = 0; pr < a.npairs; ++pr) {
@@ -438,7 +439,7 @@ __global__ void __launch_bounds__(512) k_adj_xpairs(AdjPairArgs a, Geo g, const 
       C v[8];
 #pragma unroll
       for (int r = 0; r < 8; ++r) v[r] = fbuf[pidx(gt + G * r)];
-      fft_regs<double, 1, WAVE>(v, fbuf, g.nx, g.lognx, tw, gt);
+      fft_regs<double, 1, WAVE>(v, fbuf, g.nx, g.lognx, twl, gt);
       const long long ro = (long long)j * g.pp + (long long)(k0 + m0 + grp) * g.nx;
       double* oa = a.outA[pr] + ro;
       double* ob = a.outB[pr] + ro;

@@ -590,7 +590,8 @@ int x_ngroups(const dgpu_handle* h, int jobs, int max_threads = 512) {
   const Geo& g = h->g;
   const int G = g.nx / 8;
   int ng = jobs;
-  auto lds = [&](int n) { return (size_t)n * x_group_stride(g.nx) * sizeof(cpx<R>); };
+  // the group buffers and the twiddle table (lds_twiddles)
+  auto lds = [&](int n) { return ((size_t)n * x_group_stride(g.nx) + g.nx) * sizeof(cpx<R>); };
   while (ng > 1 && (ng * G > max_threads || lds(ng) > h->lds_limit)) ng /= 2;
   if (ng * G > max_threads || lds(ng) > h->lds_limit)
     throw ApiError{DGPU_EUNSUP, "Nx too large for the x-transform pass"};
@@ -598,8 +599,8 @@ int x_ngroups(const dgpu_handle* h, int jobs, int max_threads = 512) {
 }
 
 template <typename R>
-size_t x_lds(const dgpu_handle* h, int ng) {
-  return (size_t)ng * x_group_stride(h->g.nx) * sizeof(cpx<R>);
+size_t x_lds(const dgpu_handle* h, int ng, bool table = true) {
+  return ((size_t)ng * x_group_stride(h->g.nx) + (table ? h->g.nx : 0)) * sizeof(cpx<R>);
 }
 
 template <typename R>
@@ -609,7 +610,7 @@ int z_ngroups(const dgpu_handle* h) {
   if (G > 512) throw ApiError{DGPU_EUNSUP, "Nz too large for the z-transform pass"};
   int ng = 256 / G;
   if (ng < 1) ng = 1;
-  auto lds = [&](int n) { return (size_t)n * fft_lds_len(g.nz) * sizeof(cpx<R>); };
+  auto lds = [&](int n) { return ((size_t)n * fft_lds_len(g.nz) + g.nz) * sizeof(cpx<R>); };
   while (ng > 1 && lds(ng) > h->lds_limit) --ng;
   if (lds(ng) > h->lds_limit) throw ApiError{DGPU_EUNSUP, "Nz too large for the z-transform pass"};
   return ng;
@@ -617,7 +618,7 @@ int z_ngroups(const dgpu_handle* h) {
 
 template <typename R>
 size_t z_lds(const dgpu_handle* h, int ng) {
-  return (size_t)ng * fft_lds_len(h->g.nz) * sizeof(cpx<R>);
+  return ((size_t)ng * fft_lds_len(h->g.nz) + h->g.nz) * sizeof(cpx<R>);
 }
 
 template <typename R>
@@ -643,7 +644,7 @@ void launch_xinv(dgpu_handle* h, const cpx<R>* T, R* u, int j0, int j1, const do
   const Geo& g = h->g;
   constexpr int KT = XTile<R>::KT;
   const int ng = x_ngroups<R>(h, KT / 2, h->kn.xpass_threads);
-  const size_t lds = x_lds<R>(h, ng);
+  const size_t lds = x_lds<R>(h, ng, XInvTable<R>::on);
   const int nf = fb ? fb->nf : 1;
   dim3 grid(g.nz / KT, j1 - j0 + 1, nf), block(ng * (g.nx / 8));
   Timed tm(h, DGPU_K_XINV, (sizeof(R) * g.pp + 2.0 * sizeof(R) * g.tp) * (j1 - j0 + 1) * nf,
@@ -1606,6 +1607,7 @@ void rk_substep_t(dgpu_handle* h, int rk_step, double time, int time_step) {
     constexpr int KT = XTile<R>::KT;
     const int ngx = x_ngroups<R>(h, KT, h->kn.xprod_threads);
     const size_t lds = x_lds<R>(h, ngx);
+    if (lds > h->lds_limit) throw ApiError{DGPU_EUNSUP, "Nx too large for the products pass"};
     dim3 grid(g.nz / KT, Ny), block(ngx * (g.nx / 8));
     // reads u1, u2, u3, th once (neighbour planes from cache); writes the product x-spectra
     Timed tm(h, DGPU_K_XPROD, (sizeof(R) * g.pp * (3 + nth) + 2.0 * sizeof(R) * g.tp * nslot) * Ny,
@@ -1679,7 +1681,7 @@ void rk_substep_t(dgpu_handle* h, int rk_step, double time, int time_step) {
     if (ng < 1) ng = 1;
     if (ng > nslot) ng = nslot;
     const int rounds = (nslot + ng - 1) / ng;
-    const size_t lds = (size_t)rounds * ng * fft_lds_len(g.nz) * sizeof(cpx<R>);
+    const size_t lds = ((size_t)rounds * ng * fft_lds_len(g.nz) + g.nz) * sizeof(cpx<R>);  // + twiddles
     if (lds > h->lds_limit) throw ApiError{DGPU_EUNSUP, "Nz too large for the spectral RHS pass"};
     dim3 grid(g.nq, Ny), block(ng * G);
     // x-spectra of the products in, cr x-spectra out; cu1..3, cp, cth, cf1..3,

@@ -23,6 +23,12 @@ struct XTile {
   static constexpr int KT = 128 / (2 * (int)sizeof(R));
 };
 
+// Whether the inverse x pass keeps its twiddle table in LDS (see k_xinv).
+template <typename R>
+struct XInvTable {
+  static constexpr bool on = sizeof(R) == 4;
+};
+
 struct Geo {
   int nx, ny, nz;     // Nx, Ny, Nz
   int lognx, lognz;   // log2 Nx, log2 Nz
@@ -90,6 +96,7 @@ __global__ void __launch_bounds__(512) k_xfwd(const R* __restrict__ u, cpx<R>* _
   const int ngroups = blockDim.x / G;
   const int grp = threadIdx.x / G, gt = threadIdx.x % G;
   cpx<R>* fbuf = smem + grp * L;
+  const cpx<R>* twl = lds_twiddles(smem + ngroups * L, tw, g.nx);
   const int j = j0 + blockIdx.y, k0 = blockIdx.x * KT;
   const R* plane = u + (long long)j * g.pp;
   cpx<R>* out = T + (long long)j * g.tp + k0;
@@ -102,7 +109,7 @@ __global__ void __launch_bounds__(512) k_xfwd(const R* __restrict__ u, cpx<R>* _
     cpx<R> v[8];
 #pragma unroll
     for (int r = 0; r < 8; ++r) v[r] = {ld_in(ra + gt + G * r), ld_in(rb + gt + G * r)};
-    fft_regs<R, -1, WAVE>(v, fbuf, g.nx, g.lognx, tw, gt);
+    fft_regs<R, -1, WAVE>(v, fbuf, g.nx, g.lognx, twl, gt);
     __syncthreads();  // every group's spectra are read below
     for (int idx = threadIdx.x; idx < g.nq * nr; idx += blockDim.x) {
       const int q = idx / nr, kk = idx - q * nr;
@@ -135,6 +142,10 @@ __global__ void __launch_bounds__(512) k_xinv(const cpx<R>* __restrict__ T, R* _
   const int ngroups = blockDim.x / G;
   const int grp = threadIdx.x / G, gt = threadIdx.x % G;
   cpx<R>* fbuf = smem + grp * L;
+  // twiddles in LDS for the FP32 pass only: in FP64 the 8 KB table costs a
+  // workgroup per CU (4 -> 3 at 512 points), and the pass went from 2.32 to
+  // 2.50 ms per step at 512x257x512 with it, against 1.39 -> 1.11 in FP32
+  const cpx<R>* twl = XInvTable<R>::on ? lds_twiddles(smem + ngroups * L, tw, g.nx) : tw;
   const int j = j0 + blockIdx.y, k0 = blockIdx.x * KT;
   const cpx<R>* in = T + (long long)j * g.tp + k0;
   R* plane = u + (long long)j * g.pp;
@@ -171,7 +182,7 @@ __global__ void __launch_bounds__(512) k_xinv(const cpx<R>* __restrict__ T, R* _
     cpx<R> v[8];
 #pragma unroll
     for (int r = 0; r < 8; ++r) v[r] = fbuf[pidx(gt + G * r)];
-    fft_regs<R, 1, WAVE>(v, fbuf, g.nx, g.lognx, tw, gt);
+    fft_regs<R, 1, WAVE>(v, fbuf, g.nx, g.lognx, twl, gt);
     R* ra = plane + (long long)(k0 + 2 * (m0 + grp)) * g.nx;
     R* rb = ra + g.nx;
     for (int i = gt; i < g.nx; i += G) {
@@ -201,6 +212,7 @@ __global__ void __launch_bounds__(512) k_zfwd(const cpx<R>* __restrict__ T, cpx<
   const int ngroups = blockDim.x / G;
   const int grp = threadIdx.x / G, gt = threadIdx.x % G;
   cpx<R>* fbuf = smem + grp * fft_lds_len(g.nz);
+  const cpx<R>* twl = lds_twiddles(smem + ngroups * fft_lds_len(g.nz), tw, g.nz);
   const int j = j0 + blockIdx.y, q = blockIdx.x * ngroups + grp;
   const bool act = q < g.nq;
   cpx<R> v[8];
@@ -212,7 +224,7 @@ __global__ void __launch_bounds__(512) k_zfwd(const cpx<R>* __restrict__ T, cpx<
 #pragma unroll
     for (int r = 0; r < 8; ++r) v[r] = {(R)0, (R)0};
   }
-  fft_regs<R, -1, WAVE>(v, fbuf, g.nz, g.lognz, tw, gt);
+  fft_regs<R, -1, WAVE>(v, fbuf, g.nz, g.lognz, twl, gt);
   if (act) {
     // 1/Nz, times s0 (kx = 0) or s1 (kx > 0): 1 for the transform itself,
     // Nx Nz W' for the transpose of the inverse transform
@@ -244,6 +256,7 @@ __global__ void __launch_bounds__(512) k_zinv(const cpx<double>* __restrict__ C,
   const int ngroups = blockDim.x / G;
   const int grp = threadIdx.x / G, gt = threadIdx.x % G;
   cpx<R>* fbuf = smem + grp * fft_lds_len(g.nz);
+  const cpx<R>* twl = lds_twiddles(smem + ngroups * fft_lds_len(g.nz), tw, g.nz);
   const int j = j0 + blockIdx.y, q = blockIdx.x * ngroups + grp;
   const bool act = q < g.nq;
   cpx<R> v[8];
@@ -262,7 +275,7 @@ __global__ void __launch_bounds__(512) k_zinv(const cpx<double>* __restrict__ C,
     }
     v[r] = {(R)(sc * z.x), (R)(sc * z.y)};
   }
-  fft_regs<R, 1, WAVE>(v, fbuf, g.nz, g.lognz, tw, gt);
+  fft_regs<R, 1, WAVE>(v, fbuf, g.nz, g.lognz, twl, gt);
   if (act) {
     cpx<R>* out = T + (long long)j * g.tp + (long long)q * g.nz;
     for (int k = gt; k < g.nz; k += G) st_out(out + k, fbuf[pidx(k)]);
@@ -327,15 +340,16 @@ __device__ __forceinline__ void pair_ranges(const ProdArgs& a, const Geo& g, int
 // each scalar (pairs 4, 5) -- and every product is formed from registers, so
 // each input row crosses the memory system once per workgroup.
 template <typename R>
-__device__ __forceinline__ void emit_pair(cpx<R>* smem, int L, const Geo& g, int ngroups, int pair,
+__device__ __forceinline__ void emit_pair(cpx<R>* smem, int L, const Geo& g, int lg_ng, int pair,
                                           R inv, const double* kx, cpx<R>* out0, cpx<R>* out1,
                                           int m0) {
+  const int ngroups = 1 << lg_ng;
   for (int idx = threadIdx.x; idx < g.nq * ngroups; idx += blockDim.x) {
-    const int q = idx / ngroups, mm = idx - q * ngroups;
+    const int q = idx >> lg_ng, mm = idx & (ngroups - 1);
     cpx<R> A, B;
     split_pair(smem + mm * L, g.nx, q, inv, A, B);
     const cpx<R> ikA = ik((R)kx[q], A);
-    const long long o = (long long)q * g.nz + m0 + mm;
+    const int o = q * g.nz + m0 + mm;  // within one plane's x-spectra
     switch (pair) {
       case 0: st_out(out0 + o, ikA + B); break;
       case 1: st_out(out0 + o, A); st_out(out1 + o, ikA + B); break;
@@ -355,19 +369,24 @@ __global__ void __launch_bounds__(512) k_xprod_all(ProdAllArgs a, Geo g,
   cpx<R>* smem = reinterpret_cast<cpx<R>*>(smem_raw);
   const int G = g.nx >> 3;  // 8 values of a row per thread
   const int L = x_group_stride(g.nx);
-  const int ngroups = blockDim.x / G;
+  const int ngroups = blockDim.x / G;  // a power of two
+  const int lg_ng = 31 - __builtin_clz(ngroups);
   const int grp = threadIdx.x / G, gt = threadIdx.x % G;
   cpx<R>* fbuf = smem + grp * L;
+  const cpx<R>* twl = lds_twiddles(smem + ngroups * L, tw, g.nx);
   const int j = j0 + blockIdx.y, k0 = blockIdx.x * KT;
   const R inv = (R)1 / (R)g.nx;
   const ProdArgs& b = a.base;
+  // plane j of each field (uniform across the block); per-thread offsets
+  // within a plane are 32-bit
+  const long long pp = g.pp;
   const R* U1 = reinterpret_cast<const R*>(b.u1);
   const R* U2 = reinterpret_cast<const R*>(b.u2);
   const R* U3 = reinterpret_cast<const R*>(b.u3);
   const R dyj = (R)b.dy[j], dyfj = (R)b.dyf[j], dyfm = (R)b.dyf[j - 1];
   const R c2dyf = (R)1 / ((R)2 * dyfj);
+  const R c2dy = (R)1 / ((R)2 * dyj), cdy = (R)1 / dyj;
   const R quarter = (R)0.25, zero = (R)0;
-  const long long pp = g.pp;
   auto outp = [&](int s) {
     return reinterpret_cast<cpx<R>*>(a.out[s]) + (long long)j * g.tp + k0;
   };
@@ -377,7 +396,7 @@ __global__ void __launch_bounds__(512) k_xprod_all(ProdAllArgs a, Geo g,
     cpx<R> v[8];
 #pragma unroll
     for (int r = 0; r < 8; ++r) v[r] = {va[r], vb[r]};
-    fft_regs<R, -1, WAVE>(v, fbuf, g.nx, g.lognx, tw, gt);
+    fft_regs<R, -1, WAVE>(v, fbuf, g.nx, g.lognx, twl, gt);
     __syncthreads();
   };
   bool r1, r2;
@@ -408,19 +427,19 @@ __global__ void __launch_bounds__(512) k_xprod_all(ProdAllArgs a, Geo g,
         vb[r] = (u1p[r] * u2p[r] + u1c[r] * u2p[r] - u1c[r] * u2c[r] - u1m[r] * u2c[r]) * c2dyf;
       }
       fft_pair(va, vb);
-      emit_pair(smem, L, g, ngroups, 0, inv, b.kx, outp(0), (cpx<R>*)nullptr, m0);
+      emit_pair(smem, L, g, lg_ng, 0, inv, b.kx, outp(0), (cpx<R>*)nullptr, m0);
       __syncthreads();
     }
     // pair 3: (W1, V2) -> slot 4 = ikx F(W1) + F(V2)                   (:356-365, 440-449)
     if (r2) {
 #pragma unroll
       for (int r = 0; r < 8; ++r) {
-        va[r] = ((dyfj * u1c[r] + dyfm * u1m[r]) / ((R)2 * dyj)) * u2c[r];
+        va[r] = ((dyfj * u1c[r] + dyfm * u1m[r]) * c2dy) * u2c[r];
         const R sp = u2c[r] + u2p[r], sm = u2c[r] + u2m[r];
-        vb[r] = (quarter * (sp * sp) - quarter * (sm * sm)) / dyj;
+        vb[r] = (quarter * (sp * sp) - quarter * (sm * sm)) * cdy;
       }
       fft_pair(va, vb);
-      emit_pair(smem, L, g, ngroups, 3, inv, b.kx, outp(4), (cpx<R>*)nullptr, m0);
+      emit_pair(smem, L, g, lg_ng, 3, inv, b.kx, outp(4), (cpx<R>*)nullptr, m0);
       __syncthreads();
     }
     // u3 at j-1, j, j+1 replaces u1 at j-1, j+1 and u2 at j-1
@@ -442,7 +461,7 @@ __global__ void __launch_bounds__(512) k_xprod_all(ProdAllArgs a, Geo g,
         vb[r] = (u3p[r] * u2p[r] + u3c[r] * u2p[r] - u3c[r] * u2c[r] - u3m[r] * u2c[r]) * c2dyf;
       }
       fft_pair(va, vb);
-      emit_pair(smem, L, g, ngroups, 1, inv, b.kx, outp(1), outp(2), m0);
+      emit_pair(smem, L, g, lg_ng, 1, inv, b.kx, outp(1), outp(2), m0);
       __syncthreads();
     }
     // pair 2: (u3u3, W3) -> slot 3 = F(u3u3), slot 5 = F(W3)            (:337-344, 377-386)
@@ -450,10 +469,10 @@ __global__ void __launch_bounds__(512) k_xprod_all(ProdAllArgs a, Geo g,
 #pragma unroll
       for (int r = 0; r < 8; ++r) {
         va[r] = r1 ? u3c[r] * u3c[r] : zero;
-        vb[r] = r2 ? ((dyfj * u3c[r] + dyfm * u3m[r]) / ((R)2 * dyj)) * u2c[r] : zero;
+        vb[r] = r2 ? ((dyfj * u3c[r] + dyfm * u3m[r]) * c2dy) * u2c[r] : zero;
       }
       fft_pair(va, vb);
-      emit_pair(smem, L, g, ngroups, 2, inv, b.kx, outp(3), outp(5), m0);
+      emit_pair(smem, L, g, lg_ng, 2, inv, b.kx, outp(3), outp(5), m0);
       __syncthreads();
     }
     // scalars: (th u1, Vt) -> slot 6+2n, (th u3, 0) -> slot 7+2n       (:557-600)
@@ -474,7 +493,7 @@ __global__ void __launch_bounds__(512) k_xprod_all(ProdAllArgs a, Geo g,
         vb[r] = (thp[r] * u2p[r] + thc[r] * u2p[r] - thc[r] * u2c[r] - thm[r] * u2c[r]) * c2dyf;
       }
       fft_pair(va, vb);
-      emit_pair(smem, L, g, ngroups, 4, inv, b.kx, outp(6 + 2 * n), (cpx<R>*)nullptr, m0);
+      emit_pair(smem, L, g, lg_ng, 4, inv, b.kx, outp(6 + 2 * n), (cpx<R>*)nullptr, m0);
       __syncthreads();
 #pragma unroll
       for (int r = 0; r < 8; ++r) {
@@ -482,7 +501,7 @@ __global__ void __launch_bounds__(512) k_xprod_all(ProdAllArgs a, Geo g,
         vb[r] = zero;
       }
       fft_pair(va, vb);
-      emit_pair(smem, L, g, ngroups, 5, inv, b.kx, outp(7 + 2 * n), (cpx<R>*)nullptr, m0);
+      emit_pair(smem, L, g, lg_ng, 5, inv, b.kx, outp(7 + 2 * n), (cpx<R>*)nullptr, m0);
       __syncthreads();
     }
   }
@@ -565,6 +584,8 @@ __global__ void __launch_bounds__(512) k_rhs(RhsArgs a, Geo g, const cpx<R>* __r
     return inT[(s - 6) >> 1];
   };
   const long long to = (long long)j * g.tp + (long long)q * g.nz;
+  const int rounds = (nslot + ngroups - 1) / ngroups;
+  const CR* twl = lds_twiddles(smem + (long long)rounds * ngroups * L, tw, g.nz);
   // --- z-transform the product spectra
   // The LDS holds one buffer per (round, group), so groups without a slot in
   // a round transform an unused buffer and every thread meets every barrier.
@@ -581,7 +602,7 @@ __global__ void __launch_bounds__(512) k_rhs(RhsArgs a, Geo g, const cpx<R>* __r
 #pragma unroll
       for (int r = 0; r < 8; ++r) v[r] = CR{(R)0, (R)0};
     }
-    fft_regs<R, -1, WAVE>(v, buf, g.nz, g.lognz, tw, gt);
+    fft_regs<R, -1, WAVE>(v, buf, g.nz, g.lognz, twl, gt);
   }
   __syncthreads();  // the combine below reads every slot
   // --- explicit terms, mode by mode
@@ -724,7 +745,7 @@ __global__ void __launch_bounds__(512) k_rhs(RhsArgs a, Geo g, const cpx<R>* __r
     CR v[8];
 #pragma unroll
     for (int r = 0; r < 8; ++r) v[r] = buf[pidx(gt + G * r)];
-    fft_regs<R, 1, WAVE>(v, buf, g.nz, g.lognz, tw, gt);
+    fft_regs<R, 1, WAVE>(v, buf, g.nz, g.lognz, twl, gt);
     if (act) {
       CR* out = reinterpret_cast<CR*>(a.Tcr[c]) + to;
       for (int k = gt; k < g.nz; k += G) out[k] = buf[pidx(k)];

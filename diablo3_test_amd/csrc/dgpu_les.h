@@ -55,6 +55,7 @@ __global__ void __launch_bounds__(512) k_zinv_op(const cpx<double>* __restrict__
   const int ngroups = blockDim.x / G;
   const int grp = threadIdx.x / G, gt = threadIdx.x % G;
   cpx<R>* fbuf = smem + grp * fft_lds_len(g.nz);
+  const cpx<R>* twl = lds_twiddles(smem + ngroups * fft_lds_len(g.nz), tw, g.nz);
   const int j = j0 + blockIdx.y, q = blockIdx.x * ngroups + grp;
   bool act = q < g.nq;
   if ((op == OP_DY_GY || op == OP_DY_C2) && j == 0) act = false;
@@ -93,7 +94,7 @@ __global__ void __launch_bounds__(512) k_zinv_op(const cpx<double>* __restrict__
     }
     v[r] = {(R)(sc * z.x), (R)(sc * z.y)};
   }
-  fft_regs<R, 1, WAVE>(v, fbuf, g.nz, g.lognz, tw, gt);
+  fft_regs<R, 1, WAVE>(v, fbuf, g.nz, g.lognz, twl, gt);
   if (q < g.nq) {
     cpx<R>* out = T + (long long)j * g.tp + (long long)q * g.nz;
     for (int k = gt; k < g.nz; k += G) st_out(out + k, fbuf[pidx(k)]);
