@@ -809,10 +809,12 @@ void to_fourier_fields(dgpu_handle* h, int nf, R* const* u, double2* const* C, c
 // The CN solve of one field on planes [ca, cb]: physical columns (r, u with
 // plane pitch pp), or with spec the dealiased spectra (r = cr, u = the
 // spectral state, FP64, 2 sp real columns a plane).
+// With r32 (spectral FP64 state only) r holds FP32 increments cr - u and
+// the solve adds the state back in FP64 (k_thomas_seg's RPLUS).
 template <typename R>
 void launch_thomas(dgpu_handle* h, int field, int rs, const R* r, R* u, int ca, int cb, double transient,
                    bool incr, const SegRow* table = nullptr, bool spec = false,
-                   const CnFields<R>* batch = nullptr) {
+                   const CnFields<R>* batch = nullptr, bool r32 = false) {
   const Geo& g = h->g;
   const int n = g.ny2;
   const SegRow* rows = (table ? table : h->rows.p) + ((size_t)field * 3 + rs) * n;
@@ -825,10 +827,15 @@ void launch_thomas(dgpu_handle* h, int field, int rs, const R* r, R* u, int ca, 
   // solve takes 4.34 ms per step with two columns and 4.70 with one.
   const int C = (h->seg_L <= 32 && h->kn.thomas_cols == 2 && (spec || g.pp % 128 == 0)) ? 2 : 1;
   dim3 grid((unsigned)((pitch + 64 * C - 1) / (64 * C)), (unsigned)nf), block(64, h->seg_S);
-  Timed tm(h, DGPU_K_THOMAS, 3.0 * sizeof(R) * pitch * n * nf);
+  Timed tm(h, DGPU_K_THOMAS, (2.0 * sizeof(R) + (r32 ? 4.0 : sizeof(R))) * pitch * n * nf);
 #define TLAUNCH(L, CC)                                                                                     \
   do {                                                                                                     \
-    if (incr)                                                                                              \
+    if (r32) {                                                                                             \
+      if constexpr (sizeof(R) == 8)                                                                        \
+        hipLaunchKernelGGL((k_thomas_seg<L, CC, double, false, false, float, true>), grid, block, 0,       \
+                           h->stream, reinterpret_cast<const float*>(r), u, rows, n, h->seg_S, pitch, ca,  \
+                           cb, transient, nullptr, nullptr, nullptr, ncol, nbc, mf);                      \
+    } else if (incr)                                                                                       \
       hipLaunchKernelGGL((k_thomas_seg<L, CC, R, true>), grid, block, 0, h->stream, r, u, rows, n,         \
                          h->seg_S, pitch, ca, cb, transient, nullptr, nullptr, nullptr, ncol, nbc, mf);   \
     else                                                                                                   \
@@ -1395,8 +1402,9 @@ void set_courant_impl(dgpu_handle* h, const dgpu_courant_params* c) {
 // ------------------------------------------------- state finiteness --
 // Arrays of doubles scanned for NaN and Inf, one per blockIdx.y.
 struct ScanList {
-  const double* a[3 * (3 + DGPU_MAX_TH) + 1];
-  long long n[3 * (3 + DGPU_MAX_TH) + 1];
+  const void* a[3 * (3 + DGPU_MAX_TH) + 1];
+  long long n[3 * (3 + DGPU_MAX_TH) + 1];  // values
+  int f32[3 * (3 + DGPU_MAX_TH) + 1];      // 1: FP32 values (the mixed form's RK carry)
 };
 constexpr int kScanBlocks = 128;  // blocks per array
 constexpr int kScanFields = 3 * (3 + DGPU_MAX_TH) + 1;  // arrays per scan, at most
@@ -1434,11 +1442,17 @@ void* scratch_address() {
 // holds a value that is not finite.  Every flag is written, so none needs
 // clearing first.
 __global__ void __launch_bounds__(256) k_nonfinite(ScanList L, int* __restrict__ flags) {
-  const double* a = L.a[blockIdx.y];
   const long long n = L.n[blockIdx.y];
   int bad = 0;
-  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
-    bad |= !isfinite(a[i]);
+  if (L.f32[blockIdx.y]) {
+    const float* a = static_cast<const float*>(L.a[blockIdx.y]);
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+      bad |= !isfinite(a[i]);
+  } else {
+    const double* a = static_cast<const double*>(L.a[blockIdx.y]);
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+      bad |= !isfinite(a[i]);
+  }
   bad = __syncthreads_or(bad);
   if (threadIdx.x == 0) flags[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = bad;
 }
@@ -1454,23 +1468,25 @@ int check_state_impl(dgpu_handle* h, bool noalloc) {
   const char* names[kScanFields];
   int nf = 0;
   const long long ns = 2LL * g.sp * g.ny2;  // doubles of a spectral field
-  auto add = [&](const void* p, long long n, const char* name) {
+  auto add = [&](const void* p, long long n, const char* name, int f32 = 0) {
     if (!p) return;
-    L.a[nf] = static_cast<const double*>(p);
+    L.a[nf] = p;
     L.n[nf] = n;
+    L.f32[nf] = f32;
     names[nf++] = name;
   };
+  const int c32 = h->precision == DGPU_MIXED ? 1 : 0;  // the carry's storage (carry32)
   static const char* thn[] = {"cth", "cfth", "kappa_t"};
   add(h->cu[0].p, ns, "cu1");
   add(h->cu[1].p, ns, "cu2");
   add(h->cu[2].p, ns, "cu3");
   add(h->cp.p, ns, "cp");
-  add(h->cf[0].p, ns, "cf1");
-  add(h->cf[1].p, ns, "cf2");
-  add(h->cf[2].p, ns, "cf3");
+  add(h->cf[0].p, ns, "cf1", c32);
+  add(h->cf[1].p, ns, "cf2", c32);
+  add(h->cf[2].p, ns, "cf3", c32);
   for (int n = 0; n < h->nth; ++n) {
     add(h->cth[n].p, ns, thn[0]);
-    add(h->cfth[n].p, ns, thn[1]);
+    add(h->cfth[n].p, ns, thn[1], c32);
   }
   if (h->p.use_les) {  // physical FP64 arrays in both forms
     const long long np = (long long)g.pp * g.ny2;
@@ -1630,6 +1646,10 @@ void rk_substep_t(dgpu_handle* h, int rk_step, double time, int time_step) {
     a.incr = (!incr || spec) ? 0 : (les ? 1 : 2);
     if (spec)  // cr in the (otherwise idle) cr x-spectra buffers, at plane pitch sp
       for (int c = 0; c < ncr; ++c) a.crs[c] = h->tpool[nslot + c].p;
+    if (spec && incr) {  // the mixed form: cr - cu as FP32 spectra
+      a.incr = 2;
+      a.crs32 = 1;
+    }
     a.cub1 = h->cu[0].p;
     a.cub2 = h->cu[1].p;
     a.cub3 = h->cu[2].p;
@@ -1686,9 +1706,10 @@ void rk_substep_t(dgpu_handle* h, int rk_step, double time, int time_step) {
     dim3 grid(g.nq, Ny), block(ng * G);
     // x-spectra of the products in, cr x-spectra out; cu1..3, cp, cth, cf1..3,
     // cfth read and cf1..3, cfth written (neighbour planes come from cache)
-    const double crbytes = spec ? 16.0 * g.sp * ncr : 2.0 * sizeof(R) * g.tp * ncr;
+    const double crbytes = spec ? (incr ? 8.0 : 16.0) * g.sp * ncr : 2.0 * sizeof(R) * g.tp * ncr;
     Timed tm(h, DGPU_K_RHS,
-             (2.0 * sizeof(R) * g.tp * nslot + crbytes + 16.0 * g.sp * (4 + 2 * nth + 3 + nth + 3 + nth)) * Ny,
+             (2.0 * sizeof(R) * g.tp * nslot + crbytes + 16.0 * g.sp * (4 + 2 * nth) +
+              2.0 * sizeof(R) * g.sp * 2 * (3 + nth)) * Ny,  // the carry cf is stored in R
              (2.0 * sizeof(R) * g.tp * nslot + (spec ? 0.0 : crbytes)) * Ny);
     hipLaunchKernelGGL((g.nz <= 512 ? k_rhs<R, true> : k_rhs<R, false>), grid, block, lds, h->stream, a, g,
                        twz_of<R>(h), 1);
@@ -1745,7 +1766,8 @@ void rk_substep_t(dgpu_handle* h, int rk_step, double time, int time_step) {
     add(1, 2, h->cu[1], 1, Ny + 1);
     add(2, 1, h->cu[2], h->jstart - 1, h->jend + 1);
     for (int n = 0; n < nth; ++n) add(3 + n, 3 + n, h->cth[n], h->jstart_th[n] - 1, h->jend_th[n] + 1);
-    launch_thomas<double>(h, 0, rs, mf.r[0], mf.u[0], mf.ca[0], mf.cb[0], transient, false, nullptr, true, &mf);
+    launch_thomas<double>(h, 0, rs, static_cast<const double*>(mf.r[0]), mf.u[0], mf.ca[0], mf.cb[0], transient,
+                          false, nullptr, true, &mf, incr);
     for (int k = 0; k < mf.nf; ++k) launch_herm0(h, reinterpret_cast<double2*>(mf.u[k]), mf.ca[k], mf.cb[k]);
   } else {
     for (int n = 0; n < nth; ++n)
@@ -2433,7 +2455,8 @@ void phys_to_ref(const dgpu_handle* h, const std::vector<double>& in, double* re
 // thread per element.  The reference-layout side is written in full (zeros
 // outside the dealiased block or the physical points), so the caller's
 // buffer is fully defined.
-__global__ void __launch_bounds__(256) k_spec_from_ref(const double2* __restrict__ ref, double2* __restrict__ dst,
+template <typename S>
+__global__ void __launch_bounds__(256) k_spec_from_ref(const double2* __restrict__ ref, S* __restrict__ dst,
                                                        Geo g) {
   const long long n = (long long)g.ny2 * g.sp;
   const long long nxc = g.nx / 2 + 1, nz2 = g.nz + 2;
@@ -2441,11 +2464,13 @@ __global__ void __launch_bounds__(256) k_spec_from_ref(const double2* __restrict
     const int j = (int)(t / g.sp);
     const int m = (int)(t - (long long)j * g.sp);
     const int q = m / g.nkk, kk = m - q * g.nkk;
-    dst[t] = ref[q + nxc * (kk + nz2 * j)];
+    const double2 v = ref[q + nxc * (kk + nz2 * j)];
+    dst[t] = S{(decltype(S::x))v.x, (decltype(S::x))v.y};
   }
 }
 
-__global__ void __launch_bounds__(256) k_spec_to_ref(const double2* __restrict__ src, double2* __restrict__ ref,
+template <typename S>
+__global__ void __launch_bounds__(256) k_spec_to_ref(const S* __restrict__ src, double2* __restrict__ ref,
                                                      Geo g) {
   const long long nxc = g.nx / 2 + 1, nz2 = g.nz + 2;
   const long long n = nxc * nz2 * g.ny2;
@@ -2454,7 +2479,10 @@ __global__ void __launch_bounds__(256) k_spec_to_ref(const double2* __restrict__
     const long long r = t - (long long)j * nxc * nz2;
     const int kk = (int)(r / nxc), q = (int)(r - (long long)kk * nxc);
     double2 v = make_double2(0.0, 0.0);
-    if (q < g.nq && kk < g.nkk) v = src[(long long)j * g.sp + (long long)q * g.nkk + kk];
+    if (q < g.nq && kk < g.nkk) {
+      const S z = src[(long long)j * g.sp + (long long)q * g.nkk + kk];
+      v = make_double2((double)z.x, (double)z.y);
+    }
     ref[t] = v;
   }
 }
@@ -2485,6 +2513,7 @@ __global__ void __launch_bounds__(256) k_phys_to_ref(const double* __restrict__ 
 bool field_buf(dgpu_handle* h, int f, int n, void** dev, bool* spec);
 bool adj_field_buf(dgpu_handle* h, int f, int n, void** dev);
 int nfield_copies(const dgpu_handle* h, int f);
+bool carry32(const dgpu_handle* h, int f);
 void adj_alloc(dgpu_handle* h);
 size_t ref_spec_doubles(const Geo& g);
 
@@ -2517,11 +2546,18 @@ void field_device_copy(dgpu_handle* h, int field, void* dev, hipStream_t cs, boo
     else
       field_buf(h, field, n, &buf, &spec);
     double* ref = static_cast<double*>(dev) + n * per;
-    if (spec && to_handle)
-      hipLaunchKernelGGL(k_spec_from_ref, grid, block, 0, cs, reinterpret_cast<const double2*>(ref),
+    const bool c32 = !adjoint && carry32(h, field);
+    if (spec && to_handle && c32)
+      hipLaunchKernelGGL(k_spec_from_ref<float2>, grid, block, 0, cs, reinterpret_cast<const double2*>(ref),
+                         static_cast<float2*>(buf), g);
+    else if (spec && to_handle)
+      hipLaunchKernelGGL(k_spec_from_ref<double2>, grid, block, 0, cs, reinterpret_cast<const double2*>(ref),
                          static_cast<double2*>(buf), g);
+    else if (spec && c32)
+      hipLaunchKernelGGL(k_spec_to_ref<float2>, grid, block, 0, cs, static_cast<const float2*>(buf),
+                         reinterpret_cast<double2*>(ref), g);
     else if (spec)
-      hipLaunchKernelGGL(k_spec_to_ref, grid, block, 0, cs, static_cast<const double2*>(buf),
+      hipLaunchKernelGGL(k_spec_to_ref<double2>, grid, block, 0, cs, static_cast<const double2*>(buf),
                          reinterpret_cast<double2*>(ref), g);
     else if (to_handle)
       hipLaunchKernelGGL(k_phys_from_ref, grid, block, 0, cs, ref, static_cast<double*>(buf), g);
@@ -2569,6 +2605,12 @@ bool field_buf(dgpu_handle* h, int f, int n, void** dev, bool* spec) {
 
 int nfield_copies(const dgpu_handle* h, int f) {
   return (f == DGPU_CTH || f == DGPU_CFTH || f == DGPU_KAPPA_T) ? h->nth : 1;
+}
+
+// The RK carry cf1..3, cfth is stored in the transform precision: FP32
+// spectra in the mixed form (k_rhs), FP64 otherwise.
+bool carry32(const dgpu_handle* h, int f) {
+  return h->precision == DGPU_MIXED && f >= DGPU_CF1 && f <= DGPU_CFTH;
 }
 
 int check_params(const dgpu_params* p, std::string& why) {
@@ -2918,7 +2960,14 @@ int dgpu_set_field(dgpu_handle* h, int field, const double* host) {
       void* dev;
       bool spec;
       field_buf(h, field, n, &dev, &spec);
-      if (spec) {
+      if (spec && carry32(h, field)) {
+        std::vector<double2> v;
+        spec_from_ref(h, host + n * per, v);
+        std::vector<float2> w(v.size());
+        for (size_t t = 0; t < v.size(); ++t) w[t] = make_float2((float)v[t].x, (float)v[t].y);
+        HIPCHECK(hipMemcpyAsync(dev, w.data(), sizeof(float2) * w.size(), hipMemcpyHostToDevice, h->stream));
+        HIPCHECK(hipStreamSynchronize(h->stream));
+      } else if (spec) {
         std::vector<double2> v;
         spec_from_ref(h, host + n * per, v);
         HIPCHECK(hipMemcpyAsync(dev, v.data(), sizeof(double2) * v.size(), hipMemcpyHostToDevice,
@@ -2947,7 +2996,14 @@ int dgpu_get_field(dgpu_handle* h, int field, double* host) {
       void* dev;
       bool spec;
       field_buf(h, field, n, &dev, &spec);
-      if (spec) {
+      if (spec && carry32(h, field)) {
+        std::vector<float2> w((size_t)h->g.ny2 * h->g.sp);
+        HIPCHECK(hipMemcpyAsync(w.data(), dev, sizeof(float2) * w.size(), hipMemcpyDeviceToHost, h->stream));
+        HIPCHECK(hipStreamSynchronize(h->stream));
+        std::vector<double2> v(w.size());
+        for (size_t t = 0; t < w.size(); ++t) v[t] = make_double2(w[t].x, w[t].y);
+        spec_to_ref(h, v, host + n * per);
+      } else if (spec) {
         std::vector<double2> v((size_t)h->g.ny2 * h->g.sp);
         HIPCHECK(hipMemcpyAsync(v.data(), dev, sizeof(double2) * v.size(), hipMemcpyDeviceToHost,
                                 h->stream));

@@ -518,8 +518,11 @@ struct RhsArgs {
   const void* X[MAXSLOT];         // x-spectra of the products (transform precision)
   void* Tcr[3 + MAXTH];           // out: x-spectra of cr1, cr3, cr2, crth_n (slot order)
   // out, instead of Tcr when set: cr itself (FP64 spectra, same slot order)
-  // for the CN solve in spectral space; no backward z transform is run
+  // for the CN solve in spectral space; no backward z transform is run.
+  // With crs32 (the mixed form) they hold cr less the state as FP32 spectra
+  // (with incr = 2), which the solve adds back to the FP64 state.
   double2* crs[3 + MAXTH];
+  int crs32;
   const double2* cu1;             // state the explicit terms read
   const double2* cu2;
   const double2* cu3;
@@ -532,6 +535,7 @@ struct RhsArgs {
   const double2* cub2;
   const double2* cub3;
   const double2* cthb[MAXTH];
+  // the RK carry, stored in the transform precision R (FP32 in the mixed form)
   double2* cf1;
   double2* cf2;
   double2* cf3;
@@ -624,12 +628,19 @@ __global__ void __launch_bounds__(512) k_rhs(RhsArgs a, Geo g, const cpx<R>* __r
       return invnz * C{(double)z.x, (double)z.y};
     };
     auto put = [&](int s, C v) {
-      if (spec)
+      if (spec && a.crs32)
+        reinterpret_cast<cpx<float>*>(a.crs[s])[o] = cpx<float>{(float)v.x, (float)v.y};
+      else if (spec)
         reinterpret_cast<C*>(a.crs[s])[o] = v;
       else
         smem[(long long)s * L + pn] = CR{(R)v.x, (R)v.y};
     };
     auto ld = [&](const double2* p) -> C { return reinterpret_cast<const C*>(p)[o]; };
+    auto ldcf = [&](const double2* p) -> C {
+      const CR z = reinterpret_cast<const CR*>(p)[o];
+      return C{(double)z.x, (double)z.y};
+    };
+    auto stcf = [&](double2* p, C v) { reinterpret_cast<CR*>(p)[o] = CR{(R)v.x, (R)v.y}; };
     C cu1 = {0, 0}, cu3 = {0, 0};
     if (in1 || a.nth > 0) {
       cu1 = reinterpret_cast<const C*>(a.cu1)[o];
@@ -665,16 +676,16 @@ __global__ void __launch_bounds__(512) k_rhs(RhsArgs a, Geo g, const cpx<R>* __r
         r3 = C{0.0, 0.0};
       }
       if (a.rk_step > 1) {
-        r1 = r1 + a.temp3 * reinterpret_cast<const C*>(a.cf1)[o];
-        r3 = r3 + a.temp3 * reinterpret_cast<const C*>(a.cf3)[o];
+        r1 = r1 + a.temp3 * ldcf(a.cf1);
+        r3 = r3 + a.temp3 * ldcf(a.cf3);
       }
       r1 = r1 - a.temp4 * ik(kxq, cp);
       r3 = r3 - a.temp4 * ik(kzk, cp);
       if (mean) r1.x -= a.temp4 * a.force;
       r1 = r1 + a.temp5 * f1;
       r3 = r3 + a.temp5 * f3;
-      reinterpret_cast<C*>(a.cf1)[o] = f1;
-      reinterpret_cast<C*>(a.cf3)[o] = f3;
+      stcf(a.cf1, f1);
+      stcf(a.cf3, f3);
       put(0, r1);
       put(1, r3);
     }
@@ -699,10 +710,10 @@ __global__ void __launch_bounds__(512) k_rhs(RhsArgs a, Geo g, const cpx<R>* __r
       C r2 = cu2;
       if (a.incr == 1) r2 = r2 - ld(a.cub2);
       if (a.incr == 2) r2 = C{0.0, 0.0};
-      if (a.rk_step > 1) r2 = r2 + a.temp3 * reinterpret_cast<const C*>(a.cf2)[o];
+      if (a.rk_step > 1) r2 = r2 + a.temp3 * ldcf(a.cf2);
       r2 = r2 - (a.temp4 / cdy) * (cpj - cpm);
       r2 = r2 + a.temp5 * f2;
-      reinterpret_cast<C*>(a.cf2)[o] = f2;
+      stcf(a.cf2, f2);
       put(2, r2);
     }
     for (int n = 0; n < a.nth; ++n) {
@@ -719,9 +730,9 @@ __global__ void __launch_bounds__(512) k_rhs(RhsArgs a, Geo g, const cpx<R>* __r
       C rt = th;
       if (a.incr == 1) rt = rt - ld(a.cthb[n]);
       if (a.incr == 2) rt = C{0.0, 0.0};
-      if (a.rk_step > 1) rt = rt + a.temp3 * reinterpret_cast<const C*>(a.cfth[n])[o];
+      if (a.rk_step > 1) rt = rt + a.temp3 * ldcf(a.cfth[n]);
       rt = rt + a.temp5 * ft;
-      reinterpret_cast<C*>(a.cfth[n])[o] = ft;
+      stcf(a.cfth[n], ft);
       put(3 + n, rt);
     }
   }
@@ -779,7 +790,7 @@ struct SegRow {
 // C adjacent columns per lane, loaded and stored as one vector of C values:
 // a wavefront touches 64*C*sizeof(T) contiguous bytes of each row (the rows
 // of one column are a plane apart, so short row accesses leave HBM idle).
-// T is the storage type of r and u (the arithmetic is FP64).  INCR is the
+// T is the storage type of u and TR that of r (the arithmetic is FP64).  INCR is the
 // increment form: r holds the right-hand side less the state (RHS pass with
 // incr), and the solve is for d = x - u with the same factors, A d = vec - A u:
 // assembled rows (A = I - E, vec = r + u + E u) give r + 2 E u, fixed rows
@@ -811,15 +822,15 @@ struct SegRow {
 constexpr int CN_MAXF = 3 + MAXTH;
 template <typename T>
 struct CnFields {
-  const T* r[CN_MAXF];
+  const void* r[CN_MAXF];  // of the kernel's TR
   T* u[CN_MAXF];
   const SegRow* rows[CN_MAXF];
   int ca[CN_MAXF], cb[CN_MAXF];
   int nf;
 };
 
-template <int L, int C, typename T, bool INCR, bool ADJ = false>
-__global__ void __launch_bounds__(1024) k_thomas_seg(const T* r,
+template <int L, int C, typename T, bool INCR, bool ADJ = false, typename TR = T, bool RPLUS = false>
+__global__ void __launch_bounds__(1024) k_thomas_seg(const TR* r,
                                                      T* __restrict__ u,
                                                      const SegRow* __restrict__ rows, int n, int S,
                                                      long long pp, int ca, int cb, double kscale,
@@ -829,9 +840,10 @@ __global__ void __launch_bounds__(1024) k_thomas_seg(const T* r,
                                                      long long ncol = 0, long long nbc = 0,
                                                      CnFields<T> mf = CnFields<T>{}) {
   typedef T VN __attribute__((ext_vector_type(C)));  // C columns, one streaming access
+  typedef TR VR __attribute__((ext_vector_type(C)));
   if (mf.nf > 0) {
     const int f = blockIdx.y;
-    r = mf.r[f];
+    r = static_cast<const TR*>(mf.r[f]);
     u = mf.u[f];
     rows = mf.rows[f];
     ca = mf.ca[f];
@@ -873,13 +885,16 @@ __global__ void __launch_bounds__(1024) k_thomas_seg(const T* r,
     if (jj < n) {
       const SegRow rw = rows[jj];
       if ((rw.kind == 1 || rw.kind == 3) && inb) {
-        const VN x = __builtin_nontemporal_load(reinterpret_cast<const VN*>(r + (long long)jj * pp + col));
+        const VR x = __builtin_nontemporal_load(reinterpret_cast<const VR*>(r + (long long)jj * pp + col));
 #pragma unroll
         for (int c = 0; c < C; ++c) {
           double val = (double)x[c];
           if (rw.kind == 1) {
             const double ex = rw.wp * (w[t + 2][c] - w[t + 1][c]) - rw.wm * (w[t + 1][c] - w[t][c]);
             val += INCR ? 2.0 * ex : ex;
+            // RPLUS: r holds the right-hand side less the state (an FP32
+            // increment); the state is added back here, in FP64
+            if (RPLUS) val += w[t + 1][c];
           }
           if (ADJ && (jj < ca || jj > cb)) {  // passes through (w is free in ADJ)
             w[t + 1][c] = val;
