@@ -58,7 +58,8 @@ struct DevBuf {
 
 // Tuning knobs.  Each handle reads them once, at dgpu_create, from the
 // environment (DGPU_SEG_TARGET, DGPU_XPASS_THREADS, DGPU_XPROD_THREADS,
-// DGPU_LES_COLS, DGPU_THOMAS_COLS, DGPU_SPECTRAL_CN, DGPU_BATCH_MB), with the
+// DGPU_LES_COLS, DGPU_THOMAS_COLS, DGPU_SPECTRAL_CN, DGPU_BATCH_MB,
+// DGPU_FFT_CONST), with the
 // defaults below when a variable is unset, and keeps its own copy: creating a
 // handle never changes how another one runs.
 struct Knobs {
@@ -88,6 +89,9 @@ struct Knobs {
   // default: at 512x257x512 every batch size measured slower than one launch
   // per pass over all planes.
   int batch_mb = 0;
+  // FP64 transform passes over 512 points with the length fixed at compile
+  // time (fft_regs_c); 0 = the run-time length everywhere.
+  int fft_const = 1;
 };
 
 Knobs knobs_from_env() {
@@ -102,6 +106,7 @@ Knobs knobs_from_env() {
   rd("DGPU_THOMAS_COLS", k.thomas_cols);
   rd("DGPU_SPECTRAL_CN", k.spectral_cn);
   rd("DGPU_BATCH_MB", k.batch_mb);
+  rd("DGPU_FFT_CONST", k.fft_const);
   k.spectral_cn = k.spectral_cn != 0;
   return k;
 }
@@ -649,8 +654,10 @@ void launch_xinv(dgpu_handle* h, const cpx<R>* T, R* u, int j0, int j1, const do
   dim3 grid(g.nz / KT, j1 - j0 + 1, nf), block(ng * (g.nx / 8));
   Timed tm(h, DGPU_K_XINV, (sizeof(R) * g.pp + 2.0 * sizeof(R) * g.tp) * (j1 - j0 + 1) * nf,
            2.0 * sizeof(R) * g.tp * (j1 - j0 + 1) * nf);
-  hipLaunchKernelGGL((g.nx <= 512 ? k_xinv<R, true> : k_xinv<R, false>), grid, block, lds, h->stream, T, u, g,
-                     twx_of<R>(h), j0, dkx, fb ? *fb : FieldBatch{});
+  auto kern = g.nx <= 512 ? k_xinv<R, true> : k_xinv<R, false>;
+  if constexpr (sizeof(R) == 8)
+    if (g.nx == 512 && h->kn.fft_const) kern = k_xinv<R, true, 9>;
+  hipLaunchKernelGGL(kern, grid, block, lds, h->stream, T, u, g, twx_of<R>(h), j0, dkx, fb ? *fb : FieldBatch{});
   HIPCHECK(hipGetLastError());
 }
 
@@ -682,9 +689,11 @@ void launch_zinv(dgpu_handle* h, const double2* C, cpx<R>* T, int j0, int j1, do
   dim3 grid((g.nq + ng - 1) / ng, j1 - j0 + 1, nf), block(ng * (g.nz / 8));
   Timed tm(h, DGPU_K_ZINV, (2.0 * sizeof(R) * g.tp + 16.0 * g.sp) * (j1 - j0 + 1) * nf,
            2.0 * sizeof(R) * g.tp * (j1 - j0 + 1) * nf);
-  hipLaunchKernelGGL((g.nz <= 512 ? k_zinv<R, true> : k_zinv<R, false>), grid, block, lds, h->stream,
-                     reinterpret_cast<const cpx<double>*>(C), T, g, twz_of<R>(h), j0, s0, s1,
-                     fb ? *fb : FieldBatch{});
+  auto kern = g.nz <= 512 ? k_zinv<R, true> : k_zinv<R, false>;
+  if constexpr (sizeof(R) == 8)
+    if (g.nz == 512 && h->kn.fft_const) kern = k_zinv<R, true, 9>;
+  hipLaunchKernelGGL(kern, grid, block, lds, h->stream, reinterpret_cast<const cpx<double>*>(C), T, g, twz_of<R>(h),
+                     j0, s0, s1, fb ? *fb : FieldBatch{});
   HIPCHECK(hipGetLastError());
 }
 
@@ -1628,8 +1637,13 @@ void rk_substep_t(dgpu_handle* h, int rk_step, double time, int time_step) {
     // reads u1, u2, u3, th once (neighbour planes from cache); writes the product x-spectra
     Timed tm(h, DGPU_K_XPROD, (sizeof(R) * g.pp * (3 + nth) + 2.0 * sizeof(R) * g.tp * nslot) * Ny,
              2.0 * sizeof(R) * g.tp * nslot * Ny);
-    hipLaunchKernelGGL((g.nx <= 512 ? k_xprod_all<R, true> : k_xprod_all<R, false>), grid, block, lds, h->stream,
-                       pa, g, twx_of<R>(h), 1);
+    // 512-point FP64 rows (the benchmark's grid) take the compile-time
+    // transform: 5.77 -> 5.54 ms per step at 512x257x512 (the FP32 pass was
+    // 3% slower with it, 2.78 -> 2.87 ms, and keeps the run-time one)
+    auto kern = g.nx <= 512 ? k_xprod_all<R, true> : k_xprod_all<R, false>;
+    if constexpr (sizeof(R) == 8)
+      if (g.nx == 512 && h->kn.fft_const) kern = k_xprod_all<R, true, 9>;
+    hipLaunchKernelGGL(kern, grid, block, lds, h->stream, pa, g, twx_of<R>(h), 1);
     HIPCHECK(hipGetLastError());
   }
 
@@ -1711,8 +1725,10 @@ void rk_substep_t(dgpu_handle* h, int rk_step, double time, int time_step) {
              (2.0 * sizeof(R) * g.tp * nslot + crbytes + 16.0 * g.sp * (4 + 2 * nth) +
               2.0 * sizeof(R) * g.sp * 2 * (3 + nth)) * Ny,  // the carry cf is stored in R
              (2.0 * sizeof(R) * g.tp * nslot + (spec ? 0.0 : crbytes)) * Ny);
-    hipLaunchKernelGGL((g.nz <= 512 ? k_rhs<R, true> : k_rhs<R, false>), grid, block, lds, h->stream, a, g,
-                       twz_of<R>(h), 1);
+    auto kern = g.nz <= 512 ? k_rhs<R, true> : k_rhs<R, false>;
+    if constexpr (sizeof(R) == 8)
+      if (g.nz == 512 && h->kn.fft_const) kern = k_rhs<R, true, 9>;
+    hipLaunchKernelGGL(kern, grid, block, lds, h->stream, a, g, twz_of<R>(h), 1);
     HIPCHECK(hipGetLastError());
   }
 
@@ -2760,6 +2776,8 @@ void create_impl(dgpu_handle* h, const double* gy) {
         (const void*)k_zinv<double, true>,    (const void*)k_zinv<double, false>,
         (const void*)k_zinv_op<double, true>, (const void*)k_zinv_op<double, false>,
         (const void*)k_xprod_all<double, true>, (const void*)k_xprod_all<double, false>,
+        (const void*)k_xprod_all<double, true, 9>, (const void*)k_rhs<double, true, 9>,
+        (const void*)k_xinv<double, true, 9>,     (const void*)k_zinv<double, true, 9>,
         (const void*)k_rhs<float, true>,      (const void*)k_rhs<float, false>,
         (const void*)k_xfwd<float, true>,     (const void*)k_xfwd<float, false>,
         (const void*)k_xinv<float, true>,     (const void*)k_xinv<float, false>,

@@ -217,6 +217,91 @@ __device__ __forceinline__ void fft_regs(cpx<R> (&v)[8], cpx<R>* buf, int N, int
   }
 }
 
+// The same transform with N = 2^LOGN a compile-time constant (N >= 64, so
+// that the group size G = N/8 and every pass's stride are multiples of 8):
+// each LDS index of a pass is then one per-thread base plus an immediate
+// offset, pidx(gt + c) = pidx(gt) + c + c/8 for c a multiple of 8.
+template <typename R, int S, int LOGR, bool WAVE, int LOGN, int LOGNS>
+__device__ __forceinline__ void stockham_pass_c(cpx<R>* buf, const cpx<R>* __restrict__ tw, int gt) {
+  constexpr int N = 1 << LOGN, RDX = 1 << LOGR, NB = 8 >> LOGR;
+  constexpr int G = N >> 3, stride = N >> LOGR, Ns = 1 << LOGNS;
+  constexpr int tstep = N >> (LOGNS + LOGR);
+  static_assert(G % 8 == 0 && stride % 8 == 0, "compile-time passes need N >= 64");
+  const int pg = pidx(gt);
+  cpx<R> v[8];
+#pragma unroll
+  for (int bb = 0; bb < NB; ++bb)
+#pragma unroll
+    for (int r = 0; r < RDX; ++r) {
+      const int c = bb * G + r * stride;
+      v[bb * RDX + r] = buf[pg + c + (c >> 3)];
+    }
+  fft_sync<WAVE>();
+#pragma unroll
+  for (int bb = 0; bb < NB; ++bb) {
+    const int j = gt + bb * G;
+    const int k = j & (Ns - 1);
+    if constexpr (LOGNS > 0) {
+#pragma unroll
+      for (int r = 1; r < RDX; ++r) {
+        cpx<R> t = tw[k * r * tstep];
+        if (S > 0) t.y = -t.y;
+        v[bb * RDX + r] = v[bb * RDX + r] * t;
+      }
+    }
+    if constexpr (LOGR == 3)
+      dft8<R, S>(v);
+    else if constexpr (LOGR == 2)
+      dft4<R, S>(v[bb * 4], v[bb * 4 + 1], v[bb * 4 + 2], v[bb * 4 + 3]);
+    else
+      dft2<R, S>(v + bb * 2);
+    const int idxD = ((j >> LOGNS) << (LOGNS + LOGR)) + k;
+    const int pd = pidx(idxD);
+#pragma unroll
+    for (int r = 0; r < RDX; ++r) {
+      // idxD + r Ns: Ns >= 8 keeps the offset a multiple of 8 (the first
+      // pass, Ns = 1, writes consecutive slots instead)
+      if constexpr (Ns >= 8)
+        buf[pd + r * Ns + ((r * Ns) >> 3)] = v[bb * RDX + r];
+      else
+        buf[pidx(idxD + r * Ns)] = v[bb * RDX + r];
+    }
+  }
+  fft_sync<WAVE>();
+}
+
+template <typename R, int S, bool WAVE, int LOGN, int LOGNS>
+__device__ __forceinline__ void fft_passes_c(cpx<R>* buf, const cpx<R>* __restrict__ tw, int gt) {
+  if constexpr (LOGNS < LOGN) {
+    constexpr int REM = LOGN - LOGNS;
+    constexpr int LR = REM >= 3 ? 3 : REM;
+    stockham_pass_c<R, S, LR, WAVE, LOGN, LOGNS>(buf, tw, gt);
+    fft_passes_c<R, S, WAVE, LOGN, LOGNS + LR>(buf, tw, gt);
+  }
+}
+
+// fft_regs for N = 2^LOGN known at compile time.
+template <typename R, int S, bool WAVE, int LOGN>
+__device__ __forceinline__ void fft_regs_c(cpx<R> (&v)[8], cpx<R>* buf, const cpx<R>* __restrict__ tw, int gt) {
+  dft8<R, S>(v);
+  fft_sync<WAVE>();
+  const int base = pidx(gt * 8);  // gt * 8 + r: 9 gt + r
+#pragma unroll
+  for (int r = 0; r < 8; ++r) buf[base + r] = v[r];
+  fft_sync<WAVE>();
+  fft_passes_c<R, S, WAVE, LOGN, 3>(buf, tw, gt);
+}
+
+// fft_regs, or fft_regs_c when LOGN > 0 fixes N = 2^LOGN at compile time.
+template <typename R, int S, bool WAVE, int LOGN>
+__device__ __forceinline__ void fft_any(cpx<R> (&v)[8], cpx<R>* buf, int N, int logN,
+                                        const cpx<R>* __restrict__ tw, int gt) {
+  if constexpr (LOGN > 0)
+    fft_regs_c<R, S, WAVE, LOGN>(v, buf, tw, gt);
+  else
+    fft_regs<R, S, WAVE>(v, buf, N, logN, tw, gt);
+}
+
 // The n-entry twiddle table copied into LDS at dst (past the transform
 // buffers), for the passes of every transform the workgroup runs: their
 // twiddle reads then come at LDS latency, not the caches' (at 512x257x512

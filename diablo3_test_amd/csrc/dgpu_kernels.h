@@ -125,7 +125,7 @@ __global__ void __launch_bounds__(512) k_xfwd(const R* __restrict__ u, cpx<R>* _
 // FFTW's c2r reads only the real part of the kx = 0 input.  Rows 2m and 2m+1
 // are synthesised together: Z = A + iB with A, B extended Hermitian.  With dkx
 // the x-spectra are multiplied by i kx first (the x derivative).
-template <typename R, bool WAVE>
+template <typename R, bool WAVE, int LOGN = 0>
 __global__ void __launch_bounds__(512) k_xinv(const cpx<R>* __restrict__ T, R* __restrict__ u,
                                                Geo g, const cpx<R>* __restrict__ tw, int j0,
                                                const double* __restrict__ dkx = nullptr,
@@ -182,7 +182,7 @@ __global__ void __launch_bounds__(512) k_xinv(const cpx<R>* __restrict__ T, R* _
     cpx<R> v[8];
 #pragma unroll
     for (int r = 0; r < 8; ++r) v[r] = fbuf[pidx(gt + G * r)];
-    fft_regs<R, 1, WAVE>(v, fbuf, g.nx, g.lognx, twl, gt);
+    fft_any<R, 1, WAVE, LOGN>(v, fbuf, g.nx, g.lognx, twl, gt);
     R* ra = plane + (long long)(k0 + 2 * (m0 + grp)) * g.nx;
     R* rb = ra + g.nx;
     for (int i = gt; i < g.nx; i += G) {
@@ -242,7 +242,7 @@ __global__ void __launch_bounds__(512) k_zfwd(const cpx<R>* __restrict__ T, cpx<
 // Unpack + zero (fft.f90:330-341) + backward z transform (:345), unnormalised.
 // s0 (kx = 0) and s1 (kx > 0) scale the input: 1 for the transform, W/(Nx Nz)
 // for the transpose of the forward transform.
-template <typename R, bool WAVE>
+template <typename R, bool WAVE, int LOGN = 0>
 __global__ void __launch_bounds__(512) k_zinv(const cpx<double>* __restrict__ C, cpx<R>* __restrict__ T,
                                                Geo g, const cpx<R>* __restrict__ tw, int j0, double s0,
                                                double s1, FieldBatch fb = FieldBatch{}) {
@@ -275,7 +275,7 @@ __global__ void __launch_bounds__(512) k_zinv(const cpx<double>* __restrict__ C,
     }
     v[r] = {(R)(sc * z.x), (R)(sc * z.y)};
   }
-  fft_regs<R, 1, WAVE>(v, fbuf, g.nz, g.lognz, twl, gt);
+  fft_any<R, 1, WAVE, LOGN>(v, fbuf, g.nz, g.lognz, twl, gt);
   if (act) {
     cpx<R>* out = T + (long long)j * g.tp + (long long)q * g.nz;
     for (int k = gt; k < g.nz; k += G) st_out(out + k, fbuf[pidx(k)]);
@@ -361,7 +361,8 @@ __device__ __forceinline__ void emit_pair(cpx<R>* smem, int L, const Geo& g, int
   }
 }
 
-template <typename R, bool WAVE>
+// LOGNX > 0: Nx = 2^LOGNX at compile time (fft_regs_c); 0: Nx from g.
+template <typename R, bool WAVE, int LOGNX = 0>
 __global__ void __launch_bounds__(512) k_xprod_all(ProdAllArgs a, Geo g,
                                                     const cpx<R>* __restrict__ tw, int j0) {
   constexpr int KT = XTile<R>::KT;
@@ -396,7 +397,10 @@ __global__ void __launch_bounds__(512) k_xprod_all(ProdAllArgs a, Geo g,
     cpx<R> v[8];
 #pragma unroll
     for (int r = 0; r < 8; ++r) v[r] = {va[r], vb[r]};
-    fft_regs<R, -1, WAVE>(v, fbuf, g.nx, g.lognx, twl, gt);
+    if constexpr (LOGNX > 0)
+      fft_regs_c<R, -1, WAVE, LOGNX>(v, fbuf, twl, gt);
+    else
+      fft_regs<R, -1, WAVE>(v, fbuf, g.nx, g.lognx, twl, gt);
     __syncthreads();
   };
   bool r1, r2;
@@ -564,7 +568,7 @@ struct RhsArgs {
   int jst[MAXTH], jet[MAXTH];
 };
 
-template <typename R, bool WAVE>
+template <typename R, bool WAVE, int LOGN = 0>
 __global__ void __launch_bounds__(512) k_rhs(RhsArgs a, Geo g, const cpx<R>* __restrict__ tw,
                                               int j0) {
   typedef cpx<double> C;
@@ -606,7 +610,7 @@ __global__ void __launch_bounds__(512) k_rhs(RhsArgs a, Geo g, const cpx<R>* __r
 #pragma unroll
       for (int r = 0; r < 8; ++r) v[r] = CR{(R)0, (R)0};
     }
-    fft_regs<R, -1, WAVE>(v, buf, g.nz, g.lognz, twl, gt);
+    fft_any<R, -1, WAVE, LOGN>(v, buf, g.nz, g.lognz, twl, gt);
   }
   __syncthreads();  // the combine below reads every slot
   // --- explicit terms, mode by mode
@@ -756,7 +760,7 @@ __global__ void __launch_bounds__(512) k_rhs(RhsArgs a, Geo g, const cpx<R>* __r
     CR v[8];
 #pragma unroll
     for (int r = 0; r < 8; ++r) v[r] = buf[pidx(gt + G * r)];
-    fft_regs<R, 1, WAVE>(v, buf, g.nz, g.lognz, twl, gt);
+    fft_any<R, 1, WAVE, LOGN>(v, buf, g.nz, g.lognz, twl, gt);
     if (act) {
       CR* out = reinterpret_cast<CR*>(a.Tcr[c]) + to;
       for (int k = gt; k < g.nz; k += G) out[k] = buf[pidx(k)];
