@@ -1415,12 +1415,13 @@ struct ScanList {
   long long n[3 * (3 + DGPU_MAX_TH) + 1];  // values
   int f32[3 * (3 + DGPU_MAX_TH) + 1];      // 1: FP32 values (the mixed form's RK carry)
 };
+static_assert(MAXTH == DGPU_MAX_TH, "the kernels' scalar cap is the ABI's");
 constexpr int kScanBlocks = 128;  // blocks per array
 constexpr int kScanFields = 3 * (3 + DGPU_MAX_TH) + 1;  // arrays per scan, at most
 
 // Scratch of dgpu_check_state_noalloc: the scan's per-block flags in static
 // device memory, so that the call allocates nothing.  Each GPU the library
-// runs on holds one copy (12.5 KiB) until the process ends; every call on
+// runs on holds one copy (29 KiB) until the process ends; every call on
 // that GPU shares it, so calls that overlap on one GPU race.  Host code
 // reaches it only through the address hipGetSymbolAddress returns for the
 // current device (scratch_address); the name itself is a host stand-in.
@@ -1479,6 +1480,7 @@ int check_state_impl(dgpu_handle* h, bool noalloc) {
   const long long ns = 2LL * g.sp * g.ny2;  // doubles of a spectral field
   auto add = [&](const void* p, long long n, const char* name, int f32 = 0) {
     if (!p) return;
+    if (nf >= kScanFields) throw ApiError{DGPU_EINVAL, "more arrays than the scan holds"};
     L.a[nf] = p;
     L.n[nf] = n;
     L.f32[nf] = f32;
@@ -1502,7 +1504,6 @@ int check_state_impl(dgpu_handle* h, bool noalloc) {
     add(h->nu_t.p, np, "nu_t");
     for (int n = 0; n < h->nth; ++n) add(h->kappa_t[n].p, np, thn[2]);
   }
-  if (nf > kScanFields) throw ApiError{DGPU_EINVAL, "more arrays than the scan holds"};
   int* flags;
   int fl_stack[kScratchInts];
   std::vector<int> fl_vec;
@@ -1548,6 +1549,56 @@ void set_front(const dgpu_handle* h, A& a) {
     a.fr_sz += p.dthdz[n];
   }
   a.fr_dr = p.delta / p.ro_inv;
+}
+
+// The launches of the RHS pass (k_rhs).  One launch holds every product slot
+// of the substep in LDS when they fit; otherwise the outputs are split: the
+// velocity (cr1, cr3 and cr2, or each on its own) and then as many scalars
+// per launch as fit.  A launch copies the twiddle table to LDS when there is
+// room for it beside the slots.
+struct RhsPart {
+  int part, th0, th1, tw_lds, ng;
+  size_t lds;
+};
+
+template <typename R>
+std::vector<RhsPart> rhs_plan(const dgpu_handle* h, int nth) {
+  const Geo& g = h->g;
+  const int G = g.nz / 8;
+  const size_t L = (size_t)fft_lds_len(g.nz);
+  // a launch of nslot slots: its group count and LDS size, with or without the table
+  auto make = [&](int part, int th0, int th1, RhsPart& out) -> bool {
+    const int nslot = ((part & 1) ? 4 : 0) + ((part & 2) ? 2 : 0) + 2 * (th1 - th0);
+    int ng = 512 / G;
+    if (ng < 1) ng = 1;
+    if (ng > nslot) ng = nslot;
+    const int rounds = (nslot + ng - 1) / ng;
+    const size_t buf = (size_t)rounds * ng * L * sizeof(cpx<R>);
+    const size_t tab = (size_t)g.nz * sizeof(cpx<R>);
+    if (buf > h->lds_limit) return false;
+    out = RhsPart{part, th0, th1, buf + tab <= h->lds_limit ? 1 : 0, ng, buf + tab <= h->lds_limit ? buf + tab : buf};
+    return true;
+  };
+  std::vector<RhsPart> plan;
+  RhsPart pt;
+  if (make(3, 0, nth, pt)) return {pt};
+  if (make(3, 0, 0, pt))
+    plan.push_back(pt);
+  else if (make(1, 0, 0, pt)) {
+    plan.push_back(pt);
+    make(2, 0, 0, pt);
+    plan.push_back(pt);
+  } else
+    throw ApiError{DGPU_EUNSUP, "Nz too large for the spectral RHS pass"};
+  for (int n0 = 0; n0 < nth;) {
+    int n1 = n0 + 1;
+    if (!make(0, n0, n1, pt)) throw ApiError{DGPU_EUNSUP, "Nz too large for the spectral RHS pass"};
+    while (n1 < nth && make(0, n0, n1 + 1, pt)) ++n1;
+    make(0, n0, n1, pt);
+    plan.push_back(pt);
+    n0 = n1;
+  }
+  return plan;
 }
 
 // ----------------------------------------------------------- the substep --
@@ -1710,14 +1761,9 @@ void rk_substep_t(dgpu_handle* h, int rk_step, double time, int time_step) {
     a.nth = nth;
     a.js = h->jstart;
     a.je = h->jend;
-    const int G = g.nz / 8;
-    int ng = 512 / G;
-    if (ng < 1) ng = 1;
-    if (ng > nslot) ng = nslot;
-    const int rounds = (nslot + ng - 1) / ng;
-    const size_t lds = ((size_t)rounds * ng * fft_lds_len(g.nz) + g.nz) * sizeof(cpx<R>);  // + twiddles
-    if (lds > h->lds_limit) throw ApiError{DGPU_EUNSUP, "Nz too large for the spectral RHS pass"};
-    dim3 grid(g.nq, Ny), block(ng * G);
+    auto kern = g.nz <= 512 ? k_rhs<R, true> : k_rhs<R, false>;
+    if constexpr (sizeof(R) == 8)
+      if (g.nz == 512 && h->kn.fft_const) kern = k_rhs<R, true, 9>;
     // x-spectra of the products in, cr x-spectra out; cu1..3, cp, cth, cf1..3,
     // cfth read and cf1..3, cfth written (neighbour planes come from cache)
     const double crbytes = spec ? (incr ? 8.0 : 16.0) * g.sp * ncr : 2.0 * sizeof(R) * g.tp * ncr;
@@ -1725,11 +1771,15 @@ void rk_substep_t(dgpu_handle* h, int rk_step, double time, int time_step) {
              (2.0 * sizeof(R) * g.tp * nslot + crbytes + 16.0 * g.sp * (4 + 2 * nth) +
               2.0 * sizeof(R) * g.sp * 2 * (3 + nth)) * Ny,  // the carry cf is stored in R
              (2.0 * sizeof(R) * g.tp * nslot + (spec ? 0.0 : crbytes)) * Ny);
-    auto kern = g.nz <= 512 ? k_rhs<R, true> : k_rhs<R, false>;
-    if constexpr (sizeof(R) == 8)
-      if (g.nz == 512 && h->kn.fft_const) kern = k_rhs<R, true, 9>;
-    hipLaunchKernelGGL(kern, grid, block, lds, h->stream, a, g, twz_of<R>(h), 1);
-    HIPCHECK(hipGetLastError());
+    for (const RhsPart& pt : rhs_plan<R>(h, nth)) {
+      a.part = pt.part;
+      a.th0 = pt.th0;
+      a.th1 = pt.th1;
+      a.tw_lds = pt.tw_lds;
+      dim3 grid(g.nq, Ny), block(pt.ng * (g.nz / 8));
+      hipLaunchKernelGGL(kern, grid, block, pt.lds, h->stream, a, g, twz_of<R>(h), 1);
+      HIPCHECK(hipGetLastError());
+    }
   }
 
   // 4. cr to physical space (second half of fft_xz_to_physical, :485-487, :621)

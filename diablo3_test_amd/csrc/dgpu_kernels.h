@@ -294,7 +294,8 @@ __global__ void __launch_bounds__(512) k_zinv(const cpx<double>* __restrict__ C,
 //   pair 3: (W1, V2)     -> X2  = ikx F(W1) + F(V2)              (W1 = interp(u1) u2)
 //   pair 4: (th u1, Vt)  -> Xt  = ikx F(th u1) + F(Vt)
 //   pair 5: (th u3, 0)   -> Xt3 = F(th u3)
-constexpr int MAXTH = 4;
+constexpr int MAXTH = 16;  // DGPU_MAX_TH; k_rhs keeps a bit per scalar in a 32-bit mask
+static_assert(MAXTH <= 32, "k_rhs holds the scalars' row flags in one 32-bit mask");
 constexpr int MAXSLOT = 6 + 2 * MAXTH;
 
 struct ProdArgs {
@@ -566,6 +567,12 @@ struct RhsArgs {
   int rk_step, nth;
   int js, je;
   int jst[MAXTH], jet[MAXTH];
+  // The outputs of this launch.  part & 1: cr1, cr3 (product slots 0-3);
+  // part & 2: cr2 (slots 4, 5); scalars th0 <= n < th1 (slots 6+2n, 7+2n).
+  // The LDS holds one buffer per slot of the launch, so a substep whose
+  // slots do not all fit is split over launches with disjoint outputs.
+  int part, th0, th1;
+  int tw_lds;  // 1: the twiddle table is copied to LDS (0: read from HBM)
 };
 
 template <typename R, bool WAVE, int LOGN = 0>
@@ -580,30 +587,42 @@ __global__ void __launch_bounds__(512) k_rhs(RhsArgs a, Geo g, const cpx<R>* __r
   const int grp = threadIdx.x / G, gt = threadIdx.x % G;
   const int j = j0 + blockIdx.y, q = blockIdx.x;
   const int L = fft_lds_len(g.nz);
-  const int nslot = 6 + 2 * a.nth;
   const bool spec = a.crs[0] != nullptr;
-  const bool in1 = j >= a.js && j <= a.je;
-  const bool in2 = j >= 2 && j <= g.ny;
-  bool inT[MAXTH];
-  for (int n = 0; n < MAXTH; ++n) inT[n] = n < a.nth && j >= a.jst[n] && j <= a.jet[n];
+  const bool p13 = (a.part & 1) != 0, p2 = (a.part & 2) != 0;
+  const bool in1 = p13 && j >= a.js && j <= a.je;
+  const bool in2 = p2 && j >= 2 && j <= g.ny;
+  unsigned inT = 0;  // bit n: scalar n of this launch has row j
+  for (int n = a.th0; n < a.th1; ++n)
+    if (j >= a.jst[n] && j <= a.jet[n]) inT |= 1u << n;
+  // local slots (LDS buffers) of the launch: its velocity slots, then the
+  // pairs of its scalars; cr slot c (0 cr1, 1 cr3, 2 cr2, 3+n crth_n) goes
+  // to local buffer lcr(c), whose products its own outputs have read first
+  const int nv = (p13 ? 4 : 0) + (p2 ? 2 : 0);
+  const int nslot = nv + 2 * (a.th1 - a.th0);
+  const int ncv = (p13 ? 2 : 0) + (p2 ? 1 : 0);
+  const int ncr = ncv + (a.th1 - a.th0);
+  auto gslot = [&](int ls) -> int { return ls < nv ? (p13 ? ls : 4 + ls) : 6 + 2 * a.th0 + (ls - nv); };
+  auto lslot = [&](int s) -> int { return s < 4 ? s : s < 6 ? s - 4 + (p13 ? 4 : 0) : nv + s - 6 - 2 * a.th0; };
+  auto gcr = [&](int lc) -> int { return lc < ncv ? (p13 ? lc : 2) : 3 + a.th0 + (lc - ncv); };
+  auto lcr = [&](int c) -> int { return c < 2 ? c : c == 2 ? (p13 ? 2 : 0) : ncv + c - 3 - a.th0; };
   auto slot_used = [&](int s) -> bool {
     if (s < 4) return in1;
     if (s < 6) return in2;
-    return inT[(s - 6) >> 1];
+    return (inT >> ((s - 6) >> 1)) & 1u;
   };
   const long long to = (long long)j * g.tp + (long long)q * g.nz;
   const int rounds = (nslot + ngroups - 1) / ngroups;
-  const CR* twl = lds_twiddles(smem + (long long)rounds * ngroups * L, tw, g.nz);
+  const CR* twl = a.tw_lds ? lds_twiddles(smem + (long long)rounds * ngroups * L, tw, g.nz) : tw;
   // --- z-transform the product spectra
   // The LDS holds one buffer per (round, group), so groups without a slot in
   // a round transform an unused buffer and every thread meets every barrier.
   for (int s0 = 0; s0 < nslot; s0 += ngroups) {
-    const int s = s0 + grp;
-    const bool act = s < nslot && slot_used(s);
-    CR* buf = smem + (long long)s * L;
+    const int ls = s0 + grp;
+    const bool act = ls < nslot && slot_used(gslot(ls));
+    CR* buf = smem + (long long)ls * L;
     CR v[8];
     if (act) {
-      const CR* col = reinterpret_cast<const CR*>(a.X[s]) + to;
+      const CR* col = reinterpret_cast<const CR*>(a.X[gslot(ls)]) + to;
 #pragma unroll
       for (int r = 0; r < 8; ++r) v[r] = col[gt + G * r];
     } else {
@@ -628,16 +647,16 @@ __global__ void __launch_bounds__(512) k_rhs(RhsArgs a, Geo g, const cpx<R>* __r
     // is written (cr slot 3+n < 6+2n, and slots 0..3 belong to u1/u3)
     const int pn = pidx(nat);
     auto Z = [&](int s) -> C {
-      const CR z = smem[(long long)s * L + pn];
+      const CR z = smem[(long long)lslot(s) * L + pn];
       return invnz * C{(double)z.x, (double)z.y};
     };
-    auto put = [&](int s, C v) {
+    auto put = [&](int c, C v) {
       if (spec && a.crs32)
-        reinterpret_cast<cpx<float>*>(a.crs[s])[o] = cpx<float>{(float)v.x, (float)v.y};
+        reinterpret_cast<cpx<float>*>(a.crs[c])[o] = cpx<float>{(float)v.x, (float)v.y};
       else if (spec)
-        reinterpret_cast<C*>(a.crs[s])[o] = v;
+        reinterpret_cast<C*>(a.crs[c])[o] = v;
       else
-        smem[(long long)s * L + pn] = CR{(R)v.x, (R)v.y};
+        smem[(long long)lcr(c) * L + pn] = CR{(R)v.x, (R)v.y};
     };
     auto ld = [&](const double2* p) -> C { return reinterpret_cast<const C*>(p)[o]; };
     auto ldcf = [&](const double2* p) -> C {
@@ -646,7 +665,7 @@ __global__ void __launch_bounds__(512) k_rhs(RhsArgs a, Geo g, const cpx<R>* __r
     };
     auto stcf = [&](double2* p, C v) { reinterpret_cast<CR*>(p)[o] = CR{(R)v.x, (R)v.y}; };
     C cu1 = {0, 0}, cu3 = {0, 0};
-    if (in1 || a.nth > 0) {
+    if (in1 || a.th1 > a.th0) {
       cu1 = reinterpret_cast<const C*>(a.cu1)[o];
       cu3 = reinterpret_cast<const C*>(a.cu3)[o];
     }
@@ -720,8 +739,8 @@ __global__ void __launch_bounds__(512) k_rhs(RhsArgs a, Geo g, const cpx<R>* __r
       stcf(a.cf2, f2);
       put(2, r2);
     }
-    for (int n = 0; n < a.nth; ++n) {
-      if (!inT[n]) continue;
+    for (int n = a.th0; n < a.th1; ++n) {
+      if (!((inT >> n) & 1u)) continue;
       C th = reinterpret_cast<const C*>(a.cth[n])[o];
       C ft = (-(a.nu / a.pr[n]) * visc) * th;
       ft = ft - a.dthdx[n] * cu1 - a.dthdz[n] * cu3;
@@ -742,27 +761,26 @@ __global__ void __launch_bounds__(512) k_rhs(RhsArgs a, Geo g, const cpx<R>* __r
   }
   if (spec) return;  // cr stays spectral
   // zero the dealiased gap of the cr slots (fft.f90:336-340)
-  const int ncr = 3 + a.nth;
-  for (int c = 0; c < ncr; ++c)
+  for (int lc = 0; lc < ncr; ++lc)
     for (int k = g.nkz + 1 + threadIdx.x; k < g.nz - g.nkz; k += blockDim.x)
-      smem[c * L + pidx(k)] = CR{(R)0, (R)0};
+      smem[lc * L + pidx(k)] = CR{(R)0, (R)0};
   // --- backward z transform of cr
   auto cr_used = [&](int c) -> bool {
     if (c < 2) return in1;
     if (c == 2) return in2;
-    return inT[c - 3];
+    return (inT >> (c - 3)) & 1u;
   };
   for (int c0 = 0; c0 < ncr; c0 += ngroups) {
-    const int c = c0 + grp;
-    const bool act = c < ncr && cr_used(c);
-    CR* buf = smem + (long long)c * L;
+    const int lc = c0 + grp;
+    const bool act = lc < ncr && cr_used(gcr(lc));
+    CR* buf = smem + (long long)lc * L;
     __syncthreads();  // the cr slots were written by every thread of the block
     CR v[8];
 #pragma unroll
     for (int r = 0; r < 8; ++r) v[r] = buf[pidx(gt + G * r)];
     fft_any<R, 1, WAVE, LOGN>(v, buf, g.nz, g.lognz, twl, gt);
     if (act) {
-      CR* out = reinterpret_cast<CR*>(a.Tcr[c]) + to;
+      CR* out = reinterpret_cast<CR*>(a.Tcr[gcr(lc)]) + to;
       for (int k = gt; k < g.nz; k += G) out[k] = buf[pidx(k)];
     }
   }

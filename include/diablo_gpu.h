@@ -39,7 +39,10 @@
 extern "C" {
 #endif
 
-#define DGPU_MAX_TH 4
+/* The most passive scalars a handle takes (the reference fixes N_th at
+   compile time, make_headers:17-21); the per-scalar arrays of dgpu_params
+   hold this many entries. */
+#define DGPU_MAX_TH 16
 
 /* Error codes. */
 #define DGPU_OK 0
@@ -227,7 +230,7 @@ int dgpu_check_state(dgpu_handle* h);
  * reference: it is for callers that cannot allocate device memory, such as
  * a tight serving loop or a harness that forbids hipMalloc).  Same scan and
  * results as dgpu_check_state, whose per-block flags it keeps in a static
- * device array of 12.5 KiB (128 blocks x 25 arrays of ints) instead of a
+ * device array of 29 KiB (128 blocks x 58 arrays of ints) instead of a
  * buffer the handle allocates on first use; scans of more arrays are
  * rejected with DGPU_EINVAL.  Each GPU holds one such array until the
  * process ends, shared by every call on that GPU: calls that overlap on one
