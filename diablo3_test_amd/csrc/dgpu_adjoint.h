@@ -387,7 +387,8 @@ struct AdjPairArgs {
   int npairs;
 };
 
-template <bool WAVE>
+// RAG: ragged last tile (Nz not a multiple of KT); rows k >= Nz are skipped.
+template <bool WAVE, bool RAG = false>
 __global__ void __launch_bounds__(512) k_adj_xpairs(AdjPairArgs a, Geo g, const cpx<double>* __restrict__ tw,
                                                     int j0) {
   typedef cpx<double> C;
@@ -413,8 +414,9 @@ __global__ void __launch_bounds__(512) k_adj_xpairs(AdjPairArgs a, Geo g, const 
       // wavefront reads 16-byte values of ngroups lines as one run
       for (int idx = threadIdx.x; idx < g.nq * ngroups; idx += blockDim.x) {
         const int q = idx >> lg_ng, mm = idx & (ngroups - 1);
-        const long long o = (long long)j * g.tp + (long long)q * g.nz + k0 + m0 + mm;
-        auto X = [&](int s) { return reinterpret_cast<const C*>(a.X[s])[o]; };
+        const bool ok = !RAG || k0 + m0 + mm < g.nz;
+        const long long o = (long long)j * g.tp + (long long)q * g.nz + (ok ? k0 + m0 + mm : 0);
+        auto X = [&](int s) { return ok ? reinterpret_cast<const C*>(a.X[s])[o] : C{0.0, 0.0}; };
         const double kxq = a.kx[q];
         C al, be;
         switch (kind) {
@@ -440,6 +442,7 @@ __global__ void __launch_bounds__(512) k_adj_xpairs(AdjPairArgs a, Geo g, const 
 #pragma unroll
       for (int r = 0; r < 8; ++r) v[r] = fbuf[pidx(gt + G * r)];
       fft_regs<double, 1, WAVE>(v, fbuf, g.nx, g.lognx, twl, gt);
+      if (RAG && k0 + m0 + grp >= g.nz) continue;  // past Nz: nothing to store (the barrier is at the loop top)
       const long long ro = (long long)j * g.pp + (long long)(k0 + m0 + grp) * g.nx;
       double* oa = a.outA[pr] + ro;
       double* ob = a.outB[pr] + ro;

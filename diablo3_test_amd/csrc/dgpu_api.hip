@@ -20,6 +20,7 @@
 #include "dgpu_init.h"
 #include "dgpu_les.h"
 #include "dgpu_les_adj.h"
+#include "dgpu_zgen.h"
 
 using namespace dgpu;
 
@@ -142,6 +143,12 @@ struct dgpu_handle {
   DevBuf<double> u[3], th[DGPU_MAX_TH], r[3], rth[DGPU_MAX_TH];
   DevBuf<double> nu_t, kappa_t[DGPU_MAX_TH];
   std::vector<DevBuf<double2>> tpool;
+  // Nz not a power of two: the z passes run the general transforms of
+  // dgpu_zgen.h with this plan, and zpool holds the z spectra of the product
+  // slots (FP64, plane pitch sp) that the RHS pass then combines
+  bool zgen = false;
+  ZPlan zplan{};
+  std::vector<DevBuf<double2>> zpool;
   DevBuf<double> sb;
   DevBuf<double2> sg;
   // projection: 1/b_j of every mode (rows 1..Ny) and the row coefficients
@@ -634,11 +641,12 @@ void launch_xfwd(dgpu_handle* h, const R* u, cpx<R>* T, int j0, int j1, const Fi
   const int ng = x_ngroups<R>(h, KT / 2, h->kn.xpass_threads);
   const size_t lds = x_lds<R>(h, ng);
   const int nf = fb ? fb->nf : 1;
-  dim3 grid(g.nz / KT, j1 - j0 + 1, nf), block(ng * (g.nx / 8));
+  dim3 grid((g.nz + KT - 1) / KT, j1 - j0 + 1, nf), block(ng * (g.nx / 8));
   Timed tm(h, DGPU_K_XFWD, (sizeof(R) * g.pp + 2.0 * sizeof(R) * g.tp) * (j1 - j0 + 1) * nf,
            2.0 * sizeof(R) * g.tp * (j1 - j0 + 1) * nf);
-  hipLaunchKernelGGL((g.nx <= 512 ? k_xfwd<R, true> : k_xfwd<R, false>), grid, block, lds, h->stream, u, T, g,
-                     twx_of<R>(h), j0, fb ? *fb : FieldBatch{});
+  auto kern = g.nx <= 512 ? k_xfwd<R, true> : k_xfwd<R, false>;
+  if (g.nz % KT) kern = g.nx <= 512 ? k_xfwd<R, true, true> : k_xfwd<R, false, true>;  // ragged last tile
+  hipLaunchKernelGGL(kern, grid, block, lds, h->stream, u, T, g, twx_of<R>(h), j0, fb ? *fb : FieldBatch{});
   HIPCHECK(hipGetLastError());
 }
 
@@ -651,13 +659,71 @@ void launch_xinv(dgpu_handle* h, const cpx<R>* T, R* u, int j0, int j1, const do
   const int ng = x_ngroups<R>(h, KT / 2, h->kn.xpass_threads);
   const size_t lds = x_lds<R>(h, ng, XInvTable<R>::on);
   const int nf = fb ? fb->nf : 1;
-  dim3 grid(g.nz / KT, j1 - j0 + 1, nf), block(ng * (g.nx / 8));
+  dim3 grid((g.nz + KT - 1) / KT, j1 - j0 + 1, nf), block(ng * (g.nx / 8));
   Timed tm(h, DGPU_K_XINV, (sizeof(R) * g.pp + 2.0 * sizeof(R) * g.tp) * (j1 - j0 + 1) * nf,
            2.0 * sizeof(R) * g.tp * (j1 - j0 + 1) * nf);
   auto kern = g.nx <= 512 ? k_xinv<R, true> : k_xinv<R, false>;
   if constexpr (sizeof(R) == 8)
     if (g.nx == 512 && h->kn.fft_const) kern = k_xinv<R, true, 9>;
+  if (g.nz % KT) kern = g.nx <= 512 ? k_xinv<R, true, 0, true> : k_xinv<R, false, 0, true>;  // ragged last tile
   hipLaunchKernelGGL(kern, grid, block, lds, h->stream, T, u, g, twx_of<R>(h), j0, dkx, fb ? *fb : FieldBatch{});
+  HIPCHECK(hipGetLastError());
+}
+
+// --- z transforms of any length (Nz not a power of two)
+// Radices of n for the general z transforms: 4s first, then 2, then the odd
+// primes in increasing order.
+ZPlan make_zplan(int n) {
+  ZPlan pl{};
+  auto push = [&](int r) {
+    if (pl.np >= ZGEN_MAXPASS) throw ApiError{DGPU_EINVAL, "nz has too many factors"};
+    pl.rad[pl.np++] = r;
+  };
+  while (n % 4 == 0) { push(4); n /= 4; }
+  while (n % 2 == 0) { push(2); n /= 2; }
+  for (int f = 3; n > 1; f += 2)
+    while (n % f == 0) { push(f); n /= f; }
+  return pl;
+}
+
+// Threads per column and columns per workgroup of the general z passes.
+template <typename R>
+void zgen_shape(const dgpu_handle* h, int& tc, int& cb, size_t& lds) {
+  const int n = h->g.nz;
+  tc = n <= 512 ? 64 : 256;
+  cb = 256 / tc;
+  const size_t per = 2 * (size_t)n * sizeof(cpx<R>);
+  while (cb > 1 && cb * per > h->lds_limit) --cb;
+  if (cb * per > h->lds_limit) throw ApiError{DGPU_EUNSUP, "Nz too large for the z-transform pass"};
+  lds = cb * per;
+}
+
+template <typename R>
+void zgen_fwd(dgpu_handle* h, const cpx<R>* T, double2* C, int j0, int j1, bool accum, double s0, double s1,
+              const FieldBatch* fb) {
+  const Geo& g = h->g;
+  int tc, cb;
+  size_t lds;
+  zgen_shape<R>(h, tc, cb, lds);
+  const int nf = fb ? fb->nf : 1;
+  dim3 grid((g.nq + cb - 1) / cb, j1 - j0 + 1, nf), block(tc * cb);
+  hipLaunchKernelGGL(k_zgen_fwd<R>, grid, block, lds, h->stream, T, reinterpret_cast<cpx<double>*>(C), g, h->zplan,
+                     twz_of<R>(h), j0, accum ? 1 : 0, (R)s0, (R)s1, tc, fb ? *fb : FieldBatch{});
+  HIPCHECK(hipGetLastError());
+}
+
+template <typename R>
+void zgen_inv(dgpu_handle* h, const double2* C, cpx<R>* T, int j0, int j1, int op, double s0, double s1,
+              const FieldBatch* fb) {
+  const Geo& g = h->g;
+  int tc, cb;
+  size_t lds;
+  zgen_shape<R>(h, tc, cb, lds);
+  const int nf = fb ? fb->nf : 1;
+  dim3 grid((g.nq + cb - 1) / cb, j1 - j0 + 1, nf), block(tc * cb);
+  hipLaunchKernelGGL(k_zgen_inv<R>, grid, block, lds, h->stream, reinterpret_cast<const cpx<double>*>(C), T, g,
+                     h->zplan, twz_of<R>(h), j0, op, h->d_kx.p, h->d_kz.p, h->d_dy.p, h->d_dyf.p, s0, s1, tc,
+                     fb ? *fb : FieldBatch{});
   HIPCHECK(hipGetLastError());
 }
 
@@ -666,12 +732,16 @@ void launch_zfwd(dgpu_handle* h, const cpx<R>* T, double2* C, int j0, int j1, bo
                  double s1 = 1.0, const FieldBatch* fb = nullptr) {
   if (j1 < j0) return;
   const Geo& g = h->g;
-  const int ng = z_ngroups<R>(h);
-  const size_t lds = z_lds<R>(h, ng);
   const int nf = fb ? fb->nf : 1;
-  dim3 grid((g.nq + ng - 1) / ng, j1 - j0 + 1, nf), block(ng * (g.nz / 8));
   Timed tm(h, DGPU_K_ZFWD, (2.0 * sizeof(R) * g.tp + (accum ? 32.0 : 16.0) * g.sp) * (j1 - j0 + 1) * nf,
            2.0 * sizeof(R) * g.tp * (j1 - j0 + 1) * nf);
+  if (h->zgen) {
+    zgen_fwd<R>(h, T, C, j0, j1, accum, s0, s1, fb);
+    return;
+  }
+  const int ng = z_ngroups<R>(h);
+  const size_t lds = z_lds<R>(h, ng);
+  dim3 grid((g.nq + ng - 1) / ng, j1 - j0 + 1, nf), block(ng * (g.nz / 8));
   hipLaunchKernelGGL((g.nz <= 512 ? k_zfwd<R, true> : k_zfwd<R, false>), grid, block, lds, h->stream, T,
                      reinterpret_cast<cpx<double>*>(C), g, twz_of<R>(h), j0, accum ? 1 : 0, (R)s0, (R)s1,
                      fb ? *fb : FieldBatch{});
@@ -683,12 +753,16 @@ void launch_zinv(dgpu_handle* h, const double2* C, cpx<R>* T, int j0, int j1, do
                  double s1 = 1.0, const FieldBatch* fb = nullptr) {
   if (j1 < j0) return;
   const Geo& g = h->g;
-  const int ng = z_ngroups<R>(h);
-  const size_t lds = z_lds<R>(h, ng);
   const int nf = fb ? fb->nf : 1;
-  dim3 grid((g.nq + ng - 1) / ng, j1 - j0 + 1, nf), block(ng * (g.nz / 8));
   Timed tm(h, DGPU_K_ZINV, (2.0 * sizeof(R) * g.tp + 16.0 * g.sp) * (j1 - j0 + 1) * nf,
            2.0 * sizeof(R) * g.tp * (j1 - j0 + 1) * nf);
+  if (h->zgen) {
+    zgen_inv<R>(h, C, T, j0, j1, OP_ID, s0, s1, fb);
+    return;
+  }
+  const int ng = z_ngroups<R>(h);
+  const size_t lds = z_lds<R>(h, ng);
+  dim3 grid((g.nq + ng - 1) / ng, j1 - j0 + 1, nf), block(ng * (g.nz / 8));
   auto kern = g.nz <= 512 ? k_zinv<R, true> : k_zinv<R, false>;
   if constexpr (sizeof(R) == 8)
     if (g.nz == 512 && h->kn.fft_const) kern = k_zinv<R, true, 9>;
@@ -702,11 +776,15 @@ void launch_zinv_op(dgpu_handle* h, const double2* C, cpx<R>* T, int j0, int j1,
                     double s1 = 1.0) {
   if (j1 < j0) return;
   const Geo& g = h->g;
+  Timed tm(h, DGPU_K_ZINV, (2.0 * sizeof(R) * g.tp + 16.0 * g.sp) * (j1 - j0 + 1),
+           2.0 * sizeof(R) * g.tp * (j1 - j0 + 1));
+  if (h->zgen) {
+    zgen_inv<R>(h, C, T, j0, j1, op, s0, s1, nullptr);
+    return;
+  }
   const int ng = z_ngroups<R>(h);
   const size_t lds = z_lds<R>(h, ng);
   dim3 grid((g.nq + ng - 1) / ng, j1 - j0 + 1), block(ng * (g.nz / 8));
-  Timed tm(h, DGPU_K_ZINV, (2.0 * sizeof(R) * g.tp + 16.0 * g.sp) * (j1 - j0 + 1),
-           2.0 * sizeof(R) * g.tp * (j1 - j0 + 1));
   hipLaunchKernelGGL((g.nz <= 512 ? k_zinv_op<R, true> : k_zinv_op<R, false>), grid, block, lds, h->stream,
                      reinterpret_cast<const cpx<double>*>(C), T, g, twz_of<R>(h), j0, op, h->d_kx.p,
                      h->d_kz.p, h->d_dy.p, h->d_dyf.p, s0, s1);
@@ -1684,7 +1762,7 @@ void rk_substep_t(dgpu_handle* h, int rk_step, double time, int time_step) {
     const int ngx = x_ngroups<R>(h, KT, h->kn.xprod_threads);
     const size_t lds = x_lds<R>(h, ngx);
     if (lds > h->lds_limit) throw ApiError{DGPU_EUNSUP, "Nx too large for the products pass"};
-    dim3 grid(g.nz / KT, Ny), block(ngx * (g.nx / 8));
+    dim3 grid((g.nz + KT - 1) / KT, Ny), block(ngx * (g.nx / 8));
     // reads u1, u2, u3, th once (neighbour planes from cache); writes the product x-spectra
     Timed tm(h, DGPU_K_XPROD, (sizeof(R) * g.pp * (3 + nth) + 2.0 * sizeof(R) * g.tp * nslot) * Ny,
              2.0 * sizeof(R) * g.tp * nslot * Ny);
@@ -1694,6 +1772,7 @@ void rk_substep_t(dgpu_handle* h, int rk_step, double time, int time_step) {
     auto kern = g.nx <= 512 ? k_xprod_all<R, true> : k_xprod_all<R, false>;
     if constexpr (sizeof(R) == 8)
       if (g.nx == 512 && h->kn.fft_const) kern = k_xprod_all<R, true, 9>;
+    if (g.nz % KT) kern = g.nx <= 512 ? k_xprod_all<R, true, 0, true> : k_xprod_all<R, false, 0, true>;
     hipLaunchKernelGGL(kern, grid, block, lds, h->stream, pa, g, twx_of<R>(h), 1);
     HIPCHECK(hipGetLastError());
   }
@@ -1771,14 +1850,49 @@ void rk_substep_t(dgpu_handle* h, int rk_step, double time, int time_step) {
              (2.0 * sizeof(R) * g.tp * nslot + crbytes + 16.0 * g.sp * (4 + 2 * nth) +
               2.0 * sizeof(R) * g.sp * 2 * (3 + nth)) * Ny,  // the carry cf is stored in R
              (2.0 * sizeof(R) * g.tp * nslot + (spec ? 0.0 : crbytes)) * Ny);
-    for (const RhsPart& pt : rhs_plan<R>(h, nth)) {
-      a.part = pt.part;
-      a.th0 = pt.th0;
-      a.th1 = pt.th1;
-      a.tw_lds = pt.tw_lds;
-      dim3 grid(g.nq, Ny), block(pt.ng * (g.nz / 8));
-      hipLaunchKernelGGL(kern, grid, block, pt.lds, h->stream, a, g, twz_of<R>(h), 1);
+    if (h->zgen) {
+      // general Nz: the z spectra of the slots first (planes 1..Ny), then a
+      // combine-only pass; without spectral solves cr goes to the slot
+      // buffers (each cr slot overwrites a slot its own outputs read first,
+      // mode by mode in the same thread) and through the general z inverse
+      for (int s0 = 0; s0 < nslot; s0 += MAXBF) {
+        FieldBatch fb{};
+        fb.nf = std::min(MAXBF, nslot - s0);
+        for (int k = 0; k < fb.nf; ++k) {
+          fb.in[k] = h->tpool[s0 + k].p;
+          fb.out[k] = h->zpool[s0 + k].p;
+        }
+        zgen_fwd<R>(h, xspec<R>(h->tpool[s0]), h->zpool[s0].p, 1, Ny, false, 1.0, 1.0, &fb);
+      }
+      for (int s = 0; s < nslot; ++s) a.X[s] = h->zpool[s].p;
+      if (!spec)
+        for (int c = 0; c < ncr; ++c) a.crs[c] = h->zpool[c].p;
+      a.part = 3;
+      a.th0 = 0;
+      a.th1 = nth;
+      hipLaunchKernelGGL((k_rhs<R, false, 0, true>), dim3(g.nq, Ny), dim3(64), 0, h->stream, a, g, twz_of<R>(h),
+                         1);
       HIPCHECK(hipGetLastError());
+      if (!spec)
+        for (int c0 = 0; c0 < ncr; c0 += MAXBF) {
+          FieldBatch fb{};
+          fb.nf = std::min(MAXBF, ncr - c0);
+          for (int k = 0; k < fb.nf; ++k) {
+            fb.in[k] = h->zpool[c0 + k].p;
+            fb.out[k] = h->tpool[nslot + c0 + k].p;
+          }
+          zgen_inv<R>(h, h->zpool[c0].p, xspec<R>(h->tpool[nslot + c0]), 1, Ny, OP_ID, 1.0, 1.0, &fb);
+        }
+    } else {
+      for (const RhsPart& pt : rhs_plan<R>(h, nth)) {
+        a.part = pt.part;
+        a.th0 = pt.th0;
+        a.th1 = pt.th1;
+        a.tw_lds = pt.tw_lds;
+        dim3 grid(g.nq, Ny), block(pt.ng * (g.nz / 8));
+        hipLaunchKernelGGL(kern, grid, block, pt.lds, h->stream, a, g, twz_of<R>(h), 1);
+        HIPCHECK(hipGetLastError());
+      }
     }
   }
 
@@ -2360,10 +2474,11 @@ void adj_substep(dgpu_handle* h, int rk_step, double time, int time_step) {
     constexpr int KT = XTile<double>::KT;
     const int ng = x_ngroups<double>(h, KT, 512);
     const size_t lds = x_lds<double>(h, ng);
-    dim3 grid(g.nz / KT, Ny), block(ng * (g.nx / 8));
+    dim3 grid((g.nz + KT - 1) / KT, Ny), block(ng * (g.nx / 8));
     Timed tm(h, DGPU_K_ADJOINT, (16.0 * g.tp * nslot + 16.0 * g.pp * npairs) * Ny, 16.0 * g.tp * nslot * Ny);
-    hipLaunchKernelGGL((g.nx <= 512 ? k_adj_xpairs<true> : k_adj_xpairs<false>), grid, block, lds, h->stream, a,
-                       g, twx_of<double>(h), 1);
+    auto kern = g.nx <= 512 ? k_adj_xpairs<true> : k_adj_xpairs<false>;
+    if (g.nz % KT) kern = g.nx <= 512 ? k_adj_xpairs<true, true> : k_adj_xpairs<false, true>;
+    hipLaunchKernelGGL(kern, grid, block, lds, h->stream, a, g, twx_of<double>(h), 1);
     HIPCHECK(hipGetLastError());
   }
   {
@@ -2680,8 +2795,9 @@ bool carry32(const dgpu_handle* h, int f) {
 }
 
 int check_params(const dgpu_params* p, std::string& why) {
-  auto pow2 = [](int n) { return n >= 16 && n <= 2048 && (n & (n - 1)) == 0; };
-  if (!pow2(p->nx) || !pow2(p->nz)) { why = "nx and nz must be powers of two in [16, 2048]"; return -1; }
+  auto pow2 = [](int n) { return n >= 16 && n <= 4096 && (n & (n - 1)) == 0; };
+  if (!pow2(p->nx)) { why = "nx must be a power of two in [16, 4096]"; return -1; }
+  if (p->nz < 16 || p->nz > 2048) { why = "nz must be in [16, 2048]"; return -1; }
   if (p->ny < 5) { why = "ny must be >= 5"; return -1; }
   if (p->n_th < 0 || p->n_th > DGPU_MAX_TH) { why = "n_th out of range"; return -1; }
   if (!(p->re > 0) || !(p->delta_t > 0)) { why = "re and delta_t must be positive"; return -1; }
@@ -2724,6 +2840,8 @@ void free_all(dgpu_handle* h) {
   h->nu_t.release();
   for (auto& b : h->tpool) b.release();
   h->tpool.clear();
+  for (auto& b : h->zpool) b.release();
+  h->zpool.clear();
   h->sb.release();
   h->sg.release();
   h->rd_ib.release();
@@ -2800,6 +2918,8 @@ void create_impl(dgpu_handle* h, const double* gy) {
   while ((1 << g.lognx) < g.nx) ++g.lognx;
   g.lognz = 0;
   while ((1 << g.lognz) < g.nz) ++g.lognz;
+  h->zgen = (1 << g.lognz) != g.nz;
+  if (h->zgen) h->zplan = make_zplan(g.nz);
   g.nkx = p.nx / 3;
   g.nkz = p.nz / 3;
   g.nq = g.nkx + 1;
@@ -2835,7 +2955,17 @@ void create_impl(dgpu_handle* h, const double* gy) {
         (const void*)k_zinv<float, true>,     (const void*)k_zinv<float, false>,
         (const void*)k_zinv_op<float, true>,  (const void*)k_zinv_op<float, false>,
         (const void*)k_xprod_all<float, true>, (const void*)k_xprod_all<float, false>,
-        (const void*)k_adj_xpairs<true>,      (const void*)k_adj_xpairs<false>};
+        (const void*)k_adj_xpairs<true>,      (const void*)k_adj_xpairs<false>,
+        (const void*)k_xfwd<double, false, true>, (const void*)k_xinv<double, false, 0, true>,
+        (const void*)k_xprod_all<double, false, 0, true>, (const void*)k_adj_xpairs<false, true>,
+        (const void*)k_xfwd<float, false, true>,  (const void*)k_xinv<float, false, 0, true>,
+        (const void*)k_xprod_all<float, false, 0, true>,
+        (const void*)k_xfwd<double, true, true>,  (const void*)k_xinv<double, true, 0, true>,
+        (const void*)k_xprod_all<double, true, 0, true>, (const void*)k_adj_xpairs<true, true>,
+        (const void*)k_xfwd<float, true, true>,   (const void*)k_xinv<float, true, 0, true>,
+        (const void*)k_xprod_all<float, true, 0, true>,
+        (const void*)k_zgen_fwd<double>, (const void*)k_zgen_inv<double>,
+        (const void*)k_zgen_fwd<float>,  (const void*)k_zgen_inv<float>};
     for (const void* k : kernels)
       HIPCHECK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds_limit));
   }
@@ -2864,6 +2994,10 @@ void create_impl(dgpu_handle* h, const double* gy) {
   const int nslot = 6 + 2 * h->nth, ncr = 3 + h->nth;
   h->tpool.resize(nslot + ncr);
   for (auto& b : h->tpool) b.alloc(nt);
+  if (h->zgen) {
+    h->zpool.resize(nslot);
+    for (auto& b : h->zpool) b.alloc(ns);
+  }
   h->sb.alloc(ns);
   h->sg.alloc(ns);
   for (int f = 0; f < 3; ++f) {

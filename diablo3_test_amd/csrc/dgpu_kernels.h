@@ -80,7 +80,9 @@ __device__ __forceinline__ void split_pair(const cpx<R>* f, int n, int q, R scal
 // spectra straight from the transform buffers as KT-long runs of T[j][q][k].
 // grid (Nz/KT, nplanes), block ngroups*Nx/8 (ngroups divides KT/2),
 // LDS ngroups*fft_lds_len(Nx) cpx.
-template <typename R, bool WAVE>
+// RAG: Nz is not a multiple of KT, so the last tile holds fewer rows; rows
+// k >= Nz are read as zero and not stored (here and in the other x passes).
+template <typename R, bool WAVE, bool RAG = false>
 __global__ void __launch_bounds__(512) k_xfwd(const R* __restrict__ u, cpx<R>* __restrict__ T,
                                                Geo g, const cpx<R>* __restrict__ tw, int j0,
                                                FieldBatch fb = FieldBatch{}) {
@@ -106,13 +108,16 @@ __global__ void __launch_bounds__(512) k_xfwd(const R* __restrict__ u, cpx<R>* _
     const int m = m0 + grp;
     const R* ra = plane + (long long)(k0 + 2 * m) * g.nx;
     const R* rb = ra + g.nx;
+    const bool oka = !RAG || k0 + 2 * m < g.nz, okb = !RAG || k0 + 2 * m + 1 < g.nz;
     cpx<R> v[8];
 #pragma unroll
-    for (int r = 0; r < 8; ++r) v[r] = {ld_in(ra + gt + G * r), ld_in(rb + gt + G * r)};
+    for (int r = 0; r < 8; ++r)
+      v[r] = {oka ? ld_in(ra + gt + G * r) : (R)0, okb ? ld_in(rb + gt + G * r) : (R)0};
     fft_regs<R, -1, WAVE>(v, fbuf, g.nx, g.lognx, twl, gt);
     __syncthreads();  // every group's spectra are read below
     for (int idx = threadIdx.x; idx < g.nq * nr; idx += blockDim.x) {
       const int q = idx / nr, kk = idx - q * nr;
+      if (RAG && k0 + 2 * m0 + kk >= g.nz) continue;
       cpx<R> A, B;
       split_pair(smem + (kk >> 1) * L, g.nx, q, inv, A, B);
       st_out(out + (long long)q * g.nz + 2 * m0 + kk, (kk & 1) ? B : A);
@@ -125,7 +130,7 @@ __global__ void __launch_bounds__(512) k_xfwd(const R* __restrict__ u, cpx<R>* _
 // FFTW's c2r reads only the real part of the kx = 0 input.  Rows 2m and 2m+1
 // are synthesised together: Z = A + iB with A, B extended Hermitian.  With dkx
 // the x-spectra are multiplied by i kx first (the x derivative).
-template <typename R, bool WAVE, int LOGN = 0>
+template <typename R, bool WAVE, int LOGN = 0, bool RAG = false>
 __global__ void __launch_bounds__(512) k_xinv(const cpx<R>* __restrict__ T, R* __restrict__ u,
                                                Geo g, const cpx<R>* __restrict__ tw, int j0,
                                                const double* __restrict__ dkx = nullptr,
@@ -158,7 +163,11 @@ __global__ void __launch_bounds__(512) k_xinv(const cpx<R>* __restrict__ T, R* _
       // streaming loads pay off for FP32 spectra (3.46 -> 2.90 ms per step at
       // 512x257x512) but not for FP64 ones (4.89 -> 5.40 ms)
       cpx<R> A, B;
-      if (sizeof(R) == 4) {
+      if (RAG) {
+        const int ka = k0 + 2 * (m0 + mm);
+        A = ka < g.nz ? src[0] : cpx<R>{(R)0, (R)0};
+        B = ka + 1 < g.nz ? src[1] : cpx<R>{(R)0, (R)0};
+      } else if (sizeof(R) == 4) {
         A = ld_in(src);
         B = ld_in(src + 1);
       } else {
@@ -185,10 +194,11 @@ __global__ void __launch_bounds__(512) k_xinv(const cpx<R>* __restrict__ T, R* _
     fft_any<R, 1, WAVE, LOGN>(v, fbuf, g.nx, g.lognx, twl, gt);
     R* ra = plane + (long long)(k0 + 2 * (m0 + grp)) * g.nx;
     R* rb = ra + g.nx;
+    const bool oka = !RAG || k0 + 2 * (m0 + grp) < g.nz, okb = !RAG || k0 + 2 * (m0 + grp) + 1 < g.nz;
     for (int i = gt; i < g.nx; i += G) {
       const cpx<R> z = fbuf[pidx(i)];
-      st_out(ra + i, z.x);
-      st_out(rb + i, z.y);
+      if (oka) st_out(ra + i, z.x);
+      if (okb) st_out(rb + i, z.y);
     }
   }
 }
@@ -343,10 +353,11 @@ __device__ __forceinline__ void pair_ranges(const ProdArgs& a, const Geo& g, int
 template <typename R>
 __device__ __forceinline__ void emit_pair(cpx<R>* smem, int L, const Geo& g, int lg_ng, int pair,
                                           R inv, const double* kx, cpx<R>* out0, cpx<R>* out1,
-                                          int m0) {
+                                          int m0, int nrow = 1 << 30) {
   const int ngroups = 1 << lg_ng;
   for (int idx = threadIdx.x; idx < g.nq * ngroups; idx += blockDim.x) {
     const int q = idx >> lg_ng, mm = idx & (ngroups - 1);
+    if (mm >= nrow) continue;  // rows past Nz of a ragged tile
     cpx<R> A, B;
     split_pair(smem + mm * L, g.nx, q, inv, A, B);
     const cpx<R> ikA = ik((R)kx[q], A);
@@ -363,7 +374,7 @@ __device__ __forceinline__ void emit_pair(cpx<R>* smem, int L, const Geo& g, int
 }
 
 // LOGNX > 0: Nx = 2^LOGNX at compile time (fft_regs_c); 0: Nx from g.
-template <typename R, bool WAVE, int LOGNX = 0>
+template <typename R, bool WAVE, int LOGNX = 0, bool RAG = false>
 __global__ void __launch_bounds__(512) k_xprod_all(ProdAllArgs a, Geo g,
                                                     const cpx<R>* __restrict__ tw, int j0) {
   constexpr int KT = XTile<R>::KT;
@@ -411,7 +422,10 @@ __global__ void __launch_bounds__(512) k_xprod_all(ProdAllArgs a, Geo g,
     pair_ranges(b, g, 3, j, r2, x);
   }
   for (int m0 = 0; m0 < KT; m0 += ngroups) {
-    const long long rowc = (long long)j * pp + (long long)(k0 + m0 + grp) * g.nx + gt;
+    // a row past Nz (ragged tile) reads row 0 and is not stored
+    const bool rowok = !RAG || k0 + m0 + grp < g.nz;
+    const int nrow = RAG ? g.nz - k0 - m0 : 1 << 30;
+    const long long rowc = (long long)j * pp + (long long)(rowok ? k0 + m0 + grp : 0) * g.nx + gt;
     R u1m[8], u1c[8], u1p[8], u2m[8], u2c[8], u2p[8];
     R va[8], vb[8];
 #pragma unroll
@@ -432,7 +446,7 @@ __global__ void __launch_bounds__(512) k_xprod_all(ProdAllArgs a, Geo g,
         vb[r] = (u1p[r] * u2p[r] + u1c[r] * u2p[r] - u1c[r] * u2c[r] - u1m[r] * u2c[r]) * c2dyf;
       }
       fft_pair(va, vb);
-      emit_pair(smem, L, g, lg_ng, 0, inv, b.kx, outp(0), (cpx<R>*)nullptr, m0);
+      emit_pair(smem, L, g, lg_ng, 0, inv, b.kx, outp(0), (cpx<R>*)nullptr, m0, nrow);
       __syncthreads();
     }
     // pair 3: (W1, V2) -> slot 4 = ikx F(W1) + F(V2)                   (:356-365, 440-449)
@@ -444,7 +458,7 @@ __global__ void __launch_bounds__(512) k_xprod_all(ProdAllArgs a, Geo g,
         vb[r] = (quarter * (sp * sp) - quarter * (sm * sm)) * cdy;
       }
       fft_pair(va, vb);
-      emit_pair(smem, L, g, lg_ng, 3, inv, b.kx, outp(4), (cpx<R>*)nullptr, m0);
+      emit_pair(smem, L, g, lg_ng, 3, inv, b.kx, outp(4), (cpx<R>*)nullptr, m0, nrow);
       __syncthreads();
     }
     // u3 at j-1, j, j+1 replaces u1 at j-1, j+1 and u2 at j-1
@@ -466,7 +480,7 @@ __global__ void __launch_bounds__(512) k_xprod_all(ProdAllArgs a, Geo g,
         vb[r] = (u3p[r] * u2p[r] + u3c[r] * u2p[r] - u3c[r] * u2c[r] - u3m[r] * u2c[r]) * c2dyf;
       }
       fft_pair(va, vb);
-      emit_pair(smem, L, g, lg_ng, 1, inv, b.kx, outp(1), outp(2), m0);
+      emit_pair(smem, L, g, lg_ng, 1, inv, b.kx, outp(1), outp(2), m0, nrow);
       __syncthreads();
     }
     // pair 2: (u3u3, W3) -> slot 3 = F(u3u3), slot 5 = F(W3)            (:337-344, 377-386)
@@ -477,7 +491,7 @@ __global__ void __launch_bounds__(512) k_xprod_all(ProdAllArgs a, Geo g,
         vb[r] = r2 ? ((dyfj * u3c[r] + dyfm * u3m[r]) * c2dy) * u2c[r] : zero;
       }
       fft_pair(va, vb);
-      emit_pair(smem, L, g, lg_ng, 2, inv, b.kx, outp(3), outp(5), m0);
+      emit_pair(smem, L, g, lg_ng, 2, inv, b.kx, outp(3), outp(5), m0, nrow);
       __syncthreads();
     }
     // scalars: (th u1, Vt) -> slot 6+2n, (th u3, 0) -> slot 7+2n       (:557-600)
@@ -498,7 +512,7 @@ __global__ void __launch_bounds__(512) k_xprod_all(ProdAllArgs a, Geo g,
         vb[r] = (thp[r] * u2p[r] + thc[r] * u2p[r] - thc[r] * u2c[r] - thm[r] * u2c[r]) * c2dyf;
       }
       fft_pair(va, vb);
-      emit_pair(smem, L, g, lg_ng, 4, inv, b.kx, outp(6 + 2 * n), (cpx<R>*)nullptr, m0);
+      emit_pair(smem, L, g, lg_ng, 4, inv, b.kx, outp(6 + 2 * n), (cpx<R>*)nullptr, m0, nrow);
       __syncthreads();
 #pragma unroll
       for (int r = 0; r < 8; ++r) {
@@ -506,7 +520,7 @@ __global__ void __launch_bounds__(512) k_xprod_all(ProdAllArgs a, Geo g,
         vb[r] = zero;
       }
       fft_pair(va, vb);
-      emit_pair(smem, L, g, lg_ng, 5, inv, b.kx, outp(7 + 2 * n), (cpx<R>*)nullptr, m0);
+      emit_pair(smem, L, g, lg_ng, 5, inv, b.kx, outp(7 + 2 * n), (cpx<R>*)nullptr, m0, nrow);
       __syncthreads();
     }
   }
@@ -575,7 +589,11 @@ struct RhsArgs {
   int tw_lds;  // 1: the twiddle table is copied to LDS (0: read from HBM)
 };
 
-template <typename R, bool WAVE, int LOGN = 0>
+// PRE: the product slots X[s] already hold their z spectra (FP64, packed,
+// scaled by 1/Nz: k_zgen_fwd for z lengths that are not powers of two); the
+// pass then only combines them, and cr goes to crs (the caller transforms
+// it further where needed).
+template <typename R, bool WAVE, int LOGN = 0, bool PRE = false>
 __global__ void __launch_bounds__(512) k_rhs(RhsArgs a, Geo g, const cpx<R>* __restrict__ tw,
                                               int j0) {
   typedef cpx<double> C;
@@ -612,11 +630,11 @@ __global__ void __launch_bounds__(512) k_rhs(RhsArgs a, Geo g, const cpx<R>* __r
   };
   const long long to = (long long)j * g.tp + (long long)q * g.nz;
   const int rounds = (nslot + ngroups - 1) / ngroups;
-  const CR* twl = a.tw_lds ? lds_twiddles(smem + (long long)rounds * ngroups * L, tw, g.nz) : tw;
+  const CR* twl = (!PRE && a.tw_lds) ? lds_twiddles(smem + (long long)rounds * ngroups * L, tw, g.nz) : tw;
   // --- z-transform the product spectra
   // The LDS holds one buffer per (round, group), so groups without a slot in
   // a round transform an unused buffer and every thread meets every barrier.
-  for (int s0 = 0; s0 < nslot; s0 += ngroups) {
+  for (int s0 = 0; s0 < (PRE ? 0 : nslot); s0 += ngroups) {
     const int ls = s0 + grp;
     const bool act = ls < nslot && slot_used(gslot(ls));
     CR* buf = smem + (long long)ls * L;
@@ -647,8 +665,12 @@ __global__ void __launch_bounds__(512) k_rhs(RhsArgs a, Geo g, const cpx<R>* __r
     // is written (cr slot 3+n < 6+2n, and slots 0..3 belong to u1/u3)
     const int pn = pidx(nat);
     auto Z = [&](int s) -> C {
-      const CR z = smem[(long long)lslot(s) * L + pn];
-      return invnz * C{(double)z.x, (double)z.y};
+      if constexpr (PRE) {
+        return reinterpret_cast<const C*>(a.X[s])[o];
+      } else {
+        const CR z = smem[(long long)lslot(s) * L + pn];
+        return invnz * C{(double)z.x, (double)z.y};
+      }
     };
     auto put = [&](int c, C v) {
       if (spec && a.crs32)
@@ -759,7 +781,7 @@ __global__ void __launch_bounds__(512) k_rhs(RhsArgs a, Geo g, const cpx<R>* __r
       put(3 + n, rt);
     }
   }
-  if (spec) return;  // cr stays spectral
+  if (PRE || spec) return;  // cr stays spectral
   // zero the dealiased gap of the cr slots (fft.f90:336-340)
   for (int lc = 0; lc < ncr; ++lc)
     for (int k = g.nkz + 1 + threadIdx.x; k < g.nz - g.nkz; k += blockDim.x)

@@ -81,7 +81,8 @@ extern "C" {
  * TOML V/W swap at parameters.f90:373-382: TOML "V_*" feeds w_bc_*).
  */
 typedef struct dgpu_params {
-  int32_t nx, ny, nz, n_th;   /* Nx, Ny (wall-normal points), Nz: Nx, Nz powers of two */
+  int32_t nx, ny, nz, n_th;   /* Nx, Ny (wall-normal points), Nz: Nx a power of two in
+                                 [16, 4096], Nz any length in [16, 2048] */
   double lx, lz;              /* periodic box lengths */
   double re;                  /* nu = 1/re (parameters.f90:101) */
   double nu_v_scale;          /* vertical/horizontal viscosity ratio */

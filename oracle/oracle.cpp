@@ -19,8 +19,10 @@
 //   for/flow.f90      RK3 coefficients (78-86)
 //   for/les.f90       les_chan / les_chan_th (in oracle_les.cpp)
 // FFTW (not vendored in the reference) is replaced by a textbook radix-2
-// Cooley-Tukey FFT with FFTW's conventions: unnormalised, forward sign -1,
-// r2c keeps k=0..N/2, c2r ignores Im of the k=0 and k=N/2 inputs.
+// Cooley-Tukey FFT (a recursive mixed-radix Cooley-Tukey for lengths that
+// are not powers of two, each prime factor by its plain DFT) with FFTW's
+// conventions: unnormalised, forward sign -1, r2c keeps k=0..N/2, c2r
+// ignores Im of the k=0 and k=N/2 inputs.
 //
 // Parity status per function (what pins it; see tests/test_oracle_*.py):
 //   fft_xz_to_fourier/physical  numpy.fft closed forms + single-mode KATs
@@ -46,6 +48,15 @@ void CFFT::init(int n_) {
   n = n_;
   logn = 0;
   while ((1 << logn) < n) ++logn;
+  pow2 = (1 << logn) == n;
+  if (!pow2) {
+    w.resize(n);
+    for (int k = 0; k < n; ++k) {
+      double a = -2.0 * M_PI * (double)k / (double)n;
+      w[k] = cplx(std::cos(a), std::sin(a));
+    }
+    return;
+  }
   w.resize(n / 2 > 0 ? n / 2 : 1);
   for (int k = 0; k < n / 2; ++k) {
     double a = -2.0 * M_PI * (double)k / (double)n;
@@ -60,7 +71,43 @@ void CFFT::init(int n_) {
   }
 }
 
+// Decimation in time over the smallest prime factor p of m (m = n / stride):
+// X[k + s m/p] = sum_r w_m^{r (k + s m/p)} DFT_{m/p}(x[r::p])[k].
+static void dft_rec(const cplx* x, int stride, cplx* out, int m, int n, const std::vector<cplx>& w,
+                    bool inverse) {
+  if (m == 1) {
+    out[0] = x[0];
+    return;
+  }
+  int p = 2;
+  while (m % p) ++p;
+  const int q = m / p;
+  std::vector<cplx> sub((size_t)m);
+  for (int r = 0; r < p; ++r) dft_rec(x + (size_t)r * stride, stride * p, sub.data() + (size_t)r * q, q, n, w, inverse);
+  const int unit = n / m;  // w_m^e = w[e * unit mod n]
+  for (int k = 0; k < q; ++k)
+    for (int s = 0; s < p; ++s) {
+      const int kk = k + s * q;
+      cplx acc(0.0, 0.0);
+      for (int r = 0; r < p; ++r) {
+        cplx t = w[(int)(((long long)r * kk * unit) % n)];
+        if (inverse) t = std::conj(t);
+        acc += sub[(size_t)r * q + k] * t;
+      }
+      out[kk] = acc;
+    }
+}
+
+void CFFT::exec_general(cplx* a, bool inverse) const {
+  std::vector<cplx> x(a, a + n);
+  dft_rec(x.data(), 1, a, n, n, w, inverse);
+}
+
 void CFFT::exec(cplx* a, bool inverse) const {
+  if (!pow2) {
+    exec_general(a, inverse);
+    return;
+  }
   for (int i = 0; i < n; ++i)
     if (i < rev[i]) std::swap(a[i], a[rev[i]]);
   for (int len = 2; len <= n; len <<= 1) {
@@ -204,11 +251,15 @@ void Solver::fft_xz_to_fourier(double* v) {
 #pragma omp parallel
   {
     std::vector<cplx> work(Nx), A(Nx / 2 + 1), B(Nx / 2 + 1);
+    // rows in pairs; with odd Nzp the last row goes alone
+    const int npair = (Nzp + 1) / 2;
 #pragma omp for schedule(static)
-    for (int jk = 0; jk < NY2 * (Nzp / 2); ++jk) {
-      int j = jk / (Nzp / 2), k = 2 * (jk % (Nzp / 2));
-      real_pair_forward(fx, &R(v, 0, k, j), &R(v, 0, k + 1, j), A.data(), B.data(), work.data());
-      for (int kk = 0; kk < 2; ++kk) {
+    for (int jk = 0; jk < NY2 * npair; ++jk) {
+      int j = jk / npair, k = 2 * (jk % npair);
+      const bool two = k + 1 < Nzp;
+      real_pair_forward(fx, &R(v, 0, k, j), two ? &R(v, 0, k + 1, j) : nullptr, A.data(), two ? B.data() : nullptr,
+                        work.data());
+      for (int kk = 0; kk < (two ? 2 : 1); ++kk) {
         const cplx* src = kk ? B.data() : A.data();
         for (int i = 0; i <= Nkx; ++i) TF(i, k + kk, j) = src[i] / (double)Nx;
         for (int i = Nkx + 1; i <= Nx / 2; ++i) TF(i, k + kk, j) = cplx(0.0, 0.0);
@@ -260,13 +311,15 @@ void Solver::fft_xz_to_physical(double* v) {
 #pragma omp parallel
   {
     std::vector<cplx> work(Nx);
+    const int npair = (Nzp + 1) / 2;
 #pragma omp for schedule(static)
-    for (int jk = 0; jk < NY2 * (Nzp / 2); ++jk) {
-      int j = jk / (Nzp / 2), k = 2 * (jk % (Nzp / 2));
-      for (int kk = 0; kk < 2; ++kk)
+    for (int jk = 0; jk < NY2 * npair; ++jk) {
+      int j = jk / npair, k = 2 * (jk % npair);
+      const bool two = k + 1 < Nzp;
+      for (int kk = 0; kk < (two ? 2 : 1); ++kk)
         for (int i = Nkx + 1; i <= Nx / 2; ++i) TF(i, k + kk, j) = cplx(0.0, 0.0);
-      real_pair_inverse(fx, &TF(0, k, j), &TF(0, k + 1, j), &R(v, 0, k, j), &R(v, 0, k + 1, j),
-                        work.data());
+      real_pair_inverse(fx, &TF(0, k, j), two ? &TF(0, k + 1, j) : nullptr, &R(v, 0, k, j),
+                        two ? &R(v, 0, k + 1, j) : nullptr, work.data());
     }
   }
 }

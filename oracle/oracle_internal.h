@@ -12,10 +12,12 @@ typedef std::complex<double> cplx;
 
 struct CFFT {
   int n = 0, logn = 0;
-  std::vector<cplx> w;
+  bool pow2 = true;       // radix-2 path; otherwise mixed-radix recursion
+  std::vector<cplx> w;    // exp(-2 pi i k / n): k < n/2 (radix 2) or k < n
   std::vector<int> rev;
   void init(int n_);
   void exec(cplx* a, bool inverse) const;
+  void exec_general(cplx* a, bool inverse) const;
 };
 
 void real_pair_forward(const CFFT& f, const double* a, const double* b, cplx* A, cplx* B,

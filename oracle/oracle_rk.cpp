@@ -425,7 +425,8 @@ static int fail(OrcHandle* h, int code, const std::string& m) {
 
 static int check_params(const dgpu_params* p, std::string& why) {
   auto pow2 = [](int n) { return n >= 8 && (n & (n - 1)) == 0; };
-  if (!pow2(p->nx) || !pow2(p->nz)) { why = "nx and nz must be powers of two >= 8"; return -1; }
+  if (!pow2(p->nx)) { why = "nx must be a power of two >= 8"; return -1; }
+  if (p->nz < 8) { why = "nz must be at least 8"; return -1; }
   if (p->ny < 5) { why = "ny must be >= 5"; return -1; }
   if (p->n_th < 0 || p->n_th > DGPU_MAX_TH) { why = "n_th out of range"; return -1; }
   if (p->use_les && p->les_model_type != 1 && p->les_model_type != 4 && p->les_model_type != 5) {

@@ -50,11 +50,12 @@ def grid_xz(p):
 
 
 # ------------------------------------------------------------------ FFT ----
+@pytest.mark.parametrize("nz", [16, 65])
 @pytest.mark.parametrize("m,n", [(0, 0), (1, 0), (0, 2), (3, 1), (2, 5)])
-def test_fft_single_mode_closed_form(m, n):
+def test_fft_single_mode_closed_form(m, n, nz):
     """cos(2 pi m x/Lx) cos(2 pi n z/Lz): packed coefficients 1/4 (1/2, 1) at
     (m, +-n), from the 1/(Nx Nz) scaling of fft.f90:262, 304, 310."""
-    p = small_params()
+    p = small_params(nz=nz)
     s = oracle_solver(p, workloads.channel_grid(p["ny"]))
     X, Z = grid_xz(p)
     f = np.cos(2 * np.pi * m * X / p["lx"]) * np.cos(2 * np.pi * n * Z / p["lz"])
@@ -97,10 +98,13 @@ def test_fft_dealias_drops_high_modes():
     assert np.abs(active(c, p)).max() < 1e-14
 
 
-@pytest.mark.parametrize("nx,nz", [(16, 16), (32, 16), (16, 64), (64, 32)])
+@pytest.mark.parametrize("nx,nz", [(16, 16), (32, 16), (16, 64), (64, 32), (16, 65), (32, 48), (16, 30),
+                                   (16, 13), (16, 169)])
 def test_fft_matches_numpy(nx, nz):
     """Random field: oracle == numpy (pocketfft) rfft/fft with the reference's
-    scaling, dealias and kz pack; inverse == numpy irfft/ifft."""
+    scaling, dealias and kz pack; inverse == numpy irfft/ifft.  The z lengths
+    that are not powers of two (65 = 5 x 13 as in the reference's front
+    setups, 48, 30, the prime 13, 13 x 13) take the mixed-radix path."""
     p = small_params(nx=nx, nz=nz)
     s = oracle_solver(p, workloads.channel_grid(p["ny"]))
     rng = np.random.default_rng(0)

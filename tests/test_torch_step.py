@@ -66,3 +66,25 @@ def test_torch_three_steps_32():
     report("torch 32x33x32 3 steps", errs)
     for k, e in errs.items():
         assert e < (1e-10 if k == "cp" else 1e-12), (k, e)
+
+
+@pytest.mark.parametrize("shape", [(16, 17, 21), (16, 16, 30)])
+@pytest.mark.parametrize("rk", [1, 3])
+def test_torch_substep_matches_oracle_general_nz(shape, rk):
+    """z lengths that are not powers of two (21 = 3 x 7, 30 = 2 x 3 x 5; an
+    even Ny): torch.fft against the oracle's mixed-radix recursion, through
+    a whole substep (packing of odd Nz, the row pairs of odd Nz in x)."""
+    nx, ny, nz = shape
+    p = workloads.channel_params(nx, ny, nz, 200.0, ri=0.1, dt=2e-3)
+    gy = workloads.channel_grid(ny)
+    s = seeded(p, gy)
+    o = oracle_solver(p, gy)
+    o.set_state(s)
+    o.rk_chan_1(rk, 0.3, 5)
+    want = o.get_state(ALL)
+    tc = TorchChannel(p, gy)
+    got = to_ref(tc.substep(from_ref(s, p), rk, 0.3), p)
+    errs = field_errors(got, want, p, names=ALL)
+    report(f"torch {nx}x{ny}x{nz} rk{rk}", errs)
+    for k, e in errs.items():
+        assert e < (1e-10 if k == "cp" else 1e-12), (k, e)
