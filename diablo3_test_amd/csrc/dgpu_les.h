@@ -521,8 +521,7 @@ __global__ void k_les_bc(LesBcArgs a, Geo g) {
 // The real CN solve when the rows depend on the eddy viscosity/diffusivity
 // (channel.f90:676-689, 736-749, 792-805, 848-861, with the explicit LES terms
 // of :519-547, 634-646): one (x,z) column per thread, the Thomas sweeps of
-// :1335-1496 run sequentially; b' goes to a scratch array and g' to the
-// solution's own storage.
+// :1335-1496 run sequentially, through two scratch arrays (sb, sg).
 struct TriRow {
   double a, b, c;   // lane-independent part of the row (all of it for fixed rows)
   double vb;        // fixed-row right-hand side (scaled by kscale when kind == 2)
@@ -533,8 +532,8 @@ struct TriRow {
   int vtype;        // 1 for u2 (GY rows: nu_t averaged to GYF)
 };
 
-// T is the storage type of r and u; the sweeps run in FP64 with b' and g' in
-// FP64 scratch.  incr has the meaning of k_thomas_seg's flag: the solve is for
+// T is the storage type of r and u; the sweeps run in FP64 with their
+// scratch in FP64.  incr has the meaning of k_thomas_seg's flag: the solve is for
 // d = x - u, with the right-hand side vec - A u formed in the forward sweep
 // from the u(j-1), u(j), u(j+1) it holds anyway (reading u back in the
 // backward sweep would put a load behind every store of the serial sweep).
@@ -542,7 +541,7 @@ struct TriRow {
 // so a wavefront's row accesses are 64 C values long.
 //
 // LesCnFields batches the solves of several fields into one launch
-// (blockIdx.y = field, each with its own b', g' scratch of n pp values; nf = 0:
+// (blockIdx.y = field, each with its own two scratch arrays of n pp values; nf = 0:
 // the single field of the plain arguments).  A solve is one sequential sweep
 // per lane, so one field fills only a few waves per CU; the batch gives the
 // CUs the other fields' sweeps to hide their latency behind.
@@ -607,36 +606,35 @@ __global__ void __launch_bounds__(256) k_thomas_les(const T* __restrict__ r, T* 
     }
     if (incr) vec -= a * um + b * uc + c * up;
   };
+  // The forward sweep stores e(j) = g'(j)/b'(j) and f(j) = c(j)/b'(j) (c with
+  // the eddy viscosity of the row's upper face), so that the backward sweep
+  // x(j) = e(j) - f(j) x(j+1) reads two scratch values a row and neither the
+  // eddy viscosity again nor divides: 64 instead of 72 bytes a point in FP64.
   VD um = zero, uc = U(0), up = (n > 1) ? U(1) : zero;
   VD a, b, c, vec;
   row(0, a, b, c, vec, um, uc, up);
-  VD bp = b, gp = vec, cprev = c;
-  *reinterpret_cast<VD*>(sb + at(0)) = bp;
-  *reinterpret_cast<VD*>(sg + at(0)) = gp;
+  VD inv = 1.0 / b, gp = vec;
+  *reinterpret_cast<VD*>(sb + at(0)) = c * inv;
+  *reinterpret_cast<VD*>(sg + at(0)) = gp * inv;
   for (int j = 1; j < n; ++j) {
     um = uc;
     uc = up;
     up = (j + 1 < n) ? U(j + 1) : zero;
+    const VD cprev = c;
     row(j, a, b, c, vec, um, uc, up);
-    const VD mlt = -a / bp;
-    bp = b + mlt * cprev;
+    const VD mlt = -a * inv;  // -a(j)/b'(j-1)
+    const VD bp = b + mlt * cprev;
     gp = vec + mlt * gp;
-    cprev = c;
-    *reinterpret_cast<VD*>(sb + at(j)) = bp;
-    *reinterpret_cast<VD*>(sg + at(j)) = gp;
+    inv = 1.0 / bp;
+    *reinterpret_cast<VD*>(sb + at(j)) = c * inv;
+    *reinterpret_cast<VD*>(sg + at(j)) = gp * inv;
   }
-  // backward: x(n-1) = g'/b', x(j) = (g'(j) - c(j) x(j+1)) / b'(j)
-  VD x = gp / bp;
+  // backward: x(n-1) = g'/b' = e(n-1), x(j) = e(j) - f(j) x(j+1)
+  VD x = gp * inv;
   auto put = [&](int j, VD v) { *reinterpret_cast<VT*>(u + at(j)) = __builtin_convertvector(v, VT); };
   if (n - 1 >= ca && n - 1 <= cb) put(n - 1, x);
   for (int j = n - 2; j >= ca; --j) {
-    const TriRow& rw = rows[j];
-    VD cc = (VD)rw.c;
-    if (rw.kind == 1) {
-      const VD nu = rw.vtype ? 0.5 * (N(j) + N(j + 1)) : N(j + 1);
-      cc -= rw.lp * nu;
-    }
-    x = (*reinterpret_cast<const VD*>(sg + at(j)) - cc * x) / *reinterpret_cast<const VD*>(sb + at(j));
+    x = *reinterpret_cast<const VD*>(sg + at(j)) - *reinterpret_cast<const VD*>(sb + at(j)) * x;
     if (j <= cb) put(j, x);
   }
 }
