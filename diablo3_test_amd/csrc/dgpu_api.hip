@@ -800,9 +800,9 @@ void launch_thomas_les(dgpu_handle* h, int field, int rs, const R* r, R* u, cons
   const TriRow* rows = h->trirows.p + ((size_t)field * 3 + rs) * n;
   const LesCnFields<R> mf = batch ? *batch : LesCnFields<R>{};
   const int nf = batch ? batch->nf : 1;
-  // reads r, u, nu_t; writes b' and g' and reads them back; writes u.  Two
-  // columns per lane (one 2-wide access per row and array) when the plane
-  // allows it.
+  // reads r, u, nu_t; writes the e, f scratch and reads it back; writes u.
+  // Two columns per lane (one 2-wide access per row and array) when the
+  // plane allows it.
   Timed tm(h, DGPU_K_THOMAS, (3.0 * sizeof(R) + 8.0 + 32.0) * g.pp * n * nf);
   if (h->kn.les_cols == 2 && g.pp % 2 == 0) {
     dim3 grid((unsigned)((g.pp / 2 + 255) / 256), (unsigned)nf), block(256);

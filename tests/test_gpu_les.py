@@ -113,3 +113,25 @@ def test_les_cfg3_shape_one_substep():
     errs = field_errors(g.get_state(SPECTRAL), o.get_state(SPECTRAL), p, names=SPECTRAL)
     report("les5 128x257x128", errs)
     assert max(errs.values()) < 1e-10
+
+
+@pytest.mark.parametrize("ny", [289, 385])
+def test_les_tall_channel(ny):
+    """The LES CN solve (and the whole LES substep) on columns of 291 and 387
+    rows against the oracle, FP64 and the mixed form."""
+    p, gy, st = les_case(5, nx=16, ny=ny, nz=16)
+    print(f"LES type 5 16x{ny}x16, 2 steps", flush=True)
+    o = prepared(oracle_solver(p, gy), st)
+    o.step(2)
+    want = o.get_state(SPECTRAL)
+    for prec in ("fp64", "mixed"):
+        g = prepared(gpu_solver(p, gy, precision=prec), st)
+        g.step(2)
+        errs = field_errors(g.get_state(SPECTRAL), want, p, names=SPECTRAL)
+        g.close()
+        report(f"les {prec} Ny={ny}", errs)
+        state = 1e-12 if prec == "fp64" else 1e-6
+        for k, e in errs.items():
+            bar = (1e-9 if prec == "fp64" else 1e-4) if k == "cp" else \
+                  (1e-10 if prec == "fp64" else 1e-4) if k.startswith("cf") else state
+            assert e < bar, (prec, k, e)
