@@ -3,6 +3,10 @@ with the eddy viscosity, the wall values of the LES stage) against central
 finite differences of the GPU forward substep, and the checkpointed
 wall-shear gradient through LES steps against finite differences.
 
+It also checks the adjoint against torch autograd through the FP64 torch
+restatement of the LES substep (tests/torch_les.py, pinned to the oracle in
+tests/test_torch_step.py).
+
 The directions are physical: random real fields transformed to spectral
 space (so the kx = 0 modes are Hermitian).  The bar is 1e-6 relative, the
 bar for the adjoint gradient.  The AMD clip max(0, .) makes models 4 and 5
@@ -15,10 +19,12 @@ carry a mean (kx = kz = 0) part, so that the gradient along them is well
 above the differences' roundoff."""
 import numpy as np
 import pytest
+import torch
 
 from diablo3_test_amd import workloads
 from diablo3_test_amd.adjoint import STATE, WallShearRun
-from helpers import active, gpu_solver, oracle_solver, prepared
+from helpers import active, gpu_solver, oracle_solver, prepared, rel_l2, report
+from torch_step import TorchChannel, from_ref, to_ref
 
 pytestmark = pytest.mark.gpu
 
@@ -131,3 +137,49 @@ def test_les_wall_shear_gradient_finite_differences():
         worst = max(worst, rel)
         print(f"  direction {i}: adjoint {ad:.12e} fd {fd:.12e} rel {rel:.2e}", flush=True)
     assert worst < 1e-6
+
+
+TORCH_VARIANTS = {
+    "dirichlet": {},
+    "neumann": dict(u_bc_ymin=1, w_bc_ymax=1, u_bc_ymin_c1=0.1, w_bc_ymax_c1=0.02, th_bc_ymax=[1],
+                    th_bc_ymax_c1=[0.2], v_bc_ymax=1),
+}
+
+
+@pytest.mark.parametrize("model", [1, 4, 5])
+@pytest.mark.parametrize("variant", sorted(TORCH_VARIANTS))
+@pytest.mark.parametrize("rk", [1, 2, 3])
+def test_les_adjoint_substep_matches_torch(model, variant, rk):
+    """One LES substep's vector-Jacobian product, 16x17x16, gate open: the HIP
+    adjoint against torch autograd of the torch restatement, every state
+    field within 1e-10 relative L2 (cp: 1e-8, the pressure's round-off
+    floor, as in the DNS check)."""
+    p, gy, st = les_case(model, 16, 17, 16, **TORCH_VARIANTS[variant])
+    o = prepared(oracle_solver(p, gy), st)
+    o.step(2)  # a live RK carry, pressure and eddy viscosity
+    s0 = o.get_state(STATE)
+    lam = physical_direction(p, gy, 4000 + rk)
+    rng = np.random.default_rng(50 + rk)
+    for k in STATE:  # plus a random non-smooth part on every active mode
+        a = active(lam[k], p)
+        a += 0.1 * (rng.standard_normal(a.shape) + 1j * rng.standard_normal(a.shape))
+    print(f"LES adjoint vs torch: model {model} {variant} rk={rk}", flush=True)
+    g = gpu_solver(p, gy)
+    g.set_state(s0)
+    g.adj_set_state(lam)
+    g.adj_rk_chan_1(rk, 0.3, 7)
+    got = g.adj_get_state(STATE)
+    g.close()
+    tc = TorchChannel(p, gy)
+    t = from_ref(s0, p)
+    for v in t.values():
+        v.requires_grad_(True)
+    out = tc.substep(t, rk, 0.3, 7)
+    lt = from_ref(lam, p)
+    L = sum((torch.conj(lt[k]) * out[k]).real.sum() for k in STATE)
+    L.backward()
+    want = to_ref({k: t[k].grad for k in STATE}, p)
+    errs = {k: rel_l2(active(got[k], p), active(want[k], p)) for k in STATE}
+    report(f"LES adj vs torch m{model} {variant} rk{rk}", errs)
+    for k, e in errs.items():
+        assert np.isfinite(e) and e < (1e-8 if k == "cp" else 1e-10), (k, e)

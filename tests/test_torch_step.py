@@ -88,3 +88,53 @@ def test_torch_substep_matches_oracle_general_nz(shape, rk):
     report(f"torch {nx}x{ny}x{nz} rk{rk}", errs)
     for k, e in errs.items():
         assert e < (1e-10 if k == "cp" else 1e-12), (k, e)
+
+
+LES_VARIANTS = {
+    "dirichlet": {},
+    "neumann": dict(u_bc_ymin=1, w_bc_ymax=1, u_bc_ymin_c1=0.1, w_bc_ymax_c1=0.02, th_bc_ymax=[1],
+                    th_bc_ymax_c1=[0.2], v_bc_ymax=1),
+    "two_scalars": dict(n_th=2, th_bc_ymin=[0, 1], th_bc_ymax=[0, 0], th_bc_ymin_c1=[0.0, 0.1],
+                        th_bc_ymax_c1=[1.0, 0.0], pr=[1.0, 0.7], dthdx=[0.0, 0.1], dthdz=[0.0, 0.0]),
+}
+
+
+def les_seeded(p, gy):
+    """A broadband state two steps into an LES run (live RK carry, pressure,
+    eddy viscosity)."""
+    st = workloads.initial_state(p, gy, "vonkarman", 3, 0.3)
+    o = prepared(oracle_solver(p, gy), st)
+    o.step(2)
+    return o.get_state(ALL)
+
+
+@pytest.mark.parametrize("model", [1, 4, 5])
+@pytest.mark.parametrize("variant", sorted(LES_VARIANTS))
+@pytest.mark.parametrize("rk", [1, 3])
+def test_torch_les_substep_matches_oracle(model, variant, rk):
+    """The LES branch of the torch substep (tests/torch_les.py: post BCs, the
+    three subgrid models, stress divergence, cross terms, scalar closures,
+    CN solves with nu_t) against the oracle, 16x17x16, gate open (restart)."""
+    p = workloads.channel_params(16, 17, 16, 2000.0, ri=0.1, dt=1e-3, les=True, les_model_type=model)
+    v = dict(LES_VARIANTS[variant])
+    if "n_th" in v:
+        p = workloads.channel_params(16, 17, 16, 2000.0, ri=0.1, dt=1e-3, les=True, les_model_type=model,
+                                     n_th=v.pop("n_th"))
+    p.update(v)
+    gy = workloads.channel_grid(p["ny"])
+    s = les_seeded(p, gy)
+    o = oracle_solver(p, gy)
+    o.set_state(s)
+    o.rk_chan_1(rk, 0.3, 7)
+    want = o.get_state(ALL)
+    nu_t = o.get_field("nu_t")
+    tc = TorchChannel(p, gy)
+    out = tc.substep(from_ref(s, p), rk, 0.3, 7)
+    got = to_ref({k: out[k] for k in ALL}, p)
+    errs = field_errors(got, want, p, names=ALL)
+    nt = out["nu_t"].detach().numpy()
+    errs["nu_t"] = rel_l2(nt[:, : p["nz"], : p["nx"]], nu_t[:, : p["nz"], : p["nx"]])
+    report(f"torch LES model {model} {variant} rk{rk}", errs)
+    assert np.abs(nu_t).max() > 0.0
+    for k, e in errs.items():
+        assert e < (1e-10 if k == "cp" else 1e-12), (k, e)

@@ -1,7 +1,7 @@
 """A small FP64 PyTorch reimplementation of the channel substep, for autograd.
 
-It restates rk_chan_1 (for/channel.f90:10-917, DNS branch) and rem_div_chan
-(:946-1023) with torch operations, so torch.autograd gives the exact
+It restates rk_chan_1 (for/channel.f90:10-917; the LES branch through
+tests/torch_les.py) and rem_div_chan (:946-1023) with torch operations, so torch.autograd gives the exact
 vector-Jacobian product of the discrete step: the reference the hand-written
 HIP adjoint is checked against.  It is test infrastructure, sized for small
 grids (the Thomas sweeps loop over y in Python).
@@ -137,8 +137,11 @@ class TorchChannel:
                 out[ny + 1] = (-1.0, 1.0, 0.0, float(dyf[ny]) * p["v_bc_ymax_c1"])
         return out
 
-    def cn_solve(self, kind, u, r, coef, time):
-        """Explicit half + implicit CN solve of one physical field (channel.f90:494-547, 656-877)."""
+    def cn_solve(self, kind, u, r, coef, time, nt=None, temp2=0.0):
+        """Explicit half + implicit CN solve of one physical field (channel.f90:494-547, 656-877).
+        nt: the eddy viscosity (or diffusivity) in physical space when the LES
+        gate is open; its terms enter both halves per column (:539-543,
+        :676-689, :736-749, :792-805, :848-861)."""
         ny, dy, dyf = self.ny, self.dy, self.dyf
         n = ny + 2
         isv = kind == "u2"
@@ -161,10 +164,19 @@ class TorchChannel:
             else:
                 lo = coef / (dy[j] * dyf[j])
                 hi = coef / (dy[j + 1] * dyf[j])
+            lo = lo * torch.ones(shape, dtype=torch.float64)
+            hi = hi * torch.ones(shape, dtype=torch.float64)
+            if nt is not None:
+                if isv:
+                    lo = lo + temp2 * 0.5 * (nt[j] + nt[j - 1]) / (dyf[j - 1] * dy[j])
+                    hi = hi + temp2 * 0.5 * (nt[j] + nt[j + 1]) / (dyf[j] * dy[j])
+                else:
+                    lo = lo + temp2 * nt[j] / (dy[j] * dyf[j])
+                    hi = hi + temp2 * nt[j + 1] / (dy[j + 1] * dyf[j])
             ex = hi * (u[j + 1] - u[j]) - lo * (u[j] - u[j - 1])
-            ml[j] = -lo * torch.ones(shape, dtype=torch.float64)
-            md[j] = (1.0 + lo + hi) * torch.ones(shape, dtype=torch.float64)
-            mu[j] = -hi * torch.ones(shape, dtype=torch.float64)
+            ml[j] = -lo
+            md[j] = 1.0 + lo + hi
+            mu[j] = -hi
             vec[j] = r[j] + ex
         for j, (l, d, uu, v) in self.bc_rows(kind, time).items():
             ml[j] = l * torch.ones(shape, dtype=torch.float64)
@@ -218,8 +230,10 @@ class TorchChannel:
         return cu1, cu2, cu3, phis
 
     # ------------------------------------------------------------ the substep
-    def substep(self, s, rk_step, time=0.0):
-        """rk_chan_1: state dict -> new state dict (the inputs are not modified)."""
+    def substep(self, s, rk_step, time=0.0, time_step=None):
+        """rk_chan_1: state dict -> new state dict (the inputs are not modified).
+        With use_les the LES stage runs when the gate of channel.f90:259 is
+        open: a restart (create_new_flow false) or time_step > 100."""
         p, ny, nth = self.p, self.ny, self.nth
         dy, dyf = self.dy, self.dyf
         rs = rk_step - 1
@@ -272,9 +286,28 @@ class TorchChannel:
                 f3 = f3 - dx * cyf * self.ikz * cu3 + dz * cyf * self.ikx * cu3 - dx * dr * u2a
                 f2 = f2 - dx * cy * self.ikz * cu2 + dz * cy * self.ikx * cu2
                 fts[n] = fts[n] - dx * cyf * self.ikz * cth[n] + dz * cyf * self.ikx * cth[n]
+        # the LES gate (channel.f90:259-289): the LES stage rewrites the wall
+        # planes of the state before its physical form is taken, adds the
+        # subgrid stresses to cf, and gives the CN solves nu_t and kappa_t
+        les = p.get("use_les", 0) and (not p.get("create_new_flow", 0) or
+                                       (time_step is not None and time_step > 100))
+        nts, kts = None, [None] * nth
+        base1, base2, base3, baseth = cu1, cu2, cu3, cth
+        if les:
+            from torch_les import les_stage
+
+            st = les_stage(self, cu1, cu2, cu3, cth, time)
+            u1, u2, u3, ths = st["u1"], st["u2"], st["u3"], st["ths"]
+            base1, base2, base3, baseth = st["cu1"], st["cu2"], st["cu3"], st["cth"]
+            nts, kts = st["nu_t"], st["kappa_t"]
+            f1 = f1 + st["df1"]
+            f2 = f2 + st["df2"]
+            f3 = f3 + st["df3"]
+            fts = [fts[n] + st["dfth"][n] for n in range(nth)]
+        else:
+            u1, u2, u3 = self.to_phys(cu1), self.to_phys(cu2), self.to_phys(cu3)
+            ths = [self.to_phys(cth[n]) for n in range(nth)]
         # nonlinear products (channel.f90:299-459, 557-608)
-        u1, u2, u3 = self.to_phys(cu1), self.to_phys(cu2), self.to_phys(cu3)
-        ths = [self.to_phys(cth[n]) for n in range(nth)]
         z = torch.zeros_like(u1[:1])
         sh = lambda a: (torch.cat([a[1:], z]), torch.cat([z, a[:-1]]))  # a(j+1), a(j-1)
         u1p, u1m = sh(u1)
@@ -316,19 +349,24 @@ class TorchChannel:
         # implicit vertical solves and back to Fourier space (channel.f90:485-896)
         out = {}
         new_th = []
+        temp2 = 0.5 * temp4
         for n in range(nth):
-            x, ca, cb = self.cn_solve(f"th{n}", ths[n], self.to_phys(crth[n]), temp1 * nvs / p["pr"][n], time)
-            new_th.append(self.splice(cth[n], x, ca, cb))
-        x2, ca, cb = self.cn_solve("u2", u2, self.to_phys(cr2), temp1 * nvs, time)
-        n2 = self.splice(cu2, x2, ca, cb)
-        x1, ca, cb = self.cn_solve("u1", u1, self.to_phys(cr1), temp1 * nvs, time)
-        n1 = self.splice(cu1, x1, ca, cb)
-        x3, ca, cb = self.cn_solve("u3", u3, self.to_phys(cr3), temp1 * nvs, time)
-        n3 = self.splice(cu3, x3, ca, cb)
+            x, ca, cb = self.cn_solve(f"th{n}", ths[n], self.to_phys(crth[n]), temp1 * nvs / p["pr"][n], time,
+                                      kts[n], temp2)
+            new_th.append(self.splice(baseth[n], x, ca, cb))
+        x2, ca, cb = self.cn_solve("u2", u2, self.to_phys(cr2), temp1 * nvs, time, nts, temp2)
+        n2 = self.splice(base2, x2, ca, cb)
+        x1, ca, cb = self.cn_solve("u1", u1, self.to_phys(cr1), temp1 * nvs, time, nts, temp2)
+        n1 = self.splice(base1, x1, ca, cb)
+        x3, ca, cb = self.cn_solve("u3", u3, self.to_phys(cr3), temp1 * nvs, time, nts, temp2)
+        n3 = self.splice(base3, x3, ca, cb)
         n1, n2, n3, phi = self.rem_div(n1, n2, n3)
         out["cu1"], out["cu2"], out["cu3"] = n1, n2, n3
         out["cp"] = cp + m1 * phi / temp4
         out["cth"] = torch.stack(new_th) if nth else cth
+        if les:
+            out["nu_t"] = nts
+            out["kappa_t"] = kts
         out["cf1"], out["cf2"], out["cf3"] = cf1, cf2, cf3
         out["cfth"] = torch.stack(cfth) if nth else cfth_old
         return out
@@ -338,10 +376,10 @@ class TorchChannel:
         m = self.rows(c_old, ca, cb)
         return m * self.to_spec(x_phys) + (1 - m) * c_old
 
-    def step(self, s, time=0.0):
+    def step(self, s, time=0.0, time_step=None):
         for rk in (1, 2, 3):
-            s = self.substep(s, rk, time)
-        return s
+            s = self.substep(s, rk, time, time_step)
+        return {k: v for k, v in s.items() if k in STATE}
 
 
 def from_ref(state, p):
