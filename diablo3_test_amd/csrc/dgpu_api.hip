@@ -93,6 +93,9 @@ struct Knobs {
   // FP64 transform passes over 512 points with the length fixed at compile
   // time (fft_regs_c); 0 = the run-time length everywhere.
   int fft_const = 1;
+  // FP64 products pass with each input phase's loads issued behind the
+  // previous pair's products, before its transform (0 = at the phase start).
+  int xprod_pf = 1;
 };
 
 Knobs knobs_from_env() {
@@ -108,6 +111,7 @@ Knobs knobs_from_env() {
   rd("DGPU_SPECTRAL_CN", k.spectral_cn);
   rd("DGPU_BATCH_MB", k.batch_mb);
   rd("DGPU_FFT_CONST", k.fft_const);
+  rd("DGPU_XPROD_PF", k.xprod_pf);
   k.spectral_cn = k.spectral_cn != 0;
   return k;
 }
@@ -1773,6 +1777,17 @@ void rk_substep_t(dgpu_handle* h, int rk_step, double time, int time_step) {
     if constexpr (sizeof(R) == 8)
       if (g.nx == 512 && h->kn.fft_const) kern = k_xprod_all<R, true, 9>;
     if (g.nz % KT) kern = g.nx <= 512 ? k_xprod_all<R, true, 0, true> : k_xprod_all<R, false, 0, true>;
+    // FP64 loads the next input phase behind each pair (5.53 -> 5.37 ms per
+    // step at 512x257x512, still two waves per SIMD at 226 VGPRs); the FP32
+    // pass keeps the phase-start loads: its 145 VGPRs with them cost a wave
+    // per SIMD (4 -> 3) and it went from 2.85 to 3.30 ms
+    if constexpr (sizeof(R) == 8)
+      if (h->kn.xprod_pf) {
+        kern = g.nx <= 512 ? k_xprod_all<R, true, 0, false, true> : k_xprod_all<R, false, 0, false, true>;
+        if (g.nx == 512 && h->kn.fft_const) kern = k_xprod_all<R, true, 9, false, true>;
+        if (g.nz % KT)
+          kern = g.nx <= 512 ? k_xprod_all<R, true, 0, true, true> : k_xprod_all<R, false, 0, true, true>;
+      }
     hipLaunchKernelGGL(kern, grid, block, lds, h->stream, pa, g, twx_of<R>(h), 1);
     HIPCHECK(hipGetLastError());
   }
@@ -2964,6 +2979,9 @@ void create_impl(dgpu_handle* h, const double* gy) {
         (const void*)k_xprod_all<double, true, 0, true>, (const void*)k_adj_xpairs<true, true>,
         (const void*)k_xfwd<float, true, true>,   (const void*)k_xinv<float, true, 0, true>,
         (const void*)k_xprod_all<float, true, 0, true>,
+        (const void*)k_xprod_all<double, true, 0, false, true>, (const void*)k_xprod_all<double, false, 0, false, true>,
+        (const void*)k_xprod_all<double, true, 9, false, true>,
+        (const void*)k_xprod_all<double, true, 0, true, true>, (const void*)k_xprod_all<double, false, 0, true, true>,
         (const void*)k_zgen_fwd<double>, (const void*)k_zgen_inv<double>,
         (const void*)k_zgen_fwd<float>,  (const void*)k_zgen_inv<float>};
     for (const void* k : kernels)

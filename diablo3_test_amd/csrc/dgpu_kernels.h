@@ -374,7 +374,10 @@ __device__ __forceinline__ void emit_pair(cpx<R>* smem, int L, const Geo& g, int
 }
 
 // LOGNX > 0: Nx = 2^LOGNX at compile time (fft_regs_c); 0: Nx from g.
-template <typename R, bool WAVE, int LOGNX = 0, bool RAG = false>
+// PF: each input phase's loads are issued into the registers the previous
+// pair has just finished with, before that pair's transform, so that they
+// are in flight while it runs (the inputs still cross the memory system once).
+template <typename R, bool WAVE, int LOGNX = 0, bool RAG = false, bool PF = false>
 __global__ void __launch_bounds__(512) k_xprod_all(ProdAllArgs a, Geo g,
                                                     const cpx<R>* __restrict__ tw, int j0) {
   constexpr int KT = XTile<R>::KT;
@@ -420,6 +423,124 @@ __global__ void __launch_bounds__(512) k_xprod_all(ProdAllArgs a, Geo g,
     bool x;
     pair_ranges(b, g, 0, j, r1, x);
     pair_ranges(b, g, 3, j, r2, x);
+  }
+  if constexpr (PF) {
+    for (int m0 = 0; m0 < KT; m0 += ngroups) {
+      const bool rowok = !RAG || k0 + m0 + grp < g.nz;
+      const int nrow = RAG ? g.nz - k0 - m0 : 1 << 30;
+      const long long rowc = (long long)j * pp + (long long)(rowok ? k0 + m0 + grp : 0) * g.nx + gt;
+      auto ld8 = [&](const R* F, long long off, R(&d)[8]) {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) d[r] = F[rowc + off + G * r];
+      };
+      auto thp_of = [&](int n) { return reinterpret_cast<const R*>(a.th[n]); };
+      // register sets, by what they hold in turn:
+      //   A: u1(j-1), u3(j-1), th(j-1)    B: u1(j)     Cs: u1(j+1), u3(j+1), th(j+1)
+      //   D: u2(j-1), u3(j)               E: u2(j)     F: u2(j+1)       T: th(j)
+      R A[8], B[8], Cs[8], D[8], E[8], F[8], T[8];
+      R va[8], vb[8];
+      ld8(U1, -pp, A);
+      ld8(U1, 0, B);
+      ld8(U1, pp, Cs);
+      ld8(U2, -pp, D);
+      ld8(U2, 0, E);
+      ld8(U2, pp, F);
+      // pair 0: (u1u1, V1) -> slot 0                                    (channel.f90:319-325, 399-407)
+      if (r1) {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+          va[r] = B[r] * B[r];
+          vb[r] = (Cs[r] * F[r] + B[r] * F[r] - B[r] * E[r] - A[r] * E[r]) * c2dyf;
+        }
+      }
+      ld8(U3, pp, Cs);  // u1(j+1) is done with
+      if (r1) {
+        fft_pair(va, vb);
+        emit_pair(smem, L, g, lg_ng, 0, inv, b.kx, outp(0), (cpx<R>*)nullptr, m0, nrow);
+        __syncthreads();
+      }
+      // pair 3: (W1, V2) -> slot 4                                      (:356-365, 440-449)
+      if (r2) {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+          va[r] = ((dyfj * B[r] + dyfm * A[r]) * c2dy) * E[r];
+          const R sp = E[r] + F[r], sm = E[r] + D[r];
+          vb[r] = (quarter * (sp * sp) - quarter * (sm * sm)) * cdy;
+        }
+      }
+      ld8(U3, -pp, A);  // u1(j-1) and u2(j-1) are done with
+      ld8(U3, 0, D);
+      if (r2) {
+        fft_pair(va, vb);
+        emit_pair(smem, L, g, lg_ng, 3, inv, b.kx, outp(4), (cpx<R>*)nullptr, m0, nrow);
+        __syncthreads();
+      }
+      // pair 1: (u3u1, V3) -> slots 1, 2                                (:298-305, 419-428)
+      if (r1) {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+          va[r] = D[r] * B[r];
+          vb[r] = (Cs[r] * F[r] + D[r] * F[r] - D[r] * E[r] - A[r] * E[r]) * c2dyf;
+        }
+      }
+      if (a.nth > 0) {  // u3(j+1) is done with
+        ld8(thp_of(0), pp, Cs);
+        ld8(thp_of(0), 0, T);
+      }
+      if (r1) {
+        fft_pair(va, vb);
+        emit_pair(smem, L, g, lg_ng, 1, inv, b.kx, outp(1), outp(2), m0, nrow);
+        __syncthreads();
+      }
+      // pair 2: (u3u3, W3) -> slots 3, 5                                (:337-344, 377-386)
+      if (r1 || r2) {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+          va[r] = r1 ? D[r] * D[r] : zero;
+          vb[r] = r2 ? ((dyfj * D[r] + dyfm * A[r]) * c2dy) * E[r] : zero;
+        }
+      }
+      if (a.nth > 0) ld8(thp_of(0), -pp, A);  // u3(j-1) is done with
+      if (r1 || r2) {
+        fft_pair(va, vb);
+        emit_pair(smem, L, g, lg_ng, 2, inv, b.kx, outp(3), outp(5), m0, nrow);
+        __syncthreads();
+      }
+      // scalars: (th u1, Vt) -> slot 6+2n, (th u3, 0) -> slot 7+2n     (:557-600)
+      // scalar n is in A, T, Cs; the next one's rows load behind its pairs
+      for (int n = 0; n < a.nth; ++n) {
+        const bool act = j >= a.jst[n] && j <= a.jet[n];  // uniform across the block
+        const bool more = n + 1 < a.nth;
+        if (act) {
+#pragma unroll
+          for (int r = 0; r < 8; ++r) {
+            va[r] = T[r] * B[r];
+            vb[r] = (Cs[r] * F[r] + T[r] * F[r] - T[r] * E[r] - A[r] * E[r]) * c2dyf;
+          }
+        }
+        if (more) {
+          ld8(thp_of(n + 1), pp, Cs);
+          ld8(thp_of(n + 1), -pp, A);
+        }
+        if (act) {
+          fft_pair(va, vb);
+          emit_pair(smem, L, g, lg_ng, 4, inv, b.kx, outp(6 + 2 * n), (cpx<R>*)nullptr, m0, nrow);
+          __syncthreads();
+#pragma unroll
+          for (int r = 0; r < 8; ++r) {
+            va[r] = T[r] * D[r];
+            vb[r] = zero;
+          }
+        }
+        if (more) ld8(thp_of(n + 1), 0, T);
+        if (act) {
+          fft_pair(va, vb);
+          emit_pair(smem, L, g, lg_ng, 5, inv, b.kx, outp(7 + 2 * n), (cpx<R>*)nullptr, m0, nrow);
+          __syncthreads();
+        }
+      }
+    }
+    return;
   }
   for (int m0 = 0; m0 < KT; m0 += ngroups) {
     // a row past Nz (ragged tile) reads row 0 and is not stored
