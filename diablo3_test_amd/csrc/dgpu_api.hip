@@ -96,6 +96,13 @@ struct Knobs {
   // FP64 products pass with each input phase's loads issued behind the
   // previous pair's products, before its transform (0 = at the phase start).
   int xprod_pf = 1;
+  // 512-point passes that run fft512_xl (the cross-lane exchange), a bit per
+  // pass (XL_* below): the products, x and z inverse and x and z forward
+  // passes in FP64 and FP32 (1 + 4 + 8 + 16 + 32 + 64).  Measured at
+  // 512x257x512: DNS FP64 16.72 -> 16.32 ms per step, mixed 11.74 -> 11.50,
+  // LES 60.6 -> 59.4.  The RHS pass keeps the Stockham form: with fft512_xl
+  // its 137 VGPRs cost a wave per SIMD (4 -> 3) and it went 4.20 -> 5.66 ms.
+  int fft_xl = 125;
 };
 
 Knobs knobs_from_env() {
@@ -112,6 +119,7 @@ Knobs knobs_from_env() {
   rd("DGPU_BATCH_MB", k.batch_mb);
   rd("DGPU_FFT_CONST", k.fft_const);
   rd("DGPU_XPROD_PF", k.xprod_pf);
+  rd("DGPU_FFT_XL", k.fft_xl);
   k.spectral_cn = k.spectral_cn != 0;
   return k;
 }
@@ -614,6 +622,14 @@ int x_ngroups(const dgpu_handle* h, int jobs, int max_threads = 512) {
   return ng;
 }
 
+// Whether a 512-point pass in precision R takes fft512_xl (knob fft_xl: a
+// bit per pass, XL_*, plus XL_FP32 for the FP32 passes).
+enum { XL_PROD = 1, XL_RHS = 2, XL_XINV = 4, XL_ZINV = 8, XL_FP32 = 16, XL_XFWD = 32, XL_ZFWD = 64 };
+template <typename R>
+bool use_xl(const dgpu_handle* h, int pass) {
+  return (h->kn.fft_xl & pass) && (sizeof(R) == 8 || (h->kn.fft_xl & XL_FP32));
+}
+
 template <typename R>
 size_t x_lds(const dgpu_handle* h, int ng, bool table = true) {
   return ((size_t)ng * x_group_stride(h->g.nx) + (table ? h->g.nx : 0)) * sizeof(cpx<R>);
@@ -649,6 +665,7 @@ void launch_xfwd(dgpu_handle* h, const R* u, cpx<R>* T, int j0, int j1, const Fi
   Timed tm(h, DGPU_K_XFWD, (sizeof(R) * g.pp + 2.0 * sizeof(R) * g.tp) * (j1 - j0 + 1) * nf,
            2.0 * sizeof(R) * g.tp * (j1 - j0 + 1) * nf);
   auto kern = g.nx <= 512 ? k_xfwd<R, true> : k_xfwd<R, false>;
+  if (g.nx == 512 && use_xl<R>(h, XL_XFWD)) kern = k_xfwd<R, true, false, FFT_XL512>;
   if (g.nz % KT) kern = g.nx <= 512 ? k_xfwd<R, true, true> : k_xfwd<R, false, true>;  // ragged last tile
   hipLaunchKernelGGL(kern, grid, block, lds, h->stream, u, T, g, twx_of<R>(h), j0, fb ? *fb : FieldBatch{});
   HIPCHECK(hipGetLastError());
@@ -669,6 +686,7 @@ void launch_xinv(dgpu_handle* h, const cpx<R>* T, R* u, int j0, int j1, const do
   auto kern = g.nx <= 512 ? k_xinv<R, true> : k_xinv<R, false>;
   if constexpr (sizeof(R) == 8)
     if (g.nx == 512 && h->kn.fft_const) kern = k_xinv<R, true, 9>;
+  if (g.nx == 512 && use_xl<R>(h, XL_XINV)) kern = k_xinv<R, true, FFT_XL512>;
   if (g.nz % KT) kern = g.nx <= 512 ? k_xinv<R, true, 0, true> : k_xinv<R, false, 0, true>;  // ragged last tile
   hipLaunchKernelGGL(kern, grid, block, lds, h->stream, T, u, g, twx_of<R>(h), j0, dkx, fb ? *fb : FieldBatch{});
   HIPCHECK(hipGetLastError());
@@ -746,9 +764,10 @@ void launch_zfwd(dgpu_handle* h, const cpx<R>* T, double2* C, int j0, int j1, bo
   const int ng = z_ngroups<R>(h);
   const size_t lds = z_lds<R>(h, ng);
   dim3 grid((g.nq + ng - 1) / ng, j1 - j0 + 1, nf), block(ng * (g.nz / 8));
-  hipLaunchKernelGGL((g.nz <= 512 ? k_zfwd<R, true> : k_zfwd<R, false>), grid, block, lds, h->stream, T,
-                     reinterpret_cast<cpx<double>*>(C), g, twz_of<R>(h), j0, accum ? 1 : 0, (R)s0, (R)s1,
-                     fb ? *fb : FieldBatch{});
+  auto kern = g.nz <= 512 ? k_zfwd<R, true> : k_zfwd<R, false>;
+  if (g.nz == 512 && use_xl<R>(h, XL_ZFWD)) kern = k_zfwd<R, true, FFT_XL512>;
+  hipLaunchKernelGGL(kern, grid, block, lds, h->stream, T, reinterpret_cast<cpx<double>*>(C), g, twz_of<R>(h), j0,
+                     accum ? 1 : 0, (R)s0, (R)s1, fb ? *fb : FieldBatch{});
   HIPCHECK(hipGetLastError());
 }
 
@@ -770,6 +789,7 @@ void launch_zinv(dgpu_handle* h, const double2* C, cpx<R>* T, int j0, int j1, do
   auto kern = g.nz <= 512 ? k_zinv<R, true> : k_zinv<R, false>;
   if constexpr (sizeof(R) == 8)
     if (g.nz == 512 && h->kn.fft_const) kern = k_zinv<R, true, 9>;
+  if (g.nz == 512 && use_xl<R>(h, XL_ZINV)) kern = k_zinv<R, true, FFT_XL512>;
   hipLaunchKernelGGL(kern, grid, block, lds, h->stream, reinterpret_cast<const cpx<double>*>(C), T, g, twz_of<R>(h),
                      j0, s0, s1, fb ? *fb : FieldBatch{});
   HIPCHECK(hipGetLastError());
@@ -789,9 +809,10 @@ void launch_zinv_op(dgpu_handle* h, const double2* C, cpx<R>* T, int j0, int j1,
   const int ng = z_ngroups<R>(h);
   const size_t lds = z_lds<R>(h, ng);
   dim3 grid((g.nq + ng - 1) / ng, j1 - j0 + 1), block(ng * (g.nz / 8));
-  hipLaunchKernelGGL((g.nz <= 512 ? k_zinv_op<R, true> : k_zinv_op<R, false>), grid, block, lds, h->stream,
-                     reinterpret_cast<const cpx<double>*>(C), T, g, twz_of<R>(h), j0, op, h->d_kx.p,
-                     h->d_kz.p, h->d_dy.p, h->d_dyf.p, s0, s1);
+  auto kern = g.nz <= 512 ? k_zinv_op<R, true> : k_zinv_op<R, false>;
+  if (g.nz == 512 && use_xl<R>(h, XL_ZINV)) kern = k_zinv_op<R, true, FFT_XL512>;
+  hipLaunchKernelGGL(kern, grid, block, lds, h->stream, reinterpret_cast<const cpx<double>*>(C), T, g, twz_of<R>(h),
+                     j0, op, h->d_kx.p, h->d_kz.p, h->d_dy.p, h->d_dyf.p, s0, s1);
   HIPCHECK(hipGetLastError());
 }
 
@@ -1788,6 +1809,12 @@ void rk_substep_t(dgpu_handle* h, int rk_step, double time, int time_step) {
         if (g.nz % KT)
           kern = g.nx <= 512 ? k_xprod_all<R, true, 0, true, true> : k_xprod_all<R, false, 0, true, true>;
       }
+    if (g.nx == 512 && g.nz % KT == 0 && use_xl<R>(h, XL_PROD)) {
+      if constexpr (sizeof(R) == 8)
+        kern = h->kn.xprod_pf ? k_xprod_all<R, true, FFT_XL512, false, true> : k_xprod_all<R, true, FFT_XL512>;
+      else
+        kern = k_xprod_all<R, true, FFT_XL512>;
+    }
     hipLaunchKernelGGL(kern, grid, block, lds, h->stream, pa, g, twx_of<R>(h), 1);
     HIPCHECK(hipGetLastError());
   }
@@ -1858,6 +1885,7 @@ void rk_substep_t(dgpu_handle* h, int rk_step, double time, int time_step) {
     auto kern = g.nz <= 512 ? k_rhs<R, true> : k_rhs<R, false>;
     if constexpr (sizeof(R) == 8)
       if (g.nz == 512 && h->kn.fft_const) kern = k_rhs<R, true, 9>;
+    if (g.nz == 512 && use_xl<R>(h, XL_RHS)) kern = k_rhs<R, true, FFT_XL512>;
     // x-spectra of the products in, cr x-spectra out; cu1..3, cp, cth, cf1..3,
     // cfth read and cf1..3, cfth written (neighbour planes come from cache)
     const double crbytes = spec ? (incr ? 8.0 : 16.0) * g.sp * ncr : 2.0 * sizeof(R) * g.tp * ncr;
@@ -2981,6 +3009,13 @@ void create_impl(dgpu_handle* h, const double* gy) {
         (const void*)k_xprod_all<float, true, 0, true>,
         (const void*)k_xprod_all<double, true, 0, false, true>, (const void*)k_xprod_all<double, false, 0, false, true>,
         (const void*)k_xprod_all<double, true, 9, false, true>,
+        (const void*)k_xprod_all<double, true, FFT_XL512, false, true>, (const void*)k_xprod_all<float, true, FFT_XL512>,
+        (const void*)k_rhs<double, true, FFT_XL512>,   (const void*)k_rhs<float, true, FFT_XL512>,
+        (const void*)k_xinv<double, true, FFT_XL512>,  (const void*)k_xinv<float, true, FFT_XL512>,
+        (const void*)k_zinv<double, true, FFT_XL512>,  (const void*)k_zinv<float, true, FFT_XL512>,
+        (const void*)k_xfwd<double, true, false, FFT_XL512>, (const void*)k_xfwd<float, true, false, FFT_XL512>,
+        (const void*)k_zfwd<double, true, FFT_XL512>,  (const void*)k_zfwd<float, true, FFT_XL512>,
+        (const void*)k_zinv_op<double, true, FFT_XL512>, (const void*)k_zinv_op<float, true, FFT_XL512>,
         (const void*)k_xprod_all<double, true, 0, true, true>, (const void*)k_xprod_all<double, false, 0, true, true>,
         (const void*)k_zgen_fwd<double>, (const void*)k_zgen_inv<double>,
         (const void*)k_zgen_fwd<float>,  (const void*)k_zgen_inv<float>};

@@ -292,11 +292,140 @@ __device__ __forceinline__ void fft_regs_c(cpx<R> (&v)[8], cpx<R>* buf, const cp
   fft_passes_c<R, S, WAVE, LOGN, 3>(buf, tw, gt);
 }
 
-// fft_regs, or fft_regs_c when LOGN > 0 fixes N = 2^LOGN at compile time.
+// ------------------------------------------ 512 points in one wavefront --
+// A 512-point transform held by the 64 lanes of one wavefront, 8 values a
+// lane, with the radix-8 steps on registers and the data exchanges between
+// them done partly across lanes.  Index n = n0 + 8 n1 + 64 n2 with n0, n1 the
+// low and high three bits of the lane and n2 the register; output
+// k = k2 + 8 k1 + 64 k0:
+//   X = sum_n0 w^(64 n0 k0) w^(8 n0 k1) w^(n0 k2)
+//         sum_n1 w^(64 n1 k1) w^(8 n1 k2) sum_n2 w^(64 n2 k2) x
+// (w = exp(S 2 pi i / 512)).  Step A is a radix-8 DFT over the registers,
+// then the twiddle w^((n0 + 8 n1) k2) = w^(lane k2).  The high lane bits (n1)
+// and the register index (k2) are then transposed without LDS: lane bit 5
+// with v_permlane32_swap, bit 4 with v_permlane16_swap, bit 3 with two DPP
+// moves (row_ror:8, each writing one half of every row through its bank
+// mask).  Step B is a radix-8 DFT over n1, then the twiddle w^(8 n0 k1).  The
+// low lane bits and the register index go through LDS (one write and one
+// read a value), step C transforms over n0, and the result is stored in
+// natural order, k = k2 + 8 k1 + 64 k0 from lane k1 + 8 k2, register k0.
+// Against the Stockham form (fft_regs_c) this replaces one of the two LDS
+// exchanges of the transform by 64 cross-lane moves per complex FP64 value
+// set: 24 LDS accesses a value instead of 40.
+
+// The 2x2 transposes of a lane bit and a register bit on 32-bit words: the
+// values of (lane bit set, a) and (lane bit clear, b) trade places.
+__device__ __forceinline__ void xl_bit5(unsigned& a, unsigned& b) {
+  const auto r = __builtin_amdgcn_permlane32_swap(a, b, false, false);
+  a = r[0];
+  b = r[1];
+}
+__device__ __forceinline__ void xl_bit4(unsigned& a, unsigned& b) {
+  const auto r = __builtin_amdgcn_permlane16_swap(a, b, false, false);
+  a = r[0];
+  b = r[1];
+}
+__device__ __forceinline__ void xl_bit3(unsigned& a, unsigned& b) {
+  // row_ror:8 (DPP control 0x128) reads lane l ^ 8 within each 16-lane row;
+  // bank mask 0xC writes lanes 8-15 of a row, 0x3 lanes 0-7
+  const int na = __builtin_amdgcn_update_dpp((int)a, (int)b, 0x128, 0xF, 0xC, false);
+  const int nb = __builtin_amdgcn_update_dpp((int)b, (int)a, 0x128, 0xF, 0x3, false);
+  a = (unsigned)na;
+  b = (unsigned)nb;
+}
+
+template <int BIT>
+__device__ __forceinline__ void xl_word(unsigned& a, unsigned& b) {
+  if constexpr (BIT == 5)
+    xl_bit5(a, b);
+  else if constexpr (BIT == 4)
+    xl_bit4(a, b);
+  else
+    xl_bit3(a, b);
+}
+
+template <int BIT>
+__device__ __forceinline__ void xl_val(double& a, double& b) {
+  const unsigned long long ua = __double_as_longlong(a), ub = __double_as_longlong(b);
+  unsigned al = (unsigned)ua, ah = (unsigned)(ua >> 32), bl = (unsigned)ub, bh = (unsigned)(ub >> 32);
+  xl_word<BIT>(al, bl);
+  xl_word<BIT>(ah, bh);
+  a = __longlong_as_double(((unsigned long long)ah << 32) | al);
+  b = __longlong_as_double(((unsigned long long)bh << 32) | bl);
+}
+template <int BIT>
+__device__ __forceinline__ void xl_val(float& a, float& b) {
+  unsigned ua = __float_as_uint(a), ub = __float_as_uint(b);
+  xl_word<BIT>(ua, ub);
+  a = __uint_as_float(ua);
+  b = __uint_as_float(ub);
+}
+
+// Transpose lane bits 3-5 with the register index of v (8 values a lane).
+template <typename R>
+__device__ __forceinline__ void xl_transpose_hi(cpx<R> (&v)[8]) {
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {  // lane bit 5 <-> register bit 2
+    xl_val<5>(v[r].x, v[r + 4].x);
+    xl_val<5>(v[r].y, v[r + 4].y);
+  }
+#pragma unroll
+  for (int r : {0, 1, 4, 5}) {  // lane bit 4 <-> register bit 1
+    xl_val<4>(v[r].x, v[r + 2].x);
+    xl_val<4>(v[r].y, v[r + 2].y);
+  }
+#pragma unroll
+  for (int r : {0, 2, 4, 6}) {  // lane bit 3 <-> register bit 0
+    xl_val<3>(v[r].x, v[r + 1].x);
+    xl_val<3>(v[r].y, v[r + 1].y);
+  }
+}
+
+// The transform: the contract of fft_regs_c with N = 512, one group per
+// wavefront (gt = lane), tw[t] = exp(-2 pi i t / 512).
+template <typename R, int S>
+__device__ __forceinline__ void fft512_xl(cpx<R> (&v)[8], cpx<R>* buf, const cpx<R>* __restrict__ tw, int gt) {
+  auto twid = [&](int t) {
+    cpx<R> c = tw[t & 511];
+    if (S > 0) c.y = -c.y;
+    return c;
+  };
+  dft8<R, S>(v);  // step A: v[k2]
+#pragma unroll
+  for (int k = 1; k < 8; ++k) v[k] = v[k] * twid(gt * k);
+  xl_transpose_hi(v);  // lane bits 3-5 = k2, v[n1]
+  dft8<R, S>(v);       // step B: v[k1]
+  const int n0 = gt & 7, hi = gt >> 3;
+#pragma unroll
+  for (int k = 1; k < 8; ++k) v[k] = v[k] * twid(8 * n0 * k);
+  // low lane bits <-> register through LDS, element n0 + 8 k1 + 64 k2
+  fft_sync<true>();  // the previous use of buf is finished
+  const int pw = pidx(n0 + 64 * hi);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) buf[pw + 9 * k] = v[k];  // pidx(e + 8k) = pidx(e) + 9k
+  fft_sync<true>();
+  const int pr = pidx(8 * n0 + 64 * hi);
+#pragma unroll
+  for (int r = 0; r < 8; ++r) v[r] = buf[pr + r];  // n0' = r: element r + 8 k1 + 64 k2
+  dft8<R, S>(v);  // step C: v[k0]
+  fft_sync<true>();
+  const int po = pidx(hi + 8 * n0);  // k = k2 + 8 k1 + 64 k0, lane = k1 + 8 k2
+#pragma unroll
+  for (int k = 0; k < 8; ++k) buf[po + 72 * k] = v[k];  // pidx(e + 64k) = pidx(e) + 72k
+  fft_sync<true>();
+}
+
+// The LOGN of fft_any that selects fft512_xl (N = 512).
+constexpr int FFT_XL512 = 100;
+
+// fft_regs, or fft_regs_c when LOGN > 0 fixes N = 2^LOGN at compile time, or
+// fft512_xl for LOGN = FFT_XL512.
 template <typename R, int S, bool WAVE, int LOGN>
 __device__ __forceinline__ void fft_any(cpx<R> (&v)[8], cpx<R>* buf, int N, int logN,
                                         const cpx<R>* __restrict__ tw, int gt) {
-  if constexpr (LOGN > 0)
+  if constexpr (LOGN == FFT_XL512)
+    fft512_xl<R, S>(v, buf, tw, gt);
+  else if constexpr (LOGN > 0)
     fft_regs_c<R, S, WAVE, LOGN>(v, buf, tw, gt);
   else
     fft_regs<R, S, WAVE>(v, buf, N, logN, tw, gt);

@@ -82,7 +82,7 @@ __device__ __forceinline__ void split_pair(const cpx<R>* f, int n, int q, R scal
 // LDS ngroups*fft_lds_len(Nx) cpx.
 // RAG: Nz is not a multiple of KT, so the last tile holds fewer rows; rows
 // k >= Nz are read as zero and not stored (here and in the other x passes).
-template <typename R, bool WAVE, bool RAG = false>
+template <typename R, bool WAVE, bool RAG = false, int LOGN = 0>
 __global__ void __launch_bounds__(512) k_xfwd(const R* __restrict__ u, cpx<R>* __restrict__ T,
                                                Geo g, const cpx<R>* __restrict__ tw, int j0,
                                                FieldBatch fb = FieldBatch{}) {
@@ -113,7 +113,7 @@ __global__ void __launch_bounds__(512) k_xfwd(const R* __restrict__ u, cpx<R>* _
 #pragma unroll
     for (int r = 0; r < 8; ++r)
       v[r] = {oka ? ld_in(ra + gt + G * r) : (R)0, okb ? ld_in(rb + gt + G * r) : (R)0};
-    fft_regs<R, -1, WAVE>(v, fbuf, g.nx, g.lognx, twl, gt);
+    fft_any<R, -1, WAVE, LOGN>(v, fbuf, g.nx, g.lognx, twl, gt);
     __syncthreads();  // every group's spectra are read below
     for (int idx = threadIdx.x; idx < g.nq * nr; idx += blockDim.x) {
       const int q = idx / nr, kk = idx - q * nr;
@@ -208,7 +208,7 @@ __global__ void __launch_bounds__(512) k_xinv(const cpx<R>* __restrict__ T, R* _
 // group; grid (ceil(nq/ngroups), nplanes), block ngroups*Nz/8.  The spectral
 // array is FP64 whatever the transform precision R; with accum the result is
 // added to it (the FP32 transform of an increment of an FP64 state).
-template <typename R, bool WAVE>
+template <typename R, bool WAVE, int LOGN = 0>
 __global__ void __launch_bounds__(512) k_zfwd(const cpx<R>* __restrict__ T, cpx<double>* __restrict__ C,
                                                Geo g, const cpx<R>* __restrict__ tw, int j0, int accum,
                                                R s0, R s1, FieldBatch fb = FieldBatch{}) {
@@ -234,7 +234,7 @@ __global__ void __launch_bounds__(512) k_zfwd(const cpx<R>* __restrict__ T, cpx<
 #pragma unroll
     for (int r = 0; r < 8; ++r) v[r] = {(R)0, (R)0};
   }
-  fft_regs<R, -1, WAVE>(v, fbuf, g.nz, g.lognz, twl, gt);
+  fft_any<R, -1, WAVE, LOGN>(v, fbuf, g.nz, g.lognz, twl, gt);
   if (act) {
     // 1/Nz, times s0 (kx = 0) or s1 (kx > 0): 1 for the transform itself,
     // Nx Nz W' for the transpose of the inverse transform
@@ -373,7 +373,8 @@ __device__ __forceinline__ void emit_pair(cpx<R>* smem, int L, const Geo& g, int
   }
 }
 
-// LOGNX > 0: Nx = 2^LOGNX at compile time (fft_regs_c); 0: Nx from g.
+// LOGNX: Nx at compile time as in fft_any (fft_regs_c, or fft512_xl for
+// FFT_XL512); 0: Nx from g.
 // PF: each input phase's loads are issued into the registers the previous
 // pair has just finished with, before that pair's transform, so that they
 // are in flight while it runs (the inputs still cross the memory system once).
@@ -412,10 +413,7 @@ __global__ void __launch_bounds__(512) k_xprod_all(ProdAllArgs a, Geo g,
     cpx<R> v[8];
 #pragma unroll
     for (int r = 0; r < 8; ++r) v[r] = {va[r], vb[r]};
-    if constexpr (LOGNX > 0)
-      fft_regs_c<R, -1, WAVE, LOGNX>(v, fbuf, twl, gt);
-    else
-      fft_regs<R, -1, WAVE>(v, fbuf, g.nx, g.lognx, twl, gt);
+    fft_any<R, -1, WAVE, LOGNX>(v, fbuf, g.nx, g.lognx, twl, gt);
     __syncthreads();
   };
   bool r1, r2;

@@ -42,7 +42,7 @@ enum DerivOp {
 // (fft.f90:319-345 applied to the derivative arrays of les.f90).  Planes
 // where the derivative is not defined (j = 0 for OP_DY_GY and OP_DY_C2,
 // j = Ny+1 for OP_DY_GYF, OP_DY_C2 and the averages) are written as zero.
-template <typename R, bool WAVE>
+template <typename R, bool WAVE, int LOGN = 0>
 __global__ void __launch_bounds__(512) k_zinv_op(const cpx<double>* __restrict__ C, cpx<R>* __restrict__ T,
                                                  Geo g, const cpx<R>* __restrict__ tw, int j0, int op,
                                                  const double* __restrict__ kx,
@@ -94,7 +94,7 @@ __global__ void __launch_bounds__(512) k_zinv_op(const cpx<double>* __restrict__
     }
     v[r] = {(R)(sc * z.x), (R)(sc * z.y)};
   }
-  fft_regs<R, 1, WAVE>(v, fbuf, g.nz, g.lognz, twl, gt);
+  fft_any<R, 1, WAVE, LOGN>(v, fbuf, g.nz, g.lognz, twl, gt);
   if (q < g.nq) {
     cpx<R>* out = T + (long long)j * g.tp + (long long)q * g.nz;
     for (int k = gt; k < g.nz; k += G) st_out(out + k, fbuf[pidx(k)]);
