@@ -388,7 +388,8 @@ struct AdjPairArgs {
 };
 
 // RAG: ragged last tile (Nz not a multiple of KT); rows k >= Nz are skipped.
-template <bool WAVE, bool RAG = false>
+// LOGN: the transform length at compile time, as in fft_any (0: Nx from g).
+template <bool WAVE, bool RAG = false, int LOGN = 0>
 __global__ void __launch_bounds__(512) k_adj_xpairs(AdjPairArgs a, Geo g, const cpx<double>* __restrict__ tw,
                                                     int j0) {
   typedef cpx<double> C;
@@ -441,7 +442,7 @@ __global__ void __launch_bounds__(512) k_adj_xpairs(AdjPairArgs a, Geo g, const 
       C v[8];
 #pragma unroll
       for (int r = 0; r < 8; ++r) v[r] = fbuf[pidx(gt + G * r)];
-      fft_regs<double, 1, WAVE>(v, fbuf, g.nx, g.lognx, twl, gt);
+      fft_any<double, 1, WAVE, LOGN>(v, fbuf, g.nx, g.lognx, twl, gt);
       if (RAG && k0 + m0 + grp >= g.nz) continue;  // past Nz: nothing to store (the barrier is at the loop top)
       const long long ro = (long long)j * g.pp + (long long)(k0 + m0 + grp) * g.nx;
       double* oa = a.outA[pr] + ro;

@@ -98,11 +98,12 @@ struct Knobs {
   int xprod_pf = 1;
   // 512-point passes that run fft512_xl (the cross-lane exchange), a bit per
   // pass (XL_* below): the products, x and z inverse and x and z forward
-  // passes in FP64 and FP32 (1 + 4 + 8 + 16 + 32 + 64).  Measured at
-  // 512x257x512: DNS FP64 16.72 -> 16.32 ms per step, mixed 11.74 -> 11.50,
-  // LES 60.6 -> 59.4.  The RHS pass keeps the Stockham form: with fft512_xl
-  // its 137 VGPRs cost a wave per SIMD (4 -> 3) and it went 4.20 -> 5.66 ms.
-  int fft_xl = 125;
+  // passes in FP64 and FP32 and the adjoint's x pass (1 + 4 + 8 + 16 + 32 +
+  // 64 + 128).  Measured at 512x257x512: DNS FP64 16.72 -> 16.32 ms per step,
+  // mixed 11.74 -> 11.50, LES 60.6 -> 59.4, adjoint 71.5 -> 70.9.  The RHS
+  // pass keeps the Stockham form: with fft512_xl its 137 VGPRs cost a wave per
+  // SIMD (4 -> 3) and it went 4.20 -> 5.66 ms.
+  int fft_xl = 253;
 };
 
 Knobs knobs_from_env() {
@@ -624,7 +625,7 @@ int x_ngroups(const dgpu_handle* h, int jobs, int max_threads = 512) {
 
 // Whether a 512-point pass in precision R takes fft512_xl (knob fft_xl: a
 // bit per pass, XL_*, plus XL_FP32 for the FP32 passes).
-enum { XL_PROD = 1, XL_RHS = 2, XL_XINV = 4, XL_ZINV = 8, XL_FP32 = 16, XL_XFWD = 32, XL_ZFWD = 64 };
+enum { XL_PROD = 1, XL_RHS = 2, XL_XINV = 4, XL_ZINV = 8, XL_FP32 = 16, XL_XFWD = 32, XL_ZFWD = 64, XL_ADJ = 128 };
 template <typename R>
 bool use_xl(const dgpu_handle* h, int pass) {
   return (h->kn.fft_xl & pass) && (sizeof(R) == 8 || (h->kn.fft_xl & XL_FP32));
@@ -2520,6 +2521,7 @@ void adj_substep(dgpu_handle* h, int rk_step, double time, int time_step) {
     dim3 grid((g.nz + KT - 1) / KT, Ny), block(ng * (g.nx / 8));
     Timed tm(h, DGPU_K_ADJOINT, (16.0 * g.tp * nslot + 16.0 * g.pp * npairs) * Ny, 16.0 * g.tp * nslot * Ny);
     auto kern = g.nx <= 512 ? k_adj_xpairs<true> : k_adj_xpairs<false>;
+    if (g.nx == 512 && use_xl<double>(h, XL_ADJ)) kern = k_adj_xpairs<true, false, FFT_XL512>;
     if (g.nz % KT) kern = g.nx <= 512 ? k_adj_xpairs<true, true> : k_adj_xpairs<false, true>;
     hipLaunchKernelGGL(kern, grid, block, lds, h->stream, a, g, twx_of<double>(h), 1);
     HIPCHECK(hipGetLastError());
@@ -3016,6 +3018,7 @@ void create_impl(dgpu_handle* h, const double* gy) {
         (const void*)k_xfwd<double, true, false, FFT_XL512>, (const void*)k_xfwd<float, true, false, FFT_XL512>,
         (const void*)k_zfwd<double, true, FFT_XL512>,  (const void*)k_zfwd<float, true, FFT_XL512>,
         (const void*)k_zinv_op<double, true, FFT_XL512>, (const void*)k_zinv_op<float, true, FFT_XL512>,
+        (const void*)k_adj_xpairs<true, false, FFT_XL512>,
         (const void*)k_xprod_all<double, true, 0, true, true>, (const void*)k_xprod_all<double, false, 0, true, true>,
         (const void*)k_zgen_fwd<double>, (const void*)k_zgen_inv<double>,
         (const void*)k_zgen_fwd<float>,  (const void*)k_zgen_inv<float>};

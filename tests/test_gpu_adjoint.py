@@ -107,6 +107,27 @@ def test_adjoint_substep_tall_channel(ny):
         assert e < 1e-11, (k, e)
 
 
+@pytest.mark.parametrize("rk", [1, 3])
+def test_adjoint_substep_nx512_matches_torch(rk):
+    """Nx = 512: the 512-point x transforms of the forward and adjoint passes
+    (the one-wavefront cross-lane form) against torch autograd, 512x17x16."""
+    p = workloads.channel_params(512, 17, 16, 200.0, ri=0.1, dt=2e-3)
+    gy = workloads.channel_grid(p["ny"])
+    s = seeded(p, gy)
+    lam = random_adjoint(p, 60 + rk)
+    print(f"adjoint substep 512x17x16 rk={rk}", flush=True)
+    g = gpu_solver(p, gy)
+    g.set_state(s)
+    g.adj_set_state(lam)
+    g.adj_rk_chan_1(rk, 0.3, 5)
+    got = g.adj_get_state(STATE)
+    want = torch_vjp(p, gy, s, lam, rk, 0.3)
+    errs = {k: rel_l2(active(got[k], p), active(want[k], p)) for k in STATE}
+    report(f"adj 512x17x16 rk{rk}", errs)
+    for k, e in errs.items():
+        assert e < 1e-11, (k, e)
+
+
 def test_wall_shear_gradient_matches_torch():
     """L over 3 steps with checkpoints every 2: HIP checkpointed adjoint vs torch autograd."""
     p = workloads.channel_params(16, 17, 16, 200.0, ri=0.1, dt=2e-3)
