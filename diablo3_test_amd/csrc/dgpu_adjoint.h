@@ -79,6 +79,8 @@ __global__ void __launch_bounds__(MAXT) k_adj_remdiv_seg(AdjRemDivArgs a, Geo g,
     return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(base) + ob);
   };
   auto off = [&](int j) { return (unsigned)(((long long)j * sp + m) * 16); };
+  const long long hp = piv_plane(g), mh = piv_index(g, q, kk);  // pivots shared by +-kz
+  auto offp = [&](int j) { return (unsigned)(((long long)j * hp + mh) * 8); };
   auto shup = [](C v, int d) { return C{__shfl_up(v.x, d), __shfl_up(v.y, d)}; };
   auto shdn = [](C v, int d) { return C{__shfl_down(v.x, d), __shfl_down(v.y, d)}; };
   auto inp = [&](int j) { return j >= a.js && j <= a.je; };
@@ -94,7 +96,7 @@ __global__ void __launch_bounds__(MAXT) k_adj_remdiv_seg(AdjRemDivArgs a, Geo g,
     d3[t] = ok ? ldc(a.a3, o) : Z;
     d2[t] = ok ? ldc(a.a2, o) : Z;
     pb[t] = (ok && inp(j)) ? ldc(a.ap, o) : Z;
-    ibv[t] = ok ? ldd(a.ib, o >> 1) : 0.0;
+    ibv[t] = ok ? ldd(a.ib, offp(j)) : 0.0;
   }
   d2[L] = (live && j0 + L <= ny) ? ldc(a.a2, off(j0 + L)) : Z;
   // phibar of each row

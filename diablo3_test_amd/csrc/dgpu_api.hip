@@ -1097,9 +1097,9 @@ void setup_projection(dgpu_handle* h) {
     r.idy = (j >= 2) ? 1.0 / h->dy[j] : 0.0;
   }
   upload(h->d_prow, rows);
-  h->rd_ib.alloc((size_t)g.ny2 * g.sp);
-  hipLaunchKernelGGL(k_remdiv_pivots, dim3((unsigned)((g.sp + 255) / 256)), dim3(256), 0, h->stream, h->rd_ib.p,
-                     h->d_prow.p, h->d_kx.p, h->d_kz.p, g);
+  h->rd_ib.alloc((size_t)g.ny2 * piv_plane(g));  // kz >= 0: the -kz modes share the pivots
+  hipLaunchKernelGGL(k_remdiv_pivots, dim3((unsigned)((piv_plane(g) + 255) / 256)), dim3(256), 0, h->stream,
+                     h->rd_ib.p, h->d_prow.p, h->d_kx.p, h->d_kz.p, g);
   HIPCHECK(hipGetLastError());
 }
 

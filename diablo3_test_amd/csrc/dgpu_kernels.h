@@ -1272,23 +1272,31 @@ struct PoisRow {
   double ml, mu, idyf, idy;
 };
 
+// The pivots depend on kx^2 + kz^2, so a mode and its -kz partner share them:
+// the table holds kz >= 0 only, nq (Nkz+1) entries a row (piv_plane), and a
+// mode (q, kk) reads entry piv_index (kk > Nkz is -kz of 2Nkz+1-kk).
+__host__ __device__ inline long long piv_plane(const Geo& g) { return (long long)g.nq * (g.nkz + 1); }
+__device__ inline long long piv_index(const Geo& g, int q, int kk) {
+  return (long long)q * (g.nkz + 1) + (kk <= g.nkz ? kk : g.nkk - kk);
+}
+
 __global__ void __launch_bounds__(256) k_remdiv_pivots(double* __restrict__ ib, const PoisRow* __restrict__ rows,
                                                        const double* __restrict__ kx,
                                                        const double* __restrict__ kz, Geo g) {
-  const long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= g.sp) return;
-  const int q = (int)((unsigned)m / (unsigned)g.nkk), kk = (int)((unsigned)m % (unsigned)g.nkk);
+  const long long mh = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long hp = piv_plane(g);
+  if (mh >= hp) return;
+  const int q = (int)((unsigned)mh / (unsigned)(g.nkz + 1)), kk = (int)((unsigned)mh % (unsigned)(g.nkz + 1));
   const double lam = kx[q] * kx[q] + kz[kk] * kz[kk];
-  const long long sp = g.sp;
   double bprev = 1.0;
-  double muprev = (m == 0) ? 0.0 : -1.0;
-  ib[sp + m] = 1.0;
+  double muprev = (mh == 0) ? 0.0 : -1.0;  // the (0,0) mode's Dirichlet row 1
+  ib[hp + mh] = 1.0;
   for (int j = 2; j <= g.ny; ++j) {
     const PoisRow rw = rows[j];
     const double md = (j < g.ny) ? -lam - rw.mu - rw.ml : -1.0;
     const double mlt = -rw.ml / bprev;
     const double b = md + mlt * muprev;
-    ib[(long long)j * sp + m] = 1.0 / b;
+    ib[(long long)j * hp + mh] = 1.0 / b;
     bprev = b;
     muprev = rw.mu;
   }
@@ -1343,6 +1351,9 @@ __global__ void __launch_bounds__(MAXT) k_remdiv_seg(RemDivArgs a, Geo g, const 
     return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(base) + ob);
   };
   auto off = [&](int j) { return (unsigned)(((long long)j * sp + m) * 16); };
+  // the pivot of this mode at row j (shared with its -kz partner)
+  const long long hp = piv_plane(g), mh = piv_index(g, q, kk);
+  auto offp = [&](int j) { return (unsigned)(((long long)j * hp + mh) * 8); };
   auto shup = [](C v, int d) { return C{__shfl_up(v.x, d), __shfl_up(v.y, d)}; };
   auto shdn = [](C v, int d) { return C{__shfl_down(v.x, d), __shfl_down(v.y, d)}; };
   // rows j0 .. j0+L-1 of cu1, cu3, cu2 (plus cu2 at j0+L) and 1/b (plus j0-1)
@@ -1356,10 +1367,10 @@ __global__ void __launch_bounds__(MAXT) k_remdiv_seg(RemDivArgs a, Geo g, const 
     d1[t] = ok ? ldc(a.cu1, o) : Z;
     d3[t] = ok ? ldc(a.cu3, o) : Z;
     d2[t] = ok ? ldc(a.cu2, o) : Z;
-    ibv[t + 1] = ok ? ldd(a.ib, o >> 1) : 0.0;
+    ibv[t + 1] = ok ? ldd(a.ib, offp(j)) : 0.0;
   }
   d2[L] = (live && j0 + L <= ny) ? ldc(a.cu2, off(j0 + L)) : Z;
-  ibv[0] = (live && j0 - 1 <= ny) ? ldd(a.ib, off(j0 - 1) >> 1) : 0.0;
+  ibv[0] = (live && j0 - 1 <= ny) ? ldd(a.ib, offp(j0 - 1)) : 0.0;
 
   // forward sweep of the segment from a zero carry: g_t = gl_t + Pl_t * g(j0-1)
   C gl[L];
