@@ -104,6 +104,13 @@ struct Knobs {
   // pass keeps the Stockham form: with fft512_xl its 137 VGPRs cost a wave per
   // SIMD (4 -> 3) and it went 4.20 -> 5.66 ms.
   int fft_xl = 253;
+  // The FP64 512-point passes whose fft512_xl forms its twiddles as powers of
+  // one table entry (same bits): the x passes -- products, x inverse, x
+  // forward, the adjoint's x pass (1 + 4 + 32 + 128).  Measured at
+  // 512x257x512 (FP64 ms per step): products 4.88 -> 4.66, x inverse 2.27 ->
+  // 2.10; the z inverse went 1.24 -> 1.31 with them and keeps the table, and
+  // the FP32 passes are unchanged either way.
+  int fft_twp = 165;
 };
 
 Knobs knobs_from_env() {
@@ -121,6 +128,7 @@ Knobs knobs_from_env() {
   rd("DGPU_FFT_CONST", k.fft_const);
   rd("DGPU_XPROD_PF", k.xprod_pf);
   rd("DGPU_FFT_XL", k.fft_xl);
+  rd("DGPU_FFT_TWP", k.fft_twp);
   k.spectral_cn = k.spectral_cn != 0;
   return k;
 }
@@ -630,6 +638,11 @@ template <typename R>
 bool use_xl(const dgpu_handle* h, int pass) {
   return (h->kn.fft_xl & pass) && (sizeof(R) == 8 || (h->kn.fft_xl & XL_FP32));
 }
+// Whether that pass forms its twiddles as powers (knob fft_twp, FP64 only).
+template <typename R>
+bool xl_powers(const dgpu_handle* h, int pass) {
+  return sizeof(R) == 8 && (h->kn.fft_twp & pass);
+}
 
 template <typename R>
 size_t x_lds(const dgpu_handle* h, int ng, bool table = true) {
@@ -666,7 +679,8 @@ void launch_xfwd(dgpu_handle* h, const R* u, cpx<R>* T, int j0, int j1, const Fi
   Timed tm(h, DGPU_K_XFWD, (sizeof(R) * g.pp + 2.0 * sizeof(R) * g.tp) * (j1 - j0 + 1) * nf,
            2.0 * sizeof(R) * g.tp * (j1 - j0 + 1) * nf);
   auto kern = g.nx <= 512 ? k_xfwd<R, true> : k_xfwd<R, false>;
-  if (g.nx == 512 && use_xl<R>(h, XL_XFWD)) kern = k_xfwd<R, true, false, FFT_XL512>;
+  if (g.nx == 512 && use_xl<R>(h, XL_XFWD))
+    kern = xl_powers<R>(h, XL_XFWD) ? k_xfwd<R, true, false, FFT_XL512P> : k_xfwd<R, true, false, FFT_XL512>;
   if (g.nz % KT) kern = g.nx <= 512 ? k_xfwd<R, true, true> : k_xfwd<R, false, true>;  // ragged last tile
   hipLaunchKernelGGL(kern, grid, block, lds, h->stream, u, T, g, twx_of<R>(h), j0, fb ? *fb : FieldBatch{});
   HIPCHECK(hipGetLastError());
@@ -687,7 +701,8 @@ void launch_xinv(dgpu_handle* h, const cpx<R>* T, R* u, int j0, int j1, const do
   auto kern = g.nx <= 512 ? k_xinv<R, true> : k_xinv<R, false>;
   if constexpr (sizeof(R) == 8)
     if (g.nx == 512 && h->kn.fft_const) kern = k_xinv<R, true, 9>;
-  if (g.nx == 512 && use_xl<R>(h, XL_XINV)) kern = k_xinv<R, true, FFT_XL512>;
+  if (g.nx == 512 && use_xl<R>(h, XL_XINV))
+    kern = xl_powers<R>(h, XL_XINV) ? k_xinv<R, true, FFT_XL512P> : k_xinv<R, true, FFT_XL512>;
   if (g.nz % KT) kern = g.nx <= 512 ? k_xinv<R, true, 0, true> : k_xinv<R, false, 0, true>;  // ragged last tile
   hipLaunchKernelGGL(kern, grid, block, lds, h->stream, T, u, g, twx_of<R>(h), j0, dkx, fb ? *fb : FieldBatch{});
   HIPCHECK(hipGetLastError());
@@ -1811,8 +1826,12 @@ void rk_substep_t(dgpu_handle* h, int rk_step, double time, int time_step) {
           kern = g.nx <= 512 ? k_xprod_all<R, true, 0, true, true> : k_xprod_all<R, false, 0, true, true>;
       }
     if (g.nx == 512 && g.nz % KT == 0 && use_xl<R>(h, XL_PROD)) {
-      if constexpr (sizeof(R) == 8)
-        kern = h->kn.xprod_pf ? k_xprod_all<R, true, FFT_XL512, false, true> : k_xprod_all<R, true, FFT_XL512>;
+      if constexpr (sizeof(R) == 8) {
+        if (xl_powers<R>(h, XL_PROD))
+          kern = h->kn.xprod_pf ? k_xprod_all<R, true, FFT_XL512P, false, true> : k_xprod_all<R, true, FFT_XL512P>;
+        else
+          kern = h->kn.xprod_pf ? k_xprod_all<R, true, FFT_XL512, false, true> : k_xprod_all<R, true, FFT_XL512>;
+      }
       else
         kern = k_xprod_all<R, true, FFT_XL512>;
     }
@@ -2521,7 +2540,8 @@ void adj_substep(dgpu_handle* h, int rk_step, double time, int time_step) {
     dim3 grid((g.nz + KT - 1) / KT, Ny), block(ng * (g.nx / 8));
     Timed tm(h, DGPU_K_ADJOINT, (16.0 * g.tp * nslot + 16.0 * g.pp * npairs) * Ny, 16.0 * g.tp * nslot * Ny);
     auto kern = g.nx <= 512 ? k_adj_xpairs<true> : k_adj_xpairs<false>;
-    if (g.nx == 512 && use_xl<double>(h, XL_ADJ)) kern = k_adj_xpairs<true, false, FFT_XL512>;
+    if (g.nx == 512 && use_xl<double>(h, XL_ADJ))
+      kern = xl_powers<double>(h, XL_ADJ) ? k_adj_xpairs<true, false, FFT_XL512P> : k_adj_xpairs<true, false, FFT_XL512>;
     if (g.nz % KT) kern = g.nx <= 512 ? k_adj_xpairs<true, true> : k_adj_xpairs<false, true>;
     hipLaunchKernelGGL(kern, grid, block, lds, h->stream, a, g, twx_of<double>(h), 1);
     HIPCHECK(hipGetLastError());
@@ -3018,7 +3038,9 @@ void create_impl(dgpu_handle* h, const double* gy) {
         (const void*)k_xfwd<double, true, false, FFT_XL512>, (const void*)k_xfwd<float, true, false, FFT_XL512>,
         (const void*)k_zfwd<double, true, FFT_XL512>,  (const void*)k_zfwd<float, true, FFT_XL512>,
         (const void*)k_zinv_op<double, true, FFT_XL512>, (const void*)k_zinv_op<float, true, FFT_XL512>,
-        (const void*)k_adj_xpairs<true, false, FFT_XL512>,
+        (const void*)k_adj_xpairs<true, false, FFT_XL512>, (const void*)k_adj_xpairs<true, false, FFT_XL512P>,
+        (const void*)k_xprod_all<double, true, FFT_XL512P, false, true>, (const void*)k_xinv<double, true, FFT_XL512P>,
+        (const void*)k_xfwd<double, true, false, FFT_XL512P>,
         (const void*)k_xprod_all<double, true, 0, true, true>, (const void*)k_xprod_all<double, false, 0, true, true>,
         (const void*)k_zgen_fwd<double>, (const void*)k_zgen_inv<double>,
         (const void*)k_zgen_fwd<float>,  (const void*)k_zgen_inv<float>};

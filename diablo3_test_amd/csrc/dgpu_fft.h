@@ -381,9 +381,27 @@ __device__ __forceinline__ void xl_transpose_hi(cpx<R> (&v)[8]) {
   }
 }
 
+// v[k] *= w^k for k = 1..7, with w^k formed by complex products from w
+// (w^2 = w w, w^4 = w^2 w^2, w^3, w^5, w^6, w^7 from those): one table read
+// instead of seven.
+template <typename R>
+__device__ __forceinline__ void twiddle_powers(cpx<R> (&v)[8], cpx<R> w1) {
+  const cpx<R> w2 = w1 * w1, w4 = w2 * w2;
+  const cpx<R> w3 = w2 * w1, w5 = w4 * w1, w6 = w4 * w2, w7 = w4 * w3;
+  v[1] = v[1] * w1;
+  v[2] = v[2] * w2;
+  v[3] = v[3] * w3;
+  v[4] = v[4] * w4;
+  v[5] = v[5] * w5;
+  v[6] = v[6] * w6;
+  v[7] = v[7] * w7;
+}
+
 // The transform: the contract of fft_regs_c with N = 512, one group per
-// wavefront (gt = lane), tw[t] = exp(-2 pi i t / 512).
-template <typename R, int S>
+// wavefront (gt = lane), tw[t] = exp(-2 pi i t / 512).  TWP: the twiddles of
+// a step are formed as powers of one table entry (twiddle_powers) instead of
+// seven table reads.
+template <typename R, int S, bool TWP = false>
 __device__ __forceinline__ void fft512_xl(cpx<R> (&v)[8], cpx<R>* buf, const cpx<R>* __restrict__ tw, int gt) {
   auto twid = [&](int t) {
     cpx<R> c = tw[t & 511];
@@ -391,13 +409,21 @@ __device__ __forceinline__ void fft512_xl(cpx<R> (&v)[8], cpx<R>* buf, const cpx
     return c;
   };
   dft8<R, S>(v);  // step A: v[k2]
+  if constexpr (TWP) {
+    twiddle_powers(v, twid(gt));
+  } else {
 #pragma unroll
-  for (int k = 1; k < 8; ++k) v[k] = v[k] * twid(gt * k);
+    for (int k = 1; k < 8; ++k) v[k] = v[k] * twid(gt * k);
+  }
   xl_transpose_hi(v);  // lane bits 3-5 = k2, v[n1]
   dft8<R, S>(v);       // step B: v[k1]
   const int n0 = gt & 7, hi = gt >> 3;
+  if constexpr (TWP) {
+    twiddle_powers(v, twid(8 * n0));
+  } else {
 #pragma unroll
-  for (int k = 1; k < 8; ++k) v[k] = v[k] * twid(8 * n0 * k);
+    for (int k = 1; k < 8; ++k) v[k] = v[k] * twid(8 * n0 * k);
+  }
   // low lane bits <-> register through LDS, element n0 + 8 k1 + 64 k2
   fft_sync<true>();  // the previous use of buf is finished
   const int pw = pidx(n0 + 64 * hi);
@@ -415,16 +441,19 @@ __device__ __forceinline__ void fft512_xl(cpx<R> (&v)[8], cpx<R>* buf, const cpx
   fft_sync<true>();
 }
 
-// The LOGN of fft_any that selects fft512_xl (N = 512).
-constexpr int FFT_XL512 = 100;
+// The LOGN of fft_any that selects fft512_xl (N = 512), with its twiddles
+// read from the table (FFT_XL512) or formed as powers (FFT_XL512P).
+constexpr int FFT_XL512 = 100, FFT_XL512P = 101;
 
 // fft_regs, or fft_regs_c when LOGN > 0 fixes N = 2^LOGN at compile time, or
-// fft512_xl for LOGN = FFT_XL512.
+// fft512_xl for LOGN = FFT_XL512 or FFT_XL512P.
 template <typename R, int S, bool WAVE, int LOGN>
 __device__ __forceinline__ void fft_any(cpx<R> (&v)[8], cpx<R>* buf, int N, int logN,
                                         const cpx<R>* __restrict__ tw, int gt) {
   if constexpr (LOGN == FFT_XL512)
     fft512_xl<R, S>(v, buf, tw, gt);
+  else if constexpr (LOGN == FFT_XL512P)
+    fft512_xl<R, S, true>(v, buf, tw, gt);
   else if constexpr (LOGN > 0)
     fft_regs_c<R, S, WAVE, LOGN>(v, buf, tw, gt);
   else
