@@ -198,7 +198,7 @@ struct dgpu_handle {
   // Crank-Nicolson factor tables: [field][rk] rows, field 0 u1, 1 u2, 2 u3, 3+n th_n
   DevBuf<SegRow> rows;
   // LES (allocated when use_les): velocity/scalar gradients, model outputs,
-  // their spectra, the subgrid forcing, the pre-BC state copy, CN scratch
+  // their spectra, the subgrid forcing, the wall planes of the pre-BC state, CN scratch
   DevBuf<TriRow> trirows;              // raw CN rows [field][rk][j]
   DevBuf<double> grad[9];
   DevBuf<double> dth[3][DGPU_MAX_TH];
@@ -207,7 +207,7 @@ struct dgpu_handle {
   DevBuf<double2> ctau[7];
   DevBuf<double2> cthf[2][DGPU_MAX_TH];
   DevBuf<double2> lesf[3], lesfth[DGPU_MAX_TH];
-  DevBuf<double2> cu_pre[3], cth_pre[DGPU_MAX_TH];
+  DevBuf<double2> wpre[3 + DGPU_MAX_TH];  // planes 0, 1, Ny, Ny+1 before the LES wall values
   DevBuf<double> sbl, sgl;              // b', g' of the per-column CN solve
   // adjoint (allocated by the first adjoint call): the adjoint state, its
   // scratch, the transposed CN factor tables, and forward-state checkpoints
@@ -1747,18 +1747,21 @@ void rk_substep_t(dgpu_handle* h, int rk_step, double time, int time_step) {
   // forms), and cr never goes to physical space nor the solution back.
   const bool spec = h->kn.spectral_cn && !p.use_les;
   // The explicit spectral terms (channel.f90:39-253) read the state before
-  // les_chan applies its wall values, so the LES path keeps that copy.
-  const double2* lin_cu[3] = {h->cu[0].p, h->cu[1].p, h->cu[2].p};
-  const double2* lin_cth[DGPU_MAX_TH] = {nullptr, nullptr, nullptr, nullptr};
-  for (int n = 0; n < nth; ++n) lin_cth[n] = h->cth[n].p;
+  // les_chan applies its wall values.  Those change planes 0, 1, Ny, Ny+1
+  // only, so the LES path keeps a copy of those planes for the RHS pass.
   if (les) {
-    for (int f = 0; f < 3; ++f) {
-      copy_spec(h, h->cu_pre[f].p, h->cu[f].p);
-      lin_cu[f] = h->cu_pre[f].p;
+    WallCopy w;
+    std::memset(&w, 0, sizeof(w));
+    for (int f = 0; f < 3 + nth; ++f) {
+      w.src[f] = f < 3 ? h->cu[f].p : h->cth[f - 3].p;
+      w.dst[f] = h->wpre[f].p;
     }
-    for (int n = 0; n < nth; ++n) {
-      copy_spec(h, h->cth_pre[n].p, h->cth[n].p);
-      lin_cth[n] = h->cth_pre[n].p;
+    w.nf = 3 + nth;
+    {
+      Timed tm(h, DGPU_K_COPY, 32.0 * 4 * g.sp * w.nf);
+      hipLaunchKernelGGL(k_copy_walls, dim3((unsigned)((g.sp + 255) / 256), 4, (unsigned)w.nf), dim3(256), 0,
+                         h->stream, w, g);
+      HIPCHECK(hipGetLastError());
     }
     les_stage<R>(h, time);
     if (h->les_cap) {  // the adjoint's recomputation keeps the post-BC state
@@ -1845,9 +1848,15 @@ void rk_substep_t(dgpu_handle* h, int rk_step, double time, int time_step) {
     std::memset(&a, 0, sizeof(a));
     for (int s = 0; s < nslot; ++s) a.X[s] = h->tpool[s].p;
     for (int c = 0; c < ncr; ++c) a.Tcr[c] = h->tpool[nslot + c].p;
-    a.cu1 = lin_cu[0];
-    a.cu2 = lin_cu[1];
-    a.cu3 = lin_cu[2];
+    a.cu1 = h->cu[0].p;
+    a.cu2 = h->cu[1].p;
+    a.cu3 = h->cu[2].p;
+    if (les) {
+      a.wcu1 = h->wpre[0].p;
+      a.wcu2 = h->wpre[1].p;
+      a.wcu3 = h->wpre[2].p;
+      for (int n = 0; n < nth; ++n) a.wcth[n] = h->wpre[3 + n].p;
+    }
     a.cp = h->cp.p;
     a.incr = (!incr || spec) ? 0 : (les ? 1 : 2);
     if (spec)  // cr in the (otherwise idle) cr x-spectra buffers, at plane pitch sp
@@ -1869,7 +1878,7 @@ void rk_substep_t(dgpu_handle* h, int rk_step, double time, int time_step) {
     a.cf2 = h->cf[1].p;
     a.cf3 = h->cf[2].p;
     for (int n = 0; n < nth; ++n) {
-      a.cth[n] = lin_cth[n];
+      a.cth[n] = h->cth[n].p;
       a.cthb[n] = h->cth[n].p;
       a.cfth[n] = h->cfth[n].p;
       a.pr[n] = p.pr[n];
@@ -2933,7 +2942,7 @@ void free_all(dgpu_handle* h) {
   for (auto& b : h->tau) b.release();
   for (auto& b : h->ctau) b.release();
   for (auto& b : h->lesf) b.release();
-  for (auto& b : h->cu_pre) b.release();
+  for (auto& b : h->wpre) b.release();
   for (int n = 0; n < DGPU_MAX_TH; ++n) {
     for (int w = 0; w < 3; ++w) h->dth[w][n].release();
     for (int w = 0; w < 2; ++w) {
@@ -2941,7 +2950,6 @@ void free_all(dgpu_handle* h) {
       h->cthf[w][n].release();
     }
     h->lesfth[n].release();
-    h->cth_pre[n].release();
   }
   h->sbl.release();
   h->sgl.release();
@@ -3136,7 +3144,7 @@ void create_impl(dgpu_handle* h, const double* gy) {
     }
     for (int f = 0; f < 3; ++f) {
       h->lesf[f].alloc(ns);
-      h->cu_pre[f].alloc(ns);
+      h->wpre[f].alloc((size_t)4 * g.sp);
     }
     for (int nn = 0; nn < h->nth; ++nn) {
       for (int w = 0; w < 3; ++w) h->dth[w][nn].alloc(np);
@@ -3145,7 +3153,7 @@ void create_impl(dgpu_handle* h, const double* gy) {
         h->cthf[w][nn].alloc(ns);
       }
       h->lesfth[nn].alloc(ns);
-      h->cth_pre[nn].alloc(ns);
+      h->wpre[3 + nn].alloc((size_t)4 * g.sp);
     }
     // b', g' of the LES CN solves, one region per field (the fields of a
     // substep are solved in one launch)

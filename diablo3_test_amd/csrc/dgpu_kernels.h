@@ -666,6 +666,14 @@ struct RhsArgs {
   const double2* cu3;
   const double2* cp;
   const double2* cth[MAXTH];
+  // With LES: planes 0, 1, Ny, Ny+1 of cu1..3, cth as they were before the
+  // LES stage's wall values (4 planes each, pitch sp); the explicit terms read
+  // those planes from here and the others from cu*, cth, which the LES stage
+  // changes only on those planes.  Null otherwise.
+  const double2* wcu1;
+  const double2* wcu2;
+  const double2* wcu3;
+  const double2* wcth[MAXTH];
   // incr: cr is written less the state that was transformed to physical space
   // (cub*), so that only the O(h) increment passes through the transform
   int incr;  // 0: cr; 1: cr less cub*; 2: cr less the state read (cub* == cu*)
@@ -805,23 +813,33 @@ __global__ void __launch_bounds__(512) k_rhs(RhsArgs a, Geo g, const cpx<R>* __r
       return C{(double)z.x, (double)z.y};
     };
     auto stcf = [&](double2* p, C v) { reinterpret_cast<CR*>(p)[o] = CR{(R)v.x, (R)v.y}; };
+    // the state of a field at plane j + d before any LES wall values (the
+    // plane is uniform across the block)
+    auto st = [&](const double2* full, const double2* wall, int d) -> C {
+      const int pl = j + d;
+      if (wall && (pl <= 1 || pl >= g.ny)) {
+        const int slot = pl <= 1 ? pl : pl - g.ny + 2;
+        return reinterpret_cast<const C*>(wall)[(long long)slot * g.sp + (o - (long long)j * g.sp)];
+      }
+      return reinterpret_cast<const C*>(full)[o + (long long)d * g.sp];
+    };
     C cu1 = {0, 0}, cu3 = {0, 0};
     if (in1 || a.th1 > a.th0) {
-      cu1 = reinterpret_cast<const C*>(a.cu1)[o];
-      cu3 = reinterpret_cast<const C*>(a.cu3)[o];
+      cu1 = st(a.cu1, a.wcu1, 0);
+      cu3 = st(a.cu3, a.wcu3, 0);
     }
     if (in1) {
       C cp = reinterpret_cast<const C*>(a.cp)[o];
       C f1 = (-a.nu * visc) * cu1 + a.coriolis * cu3;
       C f3 = (-a.nu * visc) * cu3 - a.coriolis * cu1;
       for (int n = 0; n < a.nth; ++n) {
-        C th = reinterpret_cast<const C*>(a.cth[n])[o];
+        C th = st(a.cth[n], a.wcth[n], 0);
         f1 = f1 + a.grav_x * th;
         f3 = f3 + a.grav_z * th;
       }
       if (a.fr_yf) {  // forcing.f90:55-72
         const double cy = a.fr_yf[j];
-        const C u2m = 0.5 * (ld(a.cu2) + reinterpret_cast<const C*>(a.cu2)[o + g.sp]);
+        const C u2m = 0.5 * (st(a.cu2, a.wcu2, 0) + st(a.cu2, a.wcu2, 1));
         f1 = f1 - (a.fr_sx * cy) * ik(kzk, cu1) + (a.fr_sz * cy) * ik(kxq, cu1) + (a.fr_sz * a.fr_dr) * u2m;
         f3 = f3 - (a.fr_sx * cy) * ik(kzk, cu3) + (a.fr_sz * cy) * ik(kxq, cu3) - (a.fr_sx * a.fr_dr) * u2m;
       }
@@ -854,14 +872,14 @@ __global__ void __launch_bounds__(512) k_rhs(RhsArgs a, Geo g, const cpx<R>* __r
       put(1, r3);
     }
     if (in2) {
-      C cu2 = reinterpret_cast<const C*>(a.cu2)[o];
+      C cu2 = st(a.cu2, a.wcu2, 0);
       C cpj = reinterpret_cast<const C*>(a.cp)[o];
       C cpm = reinterpret_cast<const C*>(a.cp)[o - g.sp];
       C f2 = (-a.nu * visc) * cu2;
       if (!mean) {
         for (int n = 0; n < a.nth; ++n) {
-          C th = reinterpret_cast<const C*>(a.cth[n])[o];
-          C thm = reinterpret_cast<const C*>(a.cth[n])[o - g.sp];
+          C th = st(a.cth[n], a.wcth[n], 0);
+          C thm = st(a.cth[n], a.wcth[n], -1);
           f2 = f2 + (a.grav_y / (2.0 * cdy)) * (cdyfm * th + cdyf * thm);
         }
       }
@@ -882,7 +900,7 @@ __global__ void __launch_bounds__(512) k_rhs(RhsArgs a, Geo g, const cpx<R>* __r
     }
     for (int n = a.th0; n < a.th1; ++n) {
       if (!((inT >> n) & 1u)) continue;
-      C th = reinterpret_cast<const C*>(a.cth[n])[o];
+      C th = st(a.cth[n], a.wcth[n], 0);
       C ft = (-(a.nu / a.pr[n]) * visc) * th;
       ft = ft - a.dthdx[n] * cu1 - a.dthdz[n] * cu3;
       if (a.fr_yf) {  // forcing.f90:87-96
@@ -1514,6 +1532,21 @@ __global__ void __launch_bounds__(256) k_copy2(double2* __restrict__ dst, const 
                                                long long n) {
   const long long stride = (long long)gridDim.x * blockDim.x;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+// Planes 0, 1, Ny, Ny+1 of nf spectral fields into 4-plane copies (the
+// state before the LES stage's wall values, read back by the RHS pass).
+struct WallCopy {
+  const double2* src[3 + MAXTH];
+  double2* dst[3 + MAXTH];
+  int nf;
+};
+__global__ void __launch_bounds__(256) k_copy_walls(WallCopy w, Geo g) {
+  const long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= g.sp) return;
+  const int slot = blockIdx.y, f = blockIdx.z;
+  const int pl = slot <= 1 ? slot : g.ny - 2 + slot;
+  w.dst[f][(long long)slot * g.sp + m] = w.src[f][(long long)pl * g.sp + m];
 }
 
 // ---------------------------------------------------------- wall BCs ----

@@ -296,7 +296,7 @@ int dgpu_checkpoint(dgpu_handle* h, int slot, int mode);
 #define DGPU_K_BCPOST 8   /* wall boundary values */
 #define DGPU_K_LES 9      /* subgrid model pointwise kernels and forcing */
 #define DGPU_K_ADJOINT 10 /* adjoint-only kernels (projection, CN post, RHS, products) */
-#define DGPU_K_COPY 11    /* device copies of the state (checkpoints, LES pre-BC copy) */
+#define DGPU_K_COPY 11    /* device copies of the state (checkpoints, LES pre-BC wall planes) */
 #define DGPU_K_INIT 12    /* Poisson right-hand side and solve, CFL reductions */
 #define DGPU_NCLASS 13
 int dgpu_timing_enable(dgpu_handle* h, int enable);
