@@ -93,9 +93,10 @@ struct Knobs {
   // FP64 transform passes over 512 points with the length fixed at compile
   // time (fft_regs_c); 0 = the run-time length everywhere.
   int fft_const = 1;
-  // FP64 products pass with each input phase's loads issued behind the
-  // previous pair's products, before its transform (0 = at the phase start).
-  int xprod_pf = 1;
+  // Products pass with each input phase's loads issued behind the previous
+  // pair's products, before its transform: bit 1 FP64, bit 2 FP32 (the FP32
+  // pass with its 512-point cross-lane transform; 0 = at the phase start).
+  int xprod_pf = 3;
   // 512-point passes that run fft512_xl (the cross-lane exchange), a bit per
   // pass (XL_* below): the products, x and z inverse and x and z forward
   // passes in FP64 and FP32 and the adjoint's x pass (1 + 4 + 8 + 16 + 32 +
@@ -1818,11 +1819,12 @@ void rk_substep_t(dgpu_handle* h, int rk_step, double time, int time_step) {
       if (g.nx == 512 && h->kn.fft_const) kern = k_xprod_all<R, true, 9>;
     if (g.nz % KT) kern = g.nx <= 512 ? k_xprod_all<R, true, 0, true> : k_xprod_all<R, false, 0, true>;
     // FP64 loads the next input phase behind each pair (5.53 -> 5.37 ms per
-    // step at 512x257x512, still two waves per SIMD at 226 VGPRs); the FP32
-    // pass keeps the phase-start loads: its 145 VGPRs with them cost a wave
-    // per SIMD (4 -> 3) and it went from 2.85 to 3.30 ms
+    // step at 512x257x512, still two waves per SIMD at 226 VGPRs).  With the
+    // Stockham transform the FP32 pass keeps the phase-start loads: its 145
+    // VGPRs with them cost a wave per SIMD (4 -> 3), 2.85 -> 3.30 ms; with the
+    // cross-lane transform it fits 128 VGPRs and gains (below)
     if constexpr (sizeof(R) == 8)
-      if (h->kn.xprod_pf) {
+      if (h->kn.xprod_pf & 1) {
         kern = g.nx <= 512 ? k_xprod_all<R, true, 0, false, true> : k_xprod_all<R, false, 0, false, true>;
         if (g.nx == 512 && h->kn.fft_const) kern = k_xprod_all<R, true, 9, false, true>;
         if (g.nz % KT)
@@ -1830,13 +1832,14 @@ void rk_substep_t(dgpu_handle* h, int rk_step, double time, int time_step) {
       }
     if (g.nx == 512 && g.nz % KT == 0 && use_xl<R>(h, XL_PROD)) {
       if constexpr (sizeof(R) == 8) {
+        const bool pf = h->kn.xprod_pf & 1;
         if (xl_powers<R>(h, XL_PROD))
-          kern = h->kn.xprod_pf ? k_xprod_all<R, true, FFT_XL512P, false, true> : k_xprod_all<R, true, FFT_XL512P>;
+          kern = pf ? k_xprod_all<R, true, FFT_XL512P, false, true> : k_xprod_all<R, true, FFT_XL512P>;
         else
-          kern = h->kn.xprod_pf ? k_xprod_all<R, true, FFT_XL512, false, true> : k_xprod_all<R, true, FFT_XL512>;
+          kern = pf ? k_xprod_all<R, true, FFT_XL512, false, true> : k_xprod_all<R, true, FFT_XL512>;
       }
-      else
-        kern = k_xprod_all<R, true, FFT_XL512>;
+      else  // 128 VGPRs with the early loads, four waves per SIMD: 2.61 -> 2.32 ms per step
+        kern = (h->kn.xprod_pf & 2) ? k_xprod_all<R, true, FFT_XL512, false, true> : k_xprod_all<R, true, FFT_XL512>;
     }
     hipLaunchKernelGGL(kern, grid, block, lds, h->stream, pa, g, twx_of<R>(h), 1);
     HIPCHECK(hipGetLastError());
@@ -3048,6 +3051,7 @@ void create_impl(dgpu_handle* h, const double* gy) {
         (const void*)k_zinv_op<double, true, FFT_XL512>, (const void*)k_zinv_op<float, true, FFT_XL512>,
         (const void*)k_adj_xpairs<true, false, FFT_XL512>, (const void*)k_adj_xpairs<true, false, FFT_XL512P>,
         (const void*)k_xprod_all<double, true, FFT_XL512P, false, true>, (const void*)k_xinv<double, true, FFT_XL512P>,
+        (const void*)k_xprod_all<float, true, FFT_XL512, false, true>,
         (const void*)k_xfwd<double, true, false, FFT_XL512P>,
         (const void*)k_xprod_all<double, true, 0, true, true>, (const void*)k_xprod_all<double, false, 0, true, true>,
         (const void*)k_zgen_fwd<double>, (const void*)k_zgen_inv<double>,
