@@ -90,9 +90,11 @@ struct Knobs {
   // default: at 512x257x512 every batch size measured slower than one launch
   // per pass over all planes.
   int batch_mb = 0;
-  // FP64 transform passes over 512 points with the length fixed at compile
-  // time (fft_regs_c); 0 = the run-time length everywhere.
-  int fft_const = 1;
+  // Transform passes over 512 points with the length fixed at compile time
+  // (fft_regs_c): bit 1 the FP64 passes, bit 2 the FP32 RHS pass (measured
+  // at 512x257x512: 3.64 -> 3.58 ms per step); 0 = the run-time length
+  // everywhere.
+  int fft_const = 3;
   // Products pass with each input phase's loads issued behind the previous
   // pair's products, before its transform: bit 1 FP64, bit 2 FP32 (the FP32
   // pass with its 512-point cross-lane transform; 0 = at the phase start).
@@ -701,7 +703,7 @@ void launch_xinv(dgpu_handle* h, const cpx<R>* T, R* u, int j0, int j1, const do
            2.0 * sizeof(R) * g.tp * (j1 - j0 + 1) * nf);
   auto kern = g.nx <= 512 ? k_xinv<R, true> : k_xinv<R, false>;
   if constexpr (sizeof(R) == 8)
-    if (g.nx == 512 && h->kn.fft_const) kern = k_xinv<R, true, 9>;
+    if (g.nx == 512 && (h->kn.fft_const & 1)) kern = k_xinv<R, true, 9>;
   if (g.nx == 512 && use_xl<R>(h, XL_XINV))
     kern = xl_powers<R>(h, XL_XINV) ? k_xinv<R, true, FFT_XL512P> : k_xinv<R, true, FFT_XL512>;
   if (g.nz % KT) kern = g.nx <= 512 ? k_xinv<R, true, 0, true> : k_xinv<R, false, 0, true>;  // ragged last tile
@@ -805,7 +807,7 @@ void launch_zinv(dgpu_handle* h, const double2* C, cpx<R>* T, int j0, int j1, do
   dim3 grid((g.nq + ng - 1) / ng, j1 - j0 + 1, nf), block(ng * (g.nz / 8));
   auto kern = g.nz <= 512 ? k_zinv<R, true> : k_zinv<R, false>;
   if constexpr (sizeof(R) == 8)
-    if (g.nz == 512 && h->kn.fft_const) kern = k_zinv<R, true, 9>;
+    if (g.nz == 512 && (h->kn.fft_const & 1)) kern = k_zinv<R, true, 9>;
   if (g.nz == 512 && use_xl<R>(h, XL_ZINV)) kern = k_zinv<R, true, FFT_XL512>;
   hipLaunchKernelGGL(kern, grid, block, lds, h->stream, reinterpret_cast<const cpx<double>*>(C), T, g, twz_of<R>(h),
                      j0, s0, s1, fb ? *fb : FieldBatch{});
@@ -1696,7 +1698,11 @@ std::vector<RhsPart> rhs_plan(const dgpu_handle* h, int nth) {
     const size_t buf = (size_t)rounds * ng * L * sizeof(cpx<R>);
     const size_t tab = (size_t)g.nz * sizeof(cpx<R>);
     if (buf > h->lds_limit) return false;
-    out = RhsPart{part, th0, th1, buf + tab <= h->lds_limit ? 1 : 0, ng, buf + tab <= h->lds_limit ? buf + tab : buf};
+    // the table in LDS even where it leaves room for one workgroup per CU:
+    // read from the caches instead (two FP64 workgroups per CU at 512 points)
+    // the pass measured 4.37 -> 4.64 ms per step at 512x257x512
+    const bool twl = buf + tab <= h->lds_limit;
+    out = RhsPart{part, th0, th1, twl ? 1 : 0, ng, twl ? buf + tab : buf};
     return true;
   };
   std::vector<RhsPart> plan;
@@ -1816,7 +1822,7 @@ void rk_substep_t(dgpu_handle* h, int rk_step, double time, int time_step) {
     // 3% slower with it, 2.78 -> 2.87 ms, and keeps the run-time one)
     auto kern = g.nx <= 512 ? k_xprod_all<R, true> : k_xprod_all<R, false>;
     if constexpr (sizeof(R) == 8)
-      if (g.nx == 512 && h->kn.fft_const) kern = k_xprod_all<R, true, 9>;
+      if (g.nx == 512 && (h->kn.fft_const & 1)) kern = k_xprod_all<R, true, 9>;
     if (g.nz % KT) kern = g.nx <= 512 ? k_xprod_all<R, true, 0, true> : k_xprod_all<R, false, 0, true>;
     // FP64 loads the next input phase behind each pair (5.53 -> 5.37 ms per
     // step at 512x257x512, still two waves per SIMD at 226 VGPRs).  With the
@@ -1826,7 +1832,7 @@ void rk_substep_t(dgpu_handle* h, int rk_step, double time, int time_step) {
     if constexpr (sizeof(R) == 8)
       if (h->kn.xprod_pf & 1) {
         kern = g.nx <= 512 ? k_xprod_all<R, true, 0, false, true> : k_xprod_all<R, false, 0, false, true>;
-        if (g.nx == 512 && h->kn.fft_const) kern = k_xprod_all<R, true, 9, false, true>;
+        if (g.nx == 512 && (h->kn.fft_const & 1)) kern = k_xprod_all<R, true, 9, false, true>;
         if (g.nz % KT)
           kern = g.nx <= 512 ? k_xprod_all<R, true, 0, true, true> : k_xprod_all<R, false, 0, true, true>;
       }
@@ -1915,8 +1921,7 @@ void rk_substep_t(dgpu_handle* h, int rk_step, double time, int time_step) {
     a.js = h->jstart;
     a.je = h->jend;
     auto kern = g.nz <= 512 ? k_rhs<R, true> : k_rhs<R, false>;
-    if constexpr (sizeof(R) == 8)
-      if (g.nz == 512 && h->kn.fft_const) kern = k_rhs<R, true, 9>;
+    if (g.nz == 512 && (h->kn.fft_const & (sizeof(R) == 8 ? 1 : 2))) kern = k_rhs<R, true, 9>;
     if (g.nz == 512 && use_xl<R>(h, XL_RHS)) kern = k_rhs<R, true, FFT_XL512>;
     // x-spectra of the products in, cr x-spectra out; cu1..3, cp, cth, cf1..3,
     // cfth read and cf1..3, cfth written (neighbour planes come from cache)
@@ -3024,6 +3029,7 @@ void create_impl(dgpu_handle* h, const double* gy) {
         (const void*)k_xprod_all<double, true>, (const void*)k_xprod_all<double, false>,
         (const void*)k_xprod_all<double, true, 9>, (const void*)k_rhs<double, true, 9>,
         (const void*)k_xinv<double, true, 9>,     (const void*)k_zinv<double, true, 9>,
+        (const void*)k_rhs<float, true, 9>,
         (const void*)k_rhs<float, true>,      (const void*)k_rhs<float, false>,
         (const void*)k_xfwd<float, true>,     (const void*)k_xfwd<float, false>,
         (const void*)k_xinv<float, true>,     (const void*)k_xinv<float, false>,
