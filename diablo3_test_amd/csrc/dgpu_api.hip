@@ -1698,9 +1698,10 @@ std::vector<RhsPart> rhs_plan(const dgpu_handle* h, int nth) {
     const size_t buf = (size_t)rounds * ng * L * sizeof(cpx<R>);
     const size_t tab = (size_t)g.nz * sizeof(cpx<R>);
     if (buf > h->lds_limit) return false;
-    // the table in LDS even where it leaves room for one workgroup per CU:
-    // read from the caches instead (two FP64 workgroups per CU at 512 points)
-    // the pass measured 4.37 -> 4.64 ms per step at 512x257x512
+    // the table goes to LDS whenever it fits: at 512 points in FP64 the slot
+    // buffers and the table take 80 KB, so two workgroups still share a CU
+    // (as their 128 VGPRs allow), and the pass read its twiddles from the
+    // caches at 4.64 ms per step against 4.37 from LDS (512x257x512)
     const bool twl = buf + tab <= h->lds_limit;
     out = RhsPart{part, th0, th1, twl ? 1 : 0, ng, twl ? buf + tab : buf};
     return true;
@@ -3138,6 +3139,10 @@ void create_impl(dgpu_handle* h, const double* gy) {
   }
   setup_projection(h);
 
+  // the x passes and the products pass are compiled for 512-lane workgroups
+  for (int t : {h->kn.xpass_threads, h->kn.xprod_threads})
+    if (t < 64 || t > 512)
+      throw ApiError{DGPU_EINVAL, "DGPU_XPASS_THREADS and DGPU_XPROD_THREADS must lie in [64, 512]"};
   // CN factor tables (DGPU_SEG_TARGET overrides the target segment length)
   if (h->kn.seg_target < 1) throw ApiError{DGPU_EINVAL, "DGPU_SEG_TARGET must be a positive row count"};
   const int n = g.ny2;

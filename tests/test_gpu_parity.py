@@ -215,6 +215,18 @@ def test_knobs_are_per_handle(monkeypatch):
     assert xfwd_launches(spec) == 0
 
 
+@pytest.mark.parametrize("name,value", [("DGPU_XPASS_THREADS", "1024"), ("DGPU_XPROD_THREADS", "32"),
+                                        ("DGPU_SEG_TARGET", "0")])
+def test_knobs_out_of_range_rejected(monkeypatch, name, value):
+    """A knob outside what the kernels are compiled for (workgroups of at most
+    512 lanes in the x and products passes, a positive segment length) is
+    refused at handle creation instead of failing at the first launch."""
+    p, gy, *_ = workloads.workload("small_32x33x32")
+    monkeypatch.setenv(name, value)
+    with pytest.raises(DgpuError):
+        gpu_solver(p, gy)
+
+
 def test_rectangular_box_and_ny():
     p = workloads.channel_params(64, 41, 16, 300.0, ri=0.2, dt=2e-3)
     gy = workloads.channel_grid(41)
