@@ -72,9 +72,15 @@ struct Knobs {
   // kernel runs 4 waves per SIMD, so a CU holds one block, and 13 waves fill
   // more of its 16 slots.
   int seg_target = 20;
-  // Threads per workgroup of the x passes and of the products pass.
+  // Threads per workgroup of the x passes and of the products pass (FP64,
+  // FP32).  The FP64 products pass holds 228 VGPRs, two waves per SIMD: as
+  // two 256-thread workgroups per CU (each with 4 of the 8 rows of its tile,
+  // in two rounds) it measured 4.79 -> 4.71 ms per step at 512x257x512
+  // against one 512-thread group; the FP32 pass, whose 16-row tiles would
+  // then be written as 32-byte runs, keeps 512 (2.30 ms, 6.67 with 256).
   int xpass_threads = 512;
-  int xprod_threads = 512;
+  int xprod_threads = 256;
+  int xprod_threads32 = 512;
   // Columns per lane of the LES CN solve.
   int les_cols = 2;
   // Columns per lane of the FP32-storage CN solve (1 or 2).  Measured at
@@ -124,6 +130,7 @@ Knobs knobs_from_env() {
   rd("DGPU_SEG_TARGET", k.seg_target);
   rd("DGPU_XPASS_THREADS", k.xpass_threads);
   rd("DGPU_XPROD_THREADS", k.xprod_threads);
+  rd("DGPU_XPROD_THREADS32", k.xprod_threads32);
   rd("DGPU_LES_COLS", k.les_cols);
   rd("DGPU_THOMAS_COLS", k.thomas_cols);
   rd("DGPU_SPECTRAL_CN", k.spectral_cn);
@@ -1811,7 +1818,8 @@ void rk_substep_t(dgpu_handle* h, int rk_step, double time, int time_step) {
     }
     for (int s = 0; s < nslot; ++s) pa.out[s] = h->tpool[s].p;
     constexpr int KT = XTile<R>::KT;
-    const int ngx = x_ngroups<R>(h, KT, h->kn.xprod_threads);
+    // a row's group needs Nx/8 lanes whatever the limit
+    const int ngx = x_ngroups<R>(h, KT, std::max(sizeof(R) == 8 ? h->kn.xprod_threads : h->kn.xprod_threads32, g.nx / 8));
     const size_t lds = x_lds<R>(h, ngx);
     if (lds > h->lds_limit) throw ApiError{DGPU_EUNSUP, "Nx too large for the products pass"};
     dim3 grid((g.nz + KT - 1) / KT, Ny), block(ngx * (g.nx / 8));
@@ -3140,9 +3148,9 @@ void create_impl(dgpu_handle* h, const double* gy) {
   setup_projection(h);
 
   // the x passes and the products pass are compiled for 512-lane workgroups
-  for (int t : {h->kn.xpass_threads, h->kn.xprod_threads})
+  for (int t : {h->kn.xpass_threads, h->kn.xprod_threads, h->kn.xprod_threads32})
     if (t < 64 || t > 512)
-      throw ApiError{DGPU_EINVAL, "DGPU_XPASS_THREADS and DGPU_XPROD_THREADS must lie in [64, 512]"};
+      throw ApiError{DGPU_EINVAL, "DGPU_XPASS_THREADS and DGPU_XPROD_THREADS(32) must lie in [64, 512]"};
   // CN factor tables (DGPU_SEG_TARGET overrides the target segment length)
   if (h->kn.seg_target < 1) throw ApiError{DGPU_EINVAL, "DGPU_SEG_TARGET must be a positive row count"};
   const int n = g.ny2;

@@ -216,7 +216,7 @@ def test_knobs_are_per_handle(monkeypatch):
 
 
 @pytest.mark.parametrize("name,value", [("DGPU_XPASS_THREADS", "1024"), ("DGPU_XPROD_THREADS", "32"),
-                                        ("DGPU_SEG_TARGET", "0")])
+                                        ("DGPU_XPROD_THREADS32", "1024"), ("DGPU_SEG_TARGET", "0")])
 def test_knobs_out_of_range_rejected(monkeypatch, name, value):
     """A knob outside what the kernels are compiled for (workgroups of at most
     512 lanes in the x and products passes, a positive segment length) is
