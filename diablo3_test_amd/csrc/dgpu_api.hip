@@ -908,6 +908,22 @@ void to_physical_fields(dgpu_handle* h, int nf, const double2* const* C, R* cons
   launch_xinv<R>(h, T[0], u[0], 0, n2 - 1, nullptr, &xb);
 }
 
+// x inverse of several fields' x-spectra (T[k] -> u[k], planes [j0, j1], each
+// times i kx first with dkx), up to MAXBF fields a launch: one launch moves a
+// batch at the rate of the whole-state pass instead of a field's at a time
+template <typename R>
+void xinv_fields(dgpu_handle* h, int nf, cpx<R>* const* T, R* const* u, int j0, int j1, const double* dkx = nullptr) {
+  for (int k0 = 0; k0 < nf; k0 += MAXBF) {
+    FieldBatch xb{};
+    xb.nf = std::min(MAXBF, nf - k0);
+    for (int k = 0; k < xb.nf; ++k) {
+      xb.in[k] = T[k0 + k];
+      xb.out[k] = u[k0 + k];
+    }
+    launch_xinv<R>(h, T[k0], u[k0], j0, j1, dkx, &xb);
+  }
+}
+
 // physical -> spectral for one field on planes [j0, j1] (fft_xz_to_fourier,
 // fft.f90:247-315); with accum the transform is added to C
 template <typename R>
@@ -1220,23 +1236,33 @@ void les_stage(dgpu_handle* h, double time) {
   // derivatives: i kz inside the z inverse; the y differences (G12, G22,
   // G32, the scalars' T2) are formed from the physical state in k_les_model
   const bool share = h->kn.batch_mb <= 0;
-  T = xspec<R>(h->tpool[3 + nth]);  // scratch past the state's x-spectra
-  for (int m = 0; m < 9; ++m) {
-    if (gsrc[m].op == OP_DY_GY || gsrc[m].op == OP_DY_GYF) continue;
-    if (gsrc[m].op == OP_DX && share) {
-      launch_xinv<R>(h, xspec<R>(h->tpool[m / 3]), phys<R>(h->grad[m]), 0, n2 - 1, h->d_kx.p);
-      continue;
+  if (share) {
+    // the x derivatives of every field in one launch from the state's
+    // x-spectra; then the z derivatives' x-spectra reuse those buffers and
+    // go through one more batched x inverse
+    const int nd = 3 + nth;
+    cpx<R>* TX[3 + DGPU_MAX_TH];
+    R* DX[3 + DGPU_MAX_TH];
+    R* DZ[3 + DGPU_MAX_TH];
+    for (int f = 0; f < nd; ++f) {
+      TX[f] = xspec<R>(h->tpool[f]);
+      DX[f] = f < 3 ? phys<R>(h->grad[3 * f]) : phys<R>(h->dth[0][f - 3]);
+      DZ[f] = f < 3 ? phys<R>(h->grad[3 * f + 2]) : phys<R>(h->dth[2][f - 3]);
     }
+    xinv_fields<R>(h, nd, TX, DX, 0, n2 - 1, h->d_kx.p);
+    for (int f = 0; f < nd; ++f)
+      launch_zinv_op<R>(h, f < 3 ? h->cu[f].p : h->cth[f - 3].p, TX[f], 0, n2 - 1, OP_DZ);
+    xinv_fields<R>(h, nd, TX, DZ, 0, n2 - 1);
+  }
+  T = xspec<R>(h->tpool[3 + nth]);  // scratch past the state's x-spectra
+  for (int m = 0; !share && m < 9; ++m) {
+    if (gsrc[m].op == OP_DY_GY || gsrc[m].op == OP_DY_GYF) continue;
     launch_zinv_op<R>(h, gsrc[m].c, T, 0, n2 - 1, gsrc[m].op);
     launch_xinv<R>(h, T, phys<R>(h->grad[m]), 0, n2 - 1);
   }
-  for (int n = 0; n < nth; ++n) {
-    if (share) {
-      launch_xinv<R>(h, xspec<R>(h->tpool[3 + n]), phys<R>(h->dth[0][n]), 0, n2 - 1, h->d_kx.p);
-    } else {
-      launch_zinv_op<R>(h, h->cth[n].p, T, 0, n2 - 1, OP_DX);
-      launch_xinv<R>(h, T, phys<R>(h->dth[0][n]), 0, n2 - 1);
-    }
+  for (int n = 0; !share && n < nth; ++n) {
+    launch_zinv_op<R>(h, h->cth[n].p, T, 0, n2 - 1, OP_DX);
+    launch_xinv<R>(h, T, phys<R>(h->dth[0][n]), 0, n2 - 1);
     launch_zinv_op<R>(h, h->cth[n].p, T, 0, n2 - 1, OP_DZ);
     launch_xinv<R>(h, T, phys<R>(h->dth[2][n]), 0, n2 - 1);
   }
@@ -1988,11 +2014,13 @@ void rk_substep_t(dgpu_handle* h, int rk_step, double time, int time_step) {
   // 4. cr to physical space (second half of fft_xz_to_physical, :485-487, :621)
   // slot order of cr: 0 cr1, 1 cr3, 2 cr2, 3+n crth_n
   if (!spec) {
-    launch_xinv<R>(h, xspec<R>(h->tpool[nslot + 0]), phys<R>(h->r[0]), 1, Ny);
-    launch_xinv<R>(h, xspec<R>(h->tpool[nslot + 1]), phys<R>(h->r[2]), 1, Ny);
-    launch_xinv<R>(h, xspec<R>(h->tpool[nslot + 2]), phys<R>(h->r[1]), 1, Ny);
-    for (int n = 0; n < nth; ++n)
-      launch_xinv<R>(h, xspec<R>(h->tpool[nslot + 3 + n]), phys<R>(h->rth[n]), 1, Ny);
+    cpx<R>* TC[3 + DGPU_MAX_TH];
+    R* RC[3 + DGPU_MAX_TH];
+    for (int c = 0; c < 3 + nth; ++c) {
+      TC[c] = xspec<R>(h->tpool[nslot + c]);
+      RC[c] = c == 0 ? phys<R>(h->r[0]) : c == 1 ? phys<R>(h->r[2]) : c == 2 ? phys<R>(h->r[1]) : phys<R>(h->rth[c - 3]);
+    }
+    xinv_fields<R>(h, 3 + nth, TC, RC, 1, Ny);
   }
 
   // 5. implicit vertical solves (channel.f90:656-877)
