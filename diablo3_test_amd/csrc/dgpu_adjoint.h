@@ -548,6 +548,116 @@ __global__ void __launch_bounds__(256) k_adj_prod(AdjProdArgs a, Geo g) {
   }
 }
 
+// k_adj_prod marching up the planes: one thread per (x, z) point and a chunk
+// of J planes, with the j-1, j, j+1 values of every field the stencil reads
+// held in a register window.  Each value then crosses HBM once per chunk
+// (plus the chunk's two edge planes); with one thread per point the
+// neighbour planes mostly missed L2, and the pass fetched 16.2 GB a launch
+// against 8.7 GB of inputs at 512x257x512.  NT is the scalar count the
+// windows are compiled for (the launcher takes it when nth == NT); the sums
+// are those of k_adj_prod, term by term.
+template <int NT>
+__global__ void __launch_bounds__(256) k_adj_prod_march(AdjProdArgs a, Geo g, int J) {
+  const long long pp = g.pp;
+  const int n = g.ny2;
+  const long long xz = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (xz >= pp) return;
+  const int j0 = blockIdx.y * J, j1 = min(j0 + J, n);
+  struct W {
+    double m, c, p;  // planes j-1, j, j+1 (0 outside 0 .. n-1)
+  };
+  auto ld = [&](const double* f, int j) -> double { return (j >= 0 && j < n) ? f[(long long)j * pp + xz] : 0.0; };
+  auto win = [&](const double* f) -> W { return W{ld(f, j0 - 1), ld(f, j0), ld(f, j0 + 1)}; };
+  auto next = [&](W& w, const double* f, int j) {  // from plane j to j + 1
+    w.m = w.c;
+    w.c = w.p;
+    w.p = ld(f, j + 2);
+  };
+  W U1 = win(a.u1), U2 = win(a.u2), U3 = win(a.u3);
+  W B0 = win(a.B[0]), B1 = win(a.B[1]), B3 = win(a.B[3]);
+  W B2 = win(a.B[2]), A3 = win(a.A[3]);  // planes j, j+1 only
+  W TH[NT > 0 ? NT : 1], B4[NT > 0 ? NT : 1];
+#pragma unroll
+  for (int s = 0; s < NT; ++s) {
+    TH[s] = win(a.th[s]);
+    B4[s] = win(a.B[4 + 2 * s]);
+  }
+  for (int j = j0; j < j1; ++j) {
+    const long long o = (long long)j * pp + xz;
+    const bool hu = j + 1 < n, hd = j >= 1;
+    const double u1c = U1.c, u2c = U2.c, u3c = U3.c;
+    const double u1p = U1.p, u2p = U2.p, u3p = U3.p;
+    const double u1m = U1.m, u2m = U2.m, u3m = U3.m;
+    auto c_of = [&](int jj) { return 1.0 / (2.0 * a.dyf[jj]); };
+    const double cj = c_of(j), cm = hd ? c_of(j - 1) : 0.0, cp = hu ? c_of(j + 1) : 0.0;
+    const double A0 = a.A[0][o], A1 = a.A[1][o], A2 = a.A[2][o];
+    double b1 = 0.0, b2 = 0.0, b3 = 0.0;
+    // pair 0
+    b1 += 2.0 * u1c * A0;
+    b1 += cj * (u2p - u2c) * B0.c + cm * u2c * B0.m - cp * u2p * B0.p;
+    b2 += -cj * (u1c + u1m) * B0.c + cm * (u1c + u1m) * B0.m;
+    // pair 1
+    b1 += u3c * A1;
+    b3 += u1c * A1;
+    b3 += cj * (u2p - u2c) * B1.c + cm * u2c * B1.m - cp * u2p * B1.p;
+    b2 += -cj * (u3c + u3m) * B1.c + cm * (u3c + u3m) * B1.m;
+    // pair 2
+    b3 += 2.0 * u3c * A2;
+    {
+      const double i3 = hd ? (a.dyf[j] * u3c + a.dyf[j - 1] * u3m) / (2.0 * a.dy[j]) : 0.0;
+      b2 += i3 * B2.c;
+      b3 += (hd ? a.dyf[j] / (2.0 * a.dy[j]) : 0.0) * u2c * B2.c;
+      if (hu) b3 += (a.dyf[j] / (2.0 * a.dy[j + 1])) * u2p * B2.p;
+    }
+    // pair 3
+    {
+      const double i1 = hd ? (a.dyf[j] * u1c + a.dyf[j - 1] * u1m) / (2.0 * a.dy[j]) : 0.0;
+      b2 += i1 * A3.c;
+      b1 += (hd ? a.dyf[j] / (2.0 * a.dy[j]) : 0.0) * u2c * A3.c;
+      if (hu) b1 += (a.dyf[j] / (2.0 * a.dy[j + 1])) * u2p * A3.p;
+      if (hd) b2 += (0.5 * (u2p - u2m) / a.dy[j]) * B3.c;
+      if (hd) b2 += (0.5 * (u2m + u2c) / a.dy[j - 1 > 0 ? j - 1 : 1]) * B3.m;
+      if (hu) b2 += (-0.5 * (u2p + u2c) / a.dy[j + 1]) * B3.p;
+    }
+    // scalars
+#pragma unroll
+    for (int s = 0; s < NT; ++s) {
+      const double tc = TH[s].c, tm = TH[s].m;
+      const double A4 = a.A[4 + 2 * s][o], A5 = a.A[5 + 2 * s][o];
+      double bt = 0.0;
+      b1 += tc * A4;
+      bt += u1c * A4;
+      bt += cj * (u2p - u2c) * B4[s].c + cm * u2c * B4[s].m - cp * u2p * B4[s].p;
+      b2 += -cj * (tc + tm) * B4[s].c + cm * (tc + tm) * B4[s].m;
+      b3 += tc * A5;
+      bt += u3c * A5;
+      a.ubth[s][o] = a.store ? bt : a.ubth[s][o] + bt;
+    }
+    if (a.store) {
+      a.ub1[o] = b1;
+      a.ub2[o] = b2;
+      a.ub3[o] = b3;
+    } else {
+      a.ub1[o] += b1;
+      a.ub2[o] += b2;
+      a.ub3[o] += b3;
+    }
+    next(U1, a.u1, j);
+    next(U2, a.u2, j);
+    next(U3, a.u3, j);
+    next(B0, a.B[0], j);
+    next(B1, a.B[1], j);
+    next(B3, a.B[3], j);
+    next(B2, a.B[2], j);
+    next(A3, a.A[3], j);
+#pragma unroll
+    for (int s = 0; s < NT; ++s) {
+      next(TH[s], a.th[s], j);
+      next(B4[s], a.B[4 + 2 * s], j);
+    }
+  }
+}
+
 // The (0,0)-mode entries of a spectral field: a[j] (real part) over planes.
 __global__ void k_add_mean(double2* __restrict__ c, const double* __restrict__ w, Geo g) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;

@@ -2622,7 +2622,14 @@ void adj_substep(dgpu_handle* h, int rk_step, double time, int time_step) {
     a.nth = nth;
     a.store = spec ? 1 : 0;  // the spectral transposed solve wrote no ubar
     Timed tm(h, DGPU_K_ADJOINT, 8.0 * npx * (3 + nth + 2 * (4 + 2 * nth) + (spec ? 1 : 2) * (3 + nth)));
-    hipLaunchKernelGGL(k_adj_prod, grid1((long long)npx), dim3(256), 0, h->stream, a, g);
+    if (nth <= 2) {  // chunks of 16 planes per thread, every value read once a chunk
+      constexpr int J = 16;
+      const dim3 grid((unsigned)((g.pp + 255) / 256), (unsigned)((n2 + J - 1) / J));
+      auto kern = nth == 0 ? k_adj_prod_march<0> : nth == 1 ? k_adj_prod_march<1> : k_adj_prod_march<2>;
+      hipLaunchKernelGGL(kern, grid, dim3(256), 0, h->stream, a, g, J);
+    } else {
+      hipLaunchKernelGGL(k_adj_prod, grid1((long long)npx), dim3(256), 0, h->stream, a, g);
+    }
     HIPCHECK(hipGetLastError());
   }
   if (les) les_adj_stage(h, rk_step);
