@@ -404,7 +404,11 @@ __global__ void __launch_bounds__(512) k_adj_xpairs(AdjPairArgs a, Geo g, const 
   const int lg_ng = 31 - __builtin_clz(ngroups);
   const int grp = threadIdx.x / G, gt = threadIdx.x % G;
   C* fbuf = smem + grp * L;
-  const C* twl = lds_twiddles(smem + ngroups * L, tw, g.nx);
+  // the power-formed twiddles read one table entry a step, from the caches:
+  // without the table's 8 KB two 512-thread workgroups share a CU (its 106
+  // VGPRs allow them), and the adjoint's kernel class went 22.1 -> 19.7 ms per
+  // step at 512x257x512
+  const C* twl = LOGN == FFT_XL512P ? tw : lds_twiddles(smem + ngroups * L, tw, g.nx);
   const int j = j0 + blockIdx.y, k0 = blockIdx.x * KT;
   const double inv = 1.0 / g.nx;
   for (int pr = 0; pr < a.npairs; ++pr) {

@@ -683,7 +683,6 @@ void launch_xfwd(dgpu_handle* h, const R* u, cpx<R>* T, int j0, int j1, const Fi
   const Geo& g = h->g;
   constexpr int KT = XTile<R>::KT;
   const int ng = x_ngroups<R>(h, KT / 2, h->kn.xpass_threads);
-  const size_t lds = x_lds<R>(h, ng);
   const int nf = fb ? fb->nf : 1;
   dim3 grid((g.nz + KT - 1) / KT, j1 - j0 + 1, nf), block(ng * (g.nx / 8));
   Timed tm(h, DGPU_K_XFWD, (sizeof(R) * g.pp + 2.0 * sizeof(R) * g.tp) * (j1 - j0 + 1) * nf,
@@ -692,6 +691,8 @@ void launch_xfwd(dgpu_handle* h, const R* u, cpx<R>* T, int j0, int j1, const Fi
   if (g.nx == 512 && use_xl<R>(h, XL_XFWD))
     kern = xl_powers<R>(h, XL_XFWD) ? k_xfwd<R, true, false, FFT_XL512P> : k_xfwd<R, true, false, FFT_XL512>;
   if (g.nz % KT) kern = g.nx <= 512 ? k_xfwd<R, true, true> : k_xfwd<R, false, true>;  // ragged last tile
+  // the power-twiddle form keeps no table in LDS
+  const size_t lds = x_lds<R>(h, ng, kern != k_xfwd<R, true, false, FFT_XL512P>);
   hipLaunchKernelGGL(kern, grid, block, lds, h->stream, u, T, g, twx_of<R>(h), j0, fb ? *fb : FieldBatch{});
   HIPCHECK(hipGetLastError());
 }
@@ -2597,7 +2598,9 @@ void adj_substep(dgpu_handle* h, int rk_step, double time, int time_step) {
     if (g.nx == 512 && use_xl<double>(h, XL_ADJ))
       kern = xl_powers<double>(h, XL_ADJ) ? k_adj_xpairs<true, false, FFT_XL512P> : k_adj_xpairs<true, false, FFT_XL512>;
     if (g.nz % KT) kern = g.nx <= 512 ? k_adj_xpairs<true, true> : k_adj_xpairs<false, true>;
-    hipLaunchKernelGGL(kern, grid, block, lds, h->stream, a, g, twx_of<double>(h), 1);
+    // the power-twiddle form keeps no table in LDS
+    const bool twp = kern == k_adj_xpairs<true, false, FFT_XL512P>;
+    hipLaunchKernelGGL(kern, grid, block, twp ? x_lds<double>(h, ng, false) : lds, h->stream, a, g, twx_of<double>(h), 1);
     HIPCHECK(hipGetLastError());
   }
   {
