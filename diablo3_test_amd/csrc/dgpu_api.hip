@@ -115,11 +115,16 @@ struct Knobs {
   int fft_xl = 253;
   // The FP64 512-point passes whose fft512_xl forms its twiddles as powers of
   // one table entry (same bits): the x passes -- products, x inverse, x
-  // forward, the adjoint's x pass (1 + 4 + 32 + 128).  Measured at
-  // 512x257x512 (FP64 ms per step): products 4.88 -> 4.66, x inverse 2.27 ->
-  // 2.10; the z inverse went 1.24 -> 1.31 with them and keeps the table, and
-  // the FP32 passes are unchanged either way.
-  int fft_twp = 165;
+  // forward, the adjoint's x pass -- and the z inverses (1 + 4 + 32 + 128 +
+  // 8).  Measured at 512x257x512 (FP64 ms per step): products 4.88 -> 4.66, x
+  // inverse 2.27 -> 2.10.  Except in the products pass, the power forms read
+  // their one table entry a step from the caches and keep no table in LDS,
+  // which fits a fourth workgroup per CU in the z passes: the DNS z inverse
+  // stays at 1.24, the adjoint's z inverses go 5.78 -> 5.14 and the LES ones
+  // 3.21 -> 2.78; the z forward passes are split (adjoint 1.95 -> 1.78, LES
+  // 4.04 -> 4.20) and keep the table.  The FP32 passes are unchanged either
+  // way.
+  int fft_twp = 173;
 };
 
 Knobs knobs_from_env() {
@@ -673,8 +678,8 @@ int z_ngroups(const dgpu_handle* h) {
 }
 
 template <typename R>
-size_t z_lds(const dgpu_handle* h, int ng) {
-  return ((size_t)ng * fft_lds_len(h->g.nz) + h->g.nz) * sizeof(cpx<R>);
+size_t z_lds(const dgpu_handle* h, int ng, bool table = true) {
+  return ((size_t)ng * fft_lds_len(h->g.nz) + (table ? h->g.nz : 0)) * sizeof(cpx<R>);
 }
 
 template <typename R>
@@ -789,10 +794,11 @@ void launch_zfwd(dgpu_handle* h, const cpx<R>* T, double2* C, int j0, int j1, bo
     return;
   }
   const int ng = z_ngroups<R>(h);
-  const size_t lds = z_lds<R>(h, ng);
   dim3 grid((g.nq + ng - 1) / ng, j1 - j0 + 1, nf), block(ng * (g.nz / 8));
   auto kern = g.nz <= 512 ? k_zfwd<R, true> : k_zfwd<R, false>;
-  if (g.nz == 512 && use_xl<R>(h, XL_ZFWD)) kern = k_zfwd<R, true, FFT_XL512>;
+  const bool twp = g.nz == 512 && use_xl<R>(h, XL_ZFWD) && xl_powers<R>(h, XL_ZFWD);
+  if (g.nz == 512 && use_xl<R>(h, XL_ZFWD)) kern = twp ? k_zfwd<R, true, FFT_XL512P> : k_zfwd<R, true, FFT_XL512>;
+  const size_t lds = z_lds<R>(h, ng, !twp);
   hipLaunchKernelGGL(kern, grid, block, lds, h->stream, T, reinterpret_cast<cpx<double>*>(C), g, twz_of<R>(h), j0,
                      accum ? 1 : 0, (R)s0, (R)s1, fb ? *fb : FieldBatch{});
   HIPCHECK(hipGetLastError());
@@ -811,12 +817,13 @@ void launch_zinv(dgpu_handle* h, const double2* C, cpx<R>* T, int j0, int j1, do
     return;
   }
   const int ng = z_ngroups<R>(h);
-  const size_t lds = z_lds<R>(h, ng);
   dim3 grid((g.nq + ng - 1) / ng, j1 - j0 + 1, nf), block(ng * (g.nz / 8));
   auto kern = g.nz <= 512 ? k_zinv<R, true> : k_zinv<R, false>;
   if constexpr (sizeof(R) == 8)
     if (g.nz == 512 && (h->kn.fft_const & 1)) kern = k_zinv<R, true, 9>;
-  if (g.nz == 512 && use_xl<R>(h, XL_ZINV)) kern = k_zinv<R, true, FFT_XL512>;
+  const bool twp = g.nz == 512 && use_xl<R>(h, XL_ZINV) && xl_powers<R>(h, XL_ZINV);
+  if (g.nz == 512 && use_xl<R>(h, XL_ZINV)) kern = twp ? k_zinv<R, true, FFT_XL512P> : k_zinv<R, true, FFT_XL512>;
+  const size_t lds = z_lds<R>(h, ng, !twp);
   hipLaunchKernelGGL(kern, grid, block, lds, h->stream, reinterpret_cast<const cpx<double>*>(C), T, g, twz_of<R>(h),
                      j0, s0, s1, fb ? *fb : FieldBatch{});
   HIPCHECK(hipGetLastError());
@@ -834,10 +841,11 @@ void launch_zinv_op(dgpu_handle* h, const double2* C, cpx<R>* T, int j0, int j1,
     return;
   }
   const int ng = z_ngroups<R>(h);
-  const size_t lds = z_lds<R>(h, ng);
   dim3 grid((g.nq + ng - 1) / ng, j1 - j0 + 1), block(ng * (g.nz / 8));
   auto kern = g.nz <= 512 ? k_zinv_op<R, true> : k_zinv_op<R, false>;
-  if (g.nz == 512 && use_xl<R>(h, XL_ZINV)) kern = k_zinv_op<R, true, FFT_XL512>;
+  const bool twp = g.nz == 512 && use_xl<R>(h, XL_ZINV) && xl_powers<R>(h, XL_ZINV);
+  if (g.nz == 512 && use_xl<R>(h, XL_ZINV)) kern = twp ? k_zinv_op<R, true, FFT_XL512P> : k_zinv_op<R, true, FFT_XL512>;
+  const size_t lds = z_lds<R>(h, ng, !twp);
   hipLaunchKernelGGL(kern, grid, block, lds, h->stream, reinterpret_cast<const cpx<double>*>(C), T, g, twz_of<R>(h),
                      j0, op, h->d_kx.p, h->d_kz.p, h->d_dy.p, h->d_dyf.p, s0, s1);
   HIPCHECK(hipGetLastError());
@@ -3099,6 +3107,8 @@ void create_impl(dgpu_handle* h, const double* gy) {
         (const void*)k_rhs<double, true, FFT_XL512>,   (const void*)k_rhs<float, true, FFT_XL512>,
         (const void*)k_xinv<double, true, FFT_XL512>,  (const void*)k_xinv<float, true, FFT_XL512>,
         (const void*)k_zinv<double, true, FFT_XL512>,  (const void*)k_zinv<float, true, FFT_XL512>,
+        (const void*)k_zinv<double, true, FFT_XL512P>, (const void*)k_zfwd<double, true, FFT_XL512P>,
+        (const void*)k_zinv_op<double, true, FFT_XL512P>,
         (const void*)k_xfwd<double, true, false, FFT_XL512>, (const void*)k_xfwd<float, true, false, FFT_XL512>,
         (const void*)k_zfwd<double, true, FFT_XL512>,  (const void*)k_zfwd<float, true, FFT_XL512>,
         (const void*)k_zinv_op<double, true, FFT_XL512>, (const void*)k_zinv_op<float, true, FFT_XL512>,

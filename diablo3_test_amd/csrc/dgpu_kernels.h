@@ -225,7 +225,9 @@ __global__ void __launch_bounds__(512) k_zfwd(const cpx<R>* __restrict__ T, cpx<
   const int ngroups = blockDim.x / G;
   const int grp = threadIdx.x / G, gt = threadIdx.x % G;
   cpx<R>* fbuf = smem + grp * fft_lds_len(g.nz);
-  const cpx<R>* twl = lds_twiddles(smem + ngroups * fft_lds_len(g.nz), tw, g.nz);
+  // power-formed twiddles (one table entry a step) from the caches, the
+  // table's LDS left to the workgroups
+  const cpx<R>* twl = LOGN == FFT_XL512P ? tw : lds_twiddles(smem + ngroups * fft_lds_len(g.nz), tw, g.nz);
   const int j = j0 + blockIdx.y, q = blockIdx.x * ngroups + grp;
   const bool act = q < g.nq;
   cpx<R> v[8];
@@ -269,7 +271,9 @@ __global__ void __launch_bounds__(512) k_zinv(const cpx<double>* __restrict__ C,
   const int ngroups = blockDim.x / G;
   const int grp = threadIdx.x / G, gt = threadIdx.x % G;
   cpx<R>* fbuf = smem + grp * fft_lds_len(g.nz);
-  const cpx<R>* twl = lds_twiddles(smem + ngroups * fft_lds_len(g.nz), tw, g.nz);
+  // power-formed twiddles (one table entry a step) from the caches, the
+  // table's LDS left to the workgroups
+  const cpx<R>* twl = LOGN == FFT_XL512P ? tw : lds_twiddles(smem + ngroups * fft_lds_len(g.nz), tw, g.nz);
   const int j = j0 + blockIdx.y, q = blockIdx.x * ngroups + grp;
   const bool act = q < g.nq;
   cpx<R> v[8];

@@ -55,7 +55,8 @@ __global__ void __launch_bounds__(512) k_zinv_op(const cpx<double>* __restrict__
   const int ngroups = blockDim.x / G;
   const int grp = threadIdx.x / G, gt = threadIdx.x % G;
   cpx<R>* fbuf = smem + grp * fft_lds_len(g.nz);
-  const cpx<R>* twl = lds_twiddles(smem + ngroups * fft_lds_len(g.nz), tw, g.nz);
+  // power-formed twiddles from the caches, as in k_zinv
+  const cpx<R>* twl = LOGN == FFT_XL512P ? tw : lds_twiddles(smem + ngroups * fft_lds_len(g.nz), tw, g.nz);
   const int j = j0 + blockIdx.y, q = blockIdx.x * ngroups + grp;
   bool act = q < g.nq;
   if ((op == OP_DY_GY || op == OP_DY_C2) && j == 0) act = false;
