@@ -67,11 +67,12 @@ struct Knobs {
   // Rows per segment of the CN solver: one of the instantiated lengths (the
   // rows live in registers), with at most 16 segments (1024 threads per
   // block).  The target length trades registers per thread against segments
-  // per column.  At 512x257x512 20 rows (13 segments) take the spectral solves
-  // from 2.47 to 2.34 ms per step against 24 (11 segments): the 126-VGPR
-  // kernel runs 4 waves per SIMD, so a CU holds one block, and 13 waves fill
-  // more of its 16 slots.
-  int seg_target = 20;
+  // per column.  At 512x257x512 20 rows (13 segments) took the spectral
+  // solves from 2.47 to 2.34 ms per step against 24 (11 segments): the kernel
+  // runs 4 waves per SIMD, so a CU holds one block, and 13 waves fill more of
+  // its 16 slots; 17 rows (16 segments, 97 VGPRs) fill all 16 and take them
+  // from 2.24 to 2.17 (mixed 2.12 to 2.04).
+  int seg_target = 17;
   // Threads per workgroup of the x passes and of the products pass (FP64,
   // FP32).  The FP64 products pass holds 228 VGPRs, two waves per SIMD: as
   // two 256-thread workgroups per CU (each with 4 of the 8 rows of its tile,
@@ -149,7 +150,7 @@ Knobs knobs_from_env() {
 }
 
 int seg_len_for(int n, int seg_target) {
-  static const int lens[] = {8, 16, 20, 24, 28, 32, 40, 48, 56};  // the launch switches' cases
+  static const int lens[] = {8, 16, 17, 20, 24, 28, 32, 40, 48, 56};  // the launch switches' cases
   int S = (n + seg_target - 1) / seg_target;
   if (S > 16) S = 16;
   const int L0 = (n + S - 1) / S;
@@ -1008,6 +1009,7 @@ void launch_thomas(dgpu_handle* h, int field, int rs, const R* r, R* u, int ca, 
   switch (h->seg_L * 4 + C) {
     case 8 * 4 + 1: TLAUNCH(8, 1); break;
     case 16 * 4 + 1: TLAUNCH(16, 1); break;
+    case 17 * 4 + 1: TLAUNCH(17, 1); break;
     case 20 * 4 + 1: TLAUNCH(20, 1); break;
     case 24 * 4 + 1: TLAUNCH(24, 1); break;
     case 28 * 4 + 1: TLAUNCH(28, 1); break;
@@ -1017,6 +1019,7 @@ void launch_thomas(dgpu_handle* h, int field, int rs, const R* r, R* u, int ca, 
     case 56 * 4 + 1: TLAUNCH(56, 1); break;
     case 8 * 4 + 2: TLAUNCH(8, 2); break;
     case 16 * 4 + 2: TLAUNCH(16, 2); break;
+    case 17 * 4 + 2: TLAUNCH(17, 2); break;
     case 20 * 4 + 2: TLAUNCH(20, 2); break;
     case 24 * 4 + 2: TLAUNCH(24, 2); break;
     case 28 * 4 + 2: TLAUNCH(28, 2); break;
@@ -1056,6 +1059,7 @@ void launch_thomas_adj(dgpu_handle* h, int field, int rs, const double* xb, doub
   switch (h->seg_L * 4 + C) {
     case 8 * 4 + 1: TLAUNCH(8, 1); break;
     case 16 * 4 + 1: TLAUNCH(16, 1); break;
+    case 17 * 4 + 1: TLAUNCH(17, 1); break;
     case 20 * 4 + 1: TLAUNCH(20, 1); break;
     case 24 * 4 + 1: TLAUNCH(24, 1); break;
     case 28 * 4 + 1: TLAUNCH(28, 1); break;
@@ -1065,6 +1069,7 @@ void launch_thomas_adj(dgpu_handle* h, int field, int rs, const double* xb, doub
     case 56 * 4 + 1: TLAUNCH(56, 1); break;
     case 8 * 4 + 2: TLAUNCH(8, 2); break;
     case 16 * 4 + 2: TLAUNCH(16, 2); break;
+    case 17 * 4 + 2: TLAUNCH(17, 2); break;
     case 20 * 4 + 2: TLAUNCH(20, 2); break;
     case 24 * 4 + 2: TLAUNCH(24, 2); break;
     case 28 * 4 + 2: TLAUNCH(28, 2); break;
