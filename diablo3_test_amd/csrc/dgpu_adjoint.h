@@ -59,7 +59,7 @@ __global__ void __launch_bounds__(MAXT) k_adj_remdiv_seg(AdjRemDivArgs a, Geo g,
   const int w = threadIdx.x / 64, sl = (threadIdx.x % 64) / MW;
   const int nw = blockDim.x / 64;
   const long long sp = g.sp;
-  const long long m = (long long)blockIdx.x * MW + mm;
+  const long long m = (long long)xcd_block(blockIdx.x, gridDim.x) * MW + mm;  // as k_remdiv_seg
   const bool live = m < sp;
   const long long mc = live ? m : 0;
   const int q = (int)((unsigned)mc / (unsigned)g.nkk), kk = (int)((unsigned)mc % (unsigned)g.nkk);
