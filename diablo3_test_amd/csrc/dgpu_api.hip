@@ -953,11 +953,12 @@ void to_fourier(dgpu_handle* h, const R* u, double2* C, cpx<R>* T, int j0, int j
 // through the x-spectra T[k]): every field's x pass in one launch, then every
 // field's z pass.
 template <typename R>
-void to_fourier_fields(dgpu_handle* h, int nf, R* const* u, double2* const* C, cpx<R>* const* T, int j0, int j1) {
+void to_fourier_fields(dgpu_handle* h, int nf, R* const* u, double2* const* C, cpx<R>* const* T, int j0, int j1,
+                       bool accum = false) {
   for (int k0 = 0; k0 < nf; k0 += MAXBF) {
     const int nb = std::min(MAXBF, nf - k0);
     if (batch_planes<R>(h) < h->g.ny2) {  // plane batches: field by field
-      for (int k = k0; k < k0 + nb; ++k) to_fourier<R>(h, u[k], C[k], T[k], j0, j1);
+      for (int k = k0; k < k0 + nb; ++k) to_fourier<R>(h, u[k], C[k], T[k], j0, j1, accum);
       continue;
     }
     FieldBatch xb{}, zb{};
@@ -969,7 +970,7 @@ void to_fourier_fields(dgpu_handle* h, int nf, R* const* u, double2* const* C, c
     }
     xb.nf = zb.nf = nb;
     launch_xfwd<R>(h, u[k0], T[k0], j0, j1, &xb);
-    launch_zfwd<R>(h, T[k0], C[k0], j0, j1, false, 1.0, 1.0, &zb);
+    launch_zfwd<R>(h, T[k0], C[k0], j0, j1, accum, 1.0, 1.0, &zb);
   }
 }
 
@@ -2104,12 +2105,34 @@ void rk_substep_t(dgpu_handle* h, int rk_step, double time, int time_step) {
   // 6. back to Fourier space on the solved planes (channel.f90:891-896); the
   // mixed form adds the transformed increment to the FP64 state
   if (!spec) {
-    to_fourier<R>(h, phys<R>(h->u[0]), h->cu[0].p, xspec<R>(h->tpool[0]), h->jstart - 1, h->jend + 1, incr);
-    to_fourier<R>(h, phys<R>(h->u[1]), h->cu[1].p, xspec<R>(h->tpool[1]), 1, Ny + 1, incr);
-    to_fourier<R>(h, phys<R>(h->u[2]), h->cu[2].p, xspec<R>(h->tpool[2]), h->jstart - 1, h->jend + 1, incr);
-    for (int n = 0; n < nth; ++n)
-      to_fourier<R>(h, phys<R>(h->th[n]), h->cth[n].p, xspec<R>(h->tpool[3 + n]), h->jstart_th[n] - 1,
-                    h->jend_th[n] + 1, incr);
+    // fields that share their plane range go in one launch per pass
+    struct Sol {
+      R* u;
+      double2* c;
+      int j0, j1;
+    };
+    std::vector<Sol> sol;
+    sol.push_back({phys<R>(h->u[0]), h->cu[0].p, h->jstart - 1, h->jend + 1});
+    sol.push_back({phys<R>(h->u[1]), h->cu[1].p, 1, Ny + 1});
+    sol.push_back({phys<R>(h->u[2]), h->cu[2].p, h->jstart - 1, h->jend + 1});
+    for (int n = 0; n < nth; ++n) sol.push_back({phys<R>(h->th[n]), h->cth[n].p, h->jstart_th[n] - 1, h->jend_th[n] + 1});
+    std::vector<bool> done(sol.size(), false);
+    for (size_t a = 0; a < sol.size(); ++a) {
+      if (done[a]) continue;
+      R* U[3 + DGPU_MAX_TH];
+      double2* CC[3 + DGPU_MAX_TH];
+      cpx<R>* TT[3 + DGPU_MAX_TH];
+      int k = 0;
+      for (size_t b = a; b < sol.size(); ++b)
+        if (!done[b] && sol[b].j0 == sol[a].j0 && sol[b].j1 == sol[a].j1) {
+          done[b] = true;
+          U[k] = sol[b].u;
+          CC[k] = sol[b].c;
+          TT[k] = xspec<R>(h->tpool[b]);
+          ++k;
+        }
+      to_fourier_fields<R>(h, k, U, CC, TT, sol[a].j0, sol[a].j1, incr);
+    }
   }
 
   // courant on the last substep of every update_dt-th step, on the velocity
