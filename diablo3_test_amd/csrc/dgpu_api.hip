@@ -835,23 +835,24 @@ void launch_zinv(dgpu_handle* h, const double2* C, cpx<R>* T, int j0, int j1, do
 
 template <typename R>
 void launch_zinv_op(dgpu_handle* h, const double2* C, cpx<R>* T, int j0, int j1, int op, double s0 = 1.0,
-                    double s1 = 1.0) {
+                    double s1 = 1.0, const FieldBatch* fb = nullptr) {
   if (j1 < j0) return;
   const Geo& g = h->g;
-  Timed tm(h, DGPU_K_ZINV, (2.0 * sizeof(R) * g.tp + 16.0 * g.sp) * (j1 - j0 + 1),
-           2.0 * sizeof(R) * g.tp * (j1 - j0 + 1));
+  const int nf = fb ? fb->nf : 1;
+  Timed tm(h, DGPU_K_ZINV, (2.0 * sizeof(R) * g.tp + 16.0 * g.sp) * (j1 - j0 + 1) * nf,
+           2.0 * sizeof(R) * g.tp * (j1 - j0 + 1) * nf);
   if (h->zgen) {
-    zgen_inv<R>(h, C, T, j0, j1, op, s0, s1, nullptr);
+    zgen_inv<R>(h, C, T, j0, j1, op, s0, s1, fb);
     return;
   }
   const int ng = z_ngroups<R>(h);
-  dim3 grid((g.nq + ng - 1) / ng, j1 - j0 + 1), block(ng * (g.nz / 8));
+  dim3 grid((g.nq + ng - 1) / ng, j1 - j0 + 1, nf), block(ng * (g.nz / 8));
   auto kern = g.nz <= 512 ? k_zinv_op<R, true> : k_zinv_op<R, false>;
   const bool twp = g.nz == 512 && use_xl<R>(h, XL_ZINV) && xl_powers<R>(h, XL_ZINV);
   if (g.nz == 512 && use_xl<R>(h, XL_ZINV)) kern = twp ? k_zinv_op<R, true, FFT_XL512P> : k_zinv_op<R, true, FFT_XL512>;
   const size_t lds = z_lds<R>(h, ng, !twp);
   hipLaunchKernelGGL(kern, grid, block, lds, h->stream, reinterpret_cast<const cpx<double>*>(C), T, g, twz_of<R>(h),
-                     j0, op, h->d_kx.p, h->d_kz.p, h->d_dy.p, h->d_dyf.p, s0, s1);
+                     j0, op, h->d_kx.p, h->d_kz.p, h->d_dy.p, h->d_dyf.p, s0, s1, fb ? *fb : FieldBatch{});
   HIPCHECK(hipGetLastError());
 }
 
@@ -1268,8 +1269,16 @@ void les_stage(dgpu_handle* h, double time) {
       DZ[f] = f < 3 ? phys<R>(h->grad[3 * f + 2]) : phys<R>(h->dth[2][f - 3]);
     }
     xinv_fields<R>(h, nd, TX, DX, 0, n2 - 1, h->d_kx.p);
-    for (int f = 0; f < nd; ++f)
-      launch_zinv_op<R>(h, f < 3 ? h->cu[f].p : h->cth[f - 3].p, TX[f], 0, n2 - 1, OP_DZ);
+    for (int f0 = 0; f0 < nd; f0 += MAXBF) {
+      FieldBatch zb{};
+      zb.nf = std::min(MAXBF, nd - f0);
+      for (int k = 0; k < zb.nf; ++k) {
+        const int f = f0 + k;
+        zb.in[k] = f < 3 ? h->cu[f].p : h->cth[f - 3].p;
+        zb.out[k] = TX[f];
+      }
+      launch_zinv_op<R>(h, static_cast<const double2*>(zb.in[0]), TX[f0], 0, n2 - 1, OP_DZ, 1.0, 1.0, &zb);
+    }
     xinv_fields<R>(h, nd, TX, DZ, 0, n2 - 1);
   }
   T = xspec<R>(h->tpool[3 + nth]);  // scratch past the state's x-spectra
@@ -2591,6 +2600,10 @@ void adj_substep(dgpu_handle* h, int rk_step, double time, int time_step) {
   // 6. product slots: Xbar = Fz^T Zbar, Zbar_s = -fbar or ikz fbar (the
   //    combinations of k_rhs: f1 -= Z0 + ikz Z1, f3 -= Z2 + ikz Z3,
   //    f2 -= Z4 + ikz Z5, ft -= Z6 + ikz Z7)
+  //    (slot by slot, so that each source field's second read, for its
+  //    ikz slot, follows its first and comes from the last-level cache:
+  //    batched by operator, the even slots then the odd ones, the adjoint's
+  //    z inverses took 5.14 -> 5.31 ms per step at 512x257x512)
   const int nslot = 6 + 2 * nth;
   for (int s = 0; s < nslot; ++s) {
     int src;

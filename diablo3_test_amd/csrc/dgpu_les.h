@@ -48,7 +48,12 @@ __global__ void __launch_bounds__(512) k_zinv_op(const cpx<double>* __restrict__
                                                  const double* __restrict__ kx,
                                                  const double* __restrict__ kz,
                                                  const double* __restrict__ dy,
-                                                 const double* __restrict__ dyf, double s0, double s1) {
+                                                 const double* __restrict__ dyf, double s0, double s1,
+                                                 FieldBatch fb = FieldBatch{}) {
+  if (fb.nf > 0) {  // blockIdx.z picks the field
+    C = reinterpret_cast<const cpx<double>*>(fb.in[blockIdx.z]);
+    T = reinterpret_cast<cpx<R>*>(fb.out[blockIdx.z]);
+  }
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<R>* smem = reinterpret_cast<cpx<R>*>(smem_raw);
   const int G = g.nz >> 3;
