@@ -1055,6 +1055,32 @@ __global__ void __launch_bounds__(1024) k_thomas_seg(const TR* r,
   }
   __shared__ double carry_f[16][64 * C];
   __shared__ double carry_b[16][64 * C];
+  // The block's rows of the factor table (and, for the adjoint's epilogue,
+  // of the forward table) staged in LDS once.  Read from the table itself,
+  // each row's fields were scalar loads that the sweeps waited on one by one.
+  constexpr bool SPROW = ADJ && L <= 32;  // LDS for both tables
+  __shared__ SegRow srow[16 * L];
+  __shared__ SegRow sprow[SPROW ? 16 * L : 1];
+  {
+    static_assert(sizeof(SegRow) % sizeof(double) == 0, "SegRow is copied as doubles");
+    constexpr int W = sizeof(SegRow) / sizeof(double);
+    const int nt = blockDim.x * blockDim.y, tid = threadIdx.y * blockDim.x + threadIdx.x;
+    const double* src = reinterpret_cast<const double*>(rows);
+    double* dst = reinterpret_cast<double*>(srow);
+    for (int i = tid; i < n * W; i += nt) dst[i] = src[i];
+    if constexpr (SPROW) {
+      const double* ps = reinterpret_cast<const double*>(prow);
+      double* pd = reinterpret_cast<double*>(sprow);
+      for (int i = tid; i < n * W; i += nt) pd[i] = ps[i];
+    }
+  }
+  __syncthreads();
+  auto prow_at = [&](int jj) -> const SegRow& {
+    if constexpr (SPROW)
+      return sprow[jj];
+    else
+      return prow[jj];
+  };
   // blockDim.x == 64: a wavefront is one segment, so seg is wave-uniform and
   // the row-table reads below become scalar loads
   const int lane = threadIdx.x, seg = __builtin_amdgcn_readfirstlane(threadIdx.y);
@@ -1081,15 +1107,31 @@ __global__ void __launch_bounds__(1024) k_thomas_seg(const TR* r,
       for (int c = 0; c < C; ++c) w[t][c] = 0.0;
     }
   }
+  // The r rows go through a ring of RD loads in flight: the load of row t
+  // + RD is issued before row t is assembled.  Each row's load used to sit
+  // behind that row's kind test (a scalar load of the factor table), which
+  // left one round trip to memory per row; loading every row up front
+  // instead holds them all beside the u rows and spills.
+  constexpr int RD = 4;
+  auto ld_r = [&](int t) -> VR {
+    const int jj = s0 + t;
+    if (t < L && inb && jj < n)
+      return __builtin_nontemporal_load(reinterpret_cast<const VR*>(r + (long long)jj * pp + col));
+    return (VR)(TR)0;
+  };
+  VR rq[RD];
+#pragma unroll
+  for (int t = 0; t < RD; ++t) rq[t] = ld_r(t);
 #pragma unroll
   for (int t = 0; t < L; ++t) {
     const int jj = s0 + t;
+    const VR x = rq[t % RD];
+    rq[t % RD] = ld_r(t + RD);
 #pragma unroll
     for (int c = 0; c < C; ++c) v[t][c] = 0.0;
     if (jj < n) {
-      const SegRow rw = rows[jj];
+      const SegRow rw = srow[jj];
       if ((rw.kind == 1 || rw.kind == 3) && inb) {
-        const VR x = __builtin_nontemporal_load(reinterpret_cast<const VR*>(r + (long long)jj * pp + col));
 #pragma unroll
         for (int c = 0; c < C; ++c) {
           double val = (double)x[c];
@@ -1127,7 +1169,7 @@ __global__ void __launch_bounds__(1024) k_thomas_seg(const TR* r,
   for (int t = 0; t < L; ++t) {
     const int jj = s0 + t;
     if (jj < n) {
-      const double m = rows[jj].m;
+      const double m = srow[jj].m;
 #pragma unroll
       for (int c = 0; c < C; ++c) {
         h[c] = v[t][c] + m * h[c];
@@ -1150,7 +1192,7 @@ __global__ void __launch_bounds__(1024) k_thomas_seg(const TR* r,
   for (int c = 0; c < C; ++c) gin[c] = 0.0;
   for (int sp = 0; sp < seg; ++sp) {
     const int l1 = min(sp * L + L, n) - 1;
-    const double pm = rows[l1].pm;
+    const double pm = srow[l1].pm;
 #pragma unroll
     for (int c = 0; c < C; ++c) gin[c] = carry_f[sp][lane * C + c] + pm * gin[c];
   }
@@ -1158,7 +1200,7 @@ __global__ void __launch_bounds__(1024) k_thomas_seg(const TR* r,
   for (int t = 0; t < L; ++t) {
     const int jj = s0 + t;
     if (jj < n) {
-      const double pm = rows[jj].pm;
+      const double pm = srow[jj].pm;
 #pragma unroll
       for (int c = 0; c < C; ++c) v[t][c] += pm * gin[c];
     }
@@ -1171,7 +1213,7 @@ __global__ void __launch_bounds__(1024) k_thomas_seg(const TR* r,
   for (int t = L - 1; t >= 0; --t) {
     const int jj = s0 + t;
     if (jj < n) {
-      const double e = rows[jj].e, f = rows[jj].f;
+      const double e = srow[jj].e, f = srow[jj].f;
 #pragma unroll
       for (int c = 0; c < C; ++c) {
         x[c] = v[t][c] * e + f * x[c];
@@ -1186,7 +1228,7 @@ __global__ void __launch_bounds__(1024) k_thomas_seg(const TR* r,
 #pragma unroll
   for (int c = 0; c < C; ++c) xin[c] = 0.0;
   for (int sp = S - 1; sp > seg; --sp) {
-    const double qf = rows[sp * L].qf;
+    const double qf = srow[sp * L].qf;
 #pragma unroll
     for (int c = 0; c < C; ++c) xin[c] = carry_b[sp][lane * C + c] + qf * xin[c];
   }
@@ -1195,7 +1237,7 @@ __global__ void __launch_bounds__(1024) k_thomas_seg(const TR* r,
 #pragma unroll
     for (int t = 0; t < L; ++t) {
       const int jj = s0 + t;
-      const double qf = rows[jj < n ? jj : n - 1].qf;
+      const double qf = srow[jj < n ? jj : n - 1].qf;
 #pragma unroll
       for (int c = 0; c < C; ++c) v[t][c] += qf * xin[c];
     }
@@ -1220,9 +1262,9 @@ __global__ void __launch_bounds__(1024) k_thomas_seg(const TR* r,
     for (int t = 0; t < L; ++t) {
       const int jj = s0 + t;
       if (jj < n) {
-        const SegRow rj = prow[jj];
-        const bool km = jj >= 1 && prow[jj - 1].kind == 1, kp = jj + 1 < n && prow[jj + 1].kind == 1;
-        const double wpm = km ? prow[jj - 1].wp : 0.0, wmp = kp ? prow[jj + 1].wm : 0.0;
+        const SegRow rj = prow_at(jj);
+        const bool km = jj >= 1 && prow_at(jj - 1).kind == 1, kp = jj + 1 < n && prow_at(jj + 1).kind == 1;
+        const double wpm = km ? prow_at(jj - 1).wp : 0.0, wmp = kp ? prow_at(jj + 1).wm : 0.0;
         typedef double VD __attribute__((ext_vector_type(C)));
         VD rb, ub;
 #pragma unroll
@@ -1247,7 +1289,7 @@ __global__ void __launch_bounds__(1024) k_thomas_seg(const TR* r,
     for (int t = 0; t < L; ++t) {
       const int jj = s0 + t;
       if (inb && jj < n && jj >= ca && jj <= cb) {
-        const double qf = rows[jj].qf;
+        const double qf = srow[jj].qf;
         VN o;
 #pragma unroll
         for (int c = 0; c < C; ++c) o[c] = (T)(v[t][c] + qf * xin[c]);
