@@ -55,6 +55,10 @@ __global__ void __launch_bounds__(MAXT) k_adj_remdiv_seg(AdjRemDivArgs a, Geo g,
   __shared__ C tB[WMAX][MW];
   __shared__ C tL[SMAX][MW];
   __shared__ C z1s[MW];
+  // the row coefficients staged in LDS once, as in k_remdiv_seg
+  __shared__ PoisRow srw[2 + L * SMAX];
+  for (int i = threadIdx.x; i <= g.ny && i < 2 + L * SMAX; i += blockDim.x) srw[i] = rows[i];
+  __syncthreads();
   const int mm = threadIdx.x % MW, s = threadIdx.x / MW;
   const int w = threadIdx.x / 64, sl = (threadIdx.x % 64) / MW;
   const int nw = blockDim.x / 64;
@@ -104,8 +108,8 @@ __global__ void __launch_bounds__(MAXT) k_adj_remdiv_seg(AdjRemDivArgs a, Geo g,
   for (int t = 0; t < L; ++t) {
     const int j = j0 + t;
     if (j > ny) break;
-    C v = ik(kxq, d1[t]) + ik(kzk, d3[t]) - rows[j].idy * d2[t];
-    if (j + 1 <= ny) v = v + rows[j + 1].idy * d2[t + 1];
+    C v = ik(kxq, d1[t]) + ik(kzk, d3[t]) - srw[j].idy * d2[t];
+    if (j + 1 <= ny) v = v + srw[j + 1].idy * d2[t + 1];
     if (inp(j)) v = v + a.inv_h * pb[t];
     pb[t] = v;
   }
@@ -114,7 +118,7 @@ __global__ void __launch_bounds__(MAXT) k_adj_remdiv_seg(AdjRemDivArgs a, Geo g,
     if (live) {
       const unsigned o = off(1);
       z1 = ik(kxq, ldc(a.a1, o)) + ik(kzk, ldc(a.a3, o));
-      if (ny >= 2) z1 = z1 + rows[2].idy * d2[0];
+      if (ny >= 2) z1 = z1 + srw[2].idy * d2[0];
       if (inp(1)) z1 = z1 + a.inv_h * ldc(a.ap, o);
     }
     z1s[mm] = z1;
@@ -129,7 +133,7 @@ __global__ void __launch_bounds__(MAXT) k_adj_remdiv_seg(AdjRemDivArgs a, Geo g,
     for (int t = 0; t < L; ++t) {
       const int j = j0 + t;
       if (j <= ny) {
-        const double mup = (j == 2) ? mu1 : rows[j - 1].mu;
+        const double mup = (j == 2) ? mu1 : srw[j - 1].mu;
         const double f = -mup * ibv[t];
         h = ibv[t] * pb[t] + f * h;
         P = f * P;
@@ -175,7 +179,7 @@ __global__ void __launch_bounds__(MAXT) k_adj_remdiv_seg(AdjRemDivArgs a, Geo g,
     for (int t = L - 1; t >= 0; --t) {
       const int j = j0 + t;
       if (j <= ny) {
-        const double f = (j < ny) ? -rows[j + 1].ml * ibv[t] : 0.0;
+        const double f = (j < ny) ? -srw[j + 1].ml * ibv[t] : 0.0;
         x = gl[t] + f * x;
         Q = f * Q;
       }
@@ -227,8 +231,8 @@ __global__ void __launch_bounds__(MAXT) k_adj_remdiv_seg(AdjRemDivArgs a, Geo g,
     const bool dm = j - 1 >= 2 && j - 1 <= ny - 1;
     const C yprev = (t == 0) ? ym : gl[t - 1];
     C d = d2[t];
-    if (dm) d = d + rows[j - 1].idyf * yprev;
-    if (dr) d = d - rows[j].idyf * y;
+    if (dm) d = d + srw[j - 1].idyf * yprev;
+    if (dr) d = d - srw[j].idyf * y;
     stc(a.a2, o, d);
     if (dr) {
       stc(a.a1, o, d1[t] - ik(kxq, y));

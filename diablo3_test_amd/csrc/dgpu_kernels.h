@@ -1409,6 +1409,12 @@ __global__ void __launch_bounds__(MAXT) k_remdiv_seg(RemDivArgs a, Geo g, const 
   __shared__ double tQ[WMAX][MW];
   __shared__ C tB[WMAX][MW];
   __shared__ C tL[SMAX][MW];
+  // the row coefficients of the planes the block's segments cover, staged in
+  // LDS once (read from the table, the lanes of a wavefront's segments read
+  // different rows in every sweep step)
+  __shared__ PoisRow srw[2 + L * SMAX];
+  for (int i = threadIdx.x; i <= g.ny && i < 2 + L * SMAX; i += blockDim.x) srw[i] = rows[i];
+  __syncthreads();
   const int mm = threadIdx.x % MW, s = threadIdx.x / MW;
   const int w = threadIdx.x / 64, sl = (threadIdx.x % 64) / MW;
   const int nw = blockDim.x / 64;
@@ -1465,7 +1471,7 @@ __global__ void __launch_bounds__(MAXT) k_remdiv_seg(RemDivArgs a, Geo g, const 
     for (int t = 0; t < L; ++t) {
       const int j = j0 + t;
       if (j <= ny) {
-        const PoisRow rw = rows[j];
+        const PoisRow rw = srw[j];
         const double mj = -rw.ml * ibv[t];
         const C rhs = (j < ny) ? ik(kxq, d1[t]) + rw.idyf * (d2[t + 1] - d2[t]) + ik(kzk, d3[t]) : Z;
         h = rhs + mj * h;
@@ -1516,7 +1522,7 @@ __global__ void __launch_bounds__(MAXT) k_remdiv_seg(RemDivArgs a, Geo g, const 
       const int j = j0 + t;
       if (j <= ny) {
         const double e = ibv[t + 1];
-        const double f = -rows[j].mu * e;  // mu(Ny) = 0
+        const double f = -srw[j].mu * e;  // mu(Ny) = 0
         x = e * gl[t] + f * x;
         Q = f * Q;
       }
@@ -1577,7 +1583,7 @@ __global__ void __launch_bounds__(MAXT) k_remdiv_seg(RemDivArgs a, Geo g, const 
     const C phi = gl[t];
     stc(a.cu1, o, d1[t] - ik(kxq, phi));
     stc(a.cu3, o, d3[t] - ik(kzk, phi));
-    stc(a.cu2, o, d2[t] - rows[j].idy * (phi - (t == 0 ? phim : gl[t - 1])));
+    stc(a.cu2, o, d2[t] - srw[j].idy * (phi - (t == 0 ? phim : gl[t - 1])));
     if (cpu && j >= a.js && j <= a.je) stc(a.cp, o, pv[t] + a.inv_h * phi);
   }
   if (s == 0) {  // row 1: cu1, cu3 and cp (cu2 at row 1 is not part of the projection)
