@@ -1873,7 +1873,7 @@ void rk_substep_t(dgpu_handle* h, int rk_step, double time, int time_step) {
     constexpr int KT = XTile<R>::KT;
     // a row's group needs Nx/8 lanes whatever the limit
     const int ngx = x_ngroups<R>(h, KT, std::max(sizeof(R) == 8 ? h->kn.xprod_threads : h->kn.xprod_threads32, g.nx / 8));
-    const size_t lds = x_lds<R>(h, ngx);
+    const size_t lds = x_lds<R>(h, ngx) + g.nq * sizeof(R);  // + kx behind the table
     if (lds > h->lds_limit) throw ApiError{DGPU_EUNSUP, "Nx too large for the products pass"};
     dim3 grid((g.nz + KT - 1) / KT, Ny), block(ngx * (g.nx / 8));
     // reads u1, u2, u3, th once (neighbour planes from cache); writes the product x-spectra

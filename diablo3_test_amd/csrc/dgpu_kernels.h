@@ -359,7 +359,7 @@ __device__ __forceinline__ void pair_ranges(const ProdArgs& a, const Geo& g, int
 // each input row crosses the memory system once per workgroup.
 template <typename R>
 __device__ __forceinline__ void emit_pair(cpx<R>* smem, int L, const Geo& g, int lg_ng, int pair,
-                                          R inv, const double* kx, cpx<R>* out0, cpx<R>* out1,
+                                          R inv, const R* kx, cpx<R>* out0, cpx<R>* out1,
                                           int m0, int nrow = 1 << 30) {
   const int ngroups = 1 << lg_ng;
   for (int idx = threadIdx.x; idx < g.nq * ngroups; idx += blockDim.x) {
@@ -367,7 +367,7 @@ __device__ __forceinline__ void emit_pair(cpx<R>* smem, int L, const Geo& g, int
     if (mm >= nrow) continue;  // rows past Nz of a ragged tile
     cpx<R> A, B;
     split_pair(smem + mm * L, g.nx, q, inv, A, B);
-    const cpx<R> ikA = ik((R)kx[q], A);
+    const cpx<R> ikA = ik(kx[q], A);
     const int o = q * g.nz + m0 + mm;  // within one plane's x-spectra
     switch (pair) {
       case 0: st_out(out0 + o, ikA + B); break;
@@ -400,10 +400,14 @@ __global__ void __launch_bounds__(512) k_xprod_all(ProdAllArgs a, Geo g,
   // the table stays in LDS also for the power-formed twiddles (one entry a
   // step): read from the caches instead, the FP64 pass went 4.70 -> 5.18 ms
   // per step at 512x257x512 (its registers, not LDS, bound its occupancy)
-  const cpx<R>* twl = lds_twiddles(smem + ngroups * L, tw, g.nx);
+  const ProdArgs& b = a.base;
+  // kx in LDS behind the table: the emit loops then hold no global load, whose
+  // wait would also wait for the stores of the previous items
+  R* kxl = reinterpret_cast<R*>(smem + ngroups * L + g.nx);
+  for (int t = threadIdx.x; t < g.nq; t += blockDim.x) kxl[t] = (R)b.kx[t];
+  const cpx<R>* twl = lds_twiddles(smem + ngroups * L, tw, g.nx);  // synchronises
   const int j = j0 + blockIdx.y, k0 = blockIdx.x * KT;
   const R inv = (R)1 / (R)g.nx;
-  const ProdArgs& b = a.base;
   // plane j of each field (uniform across the block); per-thread offsets
   // within a plane are 32-bit
   const long long pp = g.pp;
@@ -464,7 +468,7 @@ __global__ void __launch_bounds__(512) k_xprod_all(ProdAllArgs a, Geo g,
       ld8(U3, pp, Cs);  // u1(j+1) is done with
       if (r1) {
         fft_pair(va, vb);
-        emit_pair(smem, L, g, lg_ng, 0, inv, b.kx, outp(0), (cpx<R>*)nullptr, m0, nrow);
+        emit_pair(smem, L, g, lg_ng, 0, inv, kxl, outp(0), (cpx<R>*)nullptr, m0, nrow);
         __syncthreads();
       }
       // pair 3: (W1, V2) -> slot 4                                      (:356-365, 440-449)
@@ -480,7 +484,7 @@ __global__ void __launch_bounds__(512) k_xprod_all(ProdAllArgs a, Geo g,
       ld8(U3, 0, D);
       if (r2) {
         fft_pair(va, vb);
-        emit_pair(smem, L, g, lg_ng, 3, inv, b.kx, outp(4), (cpx<R>*)nullptr, m0, nrow);
+        emit_pair(smem, L, g, lg_ng, 3, inv, kxl, outp(4), (cpx<R>*)nullptr, m0, nrow);
         __syncthreads();
       }
       // pair 1: (u3u1, V3) -> slots 1, 2                                (:298-305, 419-428)
@@ -497,7 +501,7 @@ __global__ void __launch_bounds__(512) k_xprod_all(ProdAllArgs a, Geo g,
       }
       if (r1) {
         fft_pair(va, vb);
-        emit_pair(smem, L, g, lg_ng, 1, inv, b.kx, outp(1), outp(2), m0, nrow);
+        emit_pair(smem, L, g, lg_ng, 1, inv, kxl, outp(1), outp(2), m0, nrow);
         __syncthreads();
       }
       // pair 2: (u3u3, W3) -> slots 3, 5                                (:337-344, 377-386)
@@ -511,7 +515,7 @@ __global__ void __launch_bounds__(512) k_xprod_all(ProdAllArgs a, Geo g,
       if (a.nth > 0) ld8(thp_of(0), -pp, A);  // u3(j-1) is done with
       if (r1 || r2) {
         fft_pair(va, vb);
-        emit_pair(smem, L, g, lg_ng, 2, inv, b.kx, outp(3), outp(5), m0, nrow);
+        emit_pair(smem, L, g, lg_ng, 2, inv, kxl, outp(3), outp(5), m0, nrow);
         __syncthreads();
       }
       // scalars: (th u1, Vt) -> slot 6+2n, (th u3, 0) -> slot 7+2n     (:557-600)
@@ -532,7 +536,7 @@ __global__ void __launch_bounds__(512) k_xprod_all(ProdAllArgs a, Geo g,
         }
         if (act) {
           fft_pair(va, vb);
-          emit_pair(smem, L, g, lg_ng, 4, inv, b.kx, outp(6 + 2 * n), (cpx<R>*)nullptr, m0, nrow);
+          emit_pair(smem, L, g, lg_ng, 4, inv, kxl, outp(6 + 2 * n), (cpx<R>*)nullptr, m0, nrow);
           __syncthreads();
 #pragma unroll
           for (int r = 0; r < 8; ++r) {
@@ -543,7 +547,7 @@ __global__ void __launch_bounds__(512) k_xprod_all(ProdAllArgs a, Geo g,
         if (more) ld8(thp_of(n + 1), 0, T);
         if (act) {
           fft_pair(va, vb);
-          emit_pair(smem, L, g, lg_ng, 5, inv, b.kx, outp(7 + 2 * n), (cpx<R>*)nullptr, m0, nrow);
+          emit_pair(smem, L, g, lg_ng, 5, inv, kxl, outp(7 + 2 * n), (cpx<R>*)nullptr, m0, nrow);
           __syncthreads();
         }
       }
@@ -575,7 +579,7 @@ __global__ void __launch_bounds__(512) k_xprod_all(ProdAllArgs a, Geo g,
         vb[r] = (u1p[r] * u2p[r] + u1c[r] * u2p[r] - u1c[r] * u2c[r] - u1m[r] * u2c[r]) * c2dyf;
       }
       fft_pair(va, vb);
-      emit_pair(smem, L, g, lg_ng, 0, inv, b.kx, outp(0), (cpx<R>*)nullptr, m0, nrow);
+      emit_pair(smem, L, g, lg_ng, 0, inv, kxl, outp(0), (cpx<R>*)nullptr, m0, nrow);
       __syncthreads();
     }
     // pair 3: (W1, V2) -> slot 4 = ikx F(W1) + F(V2)                   (:356-365, 440-449)
@@ -587,7 +591,7 @@ __global__ void __launch_bounds__(512) k_xprod_all(ProdAllArgs a, Geo g,
         vb[r] = (quarter * (sp * sp) - quarter * (sm * sm)) * cdy;
       }
       fft_pair(va, vb);
-      emit_pair(smem, L, g, lg_ng, 3, inv, b.kx, outp(4), (cpx<R>*)nullptr, m0, nrow);
+      emit_pair(smem, L, g, lg_ng, 3, inv, kxl, outp(4), (cpx<R>*)nullptr, m0, nrow);
       __syncthreads();
     }
     // u3 at j-1, j, j+1 replaces u1 at j-1, j+1 and u2 at j-1
@@ -609,7 +613,7 @@ __global__ void __launch_bounds__(512) k_xprod_all(ProdAllArgs a, Geo g,
         vb[r] = (u3p[r] * u2p[r] + u3c[r] * u2p[r] - u3c[r] * u2c[r] - u3m[r] * u2c[r]) * c2dyf;
       }
       fft_pair(va, vb);
-      emit_pair(smem, L, g, lg_ng, 1, inv, b.kx, outp(1), outp(2), m0, nrow);
+      emit_pair(smem, L, g, lg_ng, 1, inv, kxl, outp(1), outp(2), m0, nrow);
       __syncthreads();
     }
     // pair 2: (u3u3, W3) -> slot 3 = F(u3u3), slot 5 = F(W3)            (:337-344, 377-386)
@@ -620,7 +624,7 @@ __global__ void __launch_bounds__(512) k_xprod_all(ProdAllArgs a, Geo g,
         vb[r] = r2 ? ((dyfj * u3c[r] + dyfm * u3m[r]) * c2dy) * u2c[r] : zero;
       }
       fft_pair(va, vb);
-      emit_pair(smem, L, g, lg_ng, 2, inv, b.kx, outp(3), outp(5), m0, nrow);
+      emit_pair(smem, L, g, lg_ng, 2, inv, kxl, outp(3), outp(5), m0, nrow);
       __syncthreads();
     }
     // scalars: (th u1, Vt) -> slot 6+2n, (th u3, 0) -> slot 7+2n       (:557-600)
@@ -641,7 +645,7 @@ __global__ void __launch_bounds__(512) k_xprod_all(ProdAllArgs a, Geo g,
         vb[r] = (thp[r] * u2p[r] + thc[r] * u2p[r] - thc[r] * u2c[r] - thm[r] * u2c[r]) * c2dyf;
       }
       fft_pair(va, vb);
-      emit_pair(smem, L, g, lg_ng, 4, inv, b.kx, outp(6 + 2 * n), (cpx<R>*)nullptr, m0, nrow);
+      emit_pair(smem, L, g, lg_ng, 4, inv, kxl, outp(6 + 2 * n), (cpx<R>*)nullptr, m0, nrow);
       __syncthreads();
 #pragma unroll
       for (int r = 0; r < 8; ++r) {
@@ -649,7 +653,7 @@ __global__ void __launch_bounds__(512) k_xprod_all(ProdAllArgs a, Geo g,
         vb[r] = zero;
       }
       fft_pair(va, vb);
-      emit_pair(smem, L, g, lg_ng, 5, inv, b.kx, outp(7 + 2 * n), (cpx<R>*)nullptr, m0, nrow);
+      emit_pair(smem, L, g, lg_ng, 5, inv, kxl, outp(7 + 2 * n), (cpx<R>*)nullptr, m0, nrow);
       __syncthreads();
     }
   }
