@@ -638,12 +638,12 @@ const cpx<float>* twz_of<float>(const dgpu_handle* h) {
 // transforms one KT-row tile needs), at most max_threads threads, within the
 // LDS limit.  All of a tile's transforms in one round when they fit.
 template <typename R>
-int x_ngroups(const dgpu_handle* h, int jobs, int max_threads = 512) {
+int x_ngroups(const dgpu_handle* h, int jobs, int max_threads = 512, size_t extra = 0) {
   const Geo& g = h->g;
   const int G = g.nx / 8;
   int ng = jobs;
-  // the group buffers and the twiddle table (lds_twiddles)
-  auto lds = [&](int n) { return ((size_t)n * x_group_stride(g.nx) + g.nx) * sizeof(cpx<R>); };
+  // the group buffers and the twiddle table (lds_twiddles), plus extra bytes
+  auto lds = [&](int n) { return ((size_t)n * x_group_stride(g.nx) + g.nx) * sizeof(cpx<R>) + extra; };
   while (ng > 1 && (ng * G > max_threads || lds(ng) > h->lds_limit)) ng /= 2;
   if (ng * G > max_threads || lds(ng) > h->lds_limit)
     throw ApiError{DGPU_EUNSUP, "Nx too large for the x-transform pass"};
@@ -1872,8 +1872,10 @@ void rk_substep_t(dgpu_handle* h, int rk_step, double time, int time_step) {
     for (int s = 0; s < nslot; ++s) pa.out[s] = h->tpool[s].p;
     constexpr int KT = XTile<R>::KT;
     // a row's group needs Nx/8 lanes whatever the limit
-    const int ngx = x_ngroups<R>(h, KT, std::max(sizeof(R) == 8 ? h->kn.xprod_threads : h->kn.xprod_threads32, g.nx / 8));
-    const size_t lds = x_lds<R>(h, ngx) + g.nq * sizeof(R);  // + kx behind the table
+    const size_t kxb = g.nq * sizeof(R);  // kx behind the table
+    const int ngx = x_ngroups<R>(h, KT, std::max(sizeof(R) == 8 ? h->kn.xprod_threads : h->kn.xprod_threads32, g.nx / 8),
+                                 kxb);
+    const size_t lds = x_lds<R>(h, ngx) + kxb;
     if (lds > h->lds_limit) throw ApiError{DGPU_EUNSUP, "Nx too large for the products pass"};
     dim3 grid((g.nz + KT - 1) / KT, Ny), block(ngx * (g.nx / 8));
     // reads u1, u2, u3, th once (neighbour planes from cache); writes the product x-spectra
